@@ -1,0 +1,248 @@
+/*
+ * ppl_amd.h -- C ABI of the MI355X-native pulsar-search chain.
+ *
+ * Plain C: device pointers, sizes and a hipStream_t passed as void*.  No
+ * exception crosses this boundary; every entry point returns PPL_OK (0) or
+ * a negative PPL_E* code and leaves a message in ppl_last_error().
+ *
+ * Each entry point names the pelican-lofar interface it replaces
+ * (file:line relative to the pelican-lofar source tree).
+ */
+#ifndef PPL_AMD_H
+#define PPL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PPL_OK = 0,
+    PPL_E_ARG = -1,      /* invalid argument or shape */
+    PPL_E_HIP = -2,      /* HIP runtime error */
+    PPL_E_NOMEM = -3,    /* allocation failed */
+    PPL_E_CONFIG = -4,   /* configuration rejected (e.g. maxshift >= buffer) */
+    PPL_E_NODEV = -5     /* no GPU / not initialised */
+};
+
+/* Element types of dedispersion input buffers. */
+enum { PPL_F32 = 0, PPL_F16 = 1, PPL_U8 = 2 };
+
+/* Channeliser DFT precision. */
+enum {
+    PPL_DFT_FP32 = 0,    /* FP32 FIR + FP32 FFT (the plain method) */
+    PPL_DFT_BF16X3 = 1,  /* DFT as three BF16 MFMA GEMMs (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+    PPL_DFT_FP16X2 = 2   /* DFT as two FP16 MFMA GEMMs (hi*hi + lo*hi, data split), FP32 accumulate */
+};
+
+/* --------------------------------------------------------------- runtime */
+
+/* Replaces GPU_NVidia::initialiseResources (src/lib/src/GPU_NVidia.cpp:71-77):
+ * selects the device and prepares per-device state.  Idempotent. */
+int ppl_init(int device);
+int ppl_shutdown(void);
+const char* ppl_last_error(void);
+int ppl_version(void);
+
+/* Device / pinned memory and streams (replaces GPU_Param cudaMalloc/Memcpy,
+ * src/lib/src/GPU_Param.cpp:18,41,49).  Streams are created at the default
+ * priority. */
+int ppl_device_alloc(void** ptr, size_t bytes);
+int ppl_device_free(void* ptr);
+int ppl_host_alloc(void** ptr, size_t bytes);   /* pinned */
+int ppl_host_free(void* ptr);
+int ppl_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int ppl_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int ppl_memset_async(void* dst, int value, size_t bytes, void* stream);
+int ppl_stream_create(void** stream);
+int ppl_stream_destroy(void* stream);
+int ppl_stream_sync(void* stream);
+/* Enqueue a host callback after the work queued so far on the stream
+ * (the GPU_Job callback of src/lib/src/GPU_Manager.cpp:83-85). */
+int ppl_stream_callback(void* stream, void (*fn)(void* user, int status), void* user);
+
+/* ------------------------------------------------------------ dedispersion */
+
+typedef struct ppl_dedisp_plan ppl_dedisp_plan;
+
+typedef struct {
+    unsigned nChan;      /* channels in buffer order (nSubbands*nChannels) */
+    unsigned nsampBuf;   /* samples per buffer (2^timeBinsPerBufferPow2) */
+    unsigned tdms;       /* number of DM trials (dedispersionSamples) */
+    float dmLow;         /* dedispersionMinimum (clamped at 0) */
+    float dmStep;        /* dedispersionStepSize */
+    double tsamp;        /* block rate of the incoming Stokes blob, s */
+    double fch1;         /* frequencyChannel1, MHz (highest channel) */
+    double foff;         /* channelBandwidth, MHz (< 0: descending order) */
+    int dtype;           /* PPL_F32, PPL_F16 or PPL_U8 input buffer */
+} ppl_dedisp_config;
+
+typedef struct {
+    int maxshift;           /* DedispersionModule::maxshift() */
+    int remainingSamples;   /* (nsampBuf - maxshift) % 224 */
+    int nOut;               /* nsampBuf - maxshift - remainingSamples */
+    float mstartdm;         /* dmLow / tsamp (float) */
+    float mdmstep;          /* dmStep / tsamp (float) */
+    int tileDM, tileT;      /* DM x time tile chosen for this plan */
+} ppl_dedisp_info;
+
+/* DedispersionModule::resize (src/lib/src/DedispersionModule.cpp:109-190):
+ * builds the FP32 delay table on the host from the FP64 shift table exactly
+ * as the reference (:152-162, DedispersionKernel.cu:29,36), uploads it and
+ * picks the tile shape.  Fails with PPL_E_CONFIG when nsampBuf <= maxshift
+ * (the module's throw at :175-178) or when a delayed read would leave the
+ * channel row. */
+int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_config* cfg);
+int ppl_dedisp_plan_destroy(ppl_dedisp_plan* plan);
+int ppl_dedisp_plan_info(const ppl_dedisp_plan* plan, ppl_dedisp_info* info);
+/* Override the DM x time tile (the autotuner's hook); 0 keeps the default. */
+int ppl_dedisp_set_tile(ppl_dedisp_plan* plan, int tileDM, int tileT);
+/* Copy the int32 delay table [tdms][nChan] to host memory (for checks). */
+int ppl_dedisp_delays(const ppl_dedisp_plan* plan, int32_t* host_out);
+
+/* cacheDedisperseLoop (src/lib/DedispersionKernel.cu:53-86; declared at
+ * src/lib/src/DedispersionModule.cpp:21-24): out[dm][t] = sum over channels,
+ * in channel order with FP32 adds, of in[c][t + delay[dm][c]], t < nOut.
+ * in: [nChan][nsampBuf] of the plan's dtype; out: [tdms][nOut] float for
+ * F32/F16 input, int32 for U8 input (exact integer sums). */
+int ppl_dedisperse(const ppl_dedisp_plan* plan, const void* d_in, void* d_out, void* stream);
+
+/* The reference's C seam itself, with the reference's arguments (device
+ * pointers; maxshift includes the remainder; outbufSize in bytes), computed
+ * for every tdms (the reference's launch covers all rows only when
+ * tdms % 40 == 0).  Builds its delay table from dmShift on the device. */
+int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float* buff, float mstartdm,
+                              float mdmstep, int tdms, int numSamples, const float* dmShift,
+                              int maxshift, int i_nchans, void* stream);
+
+/* FP32 -> FP16 and FP32 -> offset-binary u8 conversion of a buffer
+ * (q = clamp(rint(x*inv_scale), -128, 127) + 128). */
+int ppl_convert_f16(const float* d_in, uint16_t* d_out, size_t n, void* stream);
+int ppl_quantise_u8(const float* d_in, uint8_t* d_out, size_t n, float inv_scale, void* stream);
+
+/* ---------------------------------------------------- dedispersion buffer */
+
+typedef struct ppl_ddbuf ppl_ddbuf;
+/* DedispersionBuffer (src/lib/src/DedispersionBuffer.cpp): a device-resident
+ * [nChan][nsampBuf] buffer of dtype, filled from Stokes-I spectra
+ * [t][nChan] (float) by transposition, whose last `overlap` samples
+ * (maxshift + remainingSamples) start the next buffer. */
+int ppl_ddbuf_create(ppl_ddbuf** buf, unsigned nChan, unsigned nsampBuf, unsigned overlap,
+                     int dtype, float inv_scale);
+int ppl_ddbuf_destroy(ppl_ddbuf* buf);
+/* addSamples (:98-179): appends spectra[first .. first+n) and returns in
+ * *consumed how many were taken before the buffer filled; *full is set to 1
+ * when the buffer holds nsampBuf samples. */
+int ppl_ddbuf_add(ppl_ddbuf* buf, const float* d_spectra, unsigned nT, unsigned first,
+                  unsigned* consumed, int* full, void* stream);
+/* copy (:65-96): carries the overlap into the start and empties the rest. */
+int ppl_ddbuf_rotate(ppl_ddbuf* buf, void* stream);
+void* ppl_ddbuf_data(ppl_ddbuf* buf);
+unsigned ppl_ddbuf_fill(const ppl_ddbuf* buf);
+
+/* -------------------------------------------------------------- detection */
+
+/* DedispersionAnalyser::analyse (src/lib/src/DedispersionAnalyser.cpp:30-203):
+ * dyadic boxcar threshold detection over data [tdms][nsamp].  Writes the
+ * events (dm, t, binFactor) as int32 triples plus their values, in the
+ * reference's order (dummy (0,0,1,0) first; then by DM, 2^binPow2 segment,
+ * level, position).  *count receives the total number of events; when it
+ * exceeds max_events only the first max_events are written.  Event buffers
+ * are host pointers; the call synchronises the stream. */
+int ppl_detect(const float* d_data, int tdms, int nsamp, int nChannels, int nSubbands,
+               float threshold, unsigned binPow2, int32_t* h_ev_ints, float* h_ev_vals,
+               int max_events, int* count, void* stream);
+/* Same over int32 u8-mode sums, dequantised as (S - 128*nChan) * scale. */
+int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float scale,
+                  int nChannels, int nSubbands, float threshold, unsigned binPow2,
+                  int32_t* h_ev_ints, float* h_ev_vals, int max_events, int* count,
+                  void* stream);
+
+/* ------------------------------------------------------------ channeliser */
+
+typedef struct ppl_ppf_plan ppl_ppf_plan;
+
+/* Window codes of ppl_ppf_coefficients. */
+enum { PPL_WIN_KAISER = 0, PPL_WIN_GAUSSIAN = 1, PPL_WIN_BLACKMAN = 2, PPL_WIN_HAMMING = 3,
+       PPL_WIN_HANN = 4 };
+
+/* PolyphaseCoefficients::genereateFilter (src/lib/src/PolyphaseCoefficients.cpp:
+ * 308-364), host code in FP64: nTaps*nCh coefficients, index t*nCh+c, with
+ * the 1/nCh scale and the (-1)^c sign.  PPL_WIN_HANN is this build's
+ * addition (0.5 - 0.5 cos(2 pi i/(n-1))); the reference has no Hann window. */
+int ppl_ppf_coefficients(unsigned nTaps, unsigned nCh, int window, double* out);
+
+typedef struct {
+    unsigned nSub, nPol, nCh, nTaps;   /* nPol must be 2 for Stokes */
+    const float* coeffs;               /* host, nTaps*nCh, index t*nCh+c */
+    int precision;                     /* PPL_DFT_* */
+    unsigned integrate;                /* integrateTimeBins (>= 1) */
+} ppl_ppf_config;
+
+/* PPFChanneliser (src/lib/src/PPFChanneliser.cpp:29-56,81-187): keeps the
+ * FIR history of every (subband, pol) resident on the device. */
+int ppl_ppf_plan_create(ppl_ppf_plan** plan, const ppl_ppf_config* cfg);
+int ppl_ppf_plan_destroy(ppl_ppf_plan* plan);
+int ppl_ppf_reset(ppl_ppf_plan* plan, void* stream);
+/* PPFChanneliser::run: complex int16 (re,im) voltages [sub][pol][nBlk*nCh]
+ * -> complex float spectra [nBlk][sub][pol][nCh]. */
+int ppl_ppf_run_i16(ppl_ppf_plan* plan, const int16_t* d_in, unsigned nBlk, float* d_spectra,
+                    void* stream);
+/* Same from complex float input (the TimeSeriesDataSetC32 blob layout). */
+int ppl_ppf_run_c32(ppl_ppf_plan* plan, const float* d_in, unsigned nBlk, float* d_spectra,
+                    void* stream);
+/* Fused PPF -> Stokes-I (StokesGenerator.cpp:88-99) -> time integration
+ * (StokesIntegrator.cpp:97-114): int16 voltages [sub][pol][nBlk*nCh] ->
+ * float [nBlk/integrate][sub][1][nCh].  nBlk must be a multiple of
+ * integrate. */
+int ppl_ppf_stokes_i16(ppl_ppf_plan* plan, const int16_t* d_in, unsigned nBlk, float* d_stokes,
+                       void* stream);
+
+/* StokesGenerator::run (src/lib/src/StokesGenerator.cpp:40-108):
+ * spectra [nBlk][nSub][2][nCh] complex -> [nBlk][nSub][nStokes][nCh]
+ * (nStokes 1: I; 4: I, Q, U, V). */
+int ppl_stokes(const float* d_spectra, unsigned nBlk, unsigned nSub, unsigned nCh,
+               unsigned nStokes, float* d_out, void* stream);
+/* StokesIntegrator::run (src/lib/src/StokesIntegrator.cpp:32-123). */
+int ppl_integrate(const float* d_in, unsigned nT, unsigned nSub, unsigned nPol, unsigned nCh,
+                  unsigned window, unsigned binCh, float* d_out, void* stream);
+
+/* ---------------------------------------------------------------- clipper */
+
+typedef struct ppl_clipper ppl_clipper;
+
+typedef struct {
+    const double* bp_values;  /* .bp file numbers: nChan, start, end, width, rms, median, poly... */
+    int n_values;
+    float crFactor;           /* channelRejectionRMS (default 10) */
+    float srFactor;           /* spectrumRejectionRMS (default 4) */
+    int maxHistory;           /* History maximum */
+    int zeroDM;               /* zeroDMing active */
+    int bandMatching;         /* Band matching (must be 1) */
+    int useMeanOverRMS;       /* computeRMSfromMean active (default 1) */
+} ppl_clipper_config;
+
+/* RFI_Clipper (src/lib/src/RFI_Clipper.cpp:22-104): bandpass model from the
+ * .bp values (BandPassAdapter.cpp:30-107), running state on the device. */
+int ppl_clipper_create(ppl_clipper** clip, const ppl_clipper_config* cfg);
+int ppl_clipper_destroy(ppl_clipper* clip);
+/* RFI_Clipper::run (:154-573) in place on I [nT][nSub][nPol][nCh]; W (may be
+ * NULL) receives the weights (always 1, :143,332). */
+int ppl_clipper_run(ppl_clipper* clip, float* d_I, float* d_W, unsigned nT, unsigned nSub,
+                    unsigned nPol, unsigned nCh, void* stream);
+/* Same, and also writes the clipped Stokes-I spectra transposed into a
+ * dedispersion buffer (fusing DedispersionBuffer::addSamples). */
+int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,
+                         ppl_ddbuf* buf, unsigned first, void* stream);
+/* Scalars after the last run: meanRunAve, rmsRunAve (WeightedSpectrumDataSet
+ * setMean/setRMS, :569-570); synchronises. */
+int ppl_clipper_stats(ppl_clipper* clip, float* mean, float* rms);
+/* Per-spectrum flags of the last run (1 = spectrum kept), host buffer. */
+int ppl_clipper_flags(ppl_clipper* clip, uint8_t* h_good, unsigned nT);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPL_AMD_H */

@@ -1,0 +1,416 @@
+"""Python access to the MI355X pulsar-search chain (libppl_amd.so).
+
+The compute lives in the C-ABI library built from csrc/ (HIP for gfx950);
+this module only binds it with ctypes and hands it device pointers and
+streams owned by PyTorch.  There is no CPU fallback: when the library or a
+GPU is missing every entry point raises.
+
+The directory name is not a Python identifier; load it with
+``load_package()`` from ``__graft_entry__`` (or importlib) as
+``pelican_lofar_amd``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libppl_amd.so")
+
+F32, F16, U8 = 0, 1, 2
+DFT_FP32, DFT_BF16X3, DFT_FP16X2 = 0, 1, 2
+DTYPE_NAMES = {"f32": F32, "f16": F16, "u8": U8}
+PRECISION_NAMES = {"fp32": DFT_FP32, "bf16x3": DFT_BF16X3, "fp16x2": DFT_FP16X2}
+
+_lib = None
+
+
+class PPLError(RuntimeError):
+    pass
+
+
+def build(force=False, jobs=8):
+    """Compile libppl_amd.so (hipcc --offload-arch=gfx950)."""
+    cmd = ["make", "-s", "-j%d" % jobs, "-C", HERE]
+    if force:
+        cmd.append("-B")
+    subprocess.check_call(cmd)
+
+
+class DedispConfig(C.Structure):
+    _fields_ = [("nChan", C.c_uint), ("nsampBuf", C.c_uint), ("tdms", C.c_uint),
+                ("dmLow", C.c_float), ("dmStep", C.c_float), ("tsamp", C.c_double),
+                ("fch1", C.c_double), ("foff", C.c_double), ("dtype", C.c_int)]
+
+
+class DedispInfo(C.Structure):
+    _fields_ = [("maxshift", C.c_int), ("remainingSamples", C.c_int), ("nOut", C.c_int),
+                ("mstartdm", C.c_float), ("mdmstep", C.c_float), ("tileDM", C.c_int),
+                ("tileT", C.c_int)]
+
+
+class PPFConfig(C.Structure):
+    _fields_ = [("nSub", C.c_uint), ("nPol", C.c_uint), ("nCh", C.c_uint), ("nTaps", C.c_uint),
+                ("coeffs", C.c_void_p), ("precision", C.c_int), ("integrate", C.c_uint)]
+
+
+class ClipperConfig(C.Structure):
+    _fields_ = [("bp_values", C.c_void_p), ("n_values", C.c_int), ("crFactor", C.c_float),
+                ("srFactor", C.c_float), ("maxHistory", C.c_int), ("zeroDM", C.c_int),
+                ("bandMatching", C.c_int), ("useMeanOverRMS", C.c_int)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/ppl_amd.h
+vp, u, i, f, d, sz = C.c_void_p, C.c_uint, C.c_int, C.c_float, C.c_double, C.c_size_t
+SIGNATURES = {
+    "ppl_init": (i, [i]),
+    "ppl_shutdown": (i, []),
+    "ppl_last_error": (C.c_char_p, []),
+    "ppl_version": (i, []),
+    "ppl_device_alloc": (i, [vp, sz]),
+    "ppl_device_free": (i, [vp]),
+    "ppl_host_alloc": (i, [vp, sz]),
+    "ppl_host_free": (i, [vp]),
+    "ppl_memcpy_h2d": (i, [vp, vp, sz, vp]),
+    "ppl_memcpy_d2h": (i, [vp, vp, sz, vp]),
+    "ppl_memset_async": (i, [vp, i, sz, vp]),
+    "ppl_stream_create": (i, [vp]),
+    "ppl_stream_destroy": (i, [vp]),
+    "ppl_stream_sync": (i, [vp]),
+    "ppl_stream_callback": (i, [vp, vp, vp]),
+    "ppl_dedisp_plan_create": (i, [vp, vp]),
+    "ppl_dedisp_plan_destroy": (i, [vp]),
+    "ppl_dedisp_plan_info": (i, [vp, vp]),
+    "ppl_dedisp_set_tile": (i, [vp, i, i]),
+    "ppl_dedisp_delays": (i, [vp, vp]),
+    "ppl_dedisperse": (i, [vp, vp, vp, vp]),
+    "ppl_cache_dedisperse_loop": (i, [vp, C.c_long, vp, f, f, i, i, vp, i, i, vp]),
+    "ppl_convert_f16": (i, [vp, vp, sz, vp]),
+    "ppl_quantise_u8": (i, [vp, vp, sz, f, vp]),
+    "ppl_ddbuf_create": (i, [vp, u, u, u, i, f]),
+    "ppl_ddbuf_destroy": (i, [vp]),
+    "ppl_ddbuf_add": (i, [vp, vp, u, u, vp, vp, vp]),
+    "ppl_ddbuf_rotate": (i, [vp, vp]),
+    "ppl_ddbuf_data": (vp, [vp]),
+    "ppl_ddbuf_fill": (u, [vp]),
+    "ppl_detect": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
+    "ppl_detect_u8": (i, [vp, i, i, i, f, i, i, f, u, vp, vp, i, vp, vp]),
+    "ppl_ppf_coefficients": (i, [u, u, i, vp]),
+    "ppl_ppf_plan_create": (i, [vp, vp]),
+    "ppl_ppf_plan_destroy": (i, [vp]),
+    "ppl_ppf_reset": (i, [vp, vp]),
+    "ppl_ppf_run_i16": (i, [vp, vp, u, vp, vp]),
+    "ppl_ppf_run_c32": (i, [vp, vp, u, vp, vp]),
+    "ppl_ppf_stokes_i16": (i, [vp, vp, u, vp, vp]),
+    "ppl_stokes": (i, [vp, u, u, u, u, vp, vp]),
+    "ppl_integrate": (i, [vp, u, u, u, u, u, u, vp, vp]),
+    "ppl_clipper_create": (i, [vp, vp]),
+    "ppl_clipper_destroy": (i, [vp]),
+    "ppl_clipper_run": (i, [vp, vp, vp, u, u, u, u, vp]),
+    "ppl_clipper_run_into": (i, [vp, vp, u, u, u, vp, u, vp]),
+    "ppl_clipper_stats": (i, [vp, vp, vp]),
+    "ppl_clipper_flags": (i, [vp, vp, u]),
+}
+
+
+def lib():
+    """Load libppl_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PPLError("libppl_amd.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise PPLError("ppl error %d: %s" % (rc, lib().ppl_last_error().decode()))
+    return rc
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def init(device=0):
+    import torch
+    if not torch.cuda.is_available():
+        raise PPLError("no GPU visible: the MI355X kernels need a device")
+    check(lib().ppl_init(device))
+
+
+# ----------------------------------------------------------- dedispersion
+class DedispPlan:
+    """DedispersionModule's kernel configuration (resize arithmetic + delay table)."""
+
+    def __init__(self, n_chan, nsamp_buf, tdms, dm_low, dm_step, tsamp, fch1, foff, dtype="f32"):
+        cfg = DedispConfig(n_chan, nsamp_buf, tdms, dm_low, dm_step, tsamp, fch1, foff,
+                           DTYPE_NAMES[dtype] if isinstance(dtype, str) else dtype)
+        h = C.c_void_p()
+        check(lib().ppl_dedisp_plan_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self.cfg = cfg
+        self.dtype = cfg.dtype
+        info = DedispInfo()
+        check(lib().ppl_dedisp_plan_info(h, C.byref(info)))
+        self.maxshift, self.rem, self.nout = info.maxshift, info.remainingSamples, info.nOut
+        self.mstartdm, self.mdmstep = info.mstartdm, info.mdmstep
+        self.tile = (info.tileDM, info.tileT)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ppl_dedisp_plan_destroy(self.h)
+            self.h = None
+
+    def set_tile(self, tile_dm, tile_t):
+        check(lib().ppl_dedisp_set_tile(self.h, tile_dm, tile_t))
+        info = DedispInfo()
+        check(lib().ppl_dedisp_plan_info(self.h, C.byref(info)))
+        self.tile = (info.tileDM, info.tileT)
+
+    def delays(self):
+        out = np.zeros((self.cfg.tdms, self.cfg.nChan), np.int32)
+        check(lib().ppl_dedisp_delays(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def out_dtype(self):
+        import torch
+        return torch.int32 if self.dtype == U8 else torch.float32
+
+    def run(self, d_in, d_out=None, stream=None):
+        import torch
+        if d_out is None:
+            d_out = torch.empty((self.cfg.tdms, self.nout), dtype=self.out_dtype(), device=d_in.device)
+        check(lib().ppl_dedisperse(self.h, _ptr(d_in), _ptr(d_out), _stream(stream)))
+        return d_out
+
+
+def cache_dedisperse_loop(outbuff, buff, mstartdm, mdmstep, tdms, num_samples, dm_shift, maxshift,
+                          nchans, stream=None):
+    check(lib().ppl_cache_dedisperse_loop(_ptr(outbuff), outbuff.numel() * 4, _ptr(buff), mstartdm,
+                                          mdmstep, tdms, num_samples, _ptr(dm_shift), maxshift,
+                                          nchans, _stream(stream)))
+
+
+def convert_f16(d_in, d_out, stream=None):
+    check(lib().ppl_convert_f16(_ptr(d_in), _ptr(d_out), d_in.numel(), _stream(stream)))
+
+
+def quantise_u8(d_in, d_out, inv_scale, stream=None):
+    check(lib().ppl_quantise_u8(_ptr(d_in), _ptr(d_out), d_in.numel(), inv_scale, _stream(stream)))
+
+
+class DDBuffer:
+    """Device-resident dedispersion buffer with overlap carry."""
+
+    def __init__(self, n_chan, nsamp_buf, overlap, dtype="f32", inv_scale=1.0):
+        h = C.c_void_p()
+        dt = DTYPE_NAMES[dtype] if isinstance(dtype, str) else dtype
+        check(lib().ppl_ddbuf_create(C.byref(h), n_chan, nsamp_buf, overlap, dt, inv_scale))
+        self.h, self.n_chan, self.nsamp, self.dtype = h, n_chan, nsamp_buf, dt
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ppl_ddbuf_destroy(self.h)
+            self.h = None
+
+    def add(self, d_spectra, first=0, stream=None):
+        consumed, full = C.c_uint(0), C.c_int(0)
+        n_t = d_spectra.shape[0]
+        check(lib().ppl_ddbuf_add(self.h, _ptr(d_spectra), n_t, first, C.byref(consumed),
+                                  C.byref(full), _stream(stream)))
+        return consumed.value, bool(full.value)
+
+    def rotate(self, stream=None):
+        check(lib().ppl_ddbuf_rotate(self.h, _stream(stream)))
+
+    def data_ptr(self):
+        return lib().ppl_ddbuf_data(self.h)
+
+    def fill(self):
+        return lib().ppl_ddbuf_fill(self.h)
+
+    def tensor(self):
+        """A torch view of the current buffer (shares device memory)."""
+        import torch
+        dt = {F32: torch.float32, F16: torch.float16, U8: torch.uint8}[self.dtype]
+        return _device_view(self.data_ptr(), (self.n_chan, self.nsamp), dt)
+
+
+def _device_view(ptr, shape, dtype):
+    import torch
+    n = int(np.prod(shape))
+    elt = torch.empty((), dtype=dtype).element_size()
+
+    class _Holder:
+        __cuda_array_interface__ = {
+            "shape": (n,), "typestr": {torch.float32: "<f4", torch.float16: "<f2",
+                                       torch.uint8: "|u1", torch.int32: "<i4"}[dtype],
+            "data": (ptr, False), "version": 3, "strides": (elt,)}
+    return torch.as_tensor(_Holder(), device="cuda").view(shape)
+
+
+# --------------------------------------------------------------- detection
+def detect(d_data, n_channels, n_subbands, threshold=6.0, bin_pow2=6, max_events=1 << 20,
+           stream=None):
+    """DedispersionAnalyser::analyse on a device [tdms][nsamp] float tensor."""
+    tdms, nsamp = d_data.shape
+    ints = np.zeros((max_events, 3), np.int32)
+    vals = np.zeros(max_events, np.float32)
+    count = C.c_int(0)
+    check(lib().ppl_detect(_ptr(d_data), tdms, nsamp, n_channels, n_subbands, threshold, bin_pow2,
+                           ints.ctypes.data_as(C.c_void_p), vals.ctypes.data_as(C.c_void_p),
+                           max_events, C.byref(count), _stream(stream)))
+    n = min(count.value, max_events)
+    return ints[:n].copy(), vals[:n].copy(), count.value
+
+
+def detect_u8(d_data, n_chan, scale, n_channels, n_subbands, threshold=6.0, bin_pow2=6,
+              max_events=1 << 20, stream=None):
+    tdms, nsamp = d_data.shape
+    ints = np.zeros((max_events, 3), np.int32)
+    vals = np.zeros(max_events, np.float32)
+    count = C.c_int(0)
+    check(lib().ppl_detect_u8(_ptr(d_data), tdms, nsamp, n_chan, scale, n_channels, n_subbands,
+                              threshold, bin_pow2, ints.ctypes.data_as(C.c_void_p),
+                              vals.ctypes.data_as(C.c_void_p), max_events, C.byref(count),
+                              _stream(stream)))
+    n = min(count.value, max_events)
+    return ints[:n].copy(), vals[:n].copy(), count.value
+
+
+# -------------------------------------------------------------- channeliser
+WINDOWS = {"kaiser": 0, "gaussian": 1, "blackman": 2, "hamming": 3, "hann": 4}
+
+
+def ppf_coefficients(n_taps, n_ch, window="kaiser"):
+    """FP64 FIR coefficients (PolyphaseCoefficients::genereateFilter), host code."""
+    out = np.zeros(n_taps * n_ch, np.float64)
+    check(lib().ppl_ppf_coefficients(n_taps, n_ch, WINDOWS[window], out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+class PPF:
+    """PPFChanneliser with device-resident FIR history."""
+
+    def __init__(self, n_sub, n_ch, n_taps, coeffs, precision="fp32", integrate=1, n_pol=2):
+        c = np.ascontiguousarray(coeffs, np.float32)
+        assert c.size == n_taps * n_ch
+        prec = PRECISION_NAMES[precision] if isinstance(precision, str) else precision
+        cfg = PPFConfig(n_sub, n_pol, n_ch, n_taps, c.ctypes.data_as(C.c_void_p), prec, integrate)
+        h = C.c_void_p()
+        check(lib().ppl_ppf_plan_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self.n_sub, self.n_ch, self.n_taps, self.integrate = n_sub, n_ch, n_taps, integrate
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ppl_ppf_plan_destroy(self.h)
+            self.h = None
+
+    def reset(self, stream=None):
+        check(lib().ppl_ppf_reset(self.h, _stream(stream)))
+
+    def spectra_i16(self, d_in, n_blk, d_out=None, stream=None):
+        import torch
+        if d_out is None:
+            d_out = torch.empty((n_blk, self.n_sub, 2, self.n_ch, 2), dtype=torch.float32,
+                                device=d_in.device)
+        check(lib().ppl_ppf_run_i16(self.h, _ptr(d_in), n_blk, _ptr(d_out), _stream(stream)))
+        return d_out
+
+    def spectra_c32(self, d_in, n_blk, d_out=None, stream=None):
+        import torch
+        if d_out is None:
+            d_out = torch.empty((n_blk, self.n_sub, 2, self.n_ch, 2), dtype=torch.float32,
+                                device=d_in.device)
+        check(lib().ppl_ppf_run_c32(self.h, _ptr(d_in), n_blk, _ptr(d_out), _stream(stream)))
+        return d_out
+
+    def stokes_i16(self, d_in, n_blk, d_out=None, stream=None):
+        import torch
+        if d_out is None:
+            d_out = torch.empty((n_blk // self.integrate, self.n_sub, 1, self.n_ch),
+                                dtype=torch.float32, device=d_in.device)
+        check(lib().ppl_ppf_stokes_i16(self.h, _ptr(d_in), n_blk, _ptr(d_out), _stream(stream)))
+        return d_out
+
+
+def stokes(d_spectra, n_stokes=1, d_out=None, stream=None):
+    import torch
+    n_blk, n_sub, _, n_ch, _ = d_spectra.shape
+    if d_out is None:
+        d_out = torch.empty((n_blk, n_sub, n_stokes, n_ch), dtype=torch.float32,
+                            device=d_spectra.device)
+    check(lib().ppl_stokes(_ptr(d_spectra), n_blk, n_sub, n_ch, n_stokes, _ptr(d_out),
+                           _stream(stream)))
+    return d_out
+
+
+def integrate(d_in, window, bin_ch=1, d_out=None, stream=None):
+    import torch
+    n_t, n_sub, n_pol, n_ch = d_in.shape
+    w = min(window, n_t)
+    if d_out is None:
+        d_out = torch.empty((n_t // w, n_sub, n_pol, n_ch // bin_ch), dtype=torch.float32,
+                            device=d_in.device)
+    check(lib().ppl_integrate(_ptr(d_in), n_t, n_sub, n_pol, n_ch, window, bin_ch, _ptr(d_out),
+                              _stream(stream)))
+    return d_out
+
+
+# ------------------------------------------------------------------ clipper
+class Clipper:
+    """RFI_Clipper with its running state on the device."""
+
+    def __init__(self, bp_values, cr=10.0, sr=4.0, history=10000, zero_dm=True,
+                 band_matching=True, mean_over_rms=True):
+        self._bp = np.ascontiguousarray(bp_values, np.float64)
+        cfg = ClipperConfig(self._bp.ctypes.data_as(C.c_void_p), self._bp.size, cr, sr, history,
+                            int(zero_dm), int(band_matching), int(mean_over_rms))
+        h = C.c_void_p()
+        check(lib().ppl_clipper_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ppl_clipper_destroy(self.h)
+            self.h = None
+
+    def run(self, d_I, d_W=None, stream=None):
+        n_t, n_sub, n_pol, n_ch = d_I.shape
+        check(lib().ppl_clipper_run(self.h, _ptr(d_I), _ptr(d_W), n_t, n_sub, n_pol, n_ch,
+                                    _stream(stream)))
+        return d_I
+
+    def run_into(self, d_I, ddbuf, first=0, stream=None):
+        n_t, n_sub, n_pol, n_ch = d_I.shape
+        assert n_pol == 1
+        check(lib().ppl_clipper_run_into(self.h, _ptr(d_I), n_t, n_sub, n_ch, ddbuf.h, first,
+                                         _stream(stream)))
+
+    def stats(self):
+        m, r = C.c_float(0), C.c_float(0)
+        check(lib().ppl_clipper_stats(self.h, C.byref(m), C.byref(r)))
+        return m.value, r.value
+
+    def flags(self, n_t):
+        out = np.zeros(n_t, np.uint8)
+        check(lib().ppl_clipper_flags(self.h, out.ctypes.data_as(C.c_void_p), n_t))
+        return out

@@ -1,0 +1,575 @@
+// ppl_channelise.hip -- polyphase channeliser, Stokes-I and time integration.
+//
+// Plain FP32 method.  A wave streams one run of L consecutive blocks of one
+// subband, both polarisations.  Each lane owns R = N/G channels of a block
+// (G = min(N, 64) lanes per block; for N < 64 a wave carries 64/N runs side
+// by side).  The FIR keeps the last nTaps blocks of its channels in
+// registers, so every input sample is read from HBM once (plus nTaps-1
+// history blocks per run), and the polyphase sum is evaluated in FP32 in the
+// reference's order (oldest block first, separate multiply and add,
+// src/lib/src/PPFChanneliser.cpp:276-289).  The N-point forward DFT is a
+// Cooley-Tukey split N = R x G: an R-point DFT in registers, twiddles, then
+// a G-point radix-2 DIF FFT across lanes with xor shuffles.  Stokes-I
+// (src/lib/src/StokesGenerator.cpp:93-99) and the integration over
+// `integrate` consecutive blocks (src/lib/src/StokesIntegrator.cpp:97-114)
+// are fused behind the FFT, so only int16 voltages are read and only the
+// integrated Stokes-I spectra are written.
+#include "ppl_internal.h"
+
+#include <cmath>
+#include <vector>
+
+struct ppl_ppf_plan {
+    ppl_ppf_config cfg;
+    float* d_coeffs = nullptr;    // [nTaps][nCh]
+    float* d_twiddle = nullptr;   // W_N^k, k < N, (re, im) pairs
+    uint32_t* d_hist[2] = {nullptr, nullptr};  // [sub][pol][nTaps-1][nCh] complex int16
+    float* d_histf[2] = {nullptr, nullptr};    // same for complex float input
+    int cur = 0;                  // which history buffer holds the latest blocks
+    void* mfma = nullptr;         // split-precision DFT state (ppl_channelise_mfma.hip)
+};
+
+namespace {
+
+using namespace ppl;
+
+struct C2 {
+    float re, im;
+};
+
+__device__ __forceinline__ C2 load_sample(const uint32_t* __restrict__ p, long idx)
+{
+    const uint32_t v = p[idx];
+    const int16_t re = (int16_t)(v & 0xffffu);
+    const int16_t im = (int16_t)(v >> 16);
+    return {(float)re, (float)im};
+}
+__device__ __forceinline__ C2 load_sample(const float2* __restrict__ p, long idx)
+{
+    const float2 v = p[idx];
+    return {v.x, v.y};
+}
+
+template <typename Tin> struct HistT;
+template <> struct HistT<uint32_t> { using type = uint32_t; };
+template <> struct HistT<float2> { using type = float2; };
+
+// G-point radix-2 decimation-in-frequency FFT across the lanes of a lane
+// group; lane l ends with X[bitrev(l)].
+template <int G>
+__device__ __forceinline__ void lane_fft(float& re, float& im, const float* twr, const float* twi)
+{
+    const int lane = threadIdx.x & 63;
+    int s = 0;
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1, ++s) {
+        const float pr = __shfl_xor(re, m, 64);
+        const float pi = __shfl_xor(im, m, 64);
+        if ((lane & m) == 0) {
+            re = re + pr;
+            im = im + pi;
+        } else {
+            const float dr = pr - re, di = pi - im;
+            re = dr * twr[s] - di * twi[s];
+            im = dr * twi[s] + di * twr[s];
+        }
+    }
+}
+
+template <int G>
+__device__ __forceinline__ int bitrev(int l)
+{
+    int r = 0;
+#pragma unroll
+    for (int b = 1; b < G; b <<= 1) {
+        r <<= 1;
+        r |= (l & b) ? 1 : 0;
+    }
+    return r;
+}
+
+// R-point DFT in registers (R = 1, 2, 4), forward.
+template <int R>
+__device__ __forceinline__ void reg_dft(C2* x)
+{
+    if constexpr (R == 2) {
+        const C2 a = x[0], b = x[1];
+        x[0] = {a.re + b.re, a.im + b.im};
+        x[1] = {a.re - b.re, a.im - b.im};
+    } else if constexpr (R == 4) {
+        const C2 a = x[0], b = x[1], c = x[2], d = x[3];
+        const C2 s02 = {a.re + c.re, a.im + c.im}, d02 = {a.re - c.re, a.im - c.im};
+        const C2 s13 = {b.re + d.re, b.im + d.im}, d13 = {b.re - d.re, b.im - d.im};
+        x[0] = {s02.re + s13.re, s02.im + s13.im};
+        x[2] = {s02.re - s13.re, s02.im - s13.im};
+        // X1 = d02 - i d13 ; X3 = d02 + i d13
+        x[1] = {d02.re + d13.im, d02.im - d13.re};
+        x[3] = {d02.re - d13.im, d02.im + d13.re};
+    }
+}
+
+// OUT: 0 = integrated Stokes-I [nBlk/integ][sub][1][N]; 1 = complex spectra
+// [nBlk][sub][2][N].  TAPS is the FIR length (compile-time, so the sliding
+// windows stay in registers).
+template <typename Tin, int N, int TAPS, int OUT>
+__global__ __launch_bounds__(256) void ppf_kernel(
+    const Tin* __restrict__ in, const typename HistT<Tin>::type* __restrict__ hist, float* __restrict__ out,
+    const float* __restrict__ coeffs, const float* __restrict__ tw, int nSub, long nBlk, int runLen,
+    int nRuns, int integ)
+{
+    constexpr int G = N < 64 ? N : 64;
+    constexpr int R = N / G;
+    constexpr int RUNS_PER_WAVE = 64 / G;
+    constexpr int H = TAPS - 1;
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / G;
+    const int l = lane % G;
+    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long wavesPerSub = (nRuns + RUNS_PER_WAVE - 1) / RUNS_PER_WAVE;
+    const int sub = (int)(waveId / wavesPerSub);
+    if (sub >= nSub) return;
+    const int run = (int)(waveId % wavesPerSub) * RUNS_PER_WAVE + grp;
+    const bool active = run < nRuns;
+    const long T = nBlk * N;  // samples per (sub, pol)
+    const Tin* x0 = in + (long)(2 * sub + 0) * T;
+    const Tin* x1 = in + (long)(2 * sub + 1) * T;
+    const auto* h0 = hist + (long)(2 * sub + 0) * H * N;
+    const auto* h1 = hist + (long)(2 * sub + 1) * H * N;
+
+    // coefficients of this lane's channels, twiddles of this lane
+    float hc[R][TAPS];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int t = 0; t < TAPS; ++t) hc[r][t] = coeffs[t * N + l + G * r];
+    float twr[6], twi[6];
+    {
+        int s = 0;
+#pragma unroll
+        for (int m = G / 2; m >= 1; m >>= 1, ++s) {
+            const int j = l & (m - 1);
+            const int k = j * (N / (2 * m));  // W_{2m}^j = W_N^{j N/(2m)}
+            twr[s] = tw[2 * k];
+            twi[s] = tw[2 * k + 1];
+        }
+    }
+    C2 rtw[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = (l * r) % N;
+        rtw[r] = {tw[2 * k], tw[2 * k + 1]};
+    }
+
+    // Sliding windows w[slot][r]: slot 0 oldest .. TAPS-1 newest.  Before the
+    // first step slots 1..TAPS-1 hold the TAPS-1 blocks preceding the run
+    // (from this chunk or from the stream's saved history).
+    C2 w0[TAPS][R], w1[TAPS][R];
+    const long b0 = (long)run * runLen;
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t)
+#pragma unroll
+        for (int r = 0; r < R; ++r) w0[t][r] = w1[t][r] = {0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+        const long b = b0 - H + t;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = l + G * r;
+            C2 a = {0.f, 0.f}, c = {0.f, 0.f};
+            if (active) {
+                if (b >= 0) {
+                    a = load_sample(x0, b * N + n);
+                    c = load_sample(x1, b * N + n);
+                } else {
+                    a = load_sample(h0, (long)(H + b) * N + n);
+                    c = load_sample(h1, (long)(H + b) * N + n);
+                }
+            }
+            w0[t + 1][r] = a;
+            w1[t + 1][r] = c;
+        }
+    }
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+    const int lastRunLen = (int)(nBlk - (long)(nRuns - 1) * runLen);
+    const int len = active ? (run == nRuns - 1 ? lastRunLen : runLen) : 0;
+    const int kLane = bitrev<G>(l);
+    // every lane runs every step: the lane FFT shuffles need the whole wave
+    for (int s = 0; s < runLen; ++s) {
+        const bool live = s < len;
+        const long b = b0 + s;
+#pragma unroll
+        for (int t = 0; t < TAPS - 1; ++t)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                w0[t][r] = w0[t + 1][r];
+                w1[t][r] = w1[t + 1][r];
+            }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = l + G * r;
+            w0[TAPS - 1][r] = live ? load_sample(x0, b * N + n) : C2{0.f, 0.f};
+            w1[TAPS - 1][r] = live ? load_sample(x1, b * N + n) : C2{0.f, 0.f};
+        }
+        // FIR, oldest block first, coefficient index t*N + n
+        C2 f0[R], f1[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            f0[r] = f1[r] = {0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < TAPS; ++t) {
+                const float coeff = hc[r][t];
+                const float a_re = w0[t][r].re * coeff, a_im = w0[t][r].im * coeff;
+                const float c_re = w1[t][r].re * coeff, c_im = w1[t][r].im * coeff;
+                f0[r].re = f0[r].re + a_re;
+                f0[r].im = f0[r].im + a_im;
+                f1[r].re = f1[r].re + c_re;
+                f1[r].im = f1[r].im + c_im;
+            }
+        }
+        // DFT: registers, twiddle, lanes
+        reg_dft<R>(f0);
+        reg_dft<R>(f1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const C2 a = f0[r], c = f1[r], t = rtw[r];
+            f0[r] = {a.re * t.re - a.im * t.im, a.re * t.im + a.im * t.re};
+            f1[r] = {c.re * t.re - c.im * t.im, c.re * t.im + c.im * t.re};
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            lane_fft<G>(f0[r].re, f0[r].im, twr, twi);
+            lane_fft<G>(f1[r].re, f1[r].im, twr, twi);
+        }
+        if constexpr (OUT == 1) {
+            if (live) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int k = r + R * kLane;
+                    float2* o = reinterpret_cast<float2*>(out) + ((long)(b * nSub + sub) * 2) * N + k;
+                    o[0] = make_float2(f0[r].re, f0[r].im);
+                    o[N] = make_float2(f1[r].re, f1[r].im);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float powerX = f0[r].re * f0[r].re + f0[r].im * f0[r].im;
+                const float powerY = f1[r].re * f1[r].re + f1[r].im * f1[r].im;
+                const float I = powerX + powerY;
+                acc[r] = acc[r] + I;
+            }
+            if ((s + 1) % integ == 0) {
+                if (live) {
+                    const long u = b / integ;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int k = r + R * kLane;
+                        out[(u * nSub + sub) * N + k] = acc[r];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+            }
+        }
+    }
+}
+
+// Save the last nTaps-1 blocks of every (sub, pol) stream (new history),
+// taking them from the chunk or, for short chunks, from the old history.
+template <typename Tin>
+__global__ void save_history_kernel(const Tin* __restrict__ in, const Tin* __restrict__ oldh,
+                                    Tin* __restrict__ newh, int nStreams, int H, int N, long nBlk)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)H * N;
+    if (i >= per * nStreams) return;
+    const long stream = i / per;
+    const long j = i % per;
+    const long slot = j / N, n = j % N;
+    const long b = nBlk - H + slot;
+    newh[i] = b >= 0 ? in[stream * nBlk * N + b * N + n] : oldh[stream * per + (H + b) * N + n];
+}
+
+template <typename Tin>
+int run_ppf(ppl_ppf_plan* p, const Tin* in, typename HistT<Tin>::type* histCur,
+            typename HistT<Tin>::type* histNext, unsigned nBlk, float* out, int mode, hipStream_t st)
+{
+    const int N = (int)p->cfg.nCh, S = (int)p->cfg.nSub, T = (int)p->cfg.nTaps;
+    const int integ = mode == 0 ? (int)p->cfg.integrate : 1;
+    PPL_CHECK_ARG(nBlk > 0, "Empty time data blob");
+    PPL_CHECK_ARG(nBlk % integ == 0, "nBlk (%u) must be a multiple of the integration (%d)", nBlk, integ);
+    // runs of L blocks, L a multiple of the integration window
+    int L = 256;
+    if (mode == 0) L = ((L + integ - 1) / integ) * integ;
+    if ((unsigned)L > nBlk) L = (int)nBlk;
+    const int nRuns = (int)cdiv(nBlk, L);
+    const int G = N < 64 ? N : 64;
+    const long wavesPerSub = (nRuns + (64 / G) - 1) / (64 / G);
+    const long waves = wavesPerSub * S;
+    const unsigned blocks = (unsigned)cdiv((unsigned)waves, 4);
+#define PPL_PPF_LAUNCH(N_, T_)                                                                          \
+    if (N == N_ && T == T_) {                                                                           \
+        if (mode == 0)                                                                                  \
+            hipLaunchKernelGGL((ppf_kernel<Tin, N_, T_, 0>), dim3(blocks), dim3(256), 0, st, in, histCur,\
+                               out, p->d_coeffs, p->d_twiddle, S, (long)nBlk, L, nRuns, integ);        \
+        else                                                                                            \
+            hipLaunchKernelGGL((ppf_kernel<Tin, N_, T_, 1>), dim3(blocks), dim3(256), 0, st, in, histCur,\
+                               out, p->d_coeffs, p->d_twiddle, S, (long)nBlk, L, nRuns, 1);            \
+        PPL_LAUNCHED();                                                                                 \
+        launched = true;                                                                                \
+    }
+#define PPL_PPF_CASE(N_) PPL_PPF_LAUNCH(N_, 4) PPL_PPF_LAUNCH(N_, 8) PPL_PPF_LAUNCH(N_, 16)
+    bool launched = false;
+    PPL_PPF_CASE(8)
+    PPL_PPF_CASE(16)
+    PPL_PPF_CASE(32)
+    PPL_PPF_CASE(64)
+    PPL_PPF_CASE(128)
+    PPL_PPF_CASE(256)
+#undef PPL_PPF_CASE
+#undef PPL_PPF_LAUNCH
+    if (!launched) {
+        set_error("channeliser: no kernel for %d channels x %d taps", N, T);
+        return PPL_E_ARG;
+    }
+    const int H = T - 1;
+    if (H > 0) {
+        const long n = (long)S * 2 * H * N;
+        hipLaunchKernelGGL((save_history_kernel<typename HistT<Tin>::type>), dim3((unsigned)cdiv((unsigned)n, 256)),
+                           dim3(256), 0, st, reinterpret_cast<const typename HistT<Tin>::type*>(in),
+                           histCur, histNext, S * 2, H, N, (long)nBlk);
+        PPL_LAUNCHED();
+    }
+    return PPL_OK;
+}
+
+bool pow2_supported(unsigned n) { return n >= 8 && n <= 256 && (n & (n - 1)) == 0; }
+
+// Stokes parameters from [nBlk][nSub][2][nCh] complex spectra,
+// src/lib/src/StokesGenerator.cpp:88-104.
+__global__ void stokes_kernel(const float2* __restrict__ in, float* __restrict__ out, long nSpec,
+                              int nCh, int nStokes)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // over nBlk*nSub*nCh
+    if (i >= nSpec * nCh) return;
+    const long spec = i / nCh;
+    const int c = (int)(i % nCh);
+    const float2 X = in[(spec * 2 + 0) * nCh + c];
+    const float2 Y = in[(spec * 2 + 1) * nCh + c];
+    const float Xr = X.x, Xi = X.y, Yr = Y.x, Yi = Y.y;
+    const float XxYstarReal = Xr * Yr + Xi * Yi;
+    const float XxYstarImag = Xi * Yr - Xr * Yi;
+    const float powerX = Xr * Xr + Xi * Xi;
+    const float powerY = Yr * Yr + Yi * Yi;
+    float* o = out + spec * nStokes * nCh + c;
+    o[0] = powerX + powerY;
+    if (nStokes == 4) {
+        o[nCh] = powerX - powerY;
+        o[2 * nCh] = 2.0f * XxYstarReal;
+        o[3 * nCh] = 2.0f * XxYstarImag;
+    }
+}
+
+// src/lib/src/StokesIntegrator.cpp:97-114: out starts at 0 and adds, for
+// each time bin of the window in order, the binCh channels of the bin.
+__global__ void integrate_kernel(const float* __restrict__ in, float* __restrict__ out, long newT,
+                                 int nSub, int nPol, int nCh, int window, int binCh)
+{
+    const int newC = nCh / binCh;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // over newT*nSub*nPol*newC
+    const long total = newT * nSub * nPol * newC;
+    if (i >= total) return;
+    const int nc = (int)(i % newC);
+    const long rest = i / newC;  // (u*nSub + s)*nPol + p
+    const int p = (int)(rest % nPol);
+    const long us = rest / nPol;
+    const int s = (int)(us % nSub);
+    const long u = us / nSub;
+    float acc = 0.0f;
+    for (int w = 0; w < window; ++w) {
+        const long t = u * window + w;
+        const float* v = in + ((t * nSub + s) * nPol + p) * nCh + (long)nc * binCh;
+        for (int c = 0; c < binCh; ++c) acc = acc + v[c];
+    }
+    out[i] = acc;
+}
+
+}  // namespace
+
+// split-precision DFT path (ppl_channelise_mfma.hip)
+int ppl_mfma_create(ppl_ppf_plan* p);
+void ppl_mfma_destroy(ppl_ppf_plan* p);
+int ppl_mfma_run(ppl_ppf_plan* p, const int16_t* d_in, const uint32_t* histCur, unsigned nBlk,
+                 float* d_out, int mode, hipStream_t st);
+
+extern "C" int ppl_ppf_plan_create(ppl_ppf_plan** plan, const ppl_ppf_config* cfg)
+{
+    PPL_CHECK_ARG(plan && cfg && cfg->coeffs, "null argument");
+    PPL_CHECK_ARG(cfg->nPol == 2, "the channeliser kernels take 2 polarisations (got %u)", cfg->nPol);
+    PPL_CHECK_ARG(pow2_supported(cfg->nCh),
+                  "outputChannelsPerSubband %u: a power of two in [8, 256] is required", cfg->nCh);
+    PPL_CHECK_ARG(cfg->nTaps == 4 || cfg->nTaps == 8 || cfg->nTaps == 16,
+                  "nTaps %u: the channeliser is built for 4, 8 or 16 taps", cfg->nTaps);
+    PPL_CHECK_ARG(cfg->nSub >= 1, "no subbands");
+    PPL_CHECK_ARG(cfg->integrate >= 1, "integrateTimeBins must be >= 1");
+    PPL_CHECK_ARG(cfg->precision == PPL_DFT_FP32 || cfg->precision == PPL_DFT_BF16X3 ||
+                      cfg->precision == PPL_DFT_FP16X2,
+                  "unknown precision %d", cfg->precision);
+    ppl_ppf_plan* p = new ppl_ppf_plan;
+    p->cfg = *cfg;
+    p->cfg.coeffs = nullptr;
+    const unsigned N = cfg->nCh, T = cfg->nTaps;
+    std::vector<float> tw(2 * N);
+    for (unsigned k = 0; k < N; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)N;
+        tw[2 * k] = (float)std::cos(a);
+        tw[2 * k + 1] = (float)std::sin(a);
+    }
+    auto fail = [&](int rc) {
+        ppl_ppf_plan_destroy(p);
+        return rc;
+    };
+    if (hipMalloc(&p->d_coeffs, sizeof(float) * N * T) != hipSuccess ||
+        hipMalloc(&p->d_twiddle, sizeof(float) * 2 * N) != hipSuccess) {
+        set_error("channeliser: device allocation failed");
+        return fail(PPL_E_NOMEM);
+    }
+    const size_t hist = (size_t)cfg->nSub * 2 * (T > 1 ? T - 1 : 1) * N;
+    for (int i = 0; i < 2; ++i) {
+        if (hipMalloc(&p->d_hist[i], hist * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&p->d_histf[i], hist * sizeof(float2)) != hipSuccess) {
+            set_error("channeliser: history allocation failed");
+            return fail(PPL_E_NOMEM);
+        }
+        if (hipMemset(p->d_hist[i], 0, hist * sizeof(uint32_t)) != hipSuccess ||
+            hipMemset(p->d_histf[i], 0, hist * sizeof(float2)) != hipSuccess)
+            return fail(PPL_E_HIP);
+    }
+    if (hipMemcpy(p->d_coeffs, cfg->coeffs, sizeof(float) * N * T, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_twiddle, tw.data(), sizeof(float) * 2 * N, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("channeliser: upload failed");
+        return fail(PPL_E_HIP);
+    }
+    if (cfg->precision != PPL_DFT_FP32) {
+        int rc = ppl_mfma_create(p);
+        if (rc != PPL_OK) return fail(rc);
+    }
+    *plan = p;
+    return PPL_OK;
+}
+
+extern "C" int ppl_ppf_plan_destroy(ppl_ppf_plan* p)
+{
+    if (!p) return PPL_OK;
+    if (p->mfma) ppl_mfma_destroy(p);
+    (void)hipFree(p->d_coeffs);
+    (void)hipFree(p->d_twiddle);
+    for (int i = 0; i < 2; ++i) {
+        (void)hipFree(p->d_hist[i]);
+        (void)hipFree(p->d_histf[i]);
+    }
+    delete p;
+    return PPL_OK;
+}
+
+extern "C" int ppl_ppf_reset(ppl_ppf_plan* p, void* stream)
+{
+    PPL_CHECK_ARG(p, "null plan");
+    const size_t hist = (size_t)p->cfg.nSub * 2 * (p->cfg.nTaps > 1 ? p->cfg.nTaps - 1 : 1) * p->cfg.nCh;
+    for (int i = 0; i < 2; ++i) {
+        PPL_HIP(hipMemsetAsync(p->d_hist[i], 0, hist * sizeof(uint32_t), stream_of(stream)));
+        PPL_HIP(hipMemsetAsync(p->d_histf[i], 0, hist * sizeof(float2), stream_of(stream)));
+    }
+    return PPL_OK;
+}
+
+extern "C" int ppl_ppf_run_i16(ppl_ppf_plan* p, const int16_t* d_in, unsigned nBlk, float* d_spectra,
+                               void* stream)
+{
+    PPL_CHECK_ARG(p && d_in && d_spectra, "null argument");
+    int rc;
+    if (p->cfg.precision != PPL_DFT_FP32)
+        rc = ppl_mfma_run(p, d_in, p->d_hist[p->cur], nBlk, d_spectra, 1, stream_of(stream));
+    else
+        rc = run_ppf<uint32_t>(p, reinterpret_cast<const uint32_t*>(d_in), p->d_hist[p->cur],
+                               p->d_hist[p->cur ^ 1], nBlk, d_spectra, 1, stream_of(stream));
+    if (rc == PPL_OK) p->cur ^= 1;
+    return rc;
+}
+
+extern "C" int ppl_ppf_run_c32(ppl_ppf_plan* p, const float* d_in, unsigned nBlk, float* d_spectra,
+                               void* stream)
+{
+    PPL_CHECK_ARG(p && d_in && d_spectra, "null argument");
+    PPL_CHECK_ARG(p->cfg.precision == PPL_DFT_FP32, "complex-float input runs the FP32 method only");
+    float2* hc = reinterpret_cast<float2*>(p->d_histf[p->cur]);
+    float2* hn = reinterpret_cast<float2*>(p->d_histf[p->cur ^ 1]);
+    int rc = run_ppf<float2>(p, reinterpret_cast<const float2*>(d_in), hc, hn, nBlk, d_spectra, 1,
+                             stream_of(stream));
+    if (rc == PPL_OK) p->cur ^= 1;
+    return rc;
+}
+
+extern "C" int ppl_ppf_stokes_i16(ppl_ppf_plan* p, const int16_t* d_in, unsigned nBlk, float* d_stokes,
+                                  void* stream)
+{
+    PPL_CHECK_ARG(p && d_in && d_stokes, "null argument");
+    int rc;
+    if (p->cfg.precision != PPL_DFT_FP32)
+        rc = ppl_mfma_run(p, d_in, p->d_hist[p->cur], nBlk, d_stokes, 0, stream_of(stream));
+    else
+        rc = run_ppf<uint32_t>(p, reinterpret_cast<const uint32_t*>(d_in), p->d_hist[p->cur],
+                               p->d_hist[p->cur ^ 1], nBlk, d_stokes, 0, stream_of(stream));
+    if (rc == PPL_OK) p->cur ^= 1;
+    return rc;
+}
+
+// history buffers for the split-precision path
+uint32_t* ppl_ppf_hist_next(ppl_ppf_plan* p) { return p->d_hist[p->cur ^ 1]; }
+const ppl_ppf_config& ppl_ppf_cfg(const ppl_ppf_plan* p) { return p->cfg; }
+const float* ppl_ppf_coeffs(const ppl_ppf_plan* p) { return p->d_coeffs; }
+void*& ppl_ppf_mfma_slot(ppl_ppf_plan* p) { return p->mfma; }
+int ppl_ppf_save_history(ppl_ppf_plan* p, const int16_t* d_in, unsigned nBlk, hipStream_t st)
+{
+    const int N = (int)p->cfg.nCh, S = (int)p->cfg.nSub, H = (int)p->cfg.nTaps - 1;
+    if (H <= 0) return PPL_OK;
+    const long n = (long)S * 2 * H * N;
+    hipLaunchKernelGGL(save_history_kernel<uint32_t>, dim3((unsigned)cdiv((unsigned)n, 256)), dim3(256), 0, st,
+                       reinterpret_cast<const uint32_t*>(d_in), p->d_hist[p->cur], p->d_hist[p->cur ^ 1],
+                       S * 2, H, N, (long)nBlk);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
+extern "C" int ppl_stokes(const float* d_spectra, unsigned nBlk, unsigned nSub, unsigned nCh,
+                          unsigned nStokes, float* d_out, void* stream)
+{
+    PPL_CHECK_ARG(d_spectra && d_out, "null argument");
+    PPL_CHECK_ARG(nStokes == 1 || nStokes == 4,
+                  "You can either generate 1 or 4 Stokes parameters (got %u)", nStokes);
+    const long nSpec = (long)nBlk * nSub;
+    const long n = nSpec * nCh;
+    if (!n) return PPL_OK;
+    hipLaunchKernelGGL(stokes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream_of(stream),
+                       reinterpret_cast<const float2*>(d_spectra), d_out, nSpec, (int)nCh, (int)nStokes);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
+extern "C" int ppl_integrate(const float* d_in, unsigned nT, unsigned nSub, unsigned nPol, unsigned nCh,
+                             unsigned window, unsigned binCh, float* d_out, void* stream)
+{
+    PPL_CHECK_ARG(d_in && d_out, "null argument");
+    PPL_CHECK_ARG(window >= 1 && binCh >= 1, "window and channel bin must be >= 1");
+    if (window > nT) window = nT;  // StokesIntegrator.cpp:53-56
+    if (!window) return PPL_OK;
+    const long newT = nT / window;
+    const long n = newT * nSub * nPol * (nCh / binCh);
+    if (!n) return PPL_OK;
+    hipLaunchKernelGGL(integrate_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream_of(stream),
+                       d_in, d_out, newT, (int)nSub, (int)nPol, (int)nCh, (int)window, (int)binCh);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}

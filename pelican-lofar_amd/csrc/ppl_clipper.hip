@@ -1,0 +1,741 @@
+// ppl_clipper.hip -- RFI clipper (src/lib/src/RFI_Clipper.cpp:154-573) on the
+// device, bit-exact with the reference's FP32/FP64 expression forms.
+//
+// The clipper is a recurrence over spectra: every spectrum updates running
+// means, ring buffers and the bandpass model that the next spectrum is
+// judged against.  The work splits in two:
+//
+//  * walk  (one workgroup, sequential over spectra): everything that feeds
+//    the recurrence -- the learning-phase band statistics, the channel mask,
+//    the ordered sum of the kept channels, the spectrum decision, the ring
+//    buffers, the last-good spectrum and the bandpass offset.  Elementwise
+//    work is spread over the workgroup; sums whose rounding the reference
+//    fixes by its loop order are taken in that order by one lane.  For each
+//    spectrum it records the state the rest of the spectrum needs.
+//  * apply (parallel over spectra): replaces clipped spectra, flattens by
+//    the bandpass, takes the ordered mean/RMS of each spectrum (one lane per
+//    spectrum, LDS-staged tiles) and normalises.  These values never feed
+//    back into the recurrence.
+//
+// The bandpass values are scale * evaluate(v_i) with evaluate's float
+// accumulation of double terms (BandPass.cpp:159-166); the terms
+// params[j] * pow(v_i, j), j >= 1, do not change after construction and are
+// tabulated on the host with the C library's pow, so only the offset
+// params[0] moves on the device.
+#include "ppl_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+using namespace ppl;
+
+constexpr int kMaxParams = 8;
+
+// ---------------------------------------------------------------- host side:
+// BandPass / BinMap restatement used to set the model up
+// (src/lib/src/BandPass.cpp:26-108, src/lib/src/BinMap.cpp:49-70,
+// src/lib/src/BandPassAdapter.cpp:76-100).
+struct HostMap {
+    unsigned nBins = 0;
+    double lower = 0.0, width = 1.0;
+    double binAssignmentNumber(int i) const { return lower + (width * i); }
+    double endValue() const { return lower + width * nBins; }
+};
+
+struct HostBandPass {
+    HostMap primary, current;
+    std::vector<float> params;
+    // values of the primary map and the map-scoped statistics
+    float primaryMean = 0, primaryMedian = 0, primaryRms = 0;
+    float curMean = 0, curMedian = 0, curRms = 0;
+    bool currentIsPrimary = true;
+
+    float evaluate(float v) const
+    {
+        float tot = 0.0f;
+        for (size_t i = 0; i < params.size(); ++i) tot += (double)params[i] * std::pow((double)v, (double)(int)i);
+        return tot;
+    }
+    std::vector<float> build(const HostMap& m, float scale) const
+    {
+        std::vector<float> d(m.nBins);
+        for (unsigned i = 0; i < m.nBins; ++i) d[i] = scale * evaluate((float)m.binAssignmentNumber(i));
+        return d;
+    }
+    // setData (:26-42): the primary map is the current map, QHash default 0
+    void setData(const HostMap& m, const std::vector<float>& p)
+    {
+        primary = current = m;
+        currentIsPrimary = true;
+        params = p;
+        // reBin(primary): scale 1, rms/median/mean = 0 * ...
+        primaryRms = 0.0f * std::sqrt(1.0 / 1.0);
+        primaryMedian = 0.0f;
+        std::vector<float> d = build(m, 1.0f);
+        float mean = 0;
+        for (unsigned i = 0; i < m.nBins; ++i) mean += d[i];
+        mean /= m.nBins;
+        primaryMean = mean;
+        std::vector<float> copy = d;
+        std::nth_element(copy.begin(), copy.begin() + copy.size() / 2, copy.end());
+        primaryMedian = copy[copy.size() / 2];
+        curMean = primaryMean;
+        curMedian = primaryMedian;
+        curRms = primaryRms;
+    }
+    // setMedian (:48-61) while the current map is the primary map: the two
+    // QHash entries are one, so the mean moves by delta twice.
+    void setMedian(float median)
+    {
+        float delta = median - curMedian;
+        if (std::fabs(delta) > 0.0f) {
+            float scale = current.width / primary.width;
+            curMedian = median;
+            curMedian = median / scale;  // _median[primary] (same entry)
+            params[0] += delta / scale;
+            curMean += delta;
+            curMean += delta / scale;
+            primaryMedian = curMedian;
+            primaryMean = curMean;
+        }
+    }
+    void setRMS(float r)
+    {
+        curRms = r;
+        if (currentIsPrimary) primaryRms = r;
+    }
+    // reBin to the run map (:81-99), a map distinct from the primary one
+    void reBin(const HostMap& m)
+    {
+        current = m;
+        currentIsPrimary = (m.nBins == primary.nBins && m.lower == primary.lower && m.width == primary.width);
+        if (currentIsPrimary) return;
+        double scale = m.width / primary.width;
+        curRms = primaryRms * std::sqrt(1.0 / scale);
+        curMedian = primaryMedian * scale;
+        curMean = primaryMean * scale;
+    }
+};
+
+// ------------------------------------------------------------ device state
+struct ClipScalars {
+    float meanRunAve, rmsRunAve, meanOverRMS, fractionBadChannels;
+    float params0, bpMean, bpRms, spectrumRMS;
+    int badSpectra, num, remainingZeros;
+    int meanStart, meanSize, rmsStart, rmsSize;
+    int initialised;
+};
+
+struct ClipConst {
+    int nBins, nSub, nPol, nCh, cap;
+    int nTerms;              // params.size() - 1 tabulated terms
+    float scale;             // float(current.width / primary.width)
+    float crFactor, srFactor;
+    float meanMinusMinimum;  // float(4 / sqrt(8.0))
+    double sqrtNBins;        // sqrt((double)nBins)
+    int zeroDM, useMeanOverRMS, maxHistory;
+};
+
+// Per-spectrum record written by the walk for the apply kernel.
+struct SpecRec {
+    float params0;     // bandpass offset used by this spectrum
+    float dataModel;
+    float meanRunAve;  // for the non-zero-DM normalisation
+    int flags;         // bit0: spectrum kept; bit1: channel replacement already applied
+};
+
+__device__ __forceinline__ float bp_value(const double* __restrict__ terms, int nTerms, int nBins, int i,
+                                          float p0, float scale)
+{
+    // evaluate(): tot = 0; tot += params[0]*1; tot += params[j]*pow(v,j)
+    float tot = 0.0f;
+    tot = (float)((double)tot + (double)p0 * 1.0);
+    for (int j = 0; j < nTerms; ++j) tot = (float)((double)tot + terms[(size_t)j * nBins + i]);
+    return scale * tot;
+}
+
+struct RingView {
+    float* buf;
+    double* prefix;  // prefix[k] = ((0 + e0) + ...) + e_{k-1} in double, valid while not full
+    int cap;
+};
+
+__device__ __forceinline__ void ring_push(RingView r, int& start, int& size, float v)
+{
+    if (r.cap == 0) return;
+    if (size < r.cap) {
+        r.buf[(start + size) % r.cap] = v;
+        r.prefix[size + 1] = r.prefix[size] + (double)v;
+        ++size;
+    } else {
+        r.buf[start] = v;
+        start = (start + 1) % r.cap;
+    }
+}
+__device__ __forceinline__ float ring_front(RingView r, int start) { return r.buf[start]; }
+__device__ __forceinline__ float ring_back(RingView r, int start, int size)
+{
+    // back() of an empty buffer is undefined in the reference; read nothing
+    return size > 0 ? r.buf[(start + size - 1) % r.cap] : 0.0f;
+}
+
+constexpr int kWalkThreads = 512;
+
+__device__ __forceinline__ float block_min(float v, float* red)
+{
+    // NaN-ignoring min: the reference's `if (x < m) m = x` never takes a NaN
+    const int tid = threadIdx.x;
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float o = __shfl_down(v, off, 64);
+        v = (o < v) ? o : v;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        float m = red[0];
+        for (int w = 1; w < kWalkThreads / 64; ++w) m = (red[w] < m) ? red[w] : m;
+        red[kWalkThreads / 64] = m;
+    }
+    __syncthreads();
+    const float r = red[kWalkThreads / 64];
+    __syncthreads();
+    return r;
+}
+
+// The sequential walk over nT spectra of one chunk.
+__global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
+    float* __restrict__ I, ClipConst K, ClipScalars* __restrict__ S, float* __restrict__ meanBuf,
+    double* __restrict__ meanPrefix, float* __restrict__ rmsBuf, double* __restrict__ rmsPrefix,
+    float* __restrict__ lastGood, const double* __restrict__ terms, float* __restrict__ bp,
+    uint8_t* __restrict__ mask, SpecRec* __restrict__ rec, int nT)
+{
+    __shared__ ClipScalars s;
+    __shared__ float red[kWalkThreads / 64 + 1];
+    __shared__ int goodCnt[8];
+    __shared__ int fillCnt;
+    __shared__ float sh_dataModel, sh_margin, sh_learnRMS;
+    __shared__ int sh_learning, sh_good, sh_fill;
+    __shared__ float bandMin[8], modelMin[8];
+    __shared__ float bandMean[8], bandMeanSq[8];
+
+    const int tid = threadIdx.x;
+    const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
+    const int cpb = nBins / 8;
+    RingView meanR{meanBuf, meanPrefix, K.cap}, rmsR{rmsBuf, rmsPrefix, K.cap};
+    if (tid == 0) s = *S;
+    __syncthreads();
+
+    auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb) + pol); };
+
+    // bandpass of the current offset
+    for (int i = tid; i < nBins; i += kWalkThreads) bp[i] = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+    __syncthreads();
+
+    for (int t = 0; t < nT; ++t) {
+        const int learning = s.rmsSize < K.cap;  // !_rmsBuffer.full()
+        if (tid < 8) {
+            goodCnt[tid] = 0;
+        }
+        if (tid == 0) fillCnt = 0;
+        // ---- learning-phase band statistics (RFI_Clipper.cpp:214-242)
+        if (learning) {
+            for (int b = 0; b < 8; ++b) {
+                float md = 1e6f, mm = 1e6f;
+                for (int i = b * cpb + tid; i < (b + 1) * cpb; i += kWalkThreads) {
+                    const int sb = i / nCh, c = i % nCh;
+                    const float v = I[idx(sb, 0, t) + c];
+                    md = (v < md) ? v : md;
+                    mm = (bp[i] < mm) ? bp[i] : mm;
+                }
+                md = block_min(md, red);
+                mm = block_min(mm, red);
+                if (tid == 0) {
+                    bandMin[b] = md;
+                    modelMin[b] = mm;
+                }
+            }
+            if (tid < 8) {
+                // ordered sums of band tid, in (s, c) order
+                float sum = 0.0f, sq = 0.0f;
+                for (int i = tid * cpb; i < (tid + 1) * cpb; ++i) {
+                    const int sb = i / nCh, c = i % nCh;
+                    const float v = I[idx(sb, 0, t) + c];
+                    sum += v;
+                    sq += (v * v);
+                }
+                bandMean[tid] = sum;
+                bandMeanSq[tid] = sq;
+            }
+            __syncthreads();
+        }
+        // ---- model and margin (:214-311)
+        if (tid == 0) {
+            float dataModel;
+            if (learning) {
+                float dmm[8], sig[8];
+                for (int b = 0; b < 8; ++b) {
+                    dmm[b] = bandMin[b] - modelMin[b];
+                    const float a = bandMeanSq[b] / (float)cpb;
+                    const float m = bandMean[b] / (float)cpb;
+                    sig[b] = (float)sqrt((double)a - (double)m * (double)m);
+                }
+                float spectrumRMS = sig[0];
+                for (int b = 1; b < 8; ++b)
+                    if (sig[b] < spectrumRMS) spectrumRMS = sig[b];  // std::min_element
+                // nth_element(..., begin()+4) of 8 values: the 5th smallest
+                for (int a = 1; a < 8; ++a) {
+                    float v = dmm[a];
+                    int j = a - 1;
+                    while (j >= 0 && dmm[j] > v) {
+                        dmm[j + 1] = dmm[j];
+                        --j;
+                    }
+                    dmm[j + 1] = v;
+                }
+                dataModel = dmm[4];
+                s.spectrumRMS = spectrumRMS;
+                ring_push(rmsR, s.rmsStart, s.rmsSize, spectrumRMS);
+                s.rmsRunAve = (float)(rmsPrefix[s.rmsSize] / (double)s.rmsSize);
+                if (s.rmsSize == K.cap) s.meanOverRMS = s.meanRunAve / s.rmsRunAve;
+                dataModel = dataModel + K.meanMinusMinimum * s.rmsRunAve;
+            } else {
+                s.rmsRunAve -= (ring_front(rmsR, s.rmsStart) / (float)s.rmsSize);
+                s.rmsRunAve += (ring_back(rmsR, s.rmsStart, s.rmsSize) / (float)K.cap);
+                if (K.useMeanOverRMS) s.spectrumRMS = fabsf(s.meanRunAve / s.meanOverRMS);
+                ring_push(rmsR, s.rmsStart, s.rmsSize, s.spectrumRMS);
+                dataModel = 0.0f;
+            }
+            sh_dataModel = dataModel;
+            sh_margin = K.crFactor * s.rmsRunAve;
+        }
+        __syncthreads();
+        const float dataModel = sh_dataModel, margin = sh_margin;
+        // ---- channel clipping (:315-342)
+        for (int i = tid; i < nBins; i += kWalkThreads) {
+            const int sb = i / nCh, c = i % nCh;
+            const long i0 = idx(sb, 0, t) + c;
+            const bool clip = I[i0] - dataModel - bp[i] > margin;
+            mask[i] = clip ? 1 : 0;
+            if (clip) {
+                const float g = lastGood[i];
+                for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t) + c] = g;
+            } else {
+                atomicAdd(&goodCnt[i / cpb], 1);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float spectrumSum = 0.0f;
+            for (int i = 0; i < nBins; ++i) {
+                if (!mask[i]) {
+                    const int sb = i / nCh, c = i % nCh;
+                    spectrumSum += I[idx(sb, 0, t) + c];
+                }
+            }
+            int total = 0;
+            for (int b = 0; b < 8; ++b) total += goodCnt[b];
+            s.fractionBadChannels += (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
+            spectrumSum /= (float)(unsigned)total;
+            unsigned badBands = 0;
+            for (int b = 0; b < 8; ++b) {
+                if ((double)(float)goodCnt[b] < 0.8 * (unsigned)cpb) ++badBands;
+                if (goodCnt[b] == 0) badBands += 4;
+            }
+            if ((double)(unsigned)total < 0.8 * (unsigned)nBins) badBands += 4;
+            // mean history (:383-400)
+            if (s.meanSize < K.cap) {
+                ring_push(meanR, s.meanStart, s.meanSize, spectrumSum);
+                s.meanRunAve = (float)(meanPrefix[s.meanSize] / (double)s.meanSize);
+            } else {
+                s.meanRunAve -= ring_front(meanR, s.meanStart) / (float)s.meanSize;
+                s.meanRunAve += ring_back(meanR, s.meanStart, s.meanSize) / (float)s.meanSize;
+                ring_push(meanR, s.meanStart, s.meanSize, spectrumSum);
+            }
+            const float tol = (float)((double)(K.srFactor * s.bpRms) / K.sqrtNBins);
+            int good = 1;
+            if (s.meanSize < 1000) {
+                good = 0;
+            } else if (spectrumSum - s.meanRunAve > tol || badBands >= 4) {
+                good = 0;
+                ++s.badSpectra;
+                // pop_back then push_back(back()) on both rings: the newest
+                // entry becomes a copy of the one before it
+                s.meanSize -= 1;
+                s.rmsSize -= 1;
+                ring_push(meanR, s.meanStart, s.meanSize, ring_back(meanR, s.meanStart, s.meanSize));
+                ring_push(rmsR, s.rmsStart, s.rmsSize, ring_back(rmsR, s.rmsStart, s.rmsSize));
+            }
+            sh_good = good;
+            sh_fill = good && s.remainingZeros != 0;
+        }
+        __syncthreads();
+        // ---- last-good spectrum (:441-469)
+        if (sh_fill) {
+            for (int i = tid; i < nBins; i += kWalkThreads) {
+                const int sb = i / nCh, c = i % nCh;
+                const float v = I[idx(sb, 0, t) + c];
+                if (lastGood[i] == 0.0f && v != 0.0f) {
+                    lastGood[i] = v;
+                    atomicAdd(&fillCnt, 1);
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            s.remainingZeros -= fillCnt;
+            SpecRec r;
+            r.params0 = s.params0;
+            r.dataModel = dataModel;
+            r.flags = (sh_good ? 1 : 0) | 2;
+            // stats (:535-560)
+            const unsigned reportStatsEvery = 10u * (unsigned)K.maxHistory;
+            if (s.num == 0) {
+                const float fractionBadSpectra =
+                    (float)(100.0 * (double)(float)s.badSpectra / (double)(float)reportStatsEvery);
+                if (fractionBadSpectra > 99.0) {
+                    s.rmsSize = 0;
+                    s.meanSize = 0;
+                    s.rmsStart = 0;
+                    s.meanStart = 0;
+                }
+                s.badSpectra = 0;
+                s.fractionBadChannels = 0.0f;
+            }
+            r.meanRunAve = s.meanRunAve;
+            rec[t] = r;
+            // setMean (BandPass.cpp:63-74) and setRMS (:44-46)
+            const float delta = s.meanRunAve - s.bpMean;
+            if (fabsf(delta) > 0.0f) {
+                s.params0 += delta / K.scale;
+                s.bpMean = s.meanRunAve;
+            }
+            s.bpRms = s.rmsRunAve;
+            s.num = (int)(((unsigned)s.num + 1u) % reportStatsEvery);
+        }
+        __syncthreads();
+        for (int i = tid; i < nBins; i += kWalkThreads)
+            bp[i] = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+        __syncthreads();
+    }
+    if (tid == 0) *S = s;
+}
+
+constexpr int kApplyTile = 64;
+
+// Parallel over spectra: clipSample, flatten, ordered mean/RMS, normalise
+// (RFI_Clipper.cpp:127-150, 418-438, 478-532).  A workgroup of 64 lanes owns
+// 64 spectra; bins are staged in 64x64 LDS tiles so that each lane can walk
+// its own spectrum in bin order while global accesses stay coalesced.
+__global__ __launch_bounds__(64) void clip_apply_kernel(float* __restrict__ I, float* __restrict__ W,
+                                                        ClipConst K, const double* __restrict__ terms,
+                                                        const SpecRec* __restrict__ rec, int nT)
+{
+    __shared__ float tile[kApplyTile][kApplyTile + 1];
+    __shared__ SpecRec recs[kApplyTile];
+    __shared__ float meanS[kApplyTile], rmsS[kApplyTile];
+    const int lane = threadIdx.x;
+    const int t0 = blockIdx.x * kApplyTile;
+    const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
+    auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb) + pol); };
+    if (t0 + lane < nT) recs[lane] = rec[t0 + lane];
+    __syncthreads();
+    const int nRows = min(kApplyTile, nT - t0);
+
+    // clipSample for rejected spectra: every pol and bin = -0.2
+    for (int r = 0; r < nRows; ++r) {
+        if (!(recs[r].flags & 1)) {
+            for (int pol = 0; pol < nPol; ++pol)
+                for (int i = lane; i < nBins; i += 64) {
+                    const int sb = i / nCh, c = i % nCh;
+                    I[idx(sb, pol, t0 + r) + c] = -0.2f;
+                }
+        }
+    }
+    if (W) {
+        for (int r = 0; r < nRows; ++r)
+            for (int pol = 0; pol < nPol; ++pol)
+                for (int i = lane; i < nBins; i += 64) {
+                    const int sb = i / nCh, c = i % nCh;
+                    W[idx(sb, pol, t0 + r) + c] = 1.0f;
+                }
+    }
+    __syncthreads();
+    // flatten pol 0 and take the ordered sums of each spectrum
+    float sum = 0.0f, sumSq = 0.0f;
+    for (int b0 = 0; b0 < nBins; b0 += kApplyTile) {
+        const int nb = min(kApplyTile, nBins - b0);
+        // load + flatten, coalesced along bins
+        for (int r = 0; r < nRows; ++r) {
+            if (lane < nb) {
+                const int i = b0 + lane;
+                const int sb = i / nCh, c = i % nCh;
+                const long gi = idx(sb, 0, t0 + r) + c;
+                const float bpv = bp_value(terms, K.nTerms, nBins, i, recs[r].params0, K.scale);
+                const float v = I[gi] - (bpv + recs[r].dataModel);
+                I[gi] = v;
+                tile[r][lane] = v;
+            }
+        }
+        __syncthreads();
+        if (lane < nRows) {
+            for (int k = 0; k < nb; ++k) {
+                const float v = tile[lane][k];
+                sum += v;
+                sumSq += v * v;
+            }
+        }
+        __syncthreads();
+    }
+    if (lane < nRows) {
+        const float mean = sum / (float)(unsigned)nBins;
+        float rms = (float)sqrt((double)(sumSq / (float)(unsigned)nBins) - (double)mean * (double)mean);
+        if (rms == 0.0f) rms = 1.0f;
+        meanS[lane] = mean;
+        rmsS[lane] = rms;
+    }
+    __syncthreads();
+    // normalise kept spectra
+    for (int r = 0; r < nRows; ++r) {
+        if (recs[r].flags & 1) {
+            const float sub = K.zeroDM == 1 ? (float)K.zeroDM * meanS[r] : recs[r].meanRunAve;
+            const float rms = rmsS[r];
+            for (int i = lane; i < nBins; i += 64) {
+                const int sb = i / nCh, c = i % nCh;
+                const long gi = idx(sb, 0, t0 + r) + c;
+                float v = I[gi];
+                v -= sub;
+                v /= rms;
+                if (v > K.crFactor) v = 0.0f;
+                I[gi] = v;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+struct ppl_clipper {
+    ppl_clipper_config cfg;
+    HostBandPass bp;
+    float startFrequency = 0, endFrequency = 0;
+    int nBins = 0;        // 0 until the first run fixes the map
+    ClipConst K{};
+    ClipScalars* d_S = nullptr;
+    float *d_meanBuf = nullptr, *d_rmsBuf = nullptr, *d_lastGood = nullptr, *d_bp = nullptr;
+    double *d_meanPrefix = nullptr, *d_rmsPrefix = nullptr, *d_terms = nullptr;
+    uint8_t* d_mask = nullptr;
+    SpecRec* d_rec = nullptr;
+    int recCap = 0;
+    std::vector<uint8_t> lastFlags;
+    int lastNT = 0;
+};
+
+namespace {
+
+void free_state(ppl_clipper* c)
+{
+    (void)hipFree(c->d_S);
+    (void)hipFree(c->d_meanBuf);
+    (void)hipFree(c->d_rmsBuf);
+    (void)hipFree(c->d_lastGood);
+    (void)hipFree(c->d_bp);
+    (void)hipFree(c->d_meanPrefix);
+    (void)hipFree(c->d_rmsPrefix);
+    (void)hipFree(c->d_terms);
+    (void)hipFree(c->d_mask);
+    (void)hipFree(c->d_rec);
+    c->d_S = nullptr;
+    c->d_meanBuf = c->d_rmsBuf = c->d_lastGood = c->d_bp = nullptr;
+    c->d_meanPrefix = c->d_rmsPrefix = c->d_terms = nullptr;
+    c->d_mask = nullptr;
+    c->d_rec = nullptr;
+}
+
+// First run: fix the run map (RFI_Clipper.cpp:184-187), rebin the model,
+// tabulate the polynomial terms and upload the initial state.
+int setup_map(ppl_clipper* c, unsigned nSub, unsigned nPol, unsigned nCh)
+{
+    const unsigned nBins = nSub * nCh;
+    HostMap m;
+    m.nBins = nBins;
+    m.lower = (double)c->startFrequency;
+    m.width = ((double)c->endFrequency - m.lower) / nBins;  // BinMap::setEnd
+    c->bp.reBin(m);
+    // _buildData and setMean both use the float width ratio
+    const float scale = (float)(c->bp.current.width / c->bp.primary.width);
+    const int nTerms = (int)c->bp.params.size() - 1;
+    std::vector<double> terms((size_t)std::max(nTerms, 0) * nBins);
+    for (int j = 1; j <= nTerms; ++j)
+        for (unsigned i = 0; i < nBins; ++i) {
+            const float v = (float)m.binAssignmentNumber(i);
+            terms[(size_t)(j - 1) * nBins + i] = (double)c->bp.params[j] * std::pow((double)v, (double)j);
+        }
+    ClipConst& K = c->K;
+    K.nBins = (int)nBins;
+    K.nSub = (int)nSub;
+    K.nPol = (int)nPol;
+    K.nCh = (int)nCh;
+    K.cap = c->cfg.maxHistory;
+    K.nTerms = nTerms;
+    K.scale = scale;
+    K.crFactor = c->cfg.crFactor;
+    K.srFactor = c->cfg.srFactor;
+    {
+        float k = 4;
+        K.meanMinusMinimum = k / std::sqrt(2.0 * k);
+    }
+    K.sqrtNBins = std::sqrt((double)nBins);
+    K.zeroDM = c->cfg.zeroDM ? 1 : 0;
+    K.useMeanOverRMS = c->cfg.useMeanOverRMS ? 1 : 0;
+    K.maxHistory = c->cfg.maxHistory;
+
+    ClipScalars S{};
+    S.params0 = c->bp.params[0];
+    S.bpMean = c->bp.curMean;
+    S.bpRms = c->bp.curRms;
+    S.remainingZeros = (int)nBins;
+    S.initialised = 1;
+
+    const size_t cap = (size_t)std::max(K.cap, 1);
+    PPL_HIP(hipMalloc(&c->d_S, sizeof(ClipScalars)));
+    PPL_HIP(hipMalloc(&c->d_meanBuf, cap * sizeof(float)));
+    PPL_HIP(hipMalloc(&c->d_rmsBuf, cap * sizeof(float)));
+    PPL_HIP(hipMalloc(&c->d_meanPrefix, (cap + 1) * sizeof(double)));
+    PPL_HIP(hipMalloc(&c->d_rmsPrefix, (cap + 1) * sizeof(double)));
+    PPL_HIP(hipMalloc(&c->d_lastGood, nBins * sizeof(float)));
+    PPL_HIP(hipMalloc(&c->d_bp, nBins * sizeof(float)));
+    PPL_HIP(hipMalloc(&c->d_mask, nBins));
+    PPL_HIP(hipMalloc(&c->d_terms, std::max<size_t>(terms.size(), 1) * sizeof(double)));
+    PPL_HIP(hipMemcpy(c->d_S, &S, sizeof S, hipMemcpyHostToDevice));
+    PPL_HIP(hipMemset(c->d_meanPrefix, 0, (cap + 1) * sizeof(double)));
+    PPL_HIP(hipMemset(c->d_rmsPrefix, 0, (cap + 1) * sizeof(double)));
+    PPL_HIP(hipMemset(c->d_lastGood, 0, nBins * sizeof(float)));
+    if (!terms.empty())
+        PPL_HIP(hipMemcpy(c->d_terms, terms.data(), terms.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->nBins = (int)nBins;
+    return PPL_OK;
+}
+
+}  // namespace
+
+extern "C" int ppl_clipper_create(ppl_clipper** out, const ppl_clipper_config* cfg)
+{
+    PPL_CHECK_ARG(out && cfg && cfg->bp_values, "null argument");
+    PPL_CHECK_ARG(cfg->n_values >= 6,
+                  "BandPassAdapter: data format wrong : got %d parameters, expecting at least 6", cfg->n_values);
+    PPL_CHECK_ARG(cfg->n_values - 5 <= kMaxParams + 1, "too many bandpass polynomial terms (%d)", cfg->n_values - 5);
+    PPL_CHECK_ARG(cfg->bandMatching,
+                  "RFI_Clipper: <Band matching=\"true\"/> is required (without it the bin width is 0)");
+    PPL_CHECK_ARG(cfg->maxHistory >= 1, "History maximum must be >= 1");
+    ppl_clipper* c = new ppl_clipper;
+    c->cfg = *cfg;
+    c->cfg.bp_values = nullptr;
+    // BandPassAdapter::deserialise :86-100, values through QVector<float>
+    std::vector<float> params;
+    for (int i = 0; i < cfg->n_values; ++i) params.push_back((float)cfg->bp_values[i]);
+    const unsigned nChan = (unsigned)params[0];
+    const float startFreq = params[1], deltaF = params[3], rms = params[4], median = params[5];
+    params.erase(params.begin(), params.begin() + 5);
+    HostMap m;
+    m.nBins = nChan;
+    m.lower = startFreq;
+    m.width = deltaF;
+    c->bp.setData(m, params);
+    c->bp.setMedian(median);
+    c->bp.setRMS(rms);
+    c->startFrequency = (float)c->bp.current.lower;
+    c->endFrequency = (float)c->bp.current.endValue();
+    *out = c;
+    return PPL_OK;
+}
+
+extern "C" int ppl_clipper_destroy(ppl_clipper* c)
+{
+    if (!c) return PPL_OK;
+    free_state(c);
+    delete c;
+    return PPL_OK;
+}
+
+static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
+                       unsigned nCh, hipStream_t st)
+{
+    PPL_CHECK_ARG(c && d_I, "null argument");
+    const unsigned nBins = nSub * nCh;
+    PPL_CHECK_ARG(nBins % 8 == 0 && nBins > 0, "RFI_Clipper: nBins (%u) must be a positive multiple of 8", nBins);
+    PPL_CHECK_ARG(nPol >= 1, "no polarisations");
+    if (c->nBins == 0) {
+        int rc = setup_map(c, nSub, nPol, nCh);
+        if (rc != PPL_OK) return rc;
+    }
+    PPL_CHECK_ARG((unsigned)c->nBins == nBins && (unsigned)c->K.nCh == nCh,
+                  "RFI_Clipper: spectrum shape changed (%u bins, was %d)", nBins, c->nBins);
+    c->K.nPol = (int)nPol;
+    if (nT == 0) return PPL_OK;
+    if ((int)nT > c->recCap) {
+        (void)hipFree(c->d_rec);
+        c->d_rec = nullptr;
+        PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * nT));
+        c->recCap = (int)nT;
+    }
+    hipLaunchKernelGGL(clip_walk_kernel, dim3(1), dim3(kWalkThreads), 0, st, d_I, c->K, c->d_S, c->d_meanBuf,
+                       c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_terms, c->d_bp,
+                       c->d_mask, c->d_rec, (int)nT);
+    PPL_LAUNCHED();
+    hipLaunchKernelGGL(clip_apply_kernel, dim3(cdiv(nT, kApplyTile)), dim3(64), 0, st, d_I, d_W, c->K,
+                       c->d_terms, c->d_rec, (int)nT);
+    PPL_LAUNCHED();
+    c->lastNT = (int)nT;
+    return PPL_OK;
+}
+
+extern "C" int ppl_clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
+                               unsigned nCh, void* stream)
+{
+    return clipper_run(c, d_I, d_W, nT, nSub, nPol, nCh, stream_of(stream));
+}
+
+extern "C" int ppl_clipper_stats(ppl_clipper* c, float* mean, float* rms)
+{
+    PPL_CHECK_ARG(c && mean && rms, "null argument");
+    if (!c->d_S) {
+        *mean = 0.0f;
+        *rms = 0.0f;
+        return PPL_OK;
+    }
+    ClipScalars S;
+    PPL_HIP(hipMemcpy(&S, c->d_S, sizeof S, hipMemcpyDeviceToHost));
+    *mean = S.meanRunAve;
+    *rms = S.rmsRunAve;
+    return PPL_OK;
+}
+
+extern "C" int ppl_clipper_flags(ppl_clipper* c, uint8_t* h_good, unsigned nT)
+{
+    PPL_CHECK_ARG(c && h_good, "null argument");
+    PPL_CHECK_ARG((int)nT <= c->lastNT, "asked for %u flags, last run had %d spectra", nT, c->lastNT);
+    std::vector<SpecRec> r(nT);
+    if (nT) PPL_HIP(hipMemcpy(r.data(), c->d_rec, sizeof(SpecRec) * nT, hipMemcpyDeviceToHost));
+    for (unsigned t = 0; t < nT; ++t) h_good[t] = (uint8_t)(r[t].flags & 1);
+    return PPL_OK;
+}
+
+// Fused variant: clip, then transpose the clipped Stokes-I spectra into a
+// dedispersion buffer (declared in ppl_buffer.hip).
+int ppl_ddbuf_add_spectra(ppl_ddbuf* buf, const float* d_spectra, unsigned nT, unsigned first, unsigned* consumed,
+                          int* full, hipStream_t st);
+
+extern "C" int ppl_clipper_run_into(ppl_clipper* c, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,
+                                    ppl_ddbuf* buf, unsigned first, void* stream)
+{
+    int rc = clipper_run(c, d_I, nullptr, nT, nSub, 1, nCh, stream_of(stream));
+    if (rc != PPL_OK) return rc;
+    unsigned consumed = 0;
+    int full = 0;
+    return ppl_ddbuf_add_spectra(buf, d_I, nT, first, &consumed, &full, stream_of(stream));
+}

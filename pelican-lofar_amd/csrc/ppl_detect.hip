@@ -1,0 +1,201 @@
+// ppl_detect.hip -- dyadic boxcar threshold detection
+// (DedispersionAnalyser::analyse, src/lib/src/DedispersionAnalyser.cpp:30-203).
+//
+// One wave per 64-sample segment of one DM row: lane j holds sample j of
+// the segment, level n sums pairs of level n-1 with xor shuffles
+// (binned[n][j] = binned[n-1][2j] + binned[n-1][2j+1], the reference's pair
+// order), and ballots give the per-level hit masks.  Pass 1 counts the hits
+// of every (DM, segment); an exclusive scan of the counts (row-major, the
+// reference's event order) gives each segment its output offset; pass 2
+// writes the events of the segments that have any.
+#include "ppl_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+using namespace ppl;
+
+constexpr int kMaxLevels = 7;  // binPow2 <= 6: 64-sample segments
+
+struct Thresholds {
+    float v[kMaxLevels];
+};
+
+template <typename T>
+__device__ __forceinline__ float sample_value(const T* data, long idx, int offset, float scale);
+template <>
+__device__ __forceinline__ float sample_value<float>(const float* data, long idx, int, float)
+{
+    return data[idx];
+}
+template <>
+__device__ __forceinline__ float sample_value<int32_t>(const int32_t* data, long idx, int offset, float scale)
+{
+    return (float)(data[idx] - offset) * scale;
+}
+
+// level hit masks of one segment; lane j of level n is valid for j < 2^(P-n)
+template <typename T>
+__device__ __forceinline__ void segment_masks(const T* data, long rowBase, int seg, int P, const Thresholds& th,
+                                              int offset, float scale, uint64_t* masks, float* vals)
+{
+    const int lane = threadIdx.x & 63;
+    const int W = 1 << P;
+    float v = 0.0f;
+    if (lane < W) v = sample_value<T>(data, rowBase + (long)seg * W + lane, offset, scale);
+    vals[0] = v;
+    masks[0] = __ballot(lane < W && v >= th.v[0]);
+    float cur = v;
+    for (int n = 1; n <= P; ++n) {
+        const int src = (2 * lane) & 63;
+        const float a = __shfl(cur, src, 64);
+        const float b = __shfl(cur, src + 1, 64);
+        cur = a + b;
+        vals[n] = cur;
+        masks[n] = __ballot(lane < (W >> n) && cur >= th.v[n]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void detect_count_kernel(const T* __restrict__ data, int tdms, int nsamp,
+                                                           int P, Thresholds th, int offset, float scale,
+                                                           int* __restrict__ counts)
+{
+    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nSeg = nsamp >> P;
+    const long total = (long)tdms * nSeg;
+    if (waveId >= total) return;
+    const int dm = (int)(waveId / nSeg), seg = (int)(waveId % nSeg);
+    uint64_t masks[kMaxLevels];
+    float vals[kMaxLevels];
+    segment_masks<T>(data, (long)dm * nsamp, seg, P, th, offset, scale, masks, vals);
+    int c = 0;
+    for (int n = 0; n <= P; ++n) c += __popcll(masks[n]);
+    if ((threadIdx.x & 63) == 0) counts[waveId] = c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void detect_write_kernel(const T* __restrict__ data, int tdms, int nsamp,
+                                                           int P, Thresholds th, int offset, float scale,
+                                                           const int* __restrict__ counts,
+                                                           const int* __restrict__ offsets, int maxEvents,
+                                                           int* __restrict__ evInts, float* __restrict__ evVals)
+{
+    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nSeg = nsamp >> P;
+    const long total = (long)tdms * nSeg;
+    if (waveId >= total) return;
+    if (counts[waveId] == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int dm = (int)(waveId / nSeg), seg = (int)(waveId % nSeg);
+    uint64_t masks[kMaxLevels];
+    float vals[kMaxLevels];
+    segment_masks<T>(data, (long)dm * nsamp, seg, P, th, offset, scale, masks, vals);
+    int base = offsets[waveId] + 1;  // slot 0 holds the dummy event
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int n = 0; n <= P; ++n) {
+        const uint64_t m = masks[n];
+        if ((m >> lane) & 1ull) {
+            const int k = base + __popcll(m & below);
+            if (k < maxEvents) {
+                const int binFactor = 1 << n;
+                evInts[3 * k + 0] = dm;
+                evInts[3 * k + 1] = seg * (1 << P) + binFactor * lane;
+                evInts[3 * k + 2] = binFactor;
+                evVals[k] = vals[n];
+            }
+        }
+        base += __popcll(m);
+    }
+}
+
+template <typename T>
+int run_detect(const T* d_data, int tdms, int nsamp, int offset, float scale, int nChannels, int nSubbands,
+               float threshold, unsigned binPow2, int32_t* h_ints, float* h_vals, int max_events, int* count,
+               hipStream_t st)
+{
+    PPL_CHECK_ARG(d_data && count, "null argument");
+    PPL_CHECK_ARG(binPow2 <= 6, "power2ForBinning %u > 6 is not supported", binPow2);
+    PPL_CHECK_ARG(tdms >= 0 && nsamp >= 0, "negative shape");
+    PPL_CHECK_ARG(max_events <= 0 || (h_ints && h_vals), "null event buffers");
+    // thresholds exactly as the reference forms them (float products)
+    Thresholds th;
+    const float rms = std::sqrt((float)nChannels * (float)nSubbands);
+    th.v[0] = threshold * rms;
+    for (unsigned n = 1; n <= binPow2; ++n) {
+        const int binFactor = 1 << n;
+        th.v[n] = threshold * rms * std::sqrt((float)binFactor);
+    }
+    const int P = (int)binPow2;
+    const int nSeg = nsamp >> P;
+    const long total = (long)tdms * nSeg;
+    int nEvents = 1;
+    if (max_events > 0) {
+        h_ints[0] = 0;
+        h_ints[1] = 0;
+        h_ints[2] = 1;
+        h_vals[0] = 0.0f;
+    }
+    if (total > 0) {
+        int *d_counts = nullptr, *d_offsets = nullptr, *d_ints = nullptr;
+        float* d_vals = nullptr;
+        void* d_tmp = nullptr;
+        size_t tmpBytes = 0;
+        PPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, d_counts, d_offsets, (int)total, st));
+        PPL_HIP(hipMallocAsync((void**)&d_counts, sizeof(int) * (total + 1), st));
+        PPL_HIP(hipMallocAsync((void**)&d_offsets, sizeof(int) * (total + 1), st));
+        PPL_HIP(hipMallocAsync(&d_tmp, tmpBytes + 16, st));
+        const unsigned blocks = (unsigned)((total + 3) / 4);
+        hipLaunchKernelGGL(detect_count_kernel<T>, dim3(blocks), dim3(256), 0, st, d_data, tdms, nsamp, P, th,
+                           offset, scale, d_counts);
+        PPL_LAUNCHED();
+        PPL_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmpBytes, d_counts, d_offsets, (int)total, st));
+        int last[2] = {0, 0};
+        PPL_HIP(hipMemcpyAsync(&last[0], d_offsets + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        PPL_HIP(hipMemcpyAsync(&last[1], d_counts + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        PPL_HIP(hipStreamSynchronize(st));
+        const int found = last[0] + last[1];
+        nEvents = 1 + found;
+        const int cap = std::min(nEvents, std::max(max_events, 0));
+        if (found > 0 && cap > 1) {
+            PPL_HIP(hipMallocAsync((void**)&d_ints, sizeof(int32_t) * 3 * cap, st));
+            PPL_HIP(hipMallocAsync((void**)&d_vals, sizeof(float) * cap, st));
+            hipLaunchKernelGGL(detect_write_kernel<T>, dim3(blocks), dim3(256), 0, st, d_data, tdms, nsamp, P,
+                               th, offset, scale, d_counts, d_offsets, cap, d_ints, d_vals);
+            PPL_LAUNCHED();
+            PPL_HIP(hipMemcpyAsync(h_ints + 3, d_ints + 3, sizeof(int32_t) * 3 * (cap - 1), hipMemcpyDeviceToHost,
+                                   st));
+            PPL_HIP(hipMemcpyAsync(h_vals + 1, d_vals + 1, sizeof(float) * (cap - 1), hipMemcpyDeviceToHost, st));
+            PPL_HIP(hipFreeAsync(d_ints, st));
+            PPL_HIP(hipFreeAsync(d_vals, st));
+        }
+        PPL_HIP(hipFreeAsync(d_counts, st));
+        PPL_HIP(hipFreeAsync(d_offsets, st));
+        PPL_HIP(hipFreeAsync(d_tmp, st));
+        PPL_HIP(hipStreamSynchronize(st));
+    }
+    *count = nEvents;
+    return PPL_OK;
+}
+
+}  // namespace
+
+extern "C" int ppl_detect(const float* d_data, int tdms, int nsamp, int nChannels, int nSubbands, float threshold,
+                          unsigned binPow2, int32_t* h_ev_ints, float* h_ev_vals, int max_events, int* count,
+                          void* stream)
+{
+    return run_detect<float>(d_data, tdms, nsamp, 0, 1.0f, nChannels, nSubbands, threshold, binPow2, h_ev_ints,
+                             h_ev_vals, max_events, count, stream_of(stream));
+}
+
+extern "C" int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float scale, int nChannels,
+                             int nSubbands, float threshold, unsigned binPow2, int32_t* h_ev_ints,
+                             float* h_ev_vals, int max_events, int* count, void* stream)
+{
+    return run_detect<int32_t>(d_data, tdms, nsamp, 128 * nChan, scale, nChannels, nSubbands, threshold, binPow2,
+                               h_ev_ints, h_ev_vals, max_events, count, stream_of(stream));
+}

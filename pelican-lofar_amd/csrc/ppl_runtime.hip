@@ -1,0 +1,142 @@
+// ppl_runtime.hip -- device selection, memory, streams and error reporting of
+// the C ABI.  Replaces the GPU_NVidia / GPU_Param plumbing of pelican-lofar
+// (src/lib/src/GPU_NVidia.cpp, src/lib/src/GPU_Param.cpp) with plain HIP.
+#include "ppl_internal.h"
+
+#include <cstring>
+
+namespace ppl {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+}  // namespace ppl
+
+using namespace ppl;
+
+extern "C" const char* ppl_last_error(void) { return g_err; }
+
+extern "C" int ppl_version(void) { return 1; }
+
+extern "C" int ppl_init(int device)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error("no HIP device available");
+        return PPL_E_NODEV;
+    }
+    PPL_CHECK_ARG(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+    PPL_HIP(hipSetDevice(device));
+    return PPL_OK;
+}
+
+extern "C" int ppl_shutdown(void)
+{
+    PPL_HIP(hipDeviceSynchronize());
+    return PPL_OK;
+}
+
+extern "C" int ppl_device_alloc(void** ptr, size_t bytes)
+{
+    PPL_CHECK_ARG(ptr, "null pointer");
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        set_error("hipMalloc(%zu) failed", bytes);
+        return PPL_E_NOMEM;
+    }
+    return PPL_OK;
+}
+
+extern "C" int ppl_device_free(void* ptr)
+{
+    if (ptr) PPL_HIP(hipFree(ptr));
+    return PPL_OK;
+}
+
+extern "C" int ppl_host_alloc(void** ptr, size_t bytes)
+{
+    PPL_CHECK_ARG(ptr, "null pointer");
+    if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        set_error("hipHostMalloc(%zu) failed", bytes);
+        return PPL_E_NOMEM;
+    }
+    return PPL_OK;
+}
+
+extern "C" int ppl_host_free(void* ptr)
+{
+    if (ptr) PPL_HIP(hipHostFree(ptr));
+    return PPL_OK;
+}
+
+extern "C" int ppl_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream)
+{
+    PPL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_of(stream)));
+    return PPL_OK;
+}
+
+extern "C" int ppl_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream)
+{
+    PPL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_of(stream)));
+    return PPL_OK;
+}
+
+extern "C" int ppl_memset_async(void* dst, int value, size_t bytes, void* stream)
+{
+    PPL_HIP(hipMemsetAsync(dst, value, bytes, stream_of(stream)));
+    return PPL_OK;
+}
+
+extern "C" int ppl_stream_create(void** stream)
+{
+    PPL_CHECK_ARG(stream, "null pointer");
+    hipStream_t s;
+    PPL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return PPL_OK;
+}
+
+extern "C" int ppl_stream_destroy(void* stream)
+{
+    if (stream) PPL_HIP(hipStreamDestroy(stream_of(stream)));
+    return PPL_OK;
+}
+
+extern "C" int ppl_stream_sync(void* stream)
+{
+    PPL_HIP(hipStreamSynchronize(stream_of(stream)));
+    return PPL_OK;
+}
+
+namespace {
+struct Cb {
+    void (*fn)(void*, int);
+    void* user;
+};
+void host_trampoline(void* p)
+{
+    Cb* cb = static_cast<Cb*>(p);
+    cb->fn(cb->user, PPL_OK);
+    delete cb;
+}
+}  // namespace
+
+extern "C" int ppl_stream_callback(void* stream, void (*fn)(void* user, int status), void* user)
+{
+    PPL_CHECK_ARG(fn, "null callback");
+    Cb* cb = new Cb{fn, user};
+    hipError_t e = hipLaunchHostFunc(stream_of(stream), host_trampoline, cb);
+    if (e != hipSuccess) {
+        delete cb;
+        set_error("hipLaunchHostFunc: %s", hipGetErrorString(e));
+        return PPL_E_HIP;
+    }
+    return PPL_OK;
+}

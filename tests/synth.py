@@ -1,0 +1,84 @@
+"""Seeded synthetic inputs for the pulsar-search chain (numpy, CPU).
+
+Conventions (build-defined, the reference leaves them open):
+- voltages are complex int16, layout [sub][pol][t][re,im] (TimeSeriesDataSetC32
+  order with block length nCh), Gaussian noise of standard deviation sigma
+  per component, rounded and saturated to int16;
+- buffer channel c' = sub*nCh + k is at frequency fch1 + foff*c' with foff < 0
+  (descending order, the reference's invert = 0 path);
+- a dispersed pulse is a set of channel-centre tone bursts: channel c' carries
+  a tone at baseband frequency (k - nCh/2)/nCh cycles/sample (the bin the
+  fftshifted channeliser maps it to) during [t0 + delay(c'), t0 + delay(c') +
+  width) integrated samples, delay = shift[c']*DM/tsamp;
+- the pulse amplitude is given as the mean excess of one channel's
+  integrated Stokes-I over its noise standard deviation.
+"""
+import numpy as np
+
+KDM = 4148.741601
+
+
+def shift_table(n_chan, fch1, foff):
+    c = np.arange(n_chan, dtype=np.float64)
+    f = fch1 + foff * c
+    return KDM * ((1.0 / f / f) - (1.0 / fch1 / fch1))
+
+
+def noise_voltages(n_sub, n_samples, sigma, seed, n_pol=2):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0.0, sigma, size=(n_sub, n_pol, n_samples, 2))
+    return np.clip(np.rint(x), -32768, 32767).astype(np.int16)
+
+
+def tone_amplitude(coeffs, n_ch, sigma, excess_sigma, integrate):
+    """Tone amplitude giving `excess_sigma` standard deviations of excess
+    integrated Stokes-I in its channel (both polarisations carry the tone)."""
+    h = np.asarray(coeffs, np.float64)
+    e_h = np.sum(h * h)
+    s = abs(np.sum(h * ((-1.0) ** (np.arange(h.size) % n_ch))))
+    m = 2.0 * sigma * sigma * e_h                # mean |X|^2 of one pol, one block
+    n_terms = 2 * integrate                      # pols x integrated blocks
+    std = np.sqrt(n_terms) * m
+    return float(np.sqrt(excess_sigma * std / (n_terms * s * s)))
+
+
+def inject_pulse_samples(volt, n_ch, fch1, foff, tsamp, samples_per_int, dm, t0, width, amp,
+                         sample0=0, seed=0):
+    """Tone-burst pulse: see module docstring.  t0, width in integrated samples."""
+    n_sub, n_pol, n_samp, _ = volt.shape
+    n_chan = n_sub * n_ch
+    shift = shift_table(n_chan, fch1, foff)
+    rng = np.random.default_rng(seed)
+    acc = volt.astype(np.float64)
+    for cp in range(n_chan):
+        s, k = divmod(cp, n_ch)
+        d = shift[cp] * dm / tsamp
+        a = int(round((t0 + d) * samples_per_int)) - sample0
+        b = a + int(round(width * samples_per_int))
+        a0, b0 = max(a, 0), min(b, n_samp)
+        if a0 >= b0:
+            continue
+        nu = (k - n_ch / 2.0) / n_ch
+        m = np.arange(a0, b0) + sample0
+        for p in range(n_pol):
+            ph = rng.uniform(0, 2 * np.pi)
+            z = amp * np.exp(1j * (2 * np.pi * nu * m + ph))
+            acc[s, p, a0:b0, 0] += z.real
+            acc[s, p, a0:b0, 1] += z.imag
+    volt[...] = np.clip(np.rint(acc), -32768, 32767).astype(np.int16)
+    return volt
+
+
+def bandpass_values(n_bins, start, width, median, rms, poly=()):
+    """Numbers of a .bp file: nChan, start, end, width, rms, median, poly..."""
+    end = start + width * (n_bins - 1)
+    return np.array([n_bins, start, end, width, rms, median] + list(poly), np.float64)
+
+
+def stokes_noise(n_t, n_sub, n_ch, level, seed, dof=16, n_pol=1):
+    """Chi-square-like Stokes-I spectra with mean `level` per bin
+    ([nT][nSub][nPol][nCh] float32), optional per-bin level array."""
+    rng = np.random.default_rng(seed)
+    lv = np.broadcast_to(np.asarray(level, np.float64), (n_sub * n_ch,)).reshape(n_sub, n_ch)
+    x = rng.chisquare(dof, size=(n_t, n_sub, n_pol, n_ch)) / dof
+    return (x * lv[None, :, None, :]).astype(np.float32)

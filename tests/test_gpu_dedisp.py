@@ -1,0 +1,140 @@
+"""GPU parity of brute-force dedispersion with the oracle: bit-exact FP32
+(channel-order sums), bit-exact FP16-input, exact int32 for u8 input."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(n_chan=128, nsamp_buf=4096, tdms=128, dm_low=0.0, dm_step=50 / 128,
+             tsamp=5.12e-6 * 16 * 4, fch1=141.5625, foff=-0.01220703125)
+
+
+def _oracle(orc, x, cfg, dtype="f32"):
+    p = orc.dedisp_plan(cfg["n_chan"], cfg["nsamp_buf"], cfg["tdms"], cfg["dm_low"], cfg["dm_step"],
+                        cfg["tsamp"], cfg["fch1"], cfg["foff"])
+    assert p["rc"] == 0
+    d = orc.dedisp_delays(p["shift"], cfg["tdms"], p["mstartdm"], p["mdmstep"])
+    return p, d
+
+
+def test_delay_table_matches_oracle(orc, ppl):
+    plan = ppl.DedispPlan(**SMALL)
+    p, d = _oracle(orc, None, SMALL)
+    assert (plan.maxshift, plan.rem, plan.nout) == (p["maxshift"], p["rem"], p["nout"])
+    assert np.array_equal(plan.delays(), d)
+
+
+@pytest.mark.parametrize("tile", [None, (32, 256), (16, 512), (8, 512), (16, 256), (64, 256)])
+def test_f32_bit_exact_small(orc, ppl, tile):
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
+    plan = ppl.DedispPlan(**SMALL)
+    if tile:
+        plan.set_tile(*tile)
+    out = plan.run(torch.from_numpy(x).cuda()).cpu().numpy()
+    p, d = _oracle(orc, x, SMALL)
+    exp = orc.dedisperse(x, d, p["nout"])
+    assert out.shape == exp.shape
+    assert np.array_equal(out, exp)
+
+
+@pytest.mark.parametrize("shape", [(37, 2048, 100, 0.7), (300, 8192, 200, 0.1), (1, 1024, 3, 2.0),
+                                   (129, 4096, 41, 0.3)])
+def test_f32_bit_exact_ragged(orc, ppl, shape):
+    n_chan, ns, tdms, step = shape
+    cfg = dict(n_chan=n_chan, nsamp_buf=ns, tdms=tdms, dm_low=0.0, dm_step=step,
+               tsamp=0.00032768, fch1=150.0, foff=-6.0 / 2048)
+    rng = np.random.default_rng(n_chan)
+    x = (rng.normal(size=(n_chan, ns)) * 3 + 1).astype(np.float32)
+    plan = ppl.DedispPlan(**cfg)
+    out = plan.run(torch.from_numpy(x).cuda()).cpu().numpy()
+    p, d = _oracle(orc, x, cfg)
+    assert np.array_equal(out, orc.dedisperse(x, d, p["nout"]))
+
+
+def test_known_answer_dm10(ppl):
+    """DedispersionModuleTest.cpp:167-168: dmAmplitude(0, 10) == nSub*nCh."""
+    nsub, nch = 32, 64
+    n = nsub * nch
+    fch1, foff, tsamp = 150.0, -6.0 / n, 0.00032768
+    gen = [int(10.0 * (4148.741601 * ((1.0 / (fch1 + foff * c) / (fch1 + foff * c)) -
+                                      (1.0 / fch1 / fch1)) / tsamp)) for c in range(n)]
+    x = np.zeros((n, 32768), np.float32)
+    for c in range(n):
+        x[c, gen[c]:gen[c] + 10] = 1.0
+    plan = ppl.DedispPlan(n, 32768, 200, 0.0, 0.1, tsamp, fch1, foff)
+    out = plan.run(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert out[100, 0] == 2048.0
+
+
+def test_f16_and_u8_exact(orc, ppl):
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    # FP16 input
+    h = torch.empty(x.shape, dtype=torch.float16, device="cuda")
+    ppl.convert_f16(xd, h)
+    hb = h.cpu().numpy().view(np.uint16)
+    assert np.array_equal(hb, orc.f32_to_f16_bits(x))
+    plan16 = ppl.DedispPlan(**SMALL, dtype="f16")
+    out16 = plan16.run(h).cpu().numpy()
+    p, d = _oracle(orc, x, SMALL)
+    assert np.array_equal(out16, orc.dedisperse(hb, d, p["nout"]))
+    # offset-binary u8 input, int32 sums
+    inv = np.float32(127.0 / 10.0)
+    q = torch.empty(x.shape, dtype=torch.uint8, device="cuda")
+    ppl.quantise_u8(xd, q, float(inv))
+    qh = q.cpu().numpy()
+    assert np.array_equal(qh, orc.quantise_u8(x, inv))
+    plan8 = ppl.DedispPlan(**SMALL, dtype="u8")
+    out8 = plan8.run(q).cpu().numpy()
+    assert out8.dtype == np.int32
+    assert np.array_equal(out8, orc.dedisperse(qh, d, p["nout"]))
+
+
+def test_reference_c_seam(orc, ppl):
+    """cacheDedisperseLoop's own arguments, for tdms not a multiple of 40."""
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
+    p, d = _oracle(orc, x, SMALL)
+    shift = torch.from_numpy(p["shift"]).cuda()
+    out = torch.full((SMALL["tdms"], p["nout"] + 5), -1.0, device="cuda")
+    ppl.cache_dedisperse_loop(out, torch.from_numpy(x).cuda(), float(p["mstartdm"]), float(p["mdmstep"]),
+                              SMALL["tdms"], SMALL["nsamp_buf"], shift, p["maxshift"] + p["rem"],
+                              SMALL["n_chan"])
+    flat = out.cpu().numpy().ravel()
+    n = SMALL["tdms"] * p["nout"]
+    assert np.array_equal(flat[:n].reshape(SMALL["tdms"], p["nout"]), orc.dedisperse(x, d, p["nout"]))
+    assert np.all(flat[n:] == 0.0)  # the rest of outbufSize is cleared, as cudaMemset did
+
+
+def test_plan_rejects_oversized_maxshift(ppl):
+    with pytest.raises(ppl.PPLError):
+        ppl.DedispPlan(3968, 4096, 2048, 0.0, 500 / 2048, 5.12e-6 * 64 * 4, 150.0, -0.1953125 / 64)
+
+
+def test_linearity_full_size(ppl):
+    """configs[1] shape: dedisperse(a + b) == dedisperse(a) + dedisperse(b) for
+    integer-valued inputs (exact in FP32), and an all-ones input sums to nChan."""
+    cfg = dict(n_chan=3968, nsamp_buf=65536, tdms=2048, dm_low=0.0, dm_step=500 / 2048,
+               tsamp=5.12e-6 * 64 * 4, fch1=150.0, foff=-0.1953125 / 64)
+    plan = ppl.DedispPlan(**cfg)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = torch.randint(-8, 8, (3968, 65536), device="cuda", generator=g).float()
+    b = torch.randint(-8, 8, (3968, 65536), device="cuda", generator=g).float()
+    oa = plan.run(a)
+    ob = plan.run(b)
+    oab = plan.run(a + b)
+    assert torch.equal(oab, oa + ob)
+    ones = torch.ones((3968, 65536), device="cuda")
+    assert torch.all(plan.run(ones) == 3968.0)
+    # a single hot channel sample lands at t = s - delay[dm][c] for every DM
+    z = torch.zeros((3968, 65536), device="cuda")
+    c, s = 3000, 30000
+    z[c, s] = 1.0
+    out = plan.run(z)
+    d = torch.from_numpy(plan.delays()[:, c].astype(np.int64)).cuda()
+    hit = out.nonzero()
+    assert hit.shape[0] == 2048
+    assert torch.equal(hit[:, 1], s - d)
