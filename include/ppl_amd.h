@@ -86,6 +86,7 @@ typedef struct {
     float mstartdm;         /* dmLow / tsamp (float) */
     float mdmstep;          /* dmStep / tsamp (float) */
     int tileDM, tileT;      /* DM x time tile chosen for this plan */
+    int chanBlock;          /* channels staged per block (0: as many as fit in LDS) */
 } ppl_dedisp_info;
 
 /* DedispersionModule::resize (src/lib/src/DedispersionModule.cpp:109-190):
@@ -99,6 +100,13 @@ int ppl_dedisp_plan_destroy(ppl_dedisp_plan* plan);
 int ppl_dedisp_plan_info(const ppl_dedisp_plan* plan, ppl_dedisp_info* info);
 /* Override the DM x time tile (the autotuner's hook); 0 keeps the default. */
 int ppl_dedisp_set_tile(ppl_dedisp_plan* plan, int tileDM, int tileT);
+/* Same with the number of channels staged per LDS block (0: as many as fit). */
+int ppl_dedisp_set_shape(ppl_dedisp_plan* plan, int tileDM, int tileT, int chanBlock);
+/* Autotuner: times every instantiated DM x time tile and channel block on
+ * the given buffers (reps launches each, HIP events on the stream), keeps the
+ * fastest for the plan and returns its time per launch in *best_ms. */
+int ppl_dedisp_autotune(ppl_dedisp_plan* plan, const void* d_in, void* d_out, int reps, void* stream,
+                        float* best_ms);
 /* Copy the int32 delay table [tdms][nChan] to host memory (for checks). */
 int ppl_dedisp_delays(const ppl_dedisp_plan* plan, int32_t* host_out);
 
@@ -241,6 +249,9 @@ int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nS
 int ppl_clipper_stats(ppl_clipper* clip, float* mean, float* rms);
 /* Per-spectrum flags of the last run (1 = spectrum kept), host buffer. */
 int ppl_clipper_flags(ppl_clipper* clip, uint8_t* h_good, unsigned nT);
+/* Spectra so far decided from the precomputed channel mask (fast) and by the
+ * full-workgroup path (slow); synchronises. */
+int ppl_clipper_counters(ppl_clipper* clip, long long* fast, long long* slow);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,9 @@ void orc_ppf_destroy(orc_ppf* p);
  *        float FFT is parity unpinned; its closed form is the DFT).
  * Writes double complex (re,im) to out. */
 void orc_ppf_run_fir32(orc_ppf* p, const int16_t* in, unsigned nBlk, double* out);
+/* FP32 FIR + FP32 radix-2 FFT, OpenMP over streams: the timed CPU baseline
+ * (complex float out [blk][sub][pol][nCh]; nCh a power of two). */
+void orc_ppf_run_f32(orc_ppf* p, const int16_t* in, unsigned nBlk, float* out, int nthreads);
 /* FP64 twin: FP64 coefficients, FP64 FIR and FP64 DFT. */
 void orc_ppf_run_f64(orc_ppf* p, const int16_t* in, unsigned nBlk, double* out);
 /* The FP32 FIR stage alone (bit-exact target), complex float out [blk][sub][pol][nCh]. */

@@ -48,6 +48,7 @@ def _declare(L):
         "orc_ppf_destroy": (None, [vp]),
         "orc_ppf_run_fir32": (None, [vp, vp, u, vp]),
         "orc_ppf_run_f64": (None, [vp, vp, u, vp]),
+        "orc_ppf_run_f32": (None, [vp, vp, u, vp, i]),
         "orc_ppf_fir_only_f32": (None, [vp, vp, u, vp]),
         "orc_stokes_i_f32": (None, [vp, u, u, u, vp]),
         "orc_stokes_i_f64": (None, [vp, u, u, u, vp]),
@@ -129,6 +130,14 @@ class PPF:
     def run_f64(self, x, n_blk):
         o = self._run(lib().orc_ppf_run_f64, x, n_blk, np.float64, 2)
         return o[..., 0] + 1j * o[..., 1]
+
+    def run_f32(self, x, n_blk, nthreads=0):
+        """FP32 FIR + FP32 FFT (CPU baseline path); float32 [blk][sub][pol][nCh][2]."""
+        n_sub, n_pol, n_ch, _ = self.shape
+        x = np.ascontiguousarray(x, np.int16)
+        out = np.zeros((n_blk, n_sub, n_pol, n_ch, 2), np.float32)
+        lib().orc_ppf_run_f32(self.h, _p(x), n_blk, _p(out), nthreads)
+        return out
 
     def fir_only_f32(self, x, n_blk):
         return self._run(lib().orc_ppf_fir_only_f32, x, n_blk, np.float32, 2)

@@ -8,6 +8,10 @@
 #include <complex>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 namespace {
 
 // PolyphaseCoefficients::_besselI0, src/lib/src/PolyphaseCoefficients.cpp:77-103
@@ -216,15 +220,27 @@ extern "C" void orc_ppf_destroy(orc_ppf* p) { delete p; }
 
 namespace {
 
-// FP64 forward DFT, unnormalised (the closed form of FFTW_FORWARD).
+// FP64 forward DFT, unnormalised (the closed form of FFTW_FORWARD), with the
+// twiddles W_N^k tabulated in long double once per N.
 void dft64(const std::complex<double>* x, unsigned N, std::complex<double>* y)
 {
+    static thread_local unsigned tabN = 0;
+    static thread_local std::vector<long double> tc, ts;
+    if (tabN != N) {
+        tc.resize(N);
+        ts.resize(N);
+        for (unsigned k = 0; k < N; ++k) {
+            long double a = -2.0L * (long double)M_PI * (long double)k / (long double)N;
+            tc[k] = cosl(a);
+            ts[k] = sinl(a);
+        }
+        tabN = N;
+    }
     for (unsigned k = 0; k < N; ++k) {
         long double re = 0, im = 0;
         for (unsigned n = 0; n < N; ++n) {
-            unsigned ph = (unsigned)(((unsigned long long)n * k) % N);
-            long double a = -2.0L * (long double)M_PI * ph / N;
-            long double c = cosl(a), s = sinl(a);
+            const unsigned ph = (unsigned)(((unsigned long long)n * k) % N);
+            const long double c = tc[ph], s = ts[ph];
             re += (long double)x[n].real() * c - (long double)x[n].imag() * s;
             im += (long double)x[n].real() * s + (long double)x[n].imag() * c;
         }
@@ -391,4 +407,54 @@ extern "C" void orc_integrate_f64(const double* in, unsigned nT, unsigned nSub, 
                                   unsigned window, unsigned binCh, double* out)
 {
     integrate<double>(in, nT, nSub, nPol, nCh, window, binCh, out);
+}
+
+// FP32 radix-2 FFT (iterative, decimation in time) standing in for the
+// reference's FFTW float plan when the CPU path is timed.
+static void fft32(std::complex<float>* x, unsigned N, const std::vector<std::complex<float>>& tw)
+{
+    for (unsigned i = 1, j = 0; i < N; ++i) {
+        unsigned bit = N >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(x[i], x[j]);
+    }
+    for (unsigned len = 2; len <= N; len <<= 1) {
+        const unsigned step = N / len;
+        for (unsigned i = 0; i < N; i += len)
+            for (unsigned k = 0; k < len / 2; ++k) {
+                const std::complex<float> w = tw[k * step];
+                const std::complex<float> u = x[i + k], v = x[i + k + len / 2] * w;
+                x[i + k] = u + v;
+                x[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// PPFChanneliser::run with an FP32 FFT, OpenMP over streams (the reference
+// parallelises over subbands, PPFChanneliser.cpp:139-183).  Writes complex
+// float spectra [blk][sub][pol][nCh].  Used to time the CPU baseline.
+extern "C" void orc_ppf_run_f32(orc_ppf* p, const int16_t* in, unsigned nBlk, float* out, int nthreads)
+{
+    const unsigned N = p->nCh, S = p->nSub, P = p->nPol;
+    std::vector<std::complex<float>> tw(N);
+    for (unsigned k = 0; k < N; ++k) {
+        const double a = -2.0 * M_PI * k / N;
+        tw[k] = {(float)std::cos(a), (float)std::sin(a)};
+    }
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+#endif
+    for (int stream = 0; stream < (int)(S * P); ++stream) {
+        const unsigned s = stream / P, pol = stream % P;
+        std::vector<std::complex<float>> filt(N);
+        const int16_t* x = in + 2 * (size_t)stream * nBlk * N;
+        for (unsigned b = 0; b < nBlk; ++b) {
+            fir_step32(p, stream, x + 2 * (size_t)b * N, filt.data());
+            fft32(filt.data(), N, tw);
+            float* o = out + 2 * (size_t)N * (P * ((size_t)S * b + s) + pol);
+            for (unsigned c = 0; c < N; ++c) { o[2 * c] = filt[c].real(); o[2 * c + 1] = filt[c].imag(); }
+        }
+    }
 }

@@ -47,7 +47,7 @@ class DedispConfig(C.Structure):
 class DedispInfo(C.Structure):
     _fields_ = [("maxshift", C.c_int), ("remainingSamples", C.c_int), ("nOut", C.c_int),
                 ("mstartdm", C.c_float), ("mdmstep", C.c_float), ("tileDM", C.c_int),
-                ("tileT", C.c_int)]
+                ("tileT", C.c_int), ("chanBlock", C.c_int)]
 
 
 class PPFConfig(C.Structure):
@@ -83,6 +83,8 @@ SIGNATURES = {
     "ppl_dedisp_plan_destroy": (i, [vp]),
     "ppl_dedisp_plan_info": (i, [vp, vp]),
     "ppl_dedisp_set_tile": (i, [vp, i, i]),
+    "ppl_dedisp_set_shape": (i, [vp, i, i, i]),
+    "ppl_dedisp_autotune": (i, [vp, vp, vp, i, vp, vp]),
     "ppl_dedisp_delays": (i, [vp, vp]),
     "ppl_dedisperse": (i, [vp, vp, vp, vp]),
     "ppl_cache_dedisperse_loop": (i, [vp, C.c_long, vp, f, f, i, i, vp, i, i, vp]),
@@ -111,6 +113,7 @@ SIGNATURES = {
     "ppl_clipper_run_into": (i, [vp, vp, u, u, u, vp, u, vp]),
     "ppl_clipper_stats": (i, [vp, vp, vp]),
     "ppl_clipper_flags": (i, [vp, vp, u]),
+    "ppl_clipper_counters": (i, [vp, vp, vp]),
 }
 
 
@@ -171,18 +174,29 @@ class DedispPlan:
         check(lib().ppl_dedisp_plan_info(h, C.byref(info)))
         self.maxshift, self.rem, self.nout = info.maxshift, info.remainingSamples, info.nOut
         self.mstartdm, self.mdmstep = info.mstartdm, info.mdmstep
-        self.tile = (info.tileDM, info.tileT)
+        self.tile = (info.tileDM, info.tileT, info.chanBlock)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.ppl_dedisp_plan_destroy(self.h)
             self.h = None
 
-    def set_tile(self, tile_dm, tile_t):
-        check(lib().ppl_dedisp_set_tile(self.h, tile_dm, tile_t))
+    def _refresh(self):
         info = DedispInfo()
         check(lib().ppl_dedisp_plan_info(self.h, C.byref(info)))
-        self.tile = (info.tileDM, info.tileT)
+        self.tile = (info.tileDM, info.tileT, info.chanBlock)
+
+    def set_tile(self, tile_dm, tile_t, chan_block=0):
+        check(lib().ppl_dedisp_set_shape(self.h, tile_dm, tile_t, chan_block))
+        self._refresh()
+
+    def autotune(self, d_in, d_out, reps=2, stream=None):
+        """Time every tile shape / channel block on these buffers and keep the fastest."""
+        best = C.c_float(0)
+        check(lib().ppl_dedisp_autotune(self.h, _ptr(d_in), _ptr(d_out), reps, _stream(stream),
+                                        C.byref(best)))
+        self._refresh()
+        return best.value
 
     def delays(self):
         out = np.zeros((self.cfg.tdms, self.cfg.nChan), np.int32)
@@ -409,6 +423,12 @@ class Clipper:
         m, r = C.c_float(0), C.c_float(0)
         check(lib().ppl_clipper_stats(self.h, C.byref(m), C.byref(r)))
         return m.value, r.value
+
+    def counters(self):
+        """(spectra decided from the precomputed mask, spectra on the full path)."""
+        a, b = C.c_longlong(0), C.c_longlong(0)
+        check(lib().ppl_clipper_counters(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def flags(self, n_t):
         out = np.zeros(n_t, np.uint8)

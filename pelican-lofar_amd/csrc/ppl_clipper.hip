@@ -129,6 +129,11 @@ struct ClipScalars {
     int badSpectra, num, remainingZeros;
     int meanStart, meanSize, rmsStart, rmsSize;
     int initialised;
+    // newest and second-newest ring entries, kept beside the rings so the
+    // walk never reads them back from memory
+    float meanBack, meanBack2, rmsBack, rmsBack2;
+    // spectra decided from the precomputed mask / by the full workgroup
+    long long nFast, nSlow;
 };
 
 struct ClipConst {
@@ -145,8 +150,10 @@ struct ClipConst {
 struct SpecRec {
     float params0;     // bandpass offset used by this spectrum
     float dataModel;
+    float margin;      // channel threshold of this spectrum
     float meanRunAve;  // for the non-zero-DM normalisation
     int flags;         // bit0: spectrum kept; bit1: channel replacement already applied
+    int ver;           // last-good version in force for this spectrum
 };
 
 __device__ __forceinline__ float bp_value(const double* __restrict__ terms, int nTerms, int nBins, int i,
@@ -165,7 +172,7 @@ struct RingView {
     int cap;
 };
 
-__device__ __forceinline__ void ring_push(RingView r, int& start, int& size, float v)
+__device__ __forceinline__ void ring_push(RingView r, int& start, int& size, float& back, float& back2, float v)
 {
     if (r.cap == 0) return;
     if (size < r.cap) {
@@ -176,15 +183,45 @@ __device__ __forceinline__ void ring_push(RingView r, int& start, int& size, flo
         r.buf[start] = v;
         start = (start + 1) % r.cap;
     }
+    back2 = back;
+    back = v;
+}
+// pop_back() then push_back(back()): the newest entry becomes a copy of the
+// one before it (RFI_Clipper.cpp:432-437)
+__device__ __forceinline__ void ring_repeat_previous(RingView r, int& start, int& size, float& back, float& back2)
+{
+    const float prev = back2;
+    size -= 1;
+    ring_push(r, start, size, back, back2, prev);
+    back2 = prev;
 }
 __device__ __forceinline__ float ring_front(RingView r, int start) { return r.buf[start]; }
-__device__ __forceinline__ float ring_back(RingView r, int start, int size)
-{
-    // back() of an empty buffer is undefined in the reference; read nothing
-    return size > 0 ? r.buf[(start + size - 1) % r.cap] : 0.0f;
-}
 
 constexpr int kWalkThreads = 512;
+constexpr int kStage = 512;      // spectra whose precomputed records are staged in LDS at a time
+constexpr int kRowChunk = 4096;  // bins of one spectrum staged in LDS for the ordered sums
+
+// Per-chunk speculation box: every spectrum of the chunk whose bandpass
+// offset lies in [p0lo, p0hi] and whose margin lies in [mlo, mhi] is judged
+// by the channel mask the precompute pass fixed for that spectrum, provided
+// no channel of it was undecided over the whole box.
+struct Box {
+    float p0lo, p0hi, mlo, mhi;
+    int valid;
+};
+
+// Precomputed per spectrum (steady state, dataModel = 0): ordered sum of the
+// channels kept for every state in the box, their count per band, and the
+// number of channels whose fate the box does not decide.
+struct PreRec {
+    float sum;
+    int undecided;
+    int fillCand;  // kept channels with a non-zero value whose last-good slot was empty at chunk start
+    int pad;
+    uint16_t cnt[8];
+};
+
+constexpr int kMaxVer = 4;  // last-good versions kept per chunk for the apply pass
 
 __device__ __forceinline__ float block_min(float v, float* red)
 {
@@ -207,50 +244,309 @@ __device__ __forceinline__ float block_min(float v, float* red)
     return r;
 }
 
-// The sequential walk over nT spectra of one chunk.
+// The box around the chunk-start state and the bandpass at its two offsets.
+__global__ void clip_bounds_kernel(const ClipScalars* __restrict__ S, ClipConst K, const double* __restrict__ terms,
+                                   Box* __restrict__ box, float* __restrict__ bpLo, float* __restrict__ bpHi)
+{
+    const ClipScalars s = *S;
+    const bool steady = s.rmsSize == K.cap && s.meanSize == K.cap;
+    const float rms = s.rmsRunAve;
+    const bool valid = steady && isfinite(rms) && rms > 0.0f && isfinite(s.params0);
+    const float dp = 1.0f * rms;
+    const float p0lo = s.params0 - dp, p0hi = s.params0 + dp;
+    const float margin = K.crFactor * rms;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        Box b;
+        b.p0lo = p0lo;
+        b.p0hi = p0hi;
+        b.mlo = margin * 0.9f;
+        b.mhi = margin * 1.1f;
+        b.valid = valid ? 1 : 0;
+        *box = b;
+    }
+    if (i < K.nBins) {
+        const float a = bp_value(terms, K.nTerms, K.nBins, i, p0lo, K.scale);
+        const float b = bp_value(terms, K.nTerms, K.nBins, i, p0hi, K.scale);
+        bpLo[i] = fminf(a, b);
+        bpHi[i] = fmaxf(a, b);
+    }
+}
+
+// One lane per spectrum, bins staged through 64x64 LDS tiles: the kept-
+// channel sum in bin order (the reference's order) for the mask that holds
+// over the whole box.  A channel is kept for every state in the box when
+// fl(I - bpLo) <= mlo, clipped for every state when fl(I - bpHi) > mhi
+// (fl(I - bp) falls as bp rises, and bp rises with the offset).
+__global__ __launch_bounds__(64) void clip_pre_kernel(const float* __restrict__ I, ClipConst K,
+                                                      const Box* __restrict__ box, const float* __restrict__ bpLo,
+                                                      const float* __restrict__ bpHi,
+                                                      const float* __restrict__ lastGood, PreRec* __restrict__ pre,
+                                                      int nT)
+{
+    __shared__ float tile[64][65];
+    __shared__ float lo[64], hi[64], lg[64];
+    const Box bx = *box;
+    if (!bx.valid) return;
+    const int lane = threadIdx.x;
+    const int t0 = blockIdx.x * 64;
+    const int nRows = min(64, nT - t0);
+    const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
+    const int cpb = nBins / 8;
+    float sum = 0.0f;
+    int undecided = 0, fillCand = 0;
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int band = 0, bandNext = cpb, curCnt = 0;
+    for (int b0 = 0; b0 < nBins; b0 += 64) {
+        const int nb = min(64, nBins - b0);
+        for (int r = 0; r < nRows; ++r) {
+            if (lane < nb) {
+                const int i = b0 + lane;
+                const int sb = i / nCh, c = i % nCh;
+                tile[r][lane] = I[(long)nCh * (nPol * ((long)nSub * (t0 + r) + sb)) + c];
+            }
+        }
+        if (lane < nb) {
+            lo[lane] = bpLo[b0 + lane];
+            hi[lane] = bpHi[b0 + lane];
+            lg[lane] = lastGood[b0 + lane];
+        }
+        __syncthreads();
+        if (lane < nRows) {
+            for (int k = 0; k < nb; ++k) {
+                const int i = b0 + k;
+                if (i == bandNext) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q == band) cnt[q] = curCnt;
+                    ++band;
+                    bandNext += cpb;
+                    curCnt = 0;
+                }
+                const float v = tile[lane][k];
+                const float eHi = v - lo[k];  // largest fl(I - bp) over the box
+                const float eLo = v - hi[k];  // smallest
+                if (eHi <= bx.mlo) {
+                    sum += v;
+                    ++curCnt;
+                    if (lg[k] == 0.0f && v != 0.0f) ++fillCand;
+                } else if (!(eLo > bx.mhi)) {
+                    ++undecided;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (lane < nRows) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == band) cnt[q] = curCnt;
+        PreRec r;
+        r.sum = sum;
+        r.undecided = undecided;
+        r.fillCand = fillCand;
+        r.pad = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r.cnt[q] = (uint16_t)cnt[q];
+        pre[t0 + lane] = r;
+    }
+}
+
+// The sequential walk over nT spectra of one chunk.  Thread 0 runs ahead
+// through spectra the precompute decided; the workgroup joins in for the
+// others (the learning phase, undecided channels, a state outside the box,
+// an incomplete last-good spectrum).
 __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     float* __restrict__ I, ClipConst K, ClipScalars* __restrict__ S, float* __restrict__ meanBuf,
     double* __restrict__ meanPrefix, float* __restrict__ rmsBuf, double* __restrict__ rmsPrefix,
-    float* __restrict__ lastGood, const double* __restrict__ terms, float* __restrict__ bp,
-    uint8_t* __restrict__ mask, SpecRec* __restrict__ rec, int nT)
+    float* __restrict__ lastGood, float* __restrict__ lastGoodVer, const double* __restrict__ terms,
+    uint8_t* __restrict__ mask, const Box* __restrict__ box, const PreRec* __restrict__ pre,
+    SpecRec* __restrict__ rec, int nT)
 {
     __shared__ ClipScalars s;
     __shared__ float red[kWalkThreads / 64 + 1];
     __shared__ int goodCnt[8];
     __shared__ int fillCnt;
-    __shared__ float sh_dataModel, sh_margin, sh_learnRMS;
-    __shared__ int sh_learning, sh_good, sh_fill;
+    __shared__ float sh_dataModel, sh_margin;
+    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_ver;
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
+    __shared__ PreRec recs[kStage];
+    __shared__ float meanFront[kStage], rmsFront[kStage];
+    __shared__ float rowChunk[kRowChunk];
 
     const int tid = threadIdx.x;
     const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
     const int cpb = nBins / 8;
     RingView meanR{meanBuf, meanPrefix, K.cap}, rmsR{rmsBuf, rmsPrefix, K.cap};
+    const Box bx = *box;
     if (tid == 0) s = *S;
     __syncthreads();
 
-    auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb) + pol); };
+    auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb)) + (long)nCh * pol; };
 
-    // bandpass of the current offset
-    for (int i = tid; i < nBins; i += kWalkThreads) bp[i] = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
-    __syncthreads();
+    int stageBase = 0, stageN = 0;
+    // Ring fronts of the staged spectra: with both rings full, the front of
+    // spectrum stageBase+k is entry start+k (each spectrum evicts one front;
+    // a rejection rewrites only the newest entry).
+    auto rmsFrontOf = [&](int t) {
+        return sh_frontsValid ? rmsFront[t - stageBase] : ring_front(rmsR, s.rmsStart);
+    };
+    auto meanFrontOf = [&](int t) {
+        return sh_frontsValid ? meanFront[t - stageBase] : ring_front(meanR, s.meanStart);
+    };
+    // Steady-state rms update and margin (RFI_Clipper.cpp:288-311), thread 0.
+    auto phaseA_steady = [&](int t, float& dataModel, float& margin) {
+        s.rmsRunAve -= (rmsFrontOf(t) / (float)s.rmsSize);
+        s.rmsRunAve += (s.rmsBack / (float)K.cap);
+        if (K.useMeanOverRMS) s.spectrumRMS = fabsf(s.meanRunAve / s.meanOverRMS);
+        ring_push(rmsR, s.rmsStart, s.rmsSize, s.rmsBack, s.rmsBack2, s.spectrumRMS);
+        dataModel = 0.0f;
+        margin = K.crFactor * s.rmsRunAve;
+    };
 
-    for (int t = 0; t < nT; ++t) {
-        const int learning = s.rmsSize < K.cap;  // !_rmsBuffer.full()
-        if (tid < 8) {
-            goodCnt[tid] = 0;
+    // Decision and bookkeeping (:345-565) for spectrum t, thread 0.  Returns
+    // whether the last-good spectrum must be filled from this spectrum.
+    auto phaseC = [&](int t, float spectrumSum, const int* cnt, float dataModel, float margin,
+                      int replaced) -> int {
+        int total = 0;
+        for (int b = 0; b < 8; ++b) total += cnt[b];
+        s.fractionBadChannels += (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
+        spectrumSum /= (float)(unsigned)total;
+        unsigned badBands = 0;
+        for (int b = 0; b < 8; ++b) {
+            if ((double)(float)cnt[b] < 0.8 * (unsigned)cpb) ++badBands;
+            if (cnt[b] == 0) badBands += 4;
         }
+        if ((double)(unsigned)total < 0.8 * (unsigned)nBins) badBands += 4;
+        if (s.meanSize < K.cap) {
+            ring_push(meanR, s.meanStart, s.meanSize, s.meanBack, s.meanBack2, spectrumSum);
+            s.meanRunAve = (float)(meanPrefix[s.meanSize] / (double)s.meanSize);
+        } else {
+            s.meanRunAve -= meanFrontOf(t) / (float)s.meanSize;
+            s.meanRunAve += s.meanBack / (float)s.meanSize;
+            ring_push(meanR, s.meanStart, s.meanSize, s.meanBack, s.meanBack2, spectrumSum);
+        }
+        const float tol = (float)((double)(K.srFactor * s.bpRms) / K.sqrtNBins);
+        int good = 1;
+        if (s.meanSize < 1000) {
+            good = 0;
+        } else if (spectrumSum - s.meanRunAve > tol || badBands >= 4) {
+            good = 0;
+            ++s.badSpectra;
+            ring_repeat_previous(meanR, s.meanStart, s.meanSize, s.meanBack, s.meanBack2);
+            ring_repeat_previous(rmsR, s.rmsStart, s.rmsSize, s.rmsBack, s.rmsBack2);
+        }
+        SpecRec r;
+        r.params0 = s.params0;
+        r.dataModel = dataModel;
+        r.margin = margin;
+        r.flags = (good ? 1 : 0) | (replaced ? 2 : 0);
+        r.ver = sh_ver;
+        const unsigned reportStatsEvery = 10u * (unsigned)K.maxHistory;
+        if (s.num == 0) {
+            const float fractionBadSpectra =
+                (float)(100.0 * (double)(float)s.badSpectra / (double)(float)reportStatsEvery);
+            if (fractionBadSpectra > 99.0) {
+                s.rmsSize = 0;
+                s.meanSize = 0;
+                s.rmsStart = 0;
+                s.meanStart = 0;
+            }
+            s.badSpectra = 0;
+            s.fractionBadChannels = 0.0f;
+        }
+        r.meanRunAve = s.meanRunAve;
+        rec[t] = r;
+        return good && s.remainingZeros != 0;
+    };
+    // setMean (BandPass.cpp:63-74) and setRMS (:44-46) at the end of a spectrum.
+    auto phaseD = [&]() {
+        const float delta = s.meanRunAve - s.bpMean;
+        if (fabsf(delta) > 0.0f) {
+            s.params0 += delta / K.scale;
+            s.bpMean = s.meanRunAve;
+        }
+        s.bpRms = s.rmsRunAve;
+        s.num = (int)(((unsigned)s.num + 1u) % (10u * (unsigned)K.maxHistory));
+    };
+
+    enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3 };
+    if (tid == 0) {
+        sh_frontsValid = 0;
+        sh_ver = 0;
+    }
+    // version 0 of the last-good spectrum: its state at chunk start
+    for (int i = tid; i < nBins; i += kWalkThreads) lastGoodVer[i] = lastGood[i];
+    __syncthreads();
+    int t = 0;
+    for (;;) {
+        if (tid == 0) {
+            int mode = MODE_END;
+            for (;;) {
+                if (t >= nT) { mode = MODE_END; break; }
+                if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
+                if (s.rmsSize < K.cap) { mode = MODE_FULL; break; }  // learning phase
+                float dataModel, margin;
+                phaseA_steady(t, dataModel, margin);
+                const int k = t - stageBase;
+                // a kept spectrum fills empty last-good slots (:446-469); the
+                // precompute counted the candidates against the chunk-start
+                // slots, a superset of the empty slots now
+                const bool fast = bx.valid && sh_ver < kMaxVer &&
+                                  (s.remainingZeros == 0 || recs[k].fillCand == 0) &&
+                                  recs[k].undecided == 0 && s.params0 >= bx.p0lo && s.params0 <= bx.p0hi &&
+                                  margin >= bx.mlo && margin <= bx.mhi;
+                if (!fast) {
+                    sh_dataModel = dataModel;
+                    sh_margin = margin;
+                    mode = MODE_BC;
+                    break;
+                }
+                int cnt[8];
+                for (int b = 0; b < 8; ++b) cnt[b] = recs[k].cnt[b];
+                phaseC(t, recs[k].sum, cnt, dataModel, margin, 0);
+                phaseD();
+                ++s.nFast;
+                ++t;
+            }
+            sh_mode = mode;
+            sh_t = t;
+        }
+        __syncthreads();
+        const int mode = sh_mode;
+        t = sh_t;
+        if (mode == MODE_END) break;
+        if (mode == MODE_STAGE) {
+            stageBase = t;
+            stageN = min(kStage, nT - t);
+            if (bx.valid)
+                for (int k = tid; k < stageN; k += kWalkThreads) recs[k] = pre[t + k];
+            const int fv = s.rmsSize == K.cap && s.meanSize == K.cap && K.cap > kStage;
+            if (fv) {
+                for (int k = tid; k < stageN; k += kWalkThreads) {
+                    rmsFront[k] = rmsBuf[(s.rmsStart + k) % K.cap];
+                    meanFront[k] = meanBuf[(s.meanStart + k) % K.cap];
+                }
+            }
+            __syncthreads();
+            if (tid == 0) sh_frontsValid = fv;
+            __syncthreads();
+            continue;
+        }
+        // ---- slow spectrum t, whole workgroup
+        if (tid < 8) goodCnt[tid] = 0;
         if (tid == 0) fillCnt = 0;
-        // ---- learning-phase band statistics (RFI_Clipper.cpp:214-242)
-        if (learning) {
+        if (mode == MODE_FULL) {
+            // learning-phase band statistics (RFI_Clipper.cpp:214-242)
             for (int b = 0; b < 8; ++b) {
                 float md = 1e6f, mm = 1e6f;
                 for (int i = b * cpb + tid; i < (b + 1) * cpb; i += kWalkThreads) {
                     const int sb = i / nCh, c = i % nCh;
                     const float v = I[idx(sb, 0, t) + c];
+                    const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
                     md = (v < md) ? v : md;
-                    mm = (bp[i] < mm) ? bp[i] : mm;
+                    mm = (bpv < mm) ? bpv : mm;
                 }
                 md = block_min(md, red);
                 mm = block_min(mm, red);
@@ -259,24 +555,31 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
                     modelMin[b] = mm;
                 }
             }
-            if (tid < 8) {
-                // ordered sums of band tid, in (s, c) order
-                float sum = 0.0f, sq = 0.0f;
-                for (int i = tid * cpb; i < (tid + 1) * cpb; ++i) {
-                    const int sb = i / nCh, c = i % nCh;
-                    const float v = I[idx(sb, 0, t) + c];
-                    sum += v;
-                    sq += (v * v);
+            // ordered band sums (band b by lane b, in bin order), rows staged in LDS
+            float bsum = 0.0f, bsq = 0.0f;
+            for (int x0 = 0; x0 < nBins; x0 += kRowChunk) {
+                const int n = min(kRowChunk, nBins - x0);
+                for (int k = tid; k < n; k += kWalkThreads) {
+                    const int i = x0 + k, sb = i / nCh, c = i % nCh;
+                    rowChunk[k] = I[idx(sb, 0, t) + c];
                 }
-                bandMean[tid] = sum;
-                bandMeanSq[tid] = sq;
+                __syncthreads();
+                if (tid < 8) {
+                    const int a = max(tid * cpb, x0) - x0, b = min((tid + 1) * cpb, x0 + n) - x0;
+                    for (int k = a; k < b; ++k) {
+                        const float v = rowChunk[k];
+                        bsum += v;
+                        bsq += (v * v);
+                    }
+                }
+                __syncthreads();
+            }
+            if (tid < 8) {
+                bandMean[tid] = bsum;
+                bandMeanSq[tid] = bsq;
             }
             __syncthreads();
-        }
-        // ---- model and margin (:214-311)
-        if (tid == 0) {
-            float dataModel;
-            if (learning) {
+            if (tid == 0) {
                 float dmm[8], sig[8];
                 for (int b = 0; b < 8; ++b) {
                     dmm[b] = bandMin[b] - modelMin[b];
@@ -287,9 +590,9 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
                 float spectrumRMS = sig[0];
                 for (int b = 1; b < 8; ++b)
                     if (sig[b] < spectrumRMS) spectrumRMS = sig[b];  // std::min_element
-                // nth_element(..., begin()+4) of 8 values: the 5th smallest
+                // nth_element(..., begin()+4): the 5th smallest of 8
                 for (int a = 1; a < 8; ++a) {
-                    float v = dmm[a];
+                    const float v = dmm[a];
                     int j = a - 1;
                     while (j >= 0 && dmm[j] > v) {
                         dmm[j + 1] = dmm[j];
@@ -297,29 +600,24 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
                     }
                     dmm[j + 1] = v;
                 }
-                dataModel = dmm[4];
+                float dataModel = dmm[4];
                 s.spectrumRMS = spectrumRMS;
-                ring_push(rmsR, s.rmsStart, s.rmsSize, spectrumRMS);
+                ring_push(rmsR, s.rmsStart, s.rmsSize, s.rmsBack, s.rmsBack2, spectrumRMS);
                 s.rmsRunAve = (float)(rmsPrefix[s.rmsSize] / (double)s.rmsSize);
                 if (s.rmsSize == K.cap) s.meanOverRMS = s.meanRunAve / s.rmsRunAve;
                 dataModel = dataModel + K.meanMinusMinimum * s.rmsRunAve;
-            } else {
-                s.rmsRunAve -= (ring_front(rmsR, s.rmsStart) / (float)s.rmsSize);
-                s.rmsRunAve += (ring_back(rmsR, s.rmsStart, s.rmsSize) / (float)K.cap);
-                if (K.useMeanOverRMS) s.spectrumRMS = fabsf(s.meanRunAve / s.meanOverRMS);
-                ring_push(rmsR, s.rmsStart, s.rmsSize, s.spectrumRMS);
-                dataModel = 0.0f;
+                sh_dataModel = dataModel;
+                sh_margin = K.crFactor * s.rmsRunAve;
             }
-            sh_dataModel = dataModel;
-            sh_margin = K.crFactor * s.rmsRunAve;
         }
         __syncthreads();
         const float dataModel = sh_dataModel, margin = sh_margin;
-        // ---- channel clipping (:315-342)
+        // channel clipping (:315-342)
         for (int i = tid; i < nBins; i += kWalkThreads) {
             const int sb = i / nCh, c = i % nCh;
             const long i0 = idx(sb, 0, t) + c;
-            const bool clip = I[i0] - dataModel - bp[i] > margin;
+            const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+            const bool clip = I[i0] - dataModel - bpv > margin;
             mask[i] = clip ? 1 : 0;
             if (clip) {
                 const float g = lastGood[i];
@@ -329,52 +627,28 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
             }
         }
         __syncthreads();
+        // ordered sum of the kept channels (:339): clipped bins staged as +0.0,
+        // which leaves a sum that starts at +0.0 unchanged
+        float spectrumSum = 0.0f;
+        for (int x0 = 0; x0 < nBins; x0 += kRowChunk) {
+            const int n = min(kRowChunk, nBins - x0);
+            for (int k = tid; k < n; k += kWalkThreads) {
+                const int i = x0 + k, sb = i / nCh, c = i % nCh;
+                rowChunk[k] = mask[i] ? 0.0f : I[idx(sb, 0, t) + c];
+            }
+            __syncthreads();
+            if (tid == 0)
+                for (int k = 0; k < n; ++k) spectrumSum += rowChunk[k];
+            __syncthreads();
+        }
         if (tid == 0) {
-            float spectrumSum = 0.0f;
-            for (int i = 0; i < nBins; ++i) {
-                if (!mask[i]) {
-                    const int sb = i / nCh, c = i % nCh;
-                    spectrumSum += I[idx(sb, 0, t) + c];
-                }
-            }
-            int total = 0;
-            for (int b = 0; b < 8; ++b) total += goodCnt[b];
-            s.fractionBadChannels += (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
-            spectrumSum /= (float)(unsigned)total;
-            unsigned badBands = 0;
-            for (int b = 0; b < 8; ++b) {
-                if ((double)(float)goodCnt[b] < 0.8 * (unsigned)cpb) ++badBands;
-                if (goodCnt[b] == 0) badBands += 4;
-            }
-            if ((double)(unsigned)total < 0.8 * (unsigned)nBins) badBands += 4;
-            // mean history (:383-400)
-            if (s.meanSize < K.cap) {
-                ring_push(meanR, s.meanStart, s.meanSize, spectrumSum);
-                s.meanRunAve = (float)(meanPrefix[s.meanSize] / (double)s.meanSize);
-            } else {
-                s.meanRunAve -= ring_front(meanR, s.meanStart) / (float)s.meanSize;
-                s.meanRunAve += ring_back(meanR, s.meanStart, s.meanSize) / (float)s.meanSize;
-                ring_push(meanR, s.meanStart, s.meanSize, spectrumSum);
-            }
-            const float tol = (float)((double)(K.srFactor * s.bpRms) / K.sqrtNBins);
-            int good = 1;
-            if (s.meanSize < 1000) {
-                good = 0;
-            } else if (spectrumSum - s.meanRunAve > tol || badBands >= 4) {
-                good = 0;
-                ++s.badSpectra;
-                // pop_back then push_back(back()) on both rings: the newest
-                // entry becomes a copy of the one before it
-                s.meanSize -= 1;
-                s.rmsSize -= 1;
-                ring_push(meanR, s.meanStart, s.meanSize, ring_back(meanR, s.meanStart, s.meanSize));
-                ring_push(rmsR, s.rmsStart, s.rmsSize, ring_back(rmsR, s.rmsStart, s.rmsSize));
-            }
-            sh_good = good;
-            sh_fill = good && s.remainingZeros != 0;
+            int cnt[8];
+            for (int b = 0; b < 8; ++b) cnt[b] = goodCnt[b];
+            sh_fill = phaseC(t, spectrumSum, cnt, dataModel, margin, 1);
         }
         __syncthreads();
-        // ---- last-good spectrum (:441-469)
+        // last-good spectrum (:441-469); every change starts a new version
+        // for the spectra that follow (the apply pass replaces from it)
         if (sh_fill) {
             for (int i = tid; i < nBins; i += kWalkThreads) {
                 const int sb = i / nCh, c = i % nCh;
@@ -385,42 +659,22 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
                 }
             }
             __syncthreads();
+            if (fillCnt > 0) {
+                const int nv = sh_ver + 1;
+                if (nv < kMaxVer)
+                    for (int i = tid; i < nBins; i += kWalkThreads) lastGoodVer[(size_t)nv * nBins + i] = lastGood[i];
+                __syncthreads();
+                if (tid == 0) sh_ver = nv;  // >= kMaxVer: the rest of the chunk takes this path
+            }
         }
         if (tid == 0) {
             s.remainingZeros -= fillCnt;
-            SpecRec r;
-            r.params0 = s.params0;
-            r.dataModel = dataModel;
-            r.flags = (sh_good ? 1 : 0) | 2;
-            // stats (:535-560)
-            const unsigned reportStatsEvery = 10u * (unsigned)K.maxHistory;
-            if (s.num == 0) {
-                const float fractionBadSpectra =
-                    (float)(100.0 * (double)(float)s.badSpectra / (double)(float)reportStatsEvery);
-                if (fractionBadSpectra > 99.0) {
-                    s.rmsSize = 0;
-                    s.meanSize = 0;
-                    s.rmsStart = 0;
-                    s.meanStart = 0;
-                }
-                s.badSpectra = 0;
-                s.fractionBadChannels = 0.0f;
-            }
-            r.meanRunAve = s.meanRunAve;
-            rec[t] = r;
-            // setMean (BandPass.cpp:63-74) and setRMS (:44-46)
-            const float delta = s.meanRunAve - s.bpMean;
-            if (fabsf(delta) > 0.0f) {
-                s.params0 += delta / K.scale;
-                s.bpMean = s.meanRunAve;
-            }
-            s.bpRms = s.rmsRunAve;
-            s.num = (int)(((unsigned)s.num + 1u) % reportStatsEvery);
+            phaseD();
+            ++s.nSlow;
+            sh_t = t + 1;
         }
         __syncthreads();
-        for (int i = tid; i < nBins; i += kWalkThreads)
-            bp[i] = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
-        __syncthreads();
+        t = sh_t;
     }
     if (tid == 0) *S = s;
 }
@@ -433,6 +687,7 @@ constexpr int kApplyTile = 64;
 // its own spectrum in bin order while global accesses stay coalesced.
 __global__ __launch_bounds__(64) void clip_apply_kernel(float* __restrict__ I, float* __restrict__ W,
                                                         ClipConst K, const double* __restrict__ terms,
+                                                        const float* __restrict__ lastGoodVer,
                                                         const SpecRec* __restrict__ rec, int nT)
 {
     __shared__ float tile[kApplyTile][kApplyTile + 1];
@@ -446,6 +701,22 @@ __global__ __launch_bounds__(64) void clip_apply_kernel(float* __restrict__ I, f
     __syncthreads();
     const int nRows = min(kApplyTile, nT - t0);
 
+    // channel clipping of kept spectra the walk judged from the precomputed
+    // mask (:319-335): same test, replacement by the (complete) last-good value
+    for (int r = 0; r < nRows; ++r) {
+        if ((recs[r].flags & 3) == 1) {
+            const float dm = recs[r].dataModel, margin = recs[r].margin, p0 = recs[r].params0;
+            for (int i = lane; i < nBins; i += 64) {
+                const int sb = i / nCh, c = i % nCh;
+                const long gi = idx(sb, 0, t0 + r) + c;
+                const float bpv = bp_value(terms, K.nTerms, nBins, i, p0, K.scale);
+                if (I[gi] - dm - bpv > margin) {
+                    const float g = lastGoodVer[(size_t)recs[r].ver * nBins + i];
+                    for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t0 + r) + c] = g;
+                }
+            }
+        }
+    }
     // clipSample for rejected spectra: every pol and bin = -0.2
     for (int r = 0; r < nRows; ++r) {
         if (!(recs[r].flags & 1)) {
@@ -526,10 +797,13 @@ struct ppl_clipper {
     int nBins = 0;        // 0 until the first run fixes the map
     ClipConst K{};
     ClipScalars* d_S = nullptr;
-    float *d_meanBuf = nullptr, *d_rmsBuf = nullptr, *d_lastGood = nullptr, *d_bp = nullptr;
+    float *d_meanBuf = nullptr, *d_rmsBuf = nullptr, *d_lastGood = nullptr;
     double *d_meanPrefix = nullptr, *d_rmsPrefix = nullptr, *d_terms = nullptr;
     uint8_t* d_mask = nullptr;
     SpecRec* d_rec = nullptr;
+    PreRec* d_pre = nullptr;
+    Box* d_box = nullptr;
+    float *d_bpLo = nullptr, *d_bpHi = nullptr, *d_lastGoodVer = nullptr;
     int recCap = 0;
     std::vector<uint8_t> lastFlags;
     int lastNT = 0;
@@ -543,14 +817,22 @@ void free_state(ppl_clipper* c)
     (void)hipFree(c->d_meanBuf);
     (void)hipFree(c->d_rmsBuf);
     (void)hipFree(c->d_lastGood);
-    (void)hipFree(c->d_bp);
     (void)hipFree(c->d_meanPrefix);
     (void)hipFree(c->d_rmsPrefix);
     (void)hipFree(c->d_terms);
     (void)hipFree(c->d_mask);
     (void)hipFree(c->d_rec);
+    (void)hipFree(c->d_pre);
+    (void)hipFree(c->d_box);
+    (void)hipFree(c->d_bpLo);
+    (void)hipFree(c->d_bpHi);
+    (void)hipFree(c->d_lastGoodVer);
+    c->d_lastGoodVer = nullptr;
+    c->d_pre = nullptr;
+    c->d_box = nullptr;
+    c->d_bpLo = c->d_bpHi = nullptr;
     c->d_S = nullptr;
-    c->d_meanBuf = c->d_rmsBuf = c->d_lastGood = c->d_bp = nullptr;
+    c->d_meanBuf = c->d_rmsBuf = c->d_lastGood = nullptr;
     c->d_meanPrefix = c->d_rmsPrefix = c->d_terms = nullptr;
     c->d_mask = nullptr;
     c->d_rec = nullptr;
@@ -608,8 +890,11 @@ int setup_map(ppl_clipper* c, unsigned nSub, unsigned nPol, unsigned nCh)
     PPL_HIP(hipMalloc(&c->d_meanPrefix, (cap + 1) * sizeof(double)));
     PPL_HIP(hipMalloc(&c->d_rmsPrefix, (cap + 1) * sizeof(double)));
     PPL_HIP(hipMalloc(&c->d_lastGood, nBins * sizeof(float)));
-    PPL_HIP(hipMalloc(&c->d_bp, nBins * sizeof(float)));
     PPL_HIP(hipMalloc(&c->d_mask, nBins));
+    PPL_HIP(hipMalloc(&c->d_box, sizeof(Box)));
+    PPL_HIP(hipMalloc(&c->d_bpLo, nBins * sizeof(float)));
+    PPL_HIP(hipMalloc(&c->d_bpHi, nBins * sizeof(float)));
+    PPL_HIP(hipMalloc(&c->d_lastGoodVer, (size_t)kMaxVer * nBins * sizeof(float)));
     PPL_HIP(hipMalloc(&c->d_terms, std::max<size_t>(terms.size(), 1) * sizeof(double)));
     PPL_HIP(hipMemcpy(c->d_S, &S, sizeof S, hipMemcpyHostToDevice));
     PPL_HIP(hipMemset(c->d_meanPrefix, 0, (cap + 1) * sizeof(double)));
@@ -679,16 +964,25 @@ static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsi
     if (nT == 0) return PPL_OK;
     if ((int)nT > c->recCap) {
         (void)hipFree(c->d_rec);
+        (void)hipFree(c->d_pre);
         c->d_rec = nullptr;
+        c->d_pre = nullptr;
         PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * nT));
+        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * nT));
         c->recCap = (int)nT;
     }
+    hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->K, c->d_terms,
+                       c->d_box, c->d_bpLo, c->d_bpHi);
+    PPL_LAUNCHED();
+    hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(nT, 64)), dim3(64), 0, st, d_I, c->K, c->d_box, c->d_bpLo,
+                       c->d_bpHi, c->d_lastGood, c->d_pre, (int)nT);
+    PPL_LAUNCHED();
     hipLaunchKernelGGL(clip_walk_kernel, dim3(1), dim3(kWalkThreads), 0, st, d_I, c->K, c->d_S, c->d_meanBuf,
-                       c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_terms, c->d_bp,
-                       c->d_mask, c->d_rec, (int)nT);
+                       c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_lastGoodVer, c->d_terms,
+                       c->d_mask, c->d_box, c->d_pre, c->d_rec, (int)nT);
     PPL_LAUNCHED();
     hipLaunchKernelGGL(clip_apply_kernel, dim3(cdiv(nT, kApplyTile)), dim3(64), 0, st, d_I, d_W, c->K,
-                       c->d_terms, c->d_rec, (int)nT);
+                       c->d_terms, c->d_lastGoodVer, c->d_rec, (int)nT);
     PPL_LAUNCHED();
     c->lastNT = (int)nT;
     return PPL_OK;
@@ -712,6 +1006,18 @@ extern "C" int ppl_clipper_stats(ppl_clipper* c, float* mean, float* rms)
     PPL_HIP(hipMemcpy(&S, c->d_S, sizeof S, hipMemcpyDeviceToHost));
     *mean = S.meanRunAve;
     *rms = S.rmsRunAve;
+    return PPL_OK;
+}
+
+extern "C" int ppl_clipper_counters(ppl_clipper* c, long long* fast, long long* slow)
+{
+    PPL_CHECK_ARG(c && fast && slow, "null argument");
+    *fast = *slow = 0;
+    if (!c->d_S) return PPL_OK;
+    ClipScalars S;
+    PPL_HIP(hipMemcpy(&S, c->d_S, sizeof S, hipMemcpyDeviceToHost));
+    *fast = S.nFast;
+    *slow = S.nSlow;
     return PPL_OK;
 }
 

@@ -9,15 +9,16 @@
 // trials; lane l accumulates samples t0 + l + 64*i (i < NT), so every LDS
 // read of a wave is 64 consecutive words (bank-conflict free) and every
 // store is a coalesced 256-byte row segment.  Channels are streamed in
-// blocks of CB: for each channel the workgroup stages the window
-// [t0 + delay[dm0][c], t0 + delay[dm0+TD-1][c] + TT) of the input row into
-// LDS with coalesced loads, then each wave reads its ND delayed 64-wide
-// slices from LDS.  Delays are wave-uniform and come from a transposed
-// table delayT[c][dm] through the scalar cache.
+// blocks of CB: for each channel the window [t0 + delay[dm0][c],
+// t0 + delay[dm0+TD-1][c] + TT) of the input row is staged in LDS, and each
+// wave reads its ND delayed 64-wide slices from there.  The FP32 form is
+// bound by LDS bandwidth (one 4-byte read per add); FP16 and u8 inputs halve
+// and quarter the bytes per add.
 #include "ppl_internal.h"
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 struct ppl_dedisp_plan {
@@ -29,6 +30,7 @@ struct ppl_dedisp_plan {
     int32_t* d_delayT = nullptr;  // device [nChan][tdmsPad]
     int tdmsPad = 0;
     int nd = 8, nt = 8;           // per-thread DM x sample register tile
+    int cb = 0;                   // channels per staged block (0: as many as fit)
     int span = 0;                 // max delay spread inside a DM tile
 };
 
@@ -41,6 +43,14 @@ template <> struct Acc<float> { using type = float; };
 template <> struct Acc<_Float16> { using type = float; };
 template <> struct Acc<uint8_t> { using type = int32_t; };
 
+// Channel rows are staged as dword streams straight into LDS
+// (global_load_lds_dword, no register round trip), in two buffers so that
+// the loads of channel block k+1 are in flight while block k is summed.  A
+// row starts at the dword holding element t0 + delay[dm0][c]; the element
+// misalignment is folded into that channel's per-DM LDS offsets, which the
+// staging step also writes (offs[c][j] = delay[dm0+j][c] - delay[dm0][c] +
+// misalignment + row base).  Every LDS read of a channel for one DM trial is
+// issued before any of them is consumed.
 template <typename Tin, int ND, int NT>
 __global__ __launch_bounds__(256) void dedisperse_kernel(
     const Tin* __restrict__ in, typename Acc<Tin>::type* __restrict__ out,
@@ -48,10 +58,14 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
     int W, int CB, int nDMTiles)
 {
     using TA = typename Acc<Tin>::type;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    Tin* lds = reinterpret_cast<Tin*>(smem_raw);
     constexpr int TT = 64 * NT;
     constexpr int TD = 4 * ND;
+    constexpr int E = (int)sizeof(Tin);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // layout: rows[2][CB][W] elements (W a multiple of 4 bytes), offs[2][CB][TD]
+    const int rowBytes = W * E;
+    Tin* rows0 = reinterpret_cast<Tin*>(smem_raw);
+    int* offs0 = reinterpret_cast<int*>(smem_raw + 2 * CB * rowBytes);
 
     const int bid = blockIdx.x;
     const int dmTile = bid % nDMTiles;
@@ -61,7 +75,8 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int dmw = dm0 + wave * ND;
+    const size_t totalBytes = (size_t)nChan * nsamp * E;
+    const unsigned char* inBytes = reinterpret_cast<const unsigned char*>(in);
 
     TA acc[ND][NT];
 #pragma unroll
@@ -69,32 +84,57 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
 #pragma unroll
         for (int i = 0; i < NT; ++i) acc[j][i] = TA(0);
 
-    for (int cb0 = 0; cb0 < nChan; cb0 += CB) {
-        const int nc = min(CB, nChan - cb0);
-        __syncthreads();
-        for (int cc = 0; cc < nc; ++cc) {
+    // stage channels [cb0, cb0+nc) into buffer `buf`: wave w takes channels w, w+4, ...
+    auto stage = [&](int buf, int cb0, int nc) {
+        Tin* rows = rows0 + (size_t)buf * CB * W;
+        int* offs = offs0 + buf * CB * TD;
+        const int rowDwords = rowBytes / 4;
+        for (int cc = wave; cc < nc; cc += 4) {
             const int c = cb0 + cc;
-            const int dmin = delayT[(size_t)c * tdmsPad + dm0];
-            const long base = (long)t0 + dmin;
-            const Tin* src = in + (size_t)c * nsamp + base;
-            const long lim = (long)nsamp - base;
-            Tin* dst = lds + cc * W;
-            for (int k = tid; k < W; k += 256) dst[k] = (k < lim) ? src[k] : Tin(0);
+            const int32_t* dl = delayT + (size_t)c * tdmsPad + dm0;
+            const int dmin = dl[0];
+            const size_t byteStart = ((size_t)c * nsamp + (size_t)t0 + (size_t)dmin) * E;
+            const size_t byte0 = byteStart & ~(size_t)3;
+            const int mis = (int)(byteStart - byte0) / E;
+            unsigned char* dstRow = reinterpret_cast<unsigned char*>(rows + cc * W);
+            for (int k0 = 0; k0 < rowDwords; k0 += 64) {
+                const size_t g = byte0 + 4 * (size_t)(k0 + lane);
+                if (k0 + lane < rowDwords && g + 4 <= totalBytes)
+                    __builtin_amdgcn_global_load_lds(inBytes + g, dstRow + 4 * k0, 4, 0, 0);
+            }
+            if (lane < TD) offs[cc * TD + lane] = dl[lane] - dmin + mis + cc * W;
         }
-        __syncthreads();
+    };
+
+    const int nBlocks = (nChan + CB - 1) / CB;
+    stage(0, 0, min(CB, nChan));
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int blk = 0; blk < nBlocks; ++blk) {
+        const int buf = blk & 1;
+        const int cb0 = blk * CB;
+        const int nc = min(CB, nChan - cb0);
+        if (blk + 1 < nBlocks) stage(buf ^ 1, cb0 + CB, min(CB, nChan - cb0 - CB));
+        const Tin* rows = rows0 + (size_t)buf * CB * W;
+        const int* offs = offs0 + buf * CB * TD + wave * ND;
         for (int cc = 0; cc < nc; ++cc) {
-            const int c = cb0 + cc;
-            const int32_t* dl = delayT + (size_t)c * tdmsPad;
-            const int dmin = dl[dm0];
-            const Tin* row = lds + cc * W + lane;
+            int o[ND];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) o[j] = offs[cc * TD + j];
 #pragma unroll
             for (int j = 0; j < ND; ++j) {
-                const Tin* p = row + (dl[dmw + j] - dmin);
+                const Tin* p = rows + o[j] + lane;
+                Tin v[NT];
 #pragma unroll
-                for (int i = 0; i < NT; ++i) acc[j][i] = acc[j][i] + (TA)p[64 * i];
+                for (int i = 0; i < NT; ++i) v[i] = p[64 * i];
+#pragma unroll
+                for (int i = 0; i < NT; ++i) acc[j][i] = acc[j][i] + (TA)v[i];
             }
         }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
     }
+    const int dmw = dm0 + wave * ND;
 #pragma unroll
     for (int j = 0; j < ND; ++j) {
         const int dm = dmw + j;
@@ -103,6 +143,300 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
             for (int i = 0; i < NT; ++i) {
                 const int t = t0 + lane + 64 * i;
                 if (t < nOut) out[(size_t)dm * nOut + t] = acc[j][i];
+            }
+        }
+    }
+}
+
+// FP16 input, packed pairs.  Lane l accumulates the sample pairs
+// (t0 + 2l + 128i, t0 + 2l + 128i + 1): one dword LDS read feeds two adds,
+// each a v_fma_mix_f32 of a half straight from the dword (fma(h, one, acc)
+// with one == 1.0 rounds once, exactly like h + acc).  A pair starting at an
+// odd element is read from a second copy of the row shifted by one element,
+// built in LDS with alignbyte from the staged dwords.
+template <int ND, int NP>
+__global__ __launch_bounds__(256) void dedisperse_f16_kernel(
+    const _Float16* __restrict__ in, float* __restrict__ out, const int32_t* __restrict__ delayT, int nChan,
+    int nsamp, int tdms, int tdmsPad, int nOut, int rowDw, int CB, int nDMTiles, float one)
+{
+    constexpr int TT = 128 * NP;
+    constexpr int TD = 4 * ND;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // per buffer: rows[CB][2][rowDw] dwords (copy A, copy B), then offs[CB][TD]
+    uint32_t* lds32 = reinterpret_cast<uint32_t*>(smem_raw);
+    const int bufDw = CB * (2 * rowDw + TD);
+
+    const int bid = blockIdx.x;
+    const int dmTile = bid % nDMTiles;
+    const int tTile = bid / nDMTiles;
+    const int t0 = tTile * TT;
+    const int dm0 = dmTile * TD;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t totalBytes = (size_t)nChan * nsamp * 2;
+    const unsigned char* inBytes = reinterpret_cast<const unsigned char*>(in);
+
+    float acc[ND][2 * NP];
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int i = 0; i < 2 * NP; ++i) acc[j][i] = 0.0f;
+
+    auto rowsOf = [&](int buf) { return lds32 + (size_t)buf * bufDw; };
+    auto offsOf = [&](int buf) { return reinterpret_cast<int*>(lds32 + (size_t)buf * bufDw + CB * 2 * rowDw); };
+    auto stage = [&](int buf, int cb0, int nc) {
+        uint32_t* rows = rowsOf(buf);
+        int* offs = offsOf(buf);
+        for (int cc = wave; cc < nc; cc += 4) {
+            const int c = cb0 + cc;
+            const int32_t* dl = delayT + (size_t)c * tdmsPad + dm0;
+            const int dmin = dl[0];
+            const size_t byteStart = ((size_t)c * nsamp + (size_t)t0 + (size_t)dmin) * 2;
+            const size_t byte0 = byteStart & ~(size_t)3;
+            const int mis = (int)(byteStart - byte0) / 2;
+            unsigned char* dst = reinterpret_cast<unsigned char*>(rows + (size_t)cc * 2 * rowDw);
+            for (int k0 = 0; k0 < rowDw; k0 += 64) {
+                const size_t g = byte0 + 4 * (size_t)(k0 + lane);
+                if (k0 + lane < rowDw && g + 4 <= totalBytes)
+                    __builtin_amdgcn_global_load_lds(inBytes + g, dst + 4 * k0, 4, 0, 0);
+            }
+            if (lane < TD) {
+                const int o = dl[lane] - dmin + mis;  // element offset in the row
+                offs[cc * TD + lane] = cc * 2 * rowDw + ((o & 1) ? rowDw : 0) + (o >> 1);
+            }
+        }
+    };
+    // copy B: dword m holds elements 2m+1, 2m+2 of the row
+    auto shift_copy = [&](int buf, int nc) {
+        uint32_t* rows = rowsOf(buf);
+        for (int k = tid; k < nc * (rowDw - 1); k += 256) {
+            const int cc = k / (rowDw - 1), m = k % (rowDw - 1);
+            uint32_t* A = rows + (size_t)cc * 2 * rowDw;
+            A[rowDw + m] = __builtin_amdgcn_alignbyte(A[m + 1], A[m], 2);
+        }
+    };
+
+    const int nBlocks = (nChan + CB - 1) / CB;
+    stage(0, 0, min(CB, nChan));
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    shift_copy(0, min(CB, nChan));
+    __syncthreads();
+    for (int blk = 0; blk < nBlocks; ++blk) {
+        const int buf = blk & 1;
+        const int cb0 = blk * CB;
+        const int nc = min(CB, nChan - cb0);
+        const bool more = blk + 1 < nBlocks;
+        if (more) stage(buf ^ 1, cb0 + CB, min(CB, nChan - cb0 - CB));
+        const uint32_t* rows = rowsOf(buf);
+        const int* offs = offsOf(buf) + wave * ND;
+        for (int cc = 0; cc < nc; ++cc) {
+            int o[ND];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) o[j] = offs[cc * TD + j];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                const uint32_t* p = rows + o[j] + lane;
+                uint32_t v[NP];
+#pragma unroll
+                for (int i = 0; i < NP; ++i) v[i] = p[64 * i];
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const _Float16 lo = __builtin_bit_cast(_Float16, (unsigned short)(v[i] & 0xffffu));
+                    const _Float16 hi = __builtin_bit_cast(_Float16, (unsigned short)(v[i] >> 16));
+                    acc[j][2 * i] = __builtin_fmaf((float)lo, one, acc[j][2 * i]);
+                    acc[j][2 * i + 1] = __builtin_fmaf((float)hi, one, acc[j][2 * i + 1]);
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (more) {
+            shift_copy(buf ^ 1, min(CB, nChan - cb0 - CB));
+            __syncthreads();
+        }
+    }
+    const int dmw = dm0 + wave * ND;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        const int dm = dmw + j;
+        if (dm < tdms) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const int t = t0 + 2 * lane + 128 * i;
+                float* o = out + (size_t)dm * nOut + t;
+                if (t + 1 < nOut) {
+                    o[0] = acc[j][2 * i];
+                    o[1] = acc[j][2 * i + 1];
+                } else if (t < nOut) {
+                    o[0] = acc[j][2 * i];
+                }
+            }
+        }
+    }
+}
+
+// u8 input, int32 sums, four samples per dword.  Lane l accumulates samples
+// t0 + 4l + 256i + {0,1,2,3}.  The row is held in LDS in four copies shifted
+// by 0..3 bytes (built with alignbyte), so every (channel, DM) read is one
+// aligned dword.  Bytes 0/2 and 1/3 of the dword are split into the 16-bit
+// halves of two 32-bit accumulators (and / perm) and two channels are added
+// at once (add3); the halves are flushed into int32 sums every 256 channels,
+// before they can overflow.  Integer sums are exact in any order.
+template <int ND, int NQ>
+__global__ __launch_bounds__(256) void dedisperse_u8_kernel(
+    const uint8_t* __restrict__ in, int32_t* __restrict__ out, const int32_t* __restrict__ delayT, int nChan,
+    int nsamp, int tdms, int tdmsPad, int nOut, int rowDw, int CB, int nDMTiles)
+{
+    constexpr int TT = 256 * NQ;
+    constexpr int TD = 4 * ND;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // per buffer: rows[CB][4][rowDw] dwords, then offs[CB][TD]
+    uint32_t* lds32 = reinterpret_cast<uint32_t*>(smem_raw);
+    const int bufDw = CB * (4 * rowDw + TD);
+
+    const int bid = blockIdx.x;
+    const int dmTile = bid % nDMTiles;
+    const int tTile = bid / nDMTiles;
+    const int t0 = tTile * TT;
+    const int dm0 = dmTile * TD;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t totalBytes = (size_t)nChan * nsamp;
+
+    int32_t acc[ND][NQ][4];
+    uint32_t h02[ND][NQ], h13[ND][NQ];
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            h02[j][i] = h13[j][i] = 0u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[j][i][q] = 0;
+        }
+    auto flush = [&]() {
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                acc[j][i][0] += (int32_t)(h02[j][i] & 0xffffu);
+                acc[j][i][2] += (int32_t)(h02[j][i] >> 16);
+                acc[j][i][1] += (int32_t)(h13[j][i] & 0xffffu);
+                acc[j][i][3] += (int32_t)(h13[j][i] >> 16);
+                h02[j][i] = h13[j][i] = 0u;
+            }
+    };
+
+    auto rowsOf = [&](int buf) { return lds32 + (size_t)buf * bufDw; };
+    auto offsOf = [&](int buf) { return reinterpret_cast<int*>(lds32 + (size_t)buf * bufDw + CB * 4 * rowDw); };
+    auto stage = [&](int buf, int cb0, int nc) {
+        uint32_t* rows = rowsOf(buf);
+        int* offs = offsOf(buf);
+        for (int cc = wave; cc < nc; cc += 4) {
+            const int c = cb0 + cc;
+            const int32_t* dl = delayT + (size_t)c * tdmsPad + dm0;
+            const int dmin = dl[0];
+            const size_t byteStart = (size_t)c * nsamp + (size_t)t0 + (size_t)dmin;
+            const size_t byte0 = byteStart & ~(size_t)3;
+            const int mis = (int)(byteStart - byte0);
+            unsigned char* dst = reinterpret_cast<unsigned char*>(rows + (size_t)cc * 4 * rowDw);
+            for (int k0 = 0; k0 < rowDw; k0 += 64) {
+                const size_t g = byte0 + 4 * (size_t)(k0 + lane);
+                if (k0 + lane < rowDw && g + 4 <= totalBytes)
+                    __builtin_amdgcn_global_load_lds(in + g, dst + 4 * k0, 4, 0, 0);
+            }
+            if (lane < TD) {
+                const int o = dl[lane] - dmin + mis;  // byte offset in the row
+                offs[cc * TD + lane] = cc * 4 * rowDw + (o & 3) * rowDw + (o >> 2);
+            }
+        }
+    };
+    auto shift_copies = [&](int buf, int nc) {
+        uint32_t* rows = rowsOf(buf);
+        for (int k = tid; k < nc * (rowDw - 1); k += 256) {
+            const int cc = k / (rowDw - 1), m = k % (rowDw - 1);
+            uint32_t* A = rows + (size_t)cc * 4 * rowDw;
+            const uint32_t a0 = A[m], a1 = A[m + 1];
+            A[rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 1);
+            A[2 * rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 2);
+            A[3 * rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 3);
+        }
+    };
+
+    const int nBlocks = (nChan + CB - 1) / CB;
+    stage(0, 0, min(CB, nChan));
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    shift_copies(0, min(CB, nChan));
+    __syncthreads();
+    int sinceFlush = 0;
+    for (int blk = 0; blk < nBlocks; ++blk) {
+        const int buf = blk & 1;
+        const int cb0 = blk * CB;
+        const int nc = min(CB, nChan - cb0);
+        const bool more = blk + 1 < nBlocks;
+        if (more) stage(buf ^ 1, cb0 + CB, min(CB, nChan - cb0 - CB));
+        const uint32_t* rows = rowsOf(buf);
+        const int* offs = offsOf(buf) + wave * ND;
+        int cc = 0;
+        for (; cc + 1 < nc; cc += 2) {
+            if (sinceFlush >= 254) {
+                flush();
+                sinceFlush = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                const uint32_t* p0 = rows + offs[cc * TD + j] + lane;
+                const uint32_t* p1 = rows + offs[(cc + 1) * TD + j] + lane;
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    const uint32_t v0 = p0[64 * i], v1 = p1[64 * i];
+                    h02[j][i] = h02[j][i] + (v0 & 0x00ff00ffu) + (v1 & 0x00ff00ffu);
+                    h13[j][i] = h13[j][i] + __builtin_amdgcn_perm(0u, v0, 0x0c030c01u) +
+                                __builtin_amdgcn_perm(0u, v1, 0x0c030c01u);
+                }
+            }
+            sinceFlush += 2;
+        }
+        if (cc < nc) {
+            if (sinceFlush >= 255) {
+                flush();
+                sinceFlush = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                const uint32_t* p0 = rows + offs[cc * TD + j] + lane;
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) {
+                    const uint32_t v0 = p0[64 * i];
+                    h02[j][i] += (v0 & 0x00ff00ffu);
+                    h13[j][i] += __builtin_amdgcn_perm(0u, v0, 0x0c030c01u);
+                }
+            }
+            sinceFlush += 1;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (more) {
+            shift_copies(buf ^ 1, min(CB, nChan - cb0 - CB));
+            __syncthreads();
+        }
+    }
+    flush();
+    const int dmw = dm0 + wave * ND;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        const int dm = dmw + j;
+        if (dm < tdms) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const int t = t0 + 4 * lane + 256 * i;
+                int32_t* o = out + (size_t)dm * nOut + t;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (t + q < nOut) o[q] = acc[j][i][q];
             }
         }
     }
@@ -126,10 +460,34 @@ __global__ void build_delays_kernel(const float* __restrict__ shift, int nChan, 
 }
 
 struct Shape {
-    int nd, nt;
+    int nd, nt;  // DM trials per wave, 64-sample groups per lane
+    int cb;      // channels per staged block (0: as many as fit)
 };
 
 constexpr int kMaxLDS = 64 * 1024;
+
+// LDS bytes per staged channel (both buffers) and the row length for a shape.
+template <typename Tin>
+void lds_geometry(const Shape& s, int span, int& rowUnits, int& perChannel)
+{
+    const int TT = 64 * s.nt, TD = 4 * s.nd;
+    if constexpr (std::is_same<Tin, _Float16>::value) {
+        // rowUnits = dwords per copy: (TT + span + misalignment) halves plus
+        // the dword the shifted copy reads past the end
+        rowUnits = (TT + span + 1 + 1) / 2 + 1;
+        perChannel = 2 * 4 * (2 * rowUnits + TD);
+    } else if constexpr (std::is_same<Tin, uint8_t>::value) {
+        // four shifted copies of (TT + span + misalignment) bytes, in dwords
+        rowUnits = (TT + span + 3 + 3) / 4 + 1;
+        perChannel = 2 * 4 * (4 * rowUnits + TD);
+    } else {
+        // rowUnits = elements, a whole number of dwords
+        int W = TT + span + 4;
+        W = (W + 15) & ~15;
+        rowUnits = W;
+        perChannel = 2 * (W * (int)sizeof(Tin) + TD * 4);
+    }
+}
 
 template <typename Tin>
 int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT, int nChan,
@@ -137,40 +495,73 @@ int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT
 {
     using TA = typename Acc<Tin>::type;
     const int TT = 64 * s.nt, TD = 4 * s.nd;
-    int W = TT + span;
-    W = (W + 15) & ~15;
-    const int rowBytes = W * (int)sizeof(Tin);
-    if (rowBytes > kMaxLDS) {
+    int rowUnits = 0, perChannel = 0;
+    lds_geometry<Tin>(s, span, rowUnits, perChannel);
+    if (perChannel > kMaxLDS) {
         set_error("dedisperse: delay spread %d too large for tile %dx%d", span, TD, TT);
         return PPL_E_CONFIG;
     }
-    int CB = std::min(16, kMaxLDS / rowBytes);
+    int CB = std::min(16, kMaxLDS / perChannel);
+    if (s.cb > 0) CB = std::min(CB, s.cb);
     CB = std::max(CB, 1);
     const int nDMTiles = (tdms + TD - 1) / TD;
     const int nTTiles = (nOut + TT - 1) / TT;
     dim3 grid((unsigned)nDMTiles * nTTiles), block(256);
-    const size_t lds = (size_t)CB * rowBytes;
+    const size_t lds = (size_t)CB * perChannel;
     TA* o = static_cast<TA*>(out);
+    if constexpr (std::is_same<Tin, _Float16>::value) {
+#define PPL_DD16_CASE(ND_, NT_)                                                                         \
+    if (s.nd == ND_ && s.nt == NT_) {                                                                   \
+        hipLaunchKernelGGL((dedisperse_f16_kernel<ND_, NT_ / 2>), grid, block, lds, stream, in, o, delayT, \
+                           nChan, nsamp, tdms, tdmsPad, nOut, rowUnits, CB, nDMTiles, 1.0f);          \
+        PPL_LAUNCHED();                                                                                 \
+        return PPL_OK;                                                                                  \
+    }
+        PPL_DD16_CASE(8, 8)
+        PPL_DD16_CASE(8, 4)
+        PPL_DD16_CASE(4, 8)
+        PPL_DD16_CASE(4, 4)
+        PPL_DD16_CASE(2, 8)
+        PPL_DD16_CASE(16, 4)
+        PPL_DD16_CASE(16, 2)
+#undef PPL_DD16_CASE
+    } else if constexpr (std::is_same<Tin, uint8_t>::value) {
+#define PPL_DD8_CASE(ND_, NT_)                                                                          \
+    if (s.nd == ND_ && s.nt == NT_) {                                                                   \
+        hipLaunchKernelGGL((dedisperse_u8_kernel<ND_, NT_ / 4>), grid, block, lds, stream, in, o, delayT,  \
+                           nChan, nsamp, tdms, tdmsPad, nOut, rowUnits, CB, nDMTiles);                \
+        PPL_LAUNCHED();                                                                                 \
+        return PPL_OK;                                                                                  \
+    }
+        PPL_DD8_CASE(8, 8)
+        PPL_DD8_CASE(8, 4)
+        PPL_DD8_CASE(4, 8)
+        PPL_DD8_CASE(4, 4)
+        PPL_DD8_CASE(2, 8)
+        PPL_DD8_CASE(16, 4)
+#undef PPL_DD8_CASE
+    } else {
 #define PPL_DD_CASE(ND_, NT_)                                                                   \
     if (s.nd == ND_ && s.nt == NT_) {                                                           \
         hipLaunchKernelGGL((dedisperse_kernel<Tin, ND_, NT_>), grid, block, lds, stream, in, o, \
-                           delayT, nChan, nsamp, tdms, tdmsPad, nOut, W, CB, nDMTiles);         \
+                           delayT, nChan, nsamp, tdms, tdmsPad, nOut, rowUnits, CB, nDMTiles);  \
         PPL_LAUNCHED();                                                                         \
         return PPL_OK;                                                                          \
     }
-    PPL_DD_CASE(8, 8)
-    PPL_DD_CASE(8, 4)
-    PPL_DD_CASE(4, 8)
-    PPL_DD_CASE(4, 4)
-    PPL_DD_CASE(2, 8)
-    PPL_DD_CASE(16, 4)
-    PPL_DD_CASE(16, 2)
+        PPL_DD_CASE(8, 8)
+        PPL_DD_CASE(8, 4)
+        PPL_DD_CASE(4, 8)
+        PPL_DD_CASE(4, 4)
+        PPL_DD_CASE(2, 8)
+        PPL_DD_CASE(16, 4)
+        PPL_DD_CASE(16, 2)
 #undef PPL_DD_CASE
+    }
     set_error("dedisperse: unsupported tile %dx%d", s.nd, s.nt);
     return PPL_E_ARG;
 }
 
-const Shape kShapes[] = {{8, 8}, {8, 4}, {4, 8}, {4, 4}, {2, 8}, {16, 4}, {16, 2}};
+const Shape kShapes[] = {{8, 8, 0}, {8, 4, 0}, {4, 8, 0}, {4, 4, 0}, {2, 8, 0}, {16, 4, 0}, {16, 2, 0}};
 
 bool shape_known(int nd, int nt)
 {
@@ -274,7 +665,7 @@ extern "C" int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_c
         p->nt = s.nt;
         const int TD = 4 * s.nd;
         const int span = tile_span(p->delays, (int)cfg->tdms, (int)N, TD);
-        if ((64 * s.nt + span + 16) * elem * 4 <= kMaxLDS) break;
+        if (4 * 2 * ((64 * s.nt + span + 20) * elem + 4 * TD) <= kMaxLDS) break;
     }
     int rc = upload_delays(p);
     if (rc != PPL_OK) {
@@ -303,20 +694,74 @@ extern "C" int ppl_dedisp_plan_info(const ppl_dedisp_plan* p, ppl_dedisp_info* i
     info->mdmstep = p->mdmstep;
     info->tileDM = 4 * p->nd;
     info->tileT = 64 * p->nt;
+    info->chanBlock = p->cb;
     return PPL_OK;
 }
 
 extern "C" int ppl_dedisp_set_tile(ppl_dedisp_plan* p, int tileDM, int tileT)
 {
+    return ppl_dedisp_set_shape(p, tileDM, tileT, 0);
+}
+
+extern "C" int ppl_dedisp_set_shape(ppl_dedisp_plan* p, int tileDM, int tileT, int chanBlock)
+{
     PPL_CHECK_ARG(p, "null plan");
     if (tileDM == 0 && tileT == 0) return PPL_OK;
     PPL_CHECK_ARG(tileDM % 4 == 0 && tileT % 64 == 0, "tile %dx%d not a multiple of 4x64", tileDM,
                   tileT);
+    PPL_CHECK_ARG(chanBlock >= 0 && chanBlock <= 16, "channel block %d outside [0, 16]", chanBlock);
     const int nd = tileDM / 4, nt = tileT / 64;
     PPL_CHECK_ARG(shape_known(nd, nt), "tile %dx%d not instantiated", tileDM, tileT);
     p->nd = nd;
     p->nt = nt;
+    p->cb = chanBlock;
     return upload_delays(p);
+}
+
+extern "C" int ppl_dedisp_autotune(ppl_dedisp_plan* p, const void* d_in, void* d_out, int reps, void* stream,
+                                   float* best_ms)
+{
+    PPL_CHECK_ARG(p && d_in && d_out, "null argument");
+    if (reps < 1) reps = 1;
+    hipStream_t st = stream_of(stream);
+    hipEvent_t e0, e1;
+    PPL_HIP(hipEventCreate(&e0));
+    PPL_HIP(hipEventCreate(&e1));
+    const int cbs[] = {2, 4, 8, 16};
+    float best = 1e30f;
+    Shape bestShape{p->nd, p->nt, p->cb};
+    for (const Shape& base : kShapes) {
+        for (int cb : cbs) {
+            if (ppl_dedisp_set_shape(p, 4 * base.nd, 64 * base.nt, cb) != PPL_OK) continue;
+            int rowUnits, perChannel;
+            const int elem = p->cfg.dtype == PPL_F32 ? 4 : p->cfg.dtype == PPL_F16 ? 2 : 1;
+            (void)elem;
+            if (p->cfg.dtype == PPL_F16)
+                lds_geometry<_Float16>(Shape{p->nd, p->nt, cb}, p->span, rowUnits, perChannel);
+            else if (p->cfg.dtype == PPL_F32)
+                lds_geometry<float>(Shape{p->nd, p->nt, cb}, p->span, rowUnits, perChannel);
+            else
+                lds_geometry<uint8_t>(Shape{p->nd, p->nt, cb}, p->span, rowUnits, perChannel);
+            if (perChannel > kMaxLDS || cb > kMaxLDS / perChannel) continue;
+            if (ppl_dedisperse(p, d_in, d_out, stream) != PPL_OK) continue;  // warm
+            PPL_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; ++r)
+                if (ppl_dedisperse(p, d_in, d_out, stream) != PPL_OK) break;
+            PPL_HIP(hipEventRecord(e1, st));
+            PPL_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PPL_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (ms < best) {
+                best = ms;
+                bestShape = Shape{p->nd, p->nt, cb};
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best_ms) *best_ms = best;
+    return ppl_dedisp_set_shape(p, 4 * bestShape.nd, 64 * bestShape.nt, bestShape.cb);
 }
 
 extern "C" int ppl_dedisp_delays(const ppl_dedisp_plan* p, int32_t* host_out)
@@ -329,7 +774,7 @@ extern "C" int ppl_dedisp_delays(const ppl_dedisp_plan* p, int32_t* host_out)
 extern "C" int ppl_dedisperse(const ppl_dedisp_plan* p, const void* d_in, void* d_out, void* stream)
 {
     PPL_CHECK_ARG(p && d_in && d_out, "null argument");
-    const Shape s{p->nd, p->nt};
+    const Shape s{p->nd, p->nt, p->cb};
     const int nChan = (int)p->cfg.nChan, nsamp = (int)p->cfg.nsampBuf, tdms = (int)p->cfg.tdms;
     hipStream_t st = stream_of(stream);
     switch (p->cfg.dtype) {
@@ -360,7 +805,7 @@ extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float*
     PPL_CHECK_ARG((long)tdms * nOut * (long)sizeof(float) <= outbufSize,
                   "output buffer (%ld bytes) smaller than tdms*nOut floats", outbufSize);
     hipStream_t st = stream_of(stream);
-    const Shape s{8, 8};
+    const Shape s{8, 8, 0};
     const int TD = 4 * s.nd;
     const int pad = ((tdms + TD - 1) / TD) * TD;
     // delay table on the device, then the spread of the first/last trial

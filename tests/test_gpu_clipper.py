@@ -17,9 +17,10 @@ def _stream(n_t, n_sub, n_ch, seed, n_pol=1):
     level = 1000.0 + 6.0 * (v - 145.0)
     x = stokes_noise(n_t, n_sub, n_ch, level, seed, n_pol=n_pol)
     rng = np.random.default_rng(seed + 1)
-    # narrowband RFI: 3 channels, bright in most spectra
-    for b in rng.choice(n_bins, 3, replace=False):
-        on = rng.random(n_t) < 0.7
+    # narrowband RFI: 3 channels bright in most spectra, 2 in every spectrum
+    # (their last-good slots never fill)
+    for k, b in enumerate(rng.choice(n_bins, 5, replace=False)):
+        on = rng.random(n_t) < (0.7 if k < 3 else 1.0)
         x[on, b // n_ch, :, b % n_ch] *= 30.0
     # broadband impulses in 1% of spectra
     imp = rng.random(n_t) < 0.01
@@ -78,3 +79,27 @@ def test_clipper_rejects_bad_shapes(ppl):
     gc = ppl.Clipper(_bp(4, 16))
     with pytest.raises(ppl.PPLError):
         gc.run(torch.zeros((10, 3, 1, 5), device="cuda"))  # 15 bins: not a multiple of 8
+
+
+def test_clipper_long_steady_state_3968_bins(orc, ppl):
+    """The LOFAR-beam width (62 x 64 bins) through learning into a long steady
+    state, where most spectra take the precomputed-mask path."""
+    n_sub, n_ch = 62, 64
+    x = _stream(4200, n_sub, n_ch, seed=33)
+    bp = _bp(n_sub, n_ch)
+    oc = orc.Clipper(bp, 10.0, 4.0, 1000, True)
+    gc = ppl.Clipper(bp, 10.0, 4.0, 1000, True)
+    for a in range(0, x.shape[0], 700):
+        chunk = np.ascontiguousarray(x[a:a + 700])
+        ref = chunk.copy()
+        oc.run(ref)
+        d = torch.from_numpy(chunk).cuda()
+        gc.run(d)
+        assert np.array_equal(gc.flags(chunk.shape[0]), oc.last_flags(chunk.shape[0])[0]), a
+        out = d.cpu().numpy()
+        assert np.array_equal(out, ref), (a, np.argwhere(out != ref)[:5])
+    st_o, sizes = oc.state()
+    assert sizes[0] == 1000 and sizes[1] == 1000
+    fast, slow = gc.counters()
+    assert fast + slow == x.shape[0] and fast > 1000  # the steady state mostly takes the fast path
+    assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])

@@ -138,3 +138,45 @@ def test_linearity_full_size(ppl):
     hit = out.nonzero()
     assert hit.shape[0] == 2048
     assert torch.equal(hit[:, 1], s - d)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16", "u8"])
+@pytest.mark.parametrize("tile", [(32, 512, 2), (64, 256, 4), (16, 512, 8), (32, 256, 16), (8, 512, 0),
+                                  (64, 128, 3)])
+def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
+    """Every instantiated tile and channel block, odd channel counts and odd
+    delay parities (the FP16 form reads pairs from a shifted copy)."""
+    cfg = dict(n_chan=53, nsamp_buf=3000, tdms=77, dm_low=0.0, dm_step=0.9, tsamp=0.00032768,
+               fch1=150.0, foff=-0.05)
+    rng = np.random.default_rng(hash(tile) % 1000)
+    x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
+    p, d = _oracle(orc, x, cfg)
+    if dtype == "f32":
+        dev, host = torch.from_numpy(x).cuda(), x
+    elif dtype == "f16":
+        host = orc.f32_to_f16_bits(x)
+        dev = torch.from_numpy(host.view(np.float16)).cuda()
+    else:
+        host = orc.quantise_u8(x, np.float32(12.7))
+        dev = torch.from_numpy(host).cuda()
+    plan = ppl.DedispPlan(**cfg, dtype=dtype)
+    plan.set_tile(*tile)
+    if dtype == "u8" and tile[1] % 256:
+        with pytest.raises(ppl.PPLError):  # the u8 form needs 4 samples per lane
+            plan.run(dev)
+        return
+    out = plan.run(dev).cpu().numpy()
+    assert np.array_equal(out, orc.dedisperse(host, d, p["nout"]))
+
+
+def test_autotuner_keeps_results_exact(orc, ppl):
+    cfg = dict(SMALL)
+    rng = np.random.default_rng(17)
+    x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
+    plan = ppl.DedispPlan(**cfg)
+    xd = torch.from_numpy(x).cuda()
+    out = torch.empty((cfg["tdms"], plan.nout), device="cuda")
+    ms = plan.autotune(xd, out, reps=1)
+    assert ms > 0 and plan.tile[0] % 4 == 0
+    p, d = _oracle(orc, x, cfg)
+    assert np.array_equal(plan.run(xd).cpu().numpy(), orc.dedisperse(x, d, p["nout"]))
