@@ -255,7 +255,7 @@ def main():
         "real_time_factor": rtf,
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "events_per_step": events,
-        "clipper_fast_slow_spectra": list(clip.counters()),
+        "clipper_walk": clip.counters(),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gen_seconds": round(t_gen, 1),

@@ -249,9 +249,17 @@ int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nS
 int ppl_clipper_stats(ppl_clipper* clip, float* mean, float* rms);
 /* Per-spectrum flags of the last run (1 = spectrum kept), host buffer. */
 int ppl_clipper_flags(ppl_clipper* clip, uint8_t* h_good, unsigned nT);
-/* Spectra so far decided from the precomputed channel mask (fast) and by the
- * full-workgroup path (slow); synchronises. */
-int ppl_clipper_counters(ppl_clipper* clip, long long* fast, long long* slow);
+/* Walk counters since creation (indices below); writes min(n, PPL_CLIP_NCOUNTERS)
+ * values and synchronises.  FAST: spectra decided from the precomputed
+ * channel mask; SLOW: spectra walked by the full workgroup, split by reason:
+ * LEARN (history not yet full), BOX (state outside the speculation box, or
+ * beyond the precomputed range), UNDECIDED (a channel the box does not
+ * decide), FILL (last-good slots to fill), VER (last-good versions used up);
+ * STOP: sub-chunks ended early to re-centre the box. */
+enum { PPL_CLIP_FAST = 0, PPL_CLIP_SLOW = 1, PPL_CLIP_LEARN = 2, PPL_CLIP_BOX = 3,
+       PPL_CLIP_UNDECIDED = 4, PPL_CLIP_FILL = 5, PPL_CLIP_VER = 6, PPL_CLIP_STOP = 7,
+       PPL_CLIP_NCOUNTERS = 8 };
+int ppl_clipper_counters(ppl_clipper* clip, long long* out, int n);
 
 #ifdef __cplusplus
 }

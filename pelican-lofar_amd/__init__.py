@@ -113,7 +113,7 @@ SIGNATURES = {
     "ppl_clipper_run_into": (i, [vp, vp, u, u, u, vp, u, vp]),
     "ppl_clipper_stats": (i, [vp, vp, vp]),
     "ppl_clipper_flags": (i, [vp, vp, u]),
-    "ppl_clipper_counters": (i, [vp, vp, vp]),
+    "ppl_clipper_counters": (i, [vp, vp, i]),
 }
 
 
@@ -424,11 +424,15 @@ class Clipper:
         check(lib().ppl_clipper_stats(self.h, C.byref(m), C.byref(r)))
         return m.value, r.value
 
+    COUNTER_NAMES = ("fast", "slow", "learn", "box", "undecided", "fill", "ver", "stop")
+
     def counters(self):
-        """(spectra decided from the precomputed mask, spectra on the full path)."""
-        a, b = C.c_longlong(0), C.c_longlong(0)
-        check(lib().ppl_clipper_counters(self.h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """Walk counters since creation: spectra decided from the precomputed
+        mask ("fast"), spectra on the full path ("slow") and the reason they
+        took it, and sub-chunks ended early ("stop")."""
+        out = (C.c_longlong * len(self.COUNTER_NAMES))()
+        check(lib().ppl_clipper_counters(self.h, out, len(self.COUNTER_NAMES)))
+        return dict(zip(self.COUNTER_NAMES, (int(v) for v in out)))
 
     def flags(self, n_t):
         out = np.zeros(n_t, np.uint8)

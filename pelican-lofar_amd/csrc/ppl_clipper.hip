@@ -35,6 +35,14 @@ namespace {
 using namespace ppl;
 
 constexpr int kMaxParams = 8;
+enum {
+    CNT_FAST = PPL_CLIP_FAST, CNT_SLOW = PPL_CLIP_SLOW, CNT_LEARN = PPL_CLIP_LEARN, CNT_BOX = PPL_CLIP_BOX,
+    CNT_UNDECIDED = PPL_CLIP_UNDECIDED, CNT_FILL = PPL_CLIP_FILL, CNT_VER = PPL_CLIP_VER, CNT_STOP = PPL_CLIP_STOP
+};
+// Spectra per sub-chunk of the walk, and sub-chunk launches beyond
+// nT / kSubLen that a run makes room for (each early end costs one).
+constexpr int kSubLen = 4096;
+constexpr int kExtraIter = 8;
 
 // ---------------------------------------------------------------- host side:
 // BandPass / BinMap restatement used to set the model up
@@ -132,8 +140,13 @@ struct ClipScalars {
     // newest and second-newest ring entries, kept beside the rings so the
     // walk never reads them back from memory
     float meanBack, meanBack2, rmsBack, rmsBack2;
-    // spectra decided from the precomputed mask / by the full workgroup
-    long long nFast, nSlow;
+    // run control: next spectrum to walk, last-good version in force,
+    // growth factor of the speculation box
+    int tNext, ver;
+    float boxGrow;
+    // spectra decided from the precomputed mask / by the full workgroup, the
+    // latter by reason, and sub-chunks ended early (ppl_clipper_counters)
+    long long count[PPL_CLIP_NCOUNTERS];
 };
 
 struct ClipConst {
@@ -201,27 +214,36 @@ constexpr int kWalkThreads = 512;
 constexpr int kStage = 512;      // spectra whose precomputed records are staged in LDS at a time
 constexpr int kRowChunk = 4096;  // bins of one spectrum staged in LDS for the ordered sums
 
-// Per-chunk speculation box: every spectrum of the chunk whose bandpass
-// offset lies in [p0lo, p0hi] and whose margin lies in [mlo, mhi] is judged
-// by the channel mask the precompute pass fixed for that spectrum, provided
-// no channel of it was undecided over the whole box.
+// Per-sub-chunk speculation box: every spectrum of the sub-chunk whose
+// bandpass offset lies in [p0lo, p0hi] and whose margin lies in [mlo, mhi]
+// is judged by the channel mask the precompute pass fixed for that
+// spectrum, provided no channel of it was undecided over the whole box.
 struct Box {
     float p0lo, p0hi, mlo, mhi;
     int valid;
 };
+// Box half-widths around the sub-chunk-start state, before the growth
+// factor: the bandpass offset moves the bandpass by +-kBoxP channel rms,
+// the margin by a relative +-kBoxM.  In steady state both follow running
+// averages over the history (10000 spectra by default), which move far less
+// than this over a sub-chunk; a state that leaves the box ends the
+// sub-chunk, and the next one starts with a box four times wider.
+constexpr float kBoxP = 0.02f;
+constexpr float kBoxM = 0.005f;
+constexpr float kMaxBoxGrow = 64.0f;
 
-// Precomputed per spectrum (steady state, dataModel = 0): ordered sum of the
-// channels kept for every state in the box, their count per band, and the
-// number of channels whose fate the box does not decide.
+// Precomputed per spectrum (steady state, dataModel = 0), for the mask that
+// holds over the whole box: the spectrum mean of the kept channels (their
+// ordered sum over their count), the increment of the bad-channel fraction
+// and the bad-band count.
 struct PreRec {
-    float sum;
-    int undecided;
-    int fillCand;  // kept channels with a non-zero value whose last-good slot was empty at chunk start
-    int pad;
-    uint16_t cnt[8];
+    float mean;
+    float fracInc;
+    int badBands;
+    int flags;  // bit0: some channel undecided; bit1: a kept non-zero channel whose last-good slot was empty
 };
 
-constexpr int kMaxVer = 4;  // last-good versions kept per chunk for the apply pass
+constexpr int kMaxVer = 64;  // last-good versions kept per run for the apply pass
 
 __device__ __forceinline__ float block_min(float v, float* red)
 {
@@ -244,28 +266,61 @@ __device__ __forceinline__ float block_min(float v, float* red)
     return r;
 }
 
-// The box around the chunk-start state and the bandpass at its two offsets.
+// Band bookkeeping of RFI_Clipper.cpp:345-372 from the kept-channel counts.
+__device__ __forceinline__ void band_stats(const int* cnt, int nBins, int cpb, int& total, int& badBands,
+                                           float& fracInc)
+{
+    total = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) total += cnt[b];
+    unsigned bad = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if ((double)(float)cnt[b] < 0.8 * (unsigned)cpb) ++bad;
+        if (cnt[b] == 0) bad += 4;
+    }
+    if ((double)(unsigned)total < 0.8 * (unsigned)nBins) bad += 4;
+    badBands = (int)bad;
+    fracInc = (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
+}
+
+// Start of a run: version 0 of the last-good spectrum is its current state.
+__global__ void clip_begin_kernel(ClipScalars* __restrict__ S, const float* __restrict__ lastGood,
+                                  float* __restrict__ lastGoodVer, int nBins)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        S->tNext = 0;
+        S->ver = 0;
+    }
+    if (i < nBins) lastGoodVer[i] = lastGood[i];
+}
+
+// The box around the sub-chunk-start state and the bandpass at its two offsets.
 __global__ void clip_bounds_kernel(const ClipScalars* __restrict__ S, ClipConst K, const double* __restrict__ terms,
-                                   Box* __restrict__ box, float* __restrict__ bpLo, float* __restrict__ bpHi)
+                                   Box* __restrict__ box, float* __restrict__ bpLo, float* __restrict__ bpHi, int nT)
 {
     const ClipScalars s = *S;
+    if (s.tNext >= nT) return;
     const bool steady = s.rmsSize == K.cap && s.meanSize == K.cap;
     const float rms = s.rmsRunAve;
-    const bool valid = steady && isfinite(rms) && rms > 0.0f && isfinite(s.params0);
-    const float dp = 1.0f * rms;
+    const float g = s.boxGrow;
+    const float dp = kBoxP * g * rms / fabsf(K.scale);
     const float p0lo = s.params0 - dp, p0hi = s.params0 + dp;
     const float margin = K.crFactor * rms;
+    const bool valid = steady && isfinite(rms) && rms > 0.0f && isfinite(s.params0) && isfinite(dp) &&
+                       p0lo < p0hi && margin > 0.0f && isfinite(margin);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         Box b;
         b.p0lo = p0lo;
         b.p0hi = p0hi;
-        b.mlo = margin * 0.9f;
-        b.mhi = margin * 1.1f;
+        b.mlo = margin * (1.0f - kBoxM * g);
+        b.mhi = margin * (1.0f + kBoxM * g);
         b.valid = valid ? 1 : 0;
         *box = b;
     }
-    if (i < K.nBins) {
+    if (i < K.nBins && valid) {
         const float a = bp_value(terms, K.nTerms, K.nBins, i, p0lo, K.scale);
         const float b = bp_value(terms, K.nTerms, K.nBins, i, p0hi, K.scale);
         bpLo[i] = fminf(a, b);
@@ -273,24 +328,30 @@ __global__ void clip_bounds_kernel(const ClipScalars* __restrict__ S, ClipConst 
     }
 }
 
-// One lane per spectrum, bins staged through 64x64 LDS tiles: the kept-
-// channel sum in bin order (the reference's order) for the mask that holds
-// over the whole box.  A channel is kept for every state in the box when
-// fl(I - bpLo) <= mlo, clipped for every state when fl(I - bpHi) > mhi
-// (fl(I - bp) falls as bp rises, and bp rises with the offset).
+// One lane per spectrum of the sub-chunk, bins staged through 64x64 LDS
+// tiles: the kept-channel sum in bin order (the reference's order) for the
+// mask that holds over the whole box.  A channel is kept for every state in
+// the box when fl(I - bpLo) <= mlo, clipped for every state when
+// fl(I - bpHi) > mhi (fl(I - bp) falls as bp rises; bp is monotone in the
+// offset).
 __global__ __launch_bounds__(64) void clip_pre_kernel(const float* __restrict__ I, ClipConst K,
+                                                      const ClipScalars* __restrict__ S,
                                                       const Box* __restrict__ box, const float* __restrict__ bpLo,
                                                       const float* __restrict__ bpHi,
                                                       const float* __restrict__ lastGood, PreRec* __restrict__ pre,
-                                                      int nT)
+                                                      int nT, int subLen)
 {
     __shared__ float tile[64][65];
     __shared__ float lo[64], hi[64], lg[64];
+    const int tStart = S->tNext;
+    if (tStart >= nT) return;
     const Box bx = *box;
     if (!bx.valid) return;
     const int lane = threadIdx.x;
-    const int t0 = blockIdx.x * 64;
-    const int nRows = min(64, nT - t0);
+    const int k0 = blockIdx.x * 64;
+    const int t0 = tStart + k0;
+    const int nRows = min(min(64, nT - t0), subLen - k0);
+    if (nRows <= 0) return;
     const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
     const int cpb = nBins / 8;
     float sum = 0.0f;
@@ -341,34 +402,38 @@ __global__ __launch_bounds__(64) void clip_pre_kernel(const float* __restrict__ 
 #pragma unroll
         for (int q = 0; q < 8; ++q)
             if (q == band) cnt[q] = curCnt;
+        int total, badBands;
+        float fracInc;
+        band_stats(cnt, nBins, cpb, total, badBands, fracInc);
         PreRec r;
-        r.sum = sum;
-        r.undecided = undecided;
-        r.fillCand = fillCand;
-        r.pad = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) r.cnt[q] = (uint16_t)cnt[q];
-        pre[t0 + lane] = r;
+        r.mean = sum / (float)(unsigned)total;
+        r.fracInc = fracInc;
+        r.badBands = badBands;
+        r.flags = (undecided ? 1 : 0) | (fillCand ? 2 : 0);
+        pre[k0 + lane] = r;
     }
 }
 
-// The sequential walk over nT spectra of one chunk.  Thread 0 runs ahead
+// The sequential walk over one sub-chunk of spectra.  Thread 0 runs ahead
 // through spectra the precompute decided; the workgroup joins in for the
 // others (the learning phase, undecided channels, a state outside the box,
-// an incomplete last-good spectrum).
+// a last-good spectrum still to fill).  The walk starts at S->tNext and
+// covers subLen spectra (all that remain when lastIter is set); a state
+// that leaves the box ends it after that spectrum, so that the next
+// sub-chunk speculates around the new state.
 __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     float* __restrict__ I, ClipConst K, ClipScalars* __restrict__ S, float* __restrict__ meanBuf,
     double* __restrict__ meanPrefix, float* __restrict__ rmsBuf, double* __restrict__ rmsPrefix,
     float* __restrict__ lastGood, float* __restrict__ lastGoodVer, const double* __restrict__ terms,
     uint8_t* __restrict__ mask, const Box* __restrict__ box, const PreRec* __restrict__ pre,
-    SpecRec* __restrict__ rec, int nT)
+    SpecRec* __restrict__ rec, int nT, int subLen, int lastIter)
 {
     __shared__ ClipScalars s;
     __shared__ float red[kWalkThreads / 64 + 1];
     __shared__ int goodCnt[8];
     __shared__ int fillCnt;
     __shared__ float sh_dataModel, sh_margin;
-    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_ver;
+    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_stop;
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
     __shared__ PreRec recs[kStage];
@@ -379,13 +444,21 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
     const int cpb = nBins / 8;
     RingView meanR{meanBuf, meanPrefix, K.cap}, rmsR{rmsBuf, rmsPrefix, K.cap};
-    const Box bx = *box;
-    if (tid == 0) s = *S;
+    if (tid == 0) {
+        s = *S;
+        sh_frontsValid = 0;
+        sh_stop = 0;
+    }
     __syncthreads();
+    const int tStart = s.tNext;
+    if (tStart >= nT) return;
+    const int tEnd = lastIter ? nT : min(nT, tStart + subLen);
+    const int preN = min(subLen, nT - tStart);  // spectra the precompute covered
+    const Box bx = *box;
 
     auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb)) + (long)nCh * pol; };
 
-    int stageBase = 0, stageN = 0;
+    int stageBase = tStart, stageN = 0;
     // Ring fronts of the staged spectra: with both rings full, the front of
     // spectrum stageBase+k is entry start+k (each spectrum evicts one front;
     // a rejection rewrites only the newest entry).
@@ -405,20 +478,12 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         margin = K.crFactor * s.rmsRunAve;
     };
 
-    // Decision and bookkeeping (:345-565) for spectrum t, thread 0.  Returns
-    // whether the last-good spectrum must be filled from this spectrum.
-    auto phaseC = [&](int t, float spectrumSum, const int* cnt, float dataModel, float margin,
+    // Decision and bookkeeping (:345-565) for spectrum t, thread 0, from the
+    // kept-channel mean and band statistics.  Returns whether the last-good
+    // spectrum must be filled from this spectrum.
+    auto phaseC = [&](int t, float spectrumSum, float fracInc, int badBands, float dataModel, float margin,
                       int replaced) -> int {
-        int total = 0;
-        for (int b = 0; b < 8; ++b) total += cnt[b];
-        s.fractionBadChannels += (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
-        spectrumSum /= (float)(unsigned)total;
-        unsigned badBands = 0;
-        for (int b = 0; b < 8; ++b) {
-            if ((double)(float)cnt[b] < 0.8 * (unsigned)cpb) ++badBands;
-            if (cnt[b] == 0) badBands += 4;
-        }
-        if ((double)(unsigned)total < 0.8 * (unsigned)nBins) badBands += 4;
+        s.fractionBadChannels += fracInc;
         if (s.meanSize < K.cap) {
             ring_push(meanR, s.meanStart, s.meanSize, s.meanBack, s.meanBack2, spectrumSum);
             s.meanRunAve = (float)(meanPrefix[s.meanSize] / (double)s.meanSize);
@@ -442,7 +507,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         r.dataModel = dataModel;
         r.margin = margin;
         r.flags = (good ? 1 : 0) | (replaced ? 2 : 0);
-        r.ver = sh_ver;
+        r.ver = s.ver;
         const unsigned reportStatsEvery = 10u * (unsigned)K.maxHistory;
         if (s.num == 0) {
             const float fractionBadSpectra =
@@ -472,42 +537,48 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     };
 
     enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3 };
-    if (tid == 0) {
-        sh_frontsValid = 0;
-        sh_ver = 0;
-    }
-    // version 0 of the last-good spectrum: its state at chunk start
-    for (int i = tid; i < nBins; i += kWalkThreads) lastGoodVer[i] = lastGood[i];
-    __syncthreads();
-    int t = 0;
+    int t = tStart;
     for (;;) {
         if (tid == 0) {
             int mode = MODE_END;
             for (;;) {
-                if (t >= nT) { mode = MODE_END; break; }
+                if (t >= tEnd || sh_stop) { mode = MODE_END; break; }
                 if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
-                if (s.rmsSize < K.cap) { mode = MODE_FULL; break; }  // learning phase
+                if (s.rmsSize < K.cap) {  // learning phase
+                    ++s.count[CNT_LEARN];
+                    mode = MODE_FULL;
+                    break;
+                }
                 float dataModel, margin;
                 phaseA_steady(t, dataModel, margin);
                 const int k = t - stageBase;
+                const bool covered = bx.valid && t - tStart < preN;
+                const bool inBox = covered && s.params0 >= bx.p0lo && s.params0 <= bx.p0hi &&
+                                   margin >= bx.mlo && margin <= bx.mhi;
                 // a kept spectrum fills empty last-good slots (:446-469); the
-                // precompute counted the candidates against the chunk-start
+                // precompute flagged candidates against the sub-chunk-start
                 // slots, a superset of the empty slots now
-                const bool fast = bx.valid && sh_ver < kMaxVer &&
-                                  (s.remainingZeros == 0 || recs[k].fillCand == 0) &&
-                                  recs[k].undecided == 0 && s.params0 >= bx.p0lo && s.params0 <= bx.p0hi &&
-                                  margin >= bx.mlo && margin <= bx.mhi;
-                if (!fast) {
+                int why = -1;
+                if (!inBox) {
+                    why = CNT_BOX;
+                    if (covered && !lastIter) sh_stop = 1;
+                } else if (recs[k].flags & 1) {
+                    why = CNT_UNDECIDED;
+                } else if (s.remainingZeros != 0 && (recs[k].flags & 2)) {
+                    why = CNT_FILL;
+                } else if (s.ver >= kMaxVer) {
+                    why = CNT_VER;
+                }
+                if (why >= 0) {
+                    ++s.count[why];
                     sh_dataModel = dataModel;
                     sh_margin = margin;
                     mode = MODE_BC;
                     break;
                 }
-                int cnt[8];
-                for (int b = 0; b < 8; ++b) cnt[b] = recs[k].cnt[b];
-                phaseC(t, recs[k].sum, cnt, dataModel, margin, 0);
+                phaseC(t, recs[k].mean, recs[k].fracInc, recs[k].badBands, dataModel, margin, 0);
                 phaseD();
-                ++s.nFast;
+                ++s.count[CNT_FAST];
                 ++t;
             }
             sh_mode = mode;
@@ -519,9 +590,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         if (mode == MODE_END) break;
         if (mode == MODE_STAGE) {
             stageBase = t;
-            stageN = min(kStage, nT - t);
+            stageN = min(kStage, tEnd - t);
+            const int nPre = min(stageN, preN - (t - tStart));
             if (bx.valid)
-                for (int k = tid; k < stageN; k += kWalkThreads) recs[k] = pre[t + k];
+                for (int k = tid; k < nPre; k += kWalkThreads) recs[k] = pre[t - tStart + k];
             const int fv = s.rmsSize == K.cap && s.meanSize == K.cap && K.cap > kStage;
             if (fv) {
                 for (int k = tid; k < stageN; k += kWalkThreads) {
@@ -644,7 +716,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         if (tid == 0) {
             int cnt[8];
             for (int b = 0; b < 8; ++b) cnt[b] = goodCnt[b];
-            sh_fill = phaseC(t, spectrumSum, cnt, dataModel, margin, 1);
+            int total, badBands;
+            float fracInc;
+            band_stats(cnt, nBins, cpb, total, badBands, fracInc);
+            sh_fill = phaseC(t, spectrumSum / (float)(unsigned)total, fracInc, badBands, dataModel, margin, 1);
         }
         __syncthreads();
         // last-good spectrum (:441-469); every change starts a new version
@@ -660,25 +735,33 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
             }
             __syncthreads();
             if (fillCnt > 0) {
-                const int nv = sh_ver + 1;
+                const int nv = s.ver + 1;
                 if (nv < kMaxVer)
                     for (int i = tid; i < nBins; i += kWalkThreads) lastGoodVer[(size_t)nv * nBins + i] = lastGood[i];
                 __syncthreads();
-                if (tid == 0) sh_ver = nv;  // >= kMaxVer: the rest of the chunk takes this path
+                if (tid == 0) s.ver = nv;  // >= kMaxVer: the rest of the run takes this path
             }
         }
         if (tid == 0) {
             s.remainingZeros -= fillCnt;
             phaseD();
-            ++s.nSlow;
+            ++s.count[CNT_SLOW];
             sh_t = t + 1;
         }
         __syncthreads();
         t = sh_t;
     }
-    if (tid == 0) *S = s;
+    if (tid == 0) {
+        if (sh_stop) {
+            ++s.count[CNT_STOP];
+            s.boxGrow = fminf(s.boxGrow * 4.0f, kMaxBoxGrow);
+        } else {
+            s.boxGrow = fmaxf(s.boxGrow * 0.5f, 1.0f);
+        }
+        s.tNext = t;
+        *S = s;
+    }
 }
-
 constexpr int kApplyTile = 64;
 
 // Parallel over spectra: clipSample, flatten, ordered mean/RMS, normalise
@@ -882,6 +965,7 @@ int setup_map(ppl_clipper* c, unsigned nSub, unsigned nPol, unsigned nCh)
     S.bpRms = c->bp.curRms;
     S.remainingZeros = (int)nBins;
     S.initialised = 1;
+    S.boxGrow = 1.0f;
 
     const size_t cap = (size_t)std::max(K.cap, 1);
     PPL_HIP(hipMalloc(&c->d_S, sizeof(ClipScalars)));
@@ -968,19 +1052,31 @@ static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsi
         c->d_rec = nullptr;
         c->d_pre = nullptr;
         PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * nT));
-        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * nT));
+        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * std::min((int)nT, kSubLen)));
         c->recCap = (int)nT;
     }
-    hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->K, c->d_terms,
-                       c->d_box, c->d_bpLo, c->d_bpHi);
+    // Sub-chunks: box around the state reached, precompute, walk.  The walk
+    // advances S->tNext by kSubLen spectra, or less when the state leaves the
+    // box; the last launch walks whatever remains.  Launches past the end
+    // return at once.
+    const int subLen = std::min((int)nT, kSubLen);
+    const int nIter = (int)cdiv(nT, (unsigned)subLen) + kExtraIter;
+    hipLaunchKernelGGL(clip_begin_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->d_lastGood,
+                       c->d_lastGoodVer, (int)nBins);
     PPL_LAUNCHED();
-    hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(nT, 64)), dim3(64), 0, st, d_I, c->K, c->d_box, c->d_bpLo,
-                       c->d_bpHi, c->d_lastGood, c->d_pre, (int)nT);
-    PPL_LAUNCHED();
-    hipLaunchKernelGGL(clip_walk_kernel, dim3(1), dim3(kWalkThreads), 0, st, d_I, c->K, c->d_S, c->d_meanBuf,
-                       c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_lastGoodVer, c->d_terms,
-                       c->d_mask, c->d_box, c->d_pre, c->d_rec, (int)nT);
-    PPL_LAUNCHED();
+    for (int it = 0; it < nIter; ++it) {
+        hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->K, c->d_terms,
+                           c->d_box, c->d_bpLo, c->d_bpHi, (int)nT);
+        PPL_LAUNCHED();
+        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(subLen, 64)), dim3(64), 0, st, d_I, c->K, c->d_S, c->d_box,
+                           c->d_bpLo, c->d_bpHi, c->d_lastGood, c->d_pre, (int)nT, subLen);
+        PPL_LAUNCHED();
+        hipLaunchKernelGGL(clip_walk_kernel, dim3(1), dim3(kWalkThreads), 0, st, d_I, c->K, c->d_S, c->d_meanBuf,
+                           c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_lastGoodVer,
+                           c->d_terms, c->d_mask, c->d_box, c->d_pre, c->d_rec, (int)nT, subLen,
+                           it == nIter - 1 ? 1 : 0);
+        PPL_LAUNCHED();
+    }
     hipLaunchKernelGGL(clip_apply_kernel, dim3(cdiv(nT, kApplyTile)), dim3(64), 0, st, d_I, d_W, c->K,
                        c->d_terms, c->d_lastGoodVer, c->d_rec, (int)nT);
     PPL_LAUNCHED();
@@ -1009,15 +1105,15 @@ extern "C" int ppl_clipper_stats(ppl_clipper* c, float* mean, float* rms)
     return PPL_OK;
 }
 
-extern "C" int ppl_clipper_counters(ppl_clipper* c, long long* fast, long long* slow)
+extern "C" int ppl_clipper_counters(ppl_clipper* c, long long* out, int n)
 {
-    PPL_CHECK_ARG(c && fast && slow, "null argument");
-    *fast = *slow = 0;
+    PPL_CHECK_ARG(c && out && n >= 0, "null argument");
+    const int m = std::min(n, (int)PPL_CLIP_NCOUNTERS);
+    for (int i = 0; i < m; ++i) out[i] = 0;
     if (!c->d_S) return PPL_OK;
     ClipScalars S;
     PPL_HIP(hipMemcpy(&S, c->d_S, sizeof S, hipMemcpyDeviceToHost));
-    *fast = S.nFast;
-    *slow = S.nSlow;
+    for (int i = 0; i < m; ++i) out[i] = S.count[i];
     return PPL_OK;
 }
 

@@ -100,6 +100,36 @@ def test_clipper_long_steady_state_3968_bins(orc, ppl):
         assert np.array_equal(out, ref), (a, np.argwhere(out != ref)[:5])
     st_o, sizes = oc.state()
     assert sizes[0] == 1000 and sizes[1] == 1000
-    fast, slow = gc.counters()
-    assert fast + slow == x.shape[0] and fast > 1000  # the steady state mostly takes the fast path
+    cnt = gc.counters()
+    assert cnt["fast"] + cnt["slow"] == x.shape[0] and cnt["fast"] > 1000, cnt  # steady state mostly fast
+    assert cnt["learn"] + cnt["box"] + cnt["undecided"] + cnt["fill"] + cnt["ver"] == cnt["slow"], cnt
+    assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
+
+
+def test_clipper_sub_chunks_and_drift(orc, ppl):
+    """Runs longer than one walk sub-chunk (4096 spectra), with a slow rise
+    of the whole band by 3% that carries the state out of the speculation
+    box: sub-chunks end early and re-centre, and every output still matches
+    the oracle bit for bit."""
+    n_sub, n_ch, n_t = 8, 16, 14000
+    x = _stream(n_t, n_sub, n_ch, seed=77)
+    ramp = np.clip((np.arange(n_t) - 6000) / 3000.0, 0.0, 1.0) * 0.03 + 1.0
+    x = (x * ramp[:, None, None, None]).astype(np.float32)
+    bp = _bp(n_sub, n_ch)
+    oc = orc.Clipper(bp, 10.0, 4.0, 1000, True)
+    gc = ppl.Clipper(bp, 10.0, 4.0, 1000, True)
+    for a in range(0, n_t, 7000):
+        chunk = np.ascontiguousarray(x[a:a + 7000])
+        ref = chunk.copy()
+        oc.run(ref)
+        d = torch.from_numpy(chunk).cuda()
+        gc.run(d)
+        assert np.array_equal(gc.flags(chunk.shape[0]), oc.last_flags(chunk.shape[0])[0]), a
+        out = d.cpu().numpy()
+        bad = np.nonzero(~((out == ref) | (np.isnan(out) & np.isnan(ref))))
+        assert bad[0].size == 0, (a, bad[0][:5], out[bad][:5], ref[bad][:5])
+    cnt = gc.counters()
+    assert cnt["fast"] + cnt["slow"] == n_t, cnt
+    assert cnt["stop"] >= 1 and cnt["fast"] > n_t // 2, cnt
+    st_o, _ = oc.state()
     assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
