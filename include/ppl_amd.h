@@ -145,6 +145,12 @@ int ppl_ddbuf_destroy(ppl_ddbuf* buf);
  * when the buffer holds nsampBuf samples. */
 int ppl_ddbuf_add(ppl_ddbuf* buf, const float* d_spectra, unsigned nT, unsigned first,
                   unsigned* consumed, int* full, void* stream);
+/* addSamples on a whole SpectrumDataSet blob [nT][nSub][nPol][nCh]: takes
+ * polarisation 0 ("dedispersion on total power only", :116) of every
+ * spectrum; nSub*nCh must equal the buffer's nChan. */
+int ppl_ddbuf_add_blob(ppl_ddbuf* buf, const float* d_spectra, unsigned nT, unsigned first,
+                       unsigned nSub, unsigned nPol, unsigned nCh, unsigned* consumed, int* full,
+                       void* stream);
 /* copy (:65-96): carries the overlap into the start and empties the rest. */
 int ppl_ddbuf_rotate(ppl_ddbuf* buf, void* stream);
 void* ppl_ddbuf_data(ppl_ddbuf* buf);

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ppl_ddbuf_create": (i, [vp, u, u, u, i, f]),
     "ppl_ddbuf_destroy": (i, [vp]),
     "ppl_ddbuf_add": (i, [vp, vp, u, u, vp, vp, vp]),
+    "ppl_ddbuf_add_blob": (i, [vp, vp, u, u, u, u, u, vp, vp, vp]),
     "ppl_ddbuf_rotate": (i, [vp, vp]),
     "ppl_ddbuf_data": (vp, [vp]),
     "ppl_ddbuf_fill": (u, [vp]),
@@ -249,6 +250,14 @@ class DDBuffer:
         n_t = d_spectra.shape[0]
         check(lib().ppl_ddbuf_add(self.h, _ptr(d_spectra), n_t, first, C.byref(consumed),
                                   C.byref(full), _stream(stream)))
+        return consumed.value, bool(full.value)
+
+    def add_blob(self, d_blob, first=0, stream=None):
+        """addSamples from a [nT][nSub][nPol][nCh] blob (polarisation 0)."""
+        consumed, full = C.c_uint(0), C.c_int(0)
+        n_t, n_sub, n_pol, n_ch = d_blob.shape
+        check(lib().ppl_ddbuf_add_blob(self.h, _ptr(d_blob), n_t, first, n_sub, n_pol, n_ch,
+                                       C.byref(consumed), C.byref(full), _stream(stream)))
         return consumed.value, bool(full.value)
 
     def rotate(self, stream=None):

@@ -33,18 +33,23 @@ template <> __device__ __forceinline__ uint8_t convert<uint8_t>(float v, float i
     return (uint8_t)((int)y + 128);
 }
 
-// spectra rows [t0, t0+nT) x nChan -> buf[c][col0 + (t - t0)]
+// Polarisation 0 of spectra [t][sub][pol][ch] (the SpectrumDataSet layout),
+// rows [t0, t0+nT), channel c = sub*nCh + ch -> buf[c][col0 + (t - t0)].
+// With nPol = 1 the rows are plain [t][nChan].
 template <typename T>
 __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ spectra, long t0, int nT,
-                                                        int nChan, T* __restrict__ buf, int nsamp, int col0,
-                                                        float inv)
+                                                        int nChan, int nCh, int nPol, T* __restrict__ buf,
+                                                        int nsamp, int col0, float inv)
 {
     __shared__ float tile[64][65];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
     const int cBase = blockIdx.x * 64, tBase = blockIdx.y * 64;
+    const long rowStride = (long)nChan * nPol;
+    const int c = cBase + tx;
+    const long cOff = nPol == 1 ? c : (long)(c / nCh) * nCh * nPol + c % nCh;
     for (int r = ty; r < 64; r += 4) {
-        const int t = tBase + r, c = cBase + tx;
-        tile[r][tx] = (t < nT && c < nChan) ? spectra[(t0 + t) * (long)nChan + c] : 0.0f;
+        const int t = tBase + r;
+        tile[r][tx] = (t < nT && c < nChan) ? spectra[(t0 + t) * rowStride + cOff] : 0.0f;
     }
     __syncthreads();
     for (int r = ty; r < 64; r += 4) {
@@ -101,11 +106,13 @@ extern "C" int ppl_ddbuf_destroy(ppl_ddbuf* b)
     return PPL_OK;
 }
 
-int ppl_ddbuf_add_spectra(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned first, unsigned* consumed,
-                          int* full, hipStream_t st)
+static int add_blob(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned first, unsigned nCh, unsigned nPol,
+                    unsigned* consumed, int* full, hipStream_t st)
 {
     PPL_CHECK_ARG(b && d_spectra && consumed && full, "null argument");
     PPL_CHECK_ARG(first <= nT, "first sample %u beyond %u", first, nT);
+    PPL_CHECK_ARG(nPol >= 1 && nCh >= 1 && b->nChan % nCh == 0,
+                  "DedispersionBuffer: %u channels per subband do not divide the buffer's %u", nCh, b->nChan);
     const unsigned room = b->nsamp - b->fill;
     const unsigned n = std::min(room, nT - first);
     if (n > 0) {
@@ -114,17 +121,17 @@ int ppl_ddbuf_add_spectra(ppl_ddbuf* b, const float* d_spectra, unsigned nT, uns
         switch (b->dtype) {
             case PPL_F32:
                 hipLaunchKernelGGL(transpose_kernel<float>, grid, dim3(256), 0, st, d_spectra, (long)first, (int)n,
-                                   (int)b->nChan, static_cast<float*>(dst), (int)b->nsamp, (int)b->fill,
+                                   (int)b->nChan, (int)nCh, (int)nPol, static_cast<float*>(dst), (int)b->nsamp, (int)b->fill,
                                    b->inv_scale);
                 break;
             case PPL_F16:
                 hipLaunchKernelGGL(transpose_kernel<_Float16>, grid, dim3(256), 0, st, d_spectra, (long)first,
-                                   (int)n, (int)b->nChan, static_cast<_Float16*>(dst), (int)b->nsamp,
+                                   (int)n, (int)b->nChan, (int)nCh, (int)nPol, static_cast<_Float16*>(dst), (int)b->nsamp,
                                    (int)b->fill, b->inv_scale);
                 break;
             default:
                 hipLaunchKernelGGL(transpose_kernel<uint8_t>, grid, dim3(256), 0, st, d_spectra, (long)first,
-                                   (int)n, (int)b->nChan, static_cast<uint8_t*>(dst), (int)b->nsamp,
+                                   (int)n, (int)b->nChan, (int)nCh, (int)nPol, static_cast<uint8_t*>(dst), (int)b->nsamp,
                                    (int)b->fill, b->inv_scale);
         }
         PPL_LAUNCHED();
@@ -135,10 +142,26 @@ int ppl_ddbuf_add_spectra(ppl_ddbuf* b, const float* d_spectra, unsigned nT, uns
     return PPL_OK;
 }
 
+int ppl_ddbuf_add_spectra(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned first, unsigned* consumed,
+                          int* full, hipStream_t st)
+{
+    return add_blob(b, d_spectra, nT, first, b ? b->nChan : 1u, 1u, consumed, full, st);
+}
+
 extern "C" int ppl_ddbuf_add(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned first, unsigned* consumed,
                              int* full, void* stream)
 {
     return ppl_ddbuf_add_spectra(b, d_spectra, nT, first, consumed, full, stream_of(stream));
+}
+
+extern "C" int ppl_ddbuf_add_blob(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned first, unsigned nSub,
+                                  unsigned nPol, unsigned nCh, unsigned* consumed, int* full, void* stream)
+{
+    PPL_CHECK_ARG(b, "null argument");
+    PPL_CHECK_ARG(nSub * nCh == b->nChan,
+                  "DedispersionBuffer: input data sample size(%u) does not match buffer sample size (%u)", nSub * nCh,
+                  b->nChan);
+    return add_blob(b, d_spectra, nT, first, nCh, nPol, consumed, full, stream_of(stream));
 }
 
 extern "C" int ppl_ddbuf_rotate(ppl_ddbuf* b, void* stream)

@@ -212,7 +212,7 @@ __device__ __forceinline__ float ring_front(RingView r, int start) { return r.bu
 
 constexpr int kWalkThreads = 512;
 constexpr int kStage = 512;      // spectra whose precomputed records are staged in LDS at a time
-constexpr int kRowChunk = 4096;  // bins of one spectrum staged in LDS for the ordered sums
+constexpr int kRowChunk = 8192;  // bins of one spectrum staged in LDS for the ordered sums
 
 // Per-sub-chunk speculation box: every spectrum of the sub-chunk whose
 // bandpass offset lies in [p0lo, p0hi] and whose margin lies in [mlo, mhi]
@@ -244,6 +244,112 @@ struct PreRec {
 };
 
 constexpr int kMaxVer = 64;  // last-good versions kept per run for the apply pass
+
+// Row-tiled passes (precompute, statistics): a workgroup of 256 threads owns
+// kTileRows spectra; bins stream through double-buffered LDS tiles of
+// kTileBins, loaded 4 bins per thread; one lane per spectrum folds the tile
+// in bin order while the other waves load and judge the next tile.
+constexpr int kTileRows = 32;
+constexpr int kTileBins = 256;
+constexpr int kTileThreads = 256;
+constexpr int kRowsPerWave = kTileRows / 4;
+
+// Loads of 4 consecutive bins (i % 4 == 0) of polarisation 0 of one spectrum.
+// mode 0: rows of contiguous bins (nPol == 1), 16-byte aligned; mode 1:
+// subbands of nCh (nCh % 4 == 0) interleaved with the other polarisations,
+// aligned; mode 2: anything else, one bin at a time.
+__device__ __forceinline__ long bin_offset(int i, int nCh, int nPol)
+{
+    return (long)(i / nCh) * nCh * nPol + i % nCh;
+}
+__device__ __forceinline__ float4 load_quad(const float* __restrict__ I, long rowOff, int i, int nCh, int nPol,
+                                            int mode)
+{
+    if (mode == 0) return *reinterpret_cast<const float4*>(I + rowOff + i);
+    if (mode == 1) return *reinterpret_cast<const float4*>(I + rowOff + bin_offset(i, nCh, nPol));
+    float4 r;
+    r.x = I[rowOff + bin_offset(i, nCh, nPol)];
+    r.y = I[rowOff + bin_offset(i + 1, nCh, nPol)];
+    r.z = I[rowOff + bin_offset(i + 2, nCh, nPol)];
+    r.w = I[rowOff + bin_offset(i + 3, nCh, nPol)];
+    return r;
+}
+__device__ __forceinline__ void store_quad(float* __restrict__ I, long rowOff, int i, int nCh, int nPol, int mode,
+                                           float4 v)
+{
+    if (mode == 0) {
+        *reinterpret_cast<float4*>(I + rowOff + i) = v;
+    } else if (mode == 1) {
+        *reinterpret_cast<float4*>(I + rowOff + bin_offset(i, nCh, nPol)) = v;
+    } else {
+        I[rowOff + bin_offset(i, nCh, nPol)] = v.x;
+        I[rowOff + bin_offset(i + 1, nCh, nPol)] = v.y;
+        I[rowOff + bin_offset(i + 2, nCh, nPol)] = v.z;
+        I[rowOff + bin_offset(i + 3, nCh, nPol)] = v.w;
+    }
+}
+__device__ __forceinline__ float& q_at(float4& q, int j) { return (&q.x)[j]; }
+__device__ __forceinline__ float q_at(const float4& q, int j) { return (&q.x)[j]; }
+
+// Polynomial terms of the bandpass for one bin held in registers (NT known
+// at compile time), or read from memory (NT = -1).
+template <int NT>
+struct BinTerms {
+    double t[NT > 0 ? NT : 1];
+    __device__ __forceinline__ void load(const double* __restrict__ terms, int, int nBins, int i)
+    {
+#pragma unroll
+        for (int j = 0; j < NT; ++j) t[j] = terms[(size_t)j * nBins + i];
+    }
+    __device__ __forceinline__ float bp(float p0, float scale) const
+    {
+        float tot = 0.0f;
+        tot = (float)((double)tot + (double)p0 * 1.0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) tot = (float)((double)tot + t[j]);
+        return scale * tot;
+    }
+};
+template <>
+struct BinTerms<-1> {
+    const double* terms;
+    int nTerms, nBins, i;
+    __device__ __forceinline__ void load(const double* __restrict__ tm, int nt, int nb, int bin)
+    {
+        terms = tm;
+        nTerms = nt;
+        nBins = nb;
+        i = bin;
+    }
+    __device__ __forceinline__ float bp(float p0, float scale) const
+    {
+        return bp_value(terms, nTerms, nBins, i, p0, scale);
+    }
+};
+
+// Correctly rounded x / d for a divisor d that stays fixed over many
+// divisions, as RN_f32((double)x * RN_f64(1/d)).  For float x and a normal
+// float d the exact quotient is never within 2^-49 (relative) of a float
+// rounding midpoint, while the double product is within 2^-52 of it, so the
+// two roundings agree; quotients in the subnormal range (where that bound
+// does not hold) take the division itself.
+struct FixedDivisor {
+    float d;
+    double rcp;
+    bool ok;
+    __device__ __forceinline__ void set(float divisor)
+    {
+        d = divisor;
+        ok = isfinite(divisor) && fabsf(divisor) >= 0x1p-126f;
+        rcp = 1.0 / (double)divisor;
+    }
+    __device__ __forceinline__ float div(float x) const
+    {
+        if (!ok) return x / d;
+        const float q = (float)((double)x * rcp);
+        return fabsf(q) < 0x1p-125f ? x / d : q;
+    }
+};
 
 __device__ __forceinline__ float block_min(float v, float* red)
 {
@@ -283,6 +389,10 @@ __device__ __forceinline__ void band_stats(const int* cnt, int nBins, int cpb, i
     badBands = (int)bad;
     fracInc = (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
 }
+
+// Band (0..7) of bin i: floor((i + 0.5) / cpb), exact in float while
+// 0.5 / cpb exceeds the 2^-21 error of the product (cpb < 2^19).
+__device__ __forceinline__ int band_of(int i, float invCpb) { return (int)(((float)i + 0.5f) * invCpb); }
 
 // Start of a run: version 0 of the last-good spectrum is its current state.
 __global__ void clip_begin_kernel(ClipScalars* __restrict__ S, const float* __restrict__ lastGood,
@@ -328,91 +438,137 @@ __global__ void clip_bounds_kernel(const ClipScalars* __restrict__ S, ClipConst 
     }
 }
 
-// One lane per spectrum of the sub-chunk, bins staged through 64x64 LDS
-// tiles: the kept-channel sum in bin order (the reference's order) for the
-// mask that holds over the whole box.  A channel is kept for every state in
-// the box when fl(I - bpLo) <= mlo, clipped for every state when
-// fl(I - bpHi) > mhi (fl(I - bp) falls as bp rises; bp is monotone in the
-// offset).
-__global__ __launch_bounds__(64) void clip_pre_kernel(const float* __restrict__ I, ClipConst K,
-                                                      const ClipScalars* __restrict__ S,
-                                                      const Box* __restrict__ box, const float* __restrict__ bpLo,
-                                                      const float* __restrict__ bpHi,
-                                                      const float* __restrict__ lastGood, PreRec* __restrict__ pre,
-                                                      int nT, int subLen)
+// Precompute over the spectra of one sub-chunk: the kept-channel sum in bin
+// order (the reference's order) for the mask that holds over the whole box,
+// the kept count per band, and flags for undecided channels and last-good
+// fill candidates.  A channel is kept for every state in the box when
+// fl(I - bpLo) <= mlo and clipped for every state when fl(I - bpHi) > mhi
+// (fl(I - bp) falls as bp rises; bp is monotone in the offset).  Clipped
+// channels enter the fold as +0.0, which leaves a sum started at +0.0
+// unchanged.
+__global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(const float* __restrict__ I, ClipConst K,
+                                                                const ClipScalars* __restrict__ S,
+                                                                const Box* __restrict__ box,
+                                                                const float* __restrict__ bpLo,
+                                                                const float* __restrict__ bpHi,
+                                                                const float* __restrict__ lastGood,
+                                                                PreRec* __restrict__ pre, int nT, int subLen,
+                                                                int ldMode)
 {
-    __shared__ float tile[64][65];
-    __shared__ float lo[64], hi[64], lg[64];
+    __shared__ float vals[2][kTileRows][kTileBins + 1];
+    __shared__ int cnt[kTileRows][8];
+    __shared__ int rowFlags[kTileRows];
     const int tStart = S->tNext;
     if (tStart >= nT) return;
     const Box bx = *box;
     if (!bx.valid) return;
-    const int lane = threadIdx.x;
-    const int k0 = blockIdx.x * 64;
+    const int k0 = blockIdx.x * kTileRows;
     const int t0 = tStart + k0;
-    const int nRows = min(min(64, nT - t0), subLen - k0);
+    const int nRows = min(min(kTileRows, nT - t0), subLen - k0);
     if (nRows <= 0) return;
-    const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
     const int cpb = nBins / 8;
-    float sum = 0.0f;
-    int undecided = 0, fillCand = 0;
-    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int band = 0, bandNext = cpb, curCnt = 0;
-    for (int b0 = 0; b0 < nBins; b0 += 64) {
-        const int nb = min(64, nBins - b0);
-        for (int r = 0; r < nRows; ++r) {
-            if (lane < nb) {
-                const int i = b0 + lane;
-                const int sb = i / nCh, c = i % nCh;
-                tile[r][lane] = I[(long)nCh * (nPol * ((long)nSub * (t0 + r) + sb)) + c];
-            }
-        }
-        if (lane < nb) {
-            lo[lane] = bpLo[b0 + lane];
-            hi[lane] = bpHi[b0 + lane];
-            lg[lane] = lastGood[b0 + lane];
-        }
-        __syncthreads();
-        if (lane < nRows) {
-            for (int k = 0; k < nb; ++k) {
-                const int i = b0 + k;
-                if (i == bandNext) {
+    const float invCpb = 1.0f / (float)cpb;
+    const long rowStride = (long)nBins * nPol;
+    const int nTiles = (nBins + kTileBins - 1) / kTileBins;
+    for (int i = tid; i < kTileRows * 8; i += kTileThreads) (&cnt[0][0])[i] = 0;
+    if (tid < kTileRows) rowFlags[tid] = 0;
+    __syncthreads();
+
+    float4 raw[kRowsPerWave];
+    auto issue = [&](int tile) {
+        const int i = tile * kTileBins + lane * 4;
+        if (i < nBins) {
 #pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (q == band) cnt[q] = curCnt;
-                    ++band;
-                    bandNext += cpb;
-                    curCnt = 0;
-                }
-                const float v = tile[lane][k];
-                const float eHi = v - lo[k];  // largest fl(I - bp) over the box
-                const float eLo = v - hi[k];  // smallest
-                if (eHi <= bx.mlo) {
-                    sum += v;
-                    ++curCnt;
-                    if (lg[k] == 0.0f && v != 0.0f) ++fillCand;
-                } else if (!(eLo > bx.mhi)) {
-                    ++undecided;
-                }
+            for (int q = 0; q < kRowsPerWave; ++q) {
+                const int r = wave + 4 * q;
+                if (r < nRows) raw[q] = load_quad(I, (long)(t0 + r) * rowStride, i, nCh, nPol, ldMode);
             }
         }
+    };
+    auto judge = [&](int tile, int buf) {
+        const int tb0 = tile * kTileBins;
+        const int i = tb0 + lane * 4;
+        const bool inb = i < nBins;
+        float4 lo4 = make_float4(0, 0, 0, 0), hi4 = lo4, lg4 = lo4;
+        if (inb) {
+            lo4 = *reinterpret_cast<const float4*>(bpLo + i);
+            hi4 = *reinterpret_cast<const float4*>(bpHi + i);
+            lg4 = *reinterpret_cast<const float4*>(lastGood + i);
+        }
+        int band[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) band[j] = band_of(i + j, invCpb);
+        const int bandLo = band_of(tb0, invCpb);
+        const int bandHi = band_of(min(tb0 + kTileBins, nBins) - 1, invCpb);
+#pragma unroll
+        for (int q = 0; q < kRowsPerWave; ++q) {
+            const int r = wave + 4 * q;
+            if (r >= nRows) break;  // uniform over the wave
+            bool keep[4];
+            bool und = false, fill = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = q_at(raw[q], j);
+                const float eHi = v - q_at(lo4, j);  // largest fl(I - bp) over the box
+                const float eLo = v - q_at(hi4, j);  // smallest
+                keep[j] = inb && eHi <= bx.mlo;
+                und |= inb && !keep[j] && !(eLo > bx.mhi);
+                fill |= keep[j] && q_at(lg4, j) == 0.0f && v != 0.0f;
+                if (inb) vals[buf][r][lane * 4 + j] = keep[j] ? v : 0.0f;
+            }
+            for (int b = bandLo; b <= bandHi; ++b) {
+                int c = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c += __popcll(__ballot(keep[j] && band[j] == b));
+                if (lane == 0) cnt[r][b] += c;
+            }
+            const bool anyUnd = __ballot(und) != 0, anyFill = __ballot(fill) != 0;
+            if (lane == 0 && (anyUnd || anyFill)) rowFlags[r] |= (anyUnd ? 1 : 0) | (anyFill ? 2 : 0);
+        }
+    };
+
+    float sum = 0.0f;
+    issue(0);
+    judge(0, 0);
+    __syncthreads();
+    for (int tile = 0; tile < nTiles; ++tile) {
+        const int buf = tile & 1;
+        if (tile + 1 < nTiles) issue(tile + 1);
+        if (wave == 0 && lane < nRows) {
+            const int nb = min(kTileBins, nBins - tile * kTileBins);
+            const float* row = &vals[buf][lane][0];
+#pragma unroll 8
+            for (int k = 0; k < nb; ++k) sum += row[k];
+        }
+        if (tile + 1 < nTiles) judge(tile + 1, buf ^ 1);
         __syncthreads();
     }
-    if (lane < nRows) {
+    if (wave == 0 && lane < nRows) {
+        int c[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == band) cnt[q] = curCnt;
+        for (int b = 0; b < 8; ++b) c[b] = cnt[lane][b];
         int total, badBands;
         float fracInc;
-        band_stats(cnt, nBins, cpb, total, badBands, fracInc);
+        band_stats(c, nBins, cpb, total, badBands, fracInc);
         PreRec r;
         r.mean = sum / (float)(unsigned)total;
         r.fracInc = fracInc;
         r.badBands = badBands;
-        r.flags = (undecided ? 1 : 0) | (fillCand ? 2 : 0);
+        r.flags = rowFlags[lane];
         pre[k0 + lane] = r;
     }
 }
+
+// The walk's steady-state scalars in registers (thread 0), with the ring
+// backs also kept divided by the history length.
+struct FastState {
+    float mRA, rA, p0, bpMean, bpRms, sRMS, fbc;
+    int badSpectra, num, meanStart, rmsStart;
+    float mBack, mBack2, rBack, rBack2;
+    float mBackC, mBack2C, rBackC, rBack2C;
+};
 
 // The sequential walk over one sub-chunk of spectra.  Thread 0 runs ahead
 // through spectra the precompute decided; the workgroup joins in for the
@@ -425,8 +581,8 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     float* __restrict__ I, ClipConst K, ClipScalars* __restrict__ S, float* __restrict__ meanBuf,
     double* __restrict__ meanPrefix, float* __restrict__ rmsBuf, double* __restrict__ rmsPrefix,
     float* __restrict__ lastGood, float* __restrict__ lastGoodVer, const double* __restrict__ terms,
-    uint8_t* __restrict__ mask, const Box* __restrict__ box, const PreRec* __restrict__ pre,
-    SpecRec* __restrict__ rec, int nT, int subLen, int lastIter)
+    const Box* __restrict__ box, const PreRec* __restrict__ pre, SpecRec* __restrict__ rec, int nT, int subLen,
+    int lastIter)
 {
     __shared__ ClipScalars s;
     __shared__ float red[kWalkThreads / 64 + 1];
@@ -437,8 +593,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
     __shared__ PreRec recs[kStage];
-    __shared__ float meanFront[kStage], rmsFront[kStage];
-    __shared__ float rowChunk[kRowChunk];
+    // staged per spectrum: ring fronts and the precomputed mean, each divided
+    // by the history length
+    __shared__ float meanFC[kStage], rmsFC[kStage], meanC[kStage];
+    __shared__ __align__(16) float rowChunk[kRowChunk];
 
     const int tid = threadIdx.x;
     const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
@@ -455,22 +613,24 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     const int tEnd = lastIter ? nT : min(nT, tStart + subLen);
     const int preN = min(subLen, nT - tStart);  // spectra the precompute covered
     const Box bx = *box;
+    const float capf = (float)K.cap;
 
     auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb)) + (long)nCh * pol; };
 
     int stageBase = tStart, stageN = 0;
-    // Ring fronts of the staged spectra: with both rings full, the front of
-    // spectrum stageBase+k is entry start+k (each spectrum evicts one front;
-    // a rejection rewrites only the newest entry).
-    auto rmsFrontOf = [&](int t) {
-        return sh_frontsValid ? rmsFront[t - stageBase] : ring_front(rmsR, s.rmsStart);
+    // Ring fronts of the staged spectra, divided by the (full) ring size:
+    // with both rings full, the front of spectrum stageBase+k is entry
+    // start+k (each spectrum evicts one front; a rejection rewrites only the
+    // newest entry).
+    auto rmsFrontDiv = [&](int t) {
+        return sh_frontsValid ? rmsFC[t - stageBase] : ring_front(rmsR, s.rmsStart) / (float)s.rmsSize;
     };
-    auto meanFrontOf = [&](int t) {
-        return sh_frontsValid ? meanFront[t - stageBase] : ring_front(meanR, s.meanStart);
+    auto meanFrontDiv = [&](int t) {
+        return sh_frontsValid ? meanFC[t - stageBase] : ring_front(meanR, s.meanStart) / (float)s.meanSize;
     };
     // Steady-state rms update and margin (RFI_Clipper.cpp:288-311), thread 0.
     auto phaseA_steady = [&](int t, float& dataModel, float& margin) {
-        s.rmsRunAve -= (rmsFrontOf(t) / (float)s.rmsSize);
+        s.rmsRunAve -= rmsFrontDiv(t);
         s.rmsRunAve += (s.rmsBack / (float)K.cap);
         if (K.useMeanOverRMS) s.spectrumRMS = fabsf(s.meanRunAve / s.meanOverRMS);
         ring_push(rmsR, s.rmsStart, s.rmsSize, s.rmsBack, s.rmsBack2, s.spectrumRMS);
@@ -488,7 +648,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
             ring_push(meanR, s.meanStart, s.meanSize, s.meanBack, s.meanBack2, spectrumSum);
             s.meanRunAve = (float)(meanPrefix[s.meanSize] / (double)s.meanSize);
         } else {
-            s.meanRunAve -= meanFrontOf(t) / (float)s.meanSize;
+            s.meanRunAve -= meanFrontDiv(t);
             s.meanRunAve += s.meanBack / (float)s.meanSize;
             ring_push(meanR, s.meanStart, s.meanSize, s.meanBack, s.meanBack2, spectrumSum);
         }
@@ -536,50 +696,193 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         s.num = (int)(((unsigned)s.num + 1u) % (10u * (unsigned)K.maxHistory));
     };
 
+    // Why spectrum t (after its rms update) cannot take the precomputed mask,
+    // or -1; sets sh_stop for a state that left the box.
+    auto slow_reason = [&](int t, float margin) -> int {
+        const int k = t - stageBase;
+        const bool covered = t - tStart < preN;
+        const bool inBox = bx.valid && covered && s.params0 >= bx.p0lo && s.params0 <= bx.p0hi &&
+                           margin >= bx.mlo && margin <= bx.mhi;
+        if (!inBox) {
+            // leave the sub-chunk to re-centre the box (or to get one, when
+            // the sub-chunk started in the learning phase)
+            if (covered && !lastIter) sh_stop = bx.valid ? 1 : 2;
+            return CNT_BOX;
+        }
+        // a kept spectrum fills empty last-good slots (:446-469); the
+        // precompute flagged candidates against the sub-chunk-start slots,
+        // a superset of the empty slots now
+        if (recs[k].flags & 1) return CNT_UNDECIDED;
+        if (s.remainingZeros != 0 && (recs[k].flags & 2)) return CNT_FILL;
+        if (s.ver >= kMaxVer) return CNT_VER;
+        return -1;
+    };
+
     enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3 };
+    const int period = 10 * K.maxHistory;
     int t = tStart;
     for (;;) {
         if (tid == 0) {
             int mode = MODE_END;
-            for (;;) {
-                if (t >= tEnd || sh_stop) { mode = MODE_END; break; }
-                if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
-                if (s.rmsSize < K.cap) {  // learning phase
-                    ++s.count[CNT_LEARN];
-                    mode = MODE_FULL;
-                    break;
+            if (sh_stop) {
+                mode = MODE_END;
+            } else if (sh_frontsValid && bx.valid && s.rmsSize == K.cap && s.meanSize == K.cap) {
+                // ---- steady state, state in registers
+                FixedDivisor byCap, byMOR, byScale;
+                byCap.set(capf);
+                byMOR.set(s.meanOverRMS);
+                byScale.set(K.scale);
+                FastState f;
+                f.mRA = s.meanRunAve;
+                f.rA = s.rmsRunAve;
+                f.p0 = s.params0;
+                f.bpMean = s.bpMean;
+                f.bpRms = s.bpRms;
+                f.sRMS = s.spectrumRMS;
+                f.fbc = s.fractionBadChannels;
+                f.badSpectra = s.badSpectra;
+                f.num = s.num;
+                f.meanStart = s.meanStart;
+                f.rmsStart = s.rmsStart;
+                f.mBack = s.meanBack;
+                f.mBack2 = s.meanBack2;
+                f.rBack = s.rmsBack;
+                f.rBack2 = s.rmsBack2;
+                f.mBackC = byCap.div(f.mBack);
+                f.mBack2C = byCap.div(f.mBack2);
+                f.rBackC = byCap.div(f.rBack);
+                f.rBack2C = byCap.div(f.rBack2);
+                const int cap = K.cap;
+                const bool capGe1000 = cap >= 1000;
+                int nFast = 0, reset = 0;
+                float margin = 0.0f;
+                for (;;) {
+                    if (t >= tEnd) { mode = MODE_END; break; }
+                    if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
+                    const int k = t - stageBase;
+                    // phaseA (:288-311)
+                    f.rA = f.rA - rmsFC[k];
+                    f.rA = f.rA + f.rBackC;
+                    if (K.useMeanOverRMS) f.sRMS = fabsf(byMOR.div(f.mRA));
+                    rmsBuf[f.rmsStart] = f.sRMS;
+                    f.rmsStart = f.rmsStart + 1 == cap ? 0 : f.rmsStart + 1;
+                    f.rBack2 = f.rBack;
+                    f.rBack2C = f.rBackC;
+                    f.rBack = f.sRMS;
+                    f.rBackC = byCap.div(f.sRMS);
+                    margin = K.crFactor * f.rA;
+                    const bool inBox = t - tStart < preN && f.p0 >= bx.p0lo && f.p0 <= bx.p0hi &&
+                                       margin >= bx.mlo && margin <= bx.mhi;
+                    const PreRec pr = recs[k];
+                    if (!inBox || (pr.flags & 1) || ((pr.flags & 2) && s.remainingZeros != 0) || s.ver >= kMaxVer) {
+                        mode = MODE_BC;
+                        break;
+                    }
+                    // phaseC (:345-440)
+                    f.fbc += pr.fracInc;
+                    f.mRA = f.mRA - meanFC[k];
+                    f.mRA = f.mRA + f.mBackC;
+                    meanBuf[f.meanStart] = pr.mean;
+                    f.meanStart = f.meanStart + 1 == cap ? 0 : f.meanStart + 1;
+                    f.mBack2 = f.mBack;
+                    f.mBack2C = f.mBackC;
+                    f.mBack = pr.mean;
+                    f.mBackC = meanC[k];
+                    const float tol = (float)((double)(K.srFactor * f.bpRms) / K.sqrtNBins);
+                    int good = capGe1000 ? 1 : 0;
+                    if (capGe1000 && (pr.mean - f.mRA > tol || pr.badBands >= 4)) {
+                        good = 0;
+                        ++f.badSpectra;
+                        // pop_back + push_back(back()) on both rings
+                        meanBuf[(f.meanStart == 0 ? cap : f.meanStart) - 1] = f.mBack2;
+                        f.mBack = f.mBack2;
+                        f.mBackC = f.mBack2C;
+                        rmsBuf[(f.rmsStart == 0 ? cap : f.rmsStart) - 1] = f.rBack2;
+                        f.rBack = f.rBack2;
+                        f.rBackC = f.rBack2C;
+                    }
+                    SpecRec r;
+                    r.params0 = f.p0;
+                    r.dataModel = 0.0f;
+                    r.margin = margin;
+                    r.flags = good;
+                    r.ver = s.ver;
+                    if (f.num == 0) {
+                        const float fractionBadSpectra =
+                            (float)(100.0 * (double)(float)f.badSpectra / (double)(float)(unsigned)period);
+                        if (fractionBadSpectra > 99.0) reset = 1;
+                        f.badSpectra = 0;
+                        f.fbc = 0.0f;
+                    }
+                    r.meanRunAve = f.mRA;
+                    rec[t] = r;
+                    // phaseD
+                    const float delta = f.mRA - f.bpMean;
+                    if (fabsf(delta) > 0.0f) {
+                        f.p0 += byScale.div(delta);
+                        f.bpMean = f.mRA;
+                    }
+                    f.bpRms = f.rA;
+                    f.num = f.num + 1 == period ? 0 : f.num + 1;
+                    ++nFast;
+                    ++t;
+                    if (reset) { mode = MODE_STAGE; break; }  // back to learning: restage
                 }
-                float dataModel, margin;
-                phaseA_steady(t, dataModel, margin);
-                const int k = t - stageBase;
-                const bool covered = bx.valid && t - tStart < preN;
-                const bool inBox = covered && s.params0 >= bx.p0lo && s.params0 <= bx.p0hi &&
-                                   margin >= bx.mlo && margin <= bx.mhi;
-                // a kept spectrum fills empty last-good slots (:446-469); the
-                // precompute flagged candidates against the sub-chunk-start
-                // slots, a superset of the empty slots now
-                int why = -1;
-                if (!inBox) {
-                    why = CNT_BOX;
-                    if (covered && !lastIter) sh_stop = 1;
-                } else if (recs[k].flags & 1) {
-                    why = CNT_UNDECIDED;
-                } else if (s.remainingZeros != 0 && (recs[k].flags & 2)) {
-                    why = CNT_FILL;
-                } else if (s.ver >= kMaxVer) {
-                    why = CNT_VER;
+                s.meanRunAve = f.mRA;
+                s.rmsRunAve = f.rA;
+                s.params0 = f.p0;
+                s.bpMean = f.bpMean;
+                s.bpRms = f.bpRms;
+                s.spectrumRMS = f.sRMS;
+                s.fractionBadChannels = f.fbc;
+                s.badSpectra = f.badSpectra;
+                s.num = f.num;
+                s.meanStart = f.meanStart;
+                s.rmsStart = f.rmsStart;
+                s.meanBack = f.mBack;
+                s.meanBack2 = f.mBack2;
+                s.rmsBack = f.rBack;
+                s.rmsBack2 = f.rBack2;
+                s.count[CNT_FAST] += nFast;
+                if (reset) {
+                    s.rmsSize = 0;
+                    s.meanSize = 0;
+                    s.rmsStart = 0;
+                    s.meanStart = 0;
                 }
-                if (why >= 0) {
-                    ++s.count[why];
-                    sh_dataModel = dataModel;
+                if (mode == MODE_BC) {
+                    // spectrum t had its rms update; decide its reason and
+                    // hand it to the workgroup
+                    const int why = slow_reason(t, margin);
+                    ++s.count[why < 0 ? CNT_BOX : why];
+                    sh_dataModel = 0.0f;
                     sh_margin = margin;
-                    mode = MODE_BC;
-                    break;
                 }
-                phaseC(t, recs[k].mean, recs[k].fracInc, recs[k].badBands, dataModel, margin, 0);
-                phaseD();
-                ++s.count[CNT_FAST];
-                ++t;
+            } else {
+                for (;;) {
+                    if (t >= tEnd || sh_stop) { mode = MODE_END; break; }
+                    if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
+                    if (s.rmsSize < K.cap) {  // learning phase
+                        ++s.count[CNT_LEARN];
+                        mode = MODE_FULL;
+                        break;
+                    }
+                    float dataModel, margin;
+                    phaseA_steady(t, dataModel, margin);
+                    const int why = slow_reason(t, margin);
+                    if (why >= 0) {
+                        ++s.count[why];
+                        sh_dataModel = dataModel;
+                        sh_margin = margin;
+                        mode = MODE_BC;
+                        break;
+                    }
+                    const int k = t - stageBase;
+                    phaseC(t, recs[k].mean, recs[k].fracInc, recs[k].badBands, dataModel, margin, 0);
+                    phaseD();
+                    ++s.count[CNT_FAST];
+                    ++t;
+                }
             }
             sh_mode = mode;
             sh_t = t;
@@ -591,14 +894,17 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         if (mode == MODE_STAGE) {
             stageBase = t;
             stageN = min(kStage, tEnd - t);
-            const int nPre = min(stageN, preN - (t - tStart));
-            if (bx.valid)
-                for (int k = tid; k < nPre; k += kWalkThreads) recs[k] = pre[t - tStart + k];
+            const int nPre = bx.valid ? min(stageN, preN - (t - tStart)) : 0;
+            for (int k = tid; k < nPre; k += kWalkThreads) {
+                const PreRec r = pre[t - tStart + k];
+                recs[k] = r;
+                meanC[k] = r.mean / capf;
+            }
             const int fv = s.rmsSize == K.cap && s.meanSize == K.cap && K.cap > kStage;
             if (fv) {
                 for (int k = tid; k < stageN; k += kWalkThreads) {
-                    rmsFront[k] = rmsBuf[(s.rmsStart + k) % K.cap];
-                    meanFront[k] = meanBuf[(s.meanStart + k) % K.cap];
+                    rmsFC[k] = rmsBuf[(s.rmsStart + k) % K.cap] / capf;
+                    meanFC[k] = meanBuf[(s.meanStart + k) % K.cap] / capf;
                 }
             }
             __syncthreads();
@@ -684,34 +990,40 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         }
         __syncthreads();
         const float dataModel = sh_dataModel, margin = sh_margin;
-        // channel clipping (:315-342)
-        for (int i = tid; i < nBins; i += kWalkThreads) {
-            const int sb = i / nCh, c = i % nCh;
-            const long i0 = idx(sb, 0, t) + c;
-            const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
-            const bool clip = I[i0] - dataModel - bpv > margin;
-            mask[i] = clip ? 1 : 0;
-            if (clip) {
-                const float g = lastGood[i];
-                for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t) + c] = g;
-            } else {
-                atomicAdd(&goodCnt[i / cpb], 1);
-            }
-        }
-        __syncthreads();
-        // ordered sum of the kept channels (:339): clipped bins staged as +0.0,
-        // which leaves a sum that starts at +0.0 unchanged
+        // channel clipping (:315-342), kept values staged in LDS in bin order
+        // (clipped bins as +0.0, which leaves a sum that starts at +0.0
+        // unchanged) and summed in that order by one lane
         float spectrumSum = 0.0f;
+        const bool oneChunk = nBins <= kRowChunk;
         for (int x0 = 0; x0 < nBins; x0 += kRowChunk) {
             const int n = min(kRowChunk, nBins - x0);
             for (int k = tid; k < n; k += kWalkThreads) {
                 const int i = x0 + k, sb = i / nCh, c = i % nCh;
-                rowChunk[k] = mask[i] ? 0.0f : I[idx(sb, 0, t) + c];
+                const long i0 = idx(sb, 0, t) + c;
+                const float v = I[i0];
+                const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+                if (v - dataModel - bpv > margin) {
+                    const float g = lastGood[i];
+                    for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t) + c] = g;
+                    rowChunk[k] = 0.0f;
+                } else {
+                    atomicAdd(&goodCnt[i / cpb], 1);
+                    rowChunk[k] = v;
+                }
             }
             __syncthreads();
-            if (tid == 0)
-                for (int k = 0; k < n; ++k) spectrumSum += rowChunk[k];
-            __syncthreads();
+            if (tid == 0) {
+                int k = 0;
+                for (; k + 4 <= n; k += 4) {
+                    const float4 q = *reinterpret_cast<const float4*>(&rowChunk[k]);
+                    spectrumSum += q.x;
+                    spectrumSum += q.y;
+                    spectrumSum += q.z;
+                    spectrumSum += q.w;
+                }
+                for (; k < n; ++k) spectrumSum += rowChunk[k];
+            }
+            if (!oneChunk) __syncthreads();
         }
         if (tid == 0) {
             int cnt[8];
@@ -722,12 +1034,19 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
             sh_fill = phaseC(t, spectrumSum / (float)(unsigned)total, fracInc, badBands, dataModel, margin, 1);
         }
         __syncthreads();
-        // last-good spectrum (:441-469); every change starts a new version
-        // for the spectra that follow (the apply pass replaces from it)
+        // last-good spectrum (:441-469): a clipped bin now holds its last-good
+        // value, so only kept non-zero bins can fill an empty slot; every
+        // change starts a new version for the spectra that follow (the apply
+        // pass replaces from it)
         if (sh_fill) {
             for (int i = tid; i < nBins; i += kWalkThreads) {
-                const int sb = i / nCh, c = i % nCh;
-                const float v = I[idx(sb, 0, t) + c];
+                float v;
+                if (oneChunk) {
+                    v = rowChunk[i];
+                } else {
+                    const int sb = i / nCh, c = i % nCh;
+                    v = I[idx(sb, 0, t) + c];
+                }
                 if (lastGood[i] == 0.0f && v != 0.0f) {
                     lastGood[i] = v;
                     atomicAdd(&fillCnt, 1);
@@ -752,121 +1071,182 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         t = sh_t;
     }
     if (tid == 0) {
-        if (sh_stop) {
+        if (sh_stop == 1) {
             ++s.count[CNT_STOP];
             s.boxGrow = fminf(s.boxGrow * 4.0f, kMaxBoxGrow);
-        } else {
+        } else if (sh_stop == 0) {
             s.boxGrow = fmaxf(s.boxGrow * 0.5f, 1.0f);
         }
         s.tNext = t;
         *S = s;
     }
 }
-constexpr int kApplyTile = 64;
 
-// Parallel over spectra: clipSample, flatten, ordered mean/RMS, normalise
-// (RFI_Clipper.cpp:127-150, 418-438, 478-532).  A workgroup of 64 lanes owns
-// 64 spectra; bins are staged in 64x64 LDS tiles so that each lane can walk
-// its own spectrum in bin order while global accesses stay coalesced.
-__global__ __launch_bounds__(64) void clip_apply_kernel(float* __restrict__ I, float* __restrict__ W,
-                                                        ClipConst K, const double* __restrict__ terms,
-                                                        const float* __restrict__ lastGoodVer,
-                                                        const SpecRec* __restrict__ rec, int nT)
+// Replacement, clipSample and flattening of one bin of one spectrum
+// (RFI_Clipper.cpp:319-335, 127-150, 478-485): returns the flattened value
+// of polarisation 0; *set = 1 when every polarisation of the bin takes the
+// value *g (replacement or clipSample), 0 when they keep their values.
+__device__ __forceinline__ float flatten_bin(float v, float bpv, const SpecRec& r, const float* __restrict__ lgv,
+                                             int nBins, int i, float* g, int* set)
 {
-    __shared__ float tile[kApplyTile][kApplyTile + 1];
-    __shared__ SpecRec recs[kApplyTile];
-    __shared__ float meanS[kApplyTile], rmsS[kApplyTile];
-    const int lane = threadIdx.x;
-    const int t0 = blockIdx.x * kApplyTile;
-    const int nBins = K.nBins, nCh = K.nCh, nSub = K.nSub, nPol = K.nPol;
-    auto idx = [&](int sb, int pol, int t) -> long { return (long)nCh * (nPol * ((long)nSub * t + sb) + pol); };
-    if (t0 + lane < nT) recs[lane] = rec[t0 + lane];
-    __syncthreads();
-    const int nRows = min(kApplyTile, nT - t0);
+    *set = 0;
+    if ((r.flags & 3) == 1) {
+        // kept spectrum the walk judged from the precomputed mask: same test,
+        // replacement by the last-good value in force for it
+        if (v - r.dataModel - bpv > r.margin) {
+            v = lgv[(size_t)r.ver * nBins + i];
+            *g = v;
+            *set = 1;
+        }
+    }
+    if (!(r.flags & 1)) {
+        v = -0.2f;
+        *g = v;
+        *set = 1;
+    }
+    return v - (bpv + r.dataModel);
+}
 
-    // channel clipping of kept spectra the walk judged from the precomputed
-    // mask (:319-335): same test, replacement by the (complete) last-good value
-    for (int r = 0; r < nRows; ++r) {
-        if ((recs[r].flags & 3) == 1) {
-            const float dm = recs[r].dataModel, margin = recs[r].margin, p0 = recs[r].params0;
-            for (int i = lane; i < nBins; i += 64) {
-                const int sb = i / nCh, c = i % nCh;
-                const long gi = idx(sb, 0, t0 + r) + c;
-                const float bpv = bp_value(terms, K.nTerms, nBins, i, p0, K.scale);
-                if (I[gi] - dm - bpv > margin) {
-                    const float g = lastGoodVer[(size_t)recs[r].ver * nBins + i];
-                    for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t0 + r) + c] = g;
-                }
-            }
-        }
-    }
-    // clipSample for rejected spectra: every pol and bin = -0.2
-    for (int r = 0; r < nRows; ++r) {
-        if (!(recs[r].flags & 1)) {
-            for (int pol = 0; pol < nPol; ++pol)
-                for (int i = lane; i < nBins; i += 64) {
-                    const int sb = i / nCh, c = i % nCh;
-                    I[idx(sb, pol, t0 + r) + c] = -0.2f;
-                }
-        }
-    }
-    if (W) {
-        for (int r = 0; r < nRows; ++r)
-            for (int pol = 0; pol < nPol; ++pol)
-                for (int i = lane; i < nBins; i += 64) {
-                    const int sb = i / nCh, c = i % nCh;
-                    W[idx(sb, pol, t0 + r) + c] = 1.0f;
-                }
-    }
+// Statistics pass over the flattened spectra: mean and rms of each spectrum
+// from its ordered sums (RFI_Clipper.cpp:486-501), row-tiled as the
+// precompute.  Writes stats[t] = (mean, rms).
+template <int NT>
+__global__ __launch_bounds__(kTileThreads) void clip_stats_kernel(const float* __restrict__ I, ClipConst K,
+                                                                  const double* __restrict__ terms,
+                                                                  const float* __restrict__ lastGoodVer,
+                                                                  const SpecRec* __restrict__ rec,
+                                                                  float2* __restrict__ stats, int nT, int ldMode)
+{
+    __shared__ float vals[2][kTileRows][kTileBins + 1];
+    __shared__ SpecRec recs[kTileRows];
+    const int t0 = blockIdx.x * kTileRows;
+    const int nRows = min(kTileRows, nT - t0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
+    const long rowStride = (long)nBins * nPol;
+    const int nTiles = (nBins + kTileBins - 1) / kTileBins;
+    if (tid < nRows) recs[tid] = rec[t0 + tid];
     __syncthreads();
-    // flatten pol 0 and take the ordered sums of each spectrum
-    float sum = 0.0f, sumSq = 0.0f;
-    for (int b0 = 0; b0 < nBins; b0 += kApplyTile) {
-        const int nb = min(kApplyTile, nBins - b0);
-        // load + flatten, coalesced along bins
-        for (int r = 0; r < nRows; ++r) {
-            if (lane < nb) {
-                const int i = b0 + lane;
-                const int sb = i / nCh, c = i % nCh;
-                const long gi = idx(sb, 0, t0 + r) + c;
-                const float bpv = bp_value(terms, K.nTerms, nBins, i, recs[r].params0, K.scale);
-                const float v = I[gi] - (bpv + recs[r].dataModel);
-                I[gi] = v;
-                tile[r][lane] = v;
+
+    float4 raw[kRowsPerWave];
+    auto issue = [&](int tile) {
+        const int i = tile * kTileBins + lane * 4;
+        if (i < nBins) {
+#pragma unroll
+            for (int q = 0; q < kRowsPerWave; ++q) {
+                const int r = wave + 4 * q;
+                if (r < nRows) raw[q] = load_quad(I, (long)(t0 + r) * rowStride, i, nCh, nPol, ldMode);
             }
         }
-        __syncthreads();
-        if (lane < nRows) {
+    };
+    auto flatten = [&](int tile, int buf) {
+        const int i = tile * kTileBins + lane * 4;
+        if (i >= nBins) return;
+        BinTerms<NT> bt[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bt[j].load(terms, K.nTerms, nBins, i + j);
+#pragma unroll
+        for (int q = 0; q < kRowsPerWave; ++q) {
+            const int r = wave + 4 * q;
+            if (r >= nRows) break;
+            const SpecRec sr = recs[r];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float g;
+                int set;
+                const float bpv = bt[j].bp(sr.params0, K.scale);
+                vals[buf][r][lane * 4 + j] = flatten_bin(q_at(raw[q], j), bpv, sr, lastGoodVer, nBins, i + j, &g, &set);
+            }
+        }
+    };
+
+    float sum = 0.0f, sumSq = 0.0f;
+    issue(0);
+    flatten(0, 0);
+    __syncthreads();
+    for (int tile = 0; tile < nTiles; ++tile) {
+        const int buf = tile & 1;
+        if (tile + 1 < nTiles) issue(tile + 1);
+        if (wave == 0 && lane < nRows) {
+            const int nb = min(kTileBins, nBins - tile * kTileBins);
+            const float* row = &vals[buf][lane][0];
+#pragma unroll 8
             for (int k = 0; k < nb; ++k) {
-                const float v = tile[lane][k];
+                const float v = row[k];
                 sum += v;
                 sumSq += v * v;
             }
         }
+        if (tile + 1 < nTiles) flatten(tile + 1, buf ^ 1);
         __syncthreads();
     }
-    if (lane < nRows) {
+    if (wave == 0 && lane < nRows) {
         const float mean = sum / (float)(unsigned)nBins;
         float rms = (float)sqrt((double)(sumSq / (float)(unsigned)nBins) - (double)mean * (double)mean);
         if (rms == 0.0f) rms = 1.0f;
-        meanS[lane] = mean;
-        rmsS[lane] = rms;
+        stats[t0 + lane] = make_float2(mean, rms);
+    }
+}
+
+constexpr int kWriteRows = 16;
+
+// Elementwise pass: replacement / clipSample on every polarisation, the
+// flattened polarisation 0 normalised for kept spectra (:502-532), weights.
+// Threads own 4 consecutive bins; a workgroup walks kWriteRows spectra.
+template <int NT>
+__global__ __launch_bounds__(256) void clip_write_kernel(float* __restrict__ I, float* __restrict__ W, ClipConst K,
+                                                         const double* __restrict__ terms,
+                                                         const float* __restrict__ lastGoodVer,
+                                                         const SpecRec* __restrict__ rec,
+                                                         const float2* __restrict__ stats, int nT, int ldMode)
+{
+    __shared__ SpecRec recs[kWriteRows];
+    __shared__ float2 st[kWriteRows];
+    const int tid = threadIdx.x;
+    const int t0 = blockIdx.y * kWriteRows;
+    const int nRows = min(kWriteRows, nT - t0);
+    if (tid < nRows) {
+        recs[tid] = rec[t0 + tid];
+        st[tid] = stats[t0 + tid];
     }
     __syncthreads();
-    // normalise kept spectra
+    const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
+    const int i = (blockIdx.x * 256 + tid) * 4;
+    if (i >= nBins) return;
+    const long rowStride = (long)nBins * nPol;
+    BinTerms<NT> bt[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bt[j].load(terms, K.nTerms, nBins, i + j);
     for (int r = 0; r < nRows; ++r) {
-        if (recs[r].flags & 1) {
-            const float sub = K.zeroDM == 1 ? (float)K.zeroDM * meanS[r] : recs[r].meanRunAve;
-            const float rms = rmsS[r];
-            for (int i = lane; i < nBins; i += 64) {
-                const int sb = i / nCh, c = i % nCh;
-                const long gi = idx(sb, 0, t0 + r) + c;
-                float v = I[gi];
+        const long rowOff = (long)(t0 + r) * rowStride;
+        const SpecRec sr = recs[r];
+        const float4 v4 = load_quad(I, rowOff, i, nCh, nPol, ldMode);
+        float4 o, g4;
+        int set[4];
+        const bool kept = sr.flags & 1;
+        const float sub = K.zeroDM == 1 ? (float)K.zeroDM * st[r].x : sr.meanRunAve;
+        const float rms = st[r].y;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float bpv = bt[j].bp(sr.params0, K.scale);
+            float v = flatten_bin(q_at(v4, j), bpv, sr, lastGoodVer, nBins, i + j, &q_at(g4, j), &set[j]);
+            if (kept) {
                 v -= sub;
                 v /= rms;
                 if (v > K.crFactor) v = 0.0f;
-                I[gi] = v;
             }
+            q_at(o, j) = v;
+        }
+        store_quad(I, rowOff, i, nCh, nPol, ldMode, o);
+        if (nPol > 1 && (set[0] | set[1] | set[2] | set[3])) {
+            for (int pol = 1; pol < nPol; ++pol)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (set[j]) I[rowOff + (long)pol * nCh + bin_offset(i + j, nCh, nPol)] = q_at(g4, j);
+        }
+        if (W) {
+            const float4 one = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            for (int pol = 0; pol < nPol; ++pol) store_quad(W + (long)pol * nCh, rowOff, i, nCh, nPol, ldMode, one);
         }
     }
 }
@@ -882,7 +1262,7 @@ struct ppl_clipper {
     ClipScalars* d_S = nullptr;
     float *d_meanBuf = nullptr, *d_rmsBuf = nullptr, *d_lastGood = nullptr;
     double *d_meanPrefix = nullptr, *d_rmsPrefix = nullptr, *d_terms = nullptr;
-    uint8_t* d_mask = nullptr;
+    float2* d_stats = nullptr;
     SpecRec* d_rec = nullptr;
     PreRec* d_pre = nullptr;
     Box* d_box = nullptr;
@@ -903,7 +1283,7 @@ void free_state(ppl_clipper* c)
     (void)hipFree(c->d_meanPrefix);
     (void)hipFree(c->d_rmsPrefix);
     (void)hipFree(c->d_terms);
-    (void)hipFree(c->d_mask);
+    (void)hipFree(c->d_stats);
     (void)hipFree(c->d_rec);
     (void)hipFree(c->d_pre);
     (void)hipFree(c->d_box);
@@ -917,7 +1297,7 @@ void free_state(ppl_clipper* c)
     c->d_S = nullptr;
     c->d_meanBuf = c->d_rmsBuf = c->d_lastGood = nullptr;
     c->d_meanPrefix = c->d_rmsPrefix = c->d_terms = nullptr;
-    c->d_mask = nullptr;
+    c->d_stats = nullptr;
     c->d_rec = nullptr;
 }
 
@@ -974,7 +1354,6 @@ int setup_map(ppl_clipper* c, unsigned nSub, unsigned nPol, unsigned nCh)
     PPL_HIP(hipMalloc(&c->d_meanPrefix, (cap + 1) * sizeof(double)));
     PPL_HIP(hipMalloc(&c->d_rmsPrefix, (cap + 1) * sizeof(double)));
     PPL_HIP(hipMalloc(&c->d_lastGood, nBins * sizeof(float)));
-    PPL_HIP(hipMalloc(&c->d_mask, nBins));
     PPL_HIP(hipMalloc(&c->d_box, sizeof(Box)));
     PPL_HIP(hipMalloc(&c->d_bpLo, nBins * sizeof(float)));
     PPL_HIP(hipMalloc(&c->d_bpHi, nBins * sizeof(float)));
@@ -1031,6 +1410,20 @@ extern "C" int ppl_clipper_destroy(ppl_clipper* c)
     return PPL_OK;
 }
 
+// The two apply passes: statistics of the flattened spectra, then the
+// elementwise replacement / flatten / normalise.
+template <int NT>
+static int launch_apply(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, int ldMode, hipStream_t st)
+{
+    hipLaunchKernelGGL(clip_stats_kernel<NT>, dim3(cdiv(nT, kTileRows)), dim3(kTileThreads), 0, st, d_I, c->K,
+                       c->d_terms, c->d_lastGoodVer, c->d_rec, c->d_stats, (int)nT, ldMode);
+    PPL_LAUNCHED();
+    hipLaunchKernelGGL(clip_write_kernel<NT>, dim3(cdiv((unsigned)c->nBins / 4, 256), cdiv(nT, kWriteRows)), dim3(256),
+                       0, st, d_I, d_W, c->K, c->d_terms, c->d_lastGoodVer, c->d_rec, c->d_stats, (int)nT, ldMode);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
 static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
                        unsigned nCh, hipStream_t st)
 {
@@ -1049,12 +1442,18 @@ static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsi
     if ((int)nT > c->recCap) {
         (void)hipFree(c->d_rec);
         (void)hipFree(c->d_pre);
+        (void)hipFree(c->d_stats);
         c->d_rec = nullptr;
         c->d_pre = nullptr;
+        c->d_stats = nullptr;
         PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * nT));
         PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * std::min((int)nT, kSubLen)));
+        PPL_HIP(hipMalloc(&c->d_stats, sizeof(float2) * nT));
         c->recCap = (int)nT;
     }
+    // access mode of the row-tiled passes (see load_quad)
+    auto aligned = [](const void* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const int ldMode = !(aligned(d_I) && aligned(d_W)) ? 2 : nPol == 1 ? 0 : nCh % 4 == 0 ? 1 : 2;
     // Sub-chunks: box around the state reached, precompute, walk.  The walk
     // advances S->tNext by kSubLen spectra, or less when the state leaves the
     // box; the last launch walks whatever remains.  Launches past the end
@@ -1068,18 +1467,25 @@ static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsi
         hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->K, c->d_terms,
                            c->d_box, c->d_bpLo, c->d_bpHi, (int)nT);
         PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(subLen, 64)), dim3(64), 0, st, d_I, c->K, c->d_S, c->d_box,
-                           c->d_bpLo, c->d_bpHi, c->d_lastGood, c->d_pre, (int)nT, subLen);
+        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(subLen, kTileRows)), dim3(kTileThreads), 0, st, d_I, c->K,
+                           c->d_S, c->d_box, c->d_bpLo, c->d_bpHi, c->d_lastGood, c->d_pre, (int)nT, subLen,
+                           ldMode);
         PPL_LAUNCHED();
         hipLaunchKernelGGL(clip_walk_kernel, dim3(1), dim3(kWalkThreads), 0, st, d_I, c->K, c->d_S, c->d_meanBuf,
                            c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_lastGoodVer,
-                           c->d_terms, c->d_mask, c->d_box, c->d_pre, c->d_rec, (int)nT, subLen,
-                           it == nIter - 1 ? 1 : 0);
+                           c->d_terms, c->d_box, c->d_pre, c->d_rec, (int)nT, subLen, it == nIter - 1 ? 1 : 0);
         PPL_LAUNCHED();
     }
-    hipLaunchKernelGGL(clip_apply_kernel, dim3(cdiv(nT, kApplyTile)), dim3(64), 0, st, d_I, d_W, c->K,
-                       c->d_terms, c->d_lastGoodVer, c->d_rec, (int)nT);
-    PPL_LAUNCHED();
+    int rc = PPL_OK;
+    switch (c->K.nTerms) {
+        case 0: rc = launch_apply<0>(c, d_I, d_W, nT, ldMode, st); break;
+        case 1: rc = launch_apply<1>(c, d_I, d_W, nT, ldMode, st); break;
+        case 2: rc = launch_apply<2>(c, d_I, d_W, nT, ldMode, st); break;
+        case 3: rc = launch_apply<3>(c, d_I, d_W, nT, ldMode, st); break;
+        case 4: rc = launch_apply<4>(c, d_I, d_W, nT, ldMode, st); break;
+        default: rc = launch_apply<-1>(c, d_I, d_W, nT, ldMode, st);
+    }
+    if (rc != PPL_OK) return rc;
     c->lastNT = (int)nT;
     return PPL_OK;
 }
