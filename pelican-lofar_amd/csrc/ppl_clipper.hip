@@ -213,6 +213,7 @@ __device__ __forceinline__ float ring_front(RingView r, int start) { return r.bu
 constexpr int kWalkThreads = 512;
 constexpr int kStage = 512;      // spectra whose precomputed records are staged in LDS at a time
 constexpr int kRowChunk = 8192;  // bins of one spectrum staged in LDS for the ordered sums
+static_assert(kStage == kWalkThreads, "stage scans use one thread per staged spectrum");
 
 // Per-sub-chunk speculation box: every spectrum of the sub-chunk whose
 // bandpass offset lies in [p0lo, p0hi] and whose margin lies in [mlo, mhi]
@@ -228,8 +229,8 @@ struct Box {
 // averages over the history (10000 spectra by default), which move far less
 // than this over a sub-chunk; a state that leaves the box ends the
 // sub-chunk, and the next one starts with a box four times wider.
-constexpr float kBoxP = 0.02f;
-constexpr float kBoxM = 0.005f;
+constexpr float kBoxP = 0.005f;
+constexpr float kBoxM = 0.001f;
 constexpr float kMaxBoxGrow = 64.0f;
 
 // Precomputed per spectrum (steady state, dataModel = 0), for the mask that
@@ -327,29 +328,6 @@ struct BinTerms<-1> {
     }
 };
 
-// Correctly rounded x / d for a divisor d that stays fixed over many
-// divisions, as RN_f32((double)x * RN_f64(1/d)).  For float x and a normal
-// float d the exact quotient is never within 2^-49 (relative) of a float
-// rounding midpoint, while the double product is within 2^-52 of it, so the
-// two roundings agree; quotients in the subnormal range (where that bound
-// does not hold) take the division itself.
-struct FixedDivisor {
-    float d;
-    double rcp;
-    bool ok;
-    __device__ __forceinline__ void set(float divisor)
-    {
-        d = divisor;
-        ok = isfinite(divisor) && fabsf(divisor) >= 0x1p-126f;
-        rcp = 1.0 / (double)divisor;
-    }
-    __device__ __forceinline__ float div(float x) const
-    {
-        if (!ok) return x / d;
-        const float q = (float)((double)x * rcp);
-        return fabsf(q) < 0x1p-125f ? x / d : q;
-    }
-};
 
 __device__ __forceinline__ float block_min(float v, float* red)
 {
@@ -388,6 +366,50 @@ __device__ __forceinline__ void band_stats(const int* cnt, int nBins, int cpb, i
     if ((double)(unsigned)total < 0.8 * (unsigned)nBins) bad += 4;
     badBands = (int)bad;
     fracInc = (float)(unsigned)(nBins - total) / (float)(unsigned)nBins;
+}
+
+// Ordered folds of an LDS row of n floats (n % 8 == 0, n >= 8) by one lane,
+// the reads of the next 8 values issued before the adds of the current 8.
+__device__ __forceinline__ float fold_sum(const float* row, int n, float sum)
+{
+    float a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = row[j];
+    for (int k = 8; k < n; k += 8) {
+        float b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = row[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sum += a[j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += a[j];
+    return sum;
+}
+__device__ __forceinline__ void fold_sum_sq(const float* row, int n, float& sum, float& sumSq)
+{
+    float a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = row[j];
+    for (int k = 8; k < n; k += 8) {
+        float b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = row[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            sum += a[j];
+            sumSq += a[j] * a[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        sum += a[j];
+        sumSq += a[j] * a[j];
+    }
 }
 
 // Band (0..7) of bin i: floor((i + 0.5) / cpb), exact in float while
@@ -538,9 +560,7 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(const float* __r
         if (tile + 1 < nTiles) issue(tile + 1);
         if (wave == 0 && lane < nRows) {
             const int nb = min(kTileBins, nBins - tile * kTileBins);
-            const float* row = &vals[buf][lane][0];
-#pragma unroll 8
-            for (int k = 0; k < nb; ++k) sum += row[k];
+            sum = fold_sum(&vals[buf][lane][0], nb, sum);
         }
         if (tile + 1 < nTiles) judge(tile + 1, buf ^ 1);
         __syncthreads();
@@ -561,14 +581,14 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(const float* __r
     }
 }
 
-// The walk's steady-state scalars in registers (thread 0), with the ring
-// backs also kept divided by the history length.
-struct FastState {
-    float mRA, rA, p0, bpMean, bpRms, sRMS, fbc;
-    int badSpectra, num, meanStart, rmsStart;
-    float mBack, mBack2, rBack, rBack2;
-    float mBackC, mBack2C, rBackC, rBack2C;
+// Staged per spectrum of the walk: the precomputed record, its mean and the
+// two ring fronts, each divided by the (full) history length.
+struct Staged {
+    float mean, fracInc;
+    int badBands, flags;
+    float meanC, meanFC, rmsFC, pad;
 };
+
 
 // The sequential walk over one sub-chunk of spectra.  Thread 0 runs ahead
 // through spectra the precompute decided; the workgroup joins in for the
@@ -592,10 +612,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_stop;
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
-    __shared__ PreRec recs[kStage];
-    // staged per spectrum: ring fronts and the precomputed mean, each divided
-    // by the history length
-    __shared__ float meanFC[kStage], rmsFC[kStage], meanC[kStage];
+    __shared__ Staged stg[kStage];
+    // first spectrum at or after k the fast run must not take: undecided
+    // channels (blkU), or undecided channels or a last-good fill (blkUF)
+    __shared__ int blkU[kStage], blkUF[kStage];
     __shared__ __align__(16) float rowChunk[kRowChunk];
 
     const int tid = threadIdx.x;
@@ -623,10 +643,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
     // start+k (each spectrum evicts one front; a rejection rewrites only the
     // newest entry).
     auto rmsFrontDiv = [&](int t) {
-        return sh_frontsValid ? rmsFC[t - stageBase] : ring_front(rmsR, s.rmsStart) / (float)s.rmsSize;
+        return sh_frontsValid ? stg[t - stageBase].rmsFC : ring_front(rmsR, s.rmsStart) / (float)s.rmsSize;
     };
     auto meanFrontDiv = [&](int t) {
-        return sh_frontsValid ? meanFC[t - stageBase] : ring_front(meanR, s.meanStart) / (float)s.meanSize;
+        return sh_frontsValid ? stg[t - stageBase].meanFC : ring_front(meanR, s.meanStart) / (float)s.meanSize;
     };
     // Steady-state rms update and margin (RFI_Clipper.cpp:288-311), thread 0.
     auto phaseA_steady = [&](int t, float& dataModel, float& margin) {
@@ -712,177 +732,211 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
         // a kept spectrum fills empty last-good slots (:446-469); the
         // precompute flagged candidates against the sub-chunk-start slots,
         // a superset of the empty slots now
-        if (recs[k].flags & 1) return CNT_UNDECIDED;
-        if (s.remainingZeros != 0 && (recs[k].flags & 2)) return CNT_FILL;
+        if (stg[k].flags & 1) return CNT_UNDECIDED;
+        if (s.remainingZeros != 0 && (stg[k].flags & 2)) return CNT_FILL;
         if (s.ver >= kMaxVer) return CNT_VER;
         return -1;
     };
 
-    enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3 };
     const int period = 10 * K.maxHistory;
+    // ---- steady-state fast run (thread 0): spectra t, t+1, ... that the
+    // precompute decided, with the scalars in registers.  Stops before the
+    // first spectrum that is blocked (undecided channels, last-good fill,
+    // outside the precomputed range) or at the end of the stage, or after
+    // the rms update of a spectrum whose state left the box (FAST_BOX, the
+    // margin in sh_margin), or after a statistics reset (FAST_RESET).
+    // Division by a divisor d fixed over the run (history length, mean/rms
+    // ratio, bandpass scale) is taken as RN_f32((double)x * RN_f64(1/d)):
+    // for float x and a normal float d the exact quotient is never within
+    // 2^-49 (relative) of a float rounding midpoint, and the double product
+    // is within 2^-52 of it, so both round to the same float.  Results in
+    // the subnormal range, where that bound does not hold, take the
+    // division itself.  The spectrum tolerance RN_f32(RN_f64(a / sqrtN))
+    // lies within one float step of RN_f32(RN_f64(a * RN_f64(1/sqrtN))),
+    // so the division is needed only when the comparison falls in that step.
+    enum { FAST_LIMIT = 0, FAST_BOX = 1, FAST_RESET = 2 };
+    const double rcpCap = 1.0 / (double)capf;
+    const double rcpScale = 1.0 / (double)K.scale;
+    const double rcpSqrtN = 1.0 / K.sqrtNBins;
+    const bool fastOK = K.cap >= 1000 && K.cap > kStage && isfinite(K.scale) && fabsf(K.scale) >= 0x1p-126f;
+    auto fast_run = [&](int& t) -> int {
+        const int k0 = t - stageBase;
+        const bool zerosLeft = s.remainingZeros != 0;
+        const int kLim = min(min(stageN, tEnd - stageBase),
+                             min(preN - (stageBase - tStart), zerosLeft ? blkUF[k0] : blkU[k0]));
+        if (k0 >= kLim) return FAST_LIMIT;
+        const float mOR = s.meanOverRMS;
+        const bool useMOR = K.useMeanOverRMS != 0;
+        if (useMOR && !(isfinite(mOR) && fabsf(mOR) >= 0x1p-126f)) return FAST_LIMIT;
+        const double rcpMOR = 1.0 / (double)mOR;
+        const int cap = K.cap;
+        float mRA = s.meanRunAve, rA = s.rmsRunAve, p0 = s.params0, bpMean = s.bpMean, bpRms = s.bpRms;
+        float sRMS = s.spectrumRMS, fbc = s.fractionBadChannels;
+        float mBack = s.meanBack, rBack = s.rmsBack, mBack2 = s.meanBack2, rBack2 = s.rmsBack2;
+        float mBackC = mBack / capf, rBackC = rBack / capf;
+        int badSpectra = s.badSpectra, num = s.num, meanStart = s.meanStart, rmsStart = s.rmsStart;
+        int status = FAST_LIMIT;
+        int k = k0;
+        Staged cur = stg[k0];
+        while (k < kLim) {
+            const Staged nxt = stg[min(k + 1, kStage - 1)];  // consumed next iteration
+            // phaseA (:288-311)
+            const float rAn = (rA - cur.rmsFC) + rBackC;
+            float sRMSn = sRMS;
+            if (useMOR) sRMSn = fabsf((float)((double)mRA * rcpMOR));
+            float rBCn = (float)((double)sRMSn * rcpCap);
+            if (!(fabsf(rBCn) >= 0x1p-125f) || (useMOR && !(sRMSn >= 0x1p-125f))) {
+                if (useMOR) sRMSn = fabsf(mRA / mOR);
+                rBCn = sRMSn / capf;
+            }
+            const float margin = K.crFactor * rAn;
+            if (!(p0 >= bx.p0lo && p0 <= bx.p0hi && margin >= bx.mlo && margin <= bx.mhi)) {
+                // commit the rms update only; the workgroup takes it from here
+                rmsBuf[rmsStart] = sRMSn;
+                rmsStart = rmsStart + 1 == cap ? 0 : rmsStart + 1;
+                rBack2 = rBack;
+                rBack = sRMSn;
+                rA = rAn;
+                sRMS = sRMSn;
+                sh_margin = margin;
+                status = FAST_BOX;
+                break;
+            }
+            // phaseC (:345-440)
+            const float mRAn = (mRA - cur.meanFC) + mBackC;
+            const float x = cur.mean - mRAn;
+            const float a32 = K.srFactor * bpRms;
+            const float tolA = (float)((double)a32 * rcpSqrtN);
+            bool gt;
+            if (tolA >= 0x1p-126f && tolA < 0x1p127f) {
+                const bool gtHi = x > __int_as_float(__float_as_int(tolA) + 1);
+                const bool gtLo = x > __int_as_float(__float_as_int(tolA) - 1);
+                gt = gtHi;
+                if (gtLo != gtHi) gt = x > (float)((double)a32 / K.sqrtNBins);
+            } else {
+                gt = x > (float)((double)a32 / K.sqrtNBins);
+            }
+            const bool good = !(gt || (cur.flags & 4));
+            // push, or push then pop_back + push_back(back()): the slot ends up
+            // with the new value or with the previous back
+            rmsBuf[rmsStart] = good ? sRMSn : rBack;
+            meanBuf[meanStart] = good ? cur.mean : mBack;
+            rmsStart = rmsStart + 1 == cap ? 0 : rmsStart + 1;
+            meanStart = meanStart + 1 == cap ? 0 : meanStart + 1;
+            rBack2 = rBack;
+            mBack2 = mBack;
+            if (good) {
+                rBack = sRMSn;
+                rBackC = rBCn;
+                mBack = cur.mean;
+                mBackC = cur.meanC;
+            }
+            badSpectra += good ? 0 : 1;
+            fbc += cur.fracInc;
+            SpecRec rr;
+            rr.params0 = p0;
+            rr.dataModel = 0.0f;
+            rr.margin = margin;
+            rr.meanRunAve = mRAn;
+            rr.flags = good ? 1 : 0;
+            rr.ver = s.ver;
+            rec[t] = rr;
+            int reset = 0;
+            if (num == 0) {
+                const float fractionBadSpectra =
+                    (float)(100.0 * (double)(float)badSpectra / (double)(float)(unsigned)period);
+                if (fractionBadSpectra > 99.0) reset = 1;
+                badSpectra = 0;
+                fbc = 0.0f;
+            }
+            // phaseD
+            const float delta = mRAn - bpMean;
+            if (fabsf(delta) > 0.0f) {
+                float qd = (float)((double)delta * rcpScale);
+                if (!(fabsf(qd) >= 0x1p-125f)) qd = delta / K.scale;
+                p0 += qd;
+                bpMean = mRAn;
+            }
+            bpRms = rAn;
+            num = num + 1 == period ? 0 : num + 1;
+            rA = rAn;
+            mRA = mRAn;
+            sRMS = sRMSn;
+            ++k;
+            ++t;
+            cur = nxt;
+            if (reset) {
+                status = FAST_RESET;
+                break;
+            }
+        }
+        s.count[CNT_FAST] += (t - stageBase) - k0;
+        s.meanRunAve = mRA;
+        s.rmsRunAve = rA;
+        s.params0 = p0;
+        s.bpMean = bpMean;
+        s.bpRms = bpRms;
+        s.spectrumRMS = sRMS;
+        s.fractionBadChannels = fbc;
+        s.badSpectra = badSpectra;
+        s.num = num;
+        s.meanStart = meanStart;
+        s.rmsStart = rmsStart;
+        s.meanBack = mBack;
+        s.rmsBack = rBack;
+        s.meanBack2 = mBack2;
+        s.rmsBack2 = rBack2;
+        if (status == FAST_RESET) {
+            s.rmsSize = 0;
+            s.meanSize = 0;
+            s.rmsStart = 0;
+            s.meanStart = 0;
+        }
+        return status;
+    };
+
+    enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3 };
     int t = tStart;
     for (;;) {
         if (tid == 0) {
             int mode = MODE_END;
-            if (sh_stop) {
-                mode = MODE_END;
-            } else if (sh_frontsValid && bx.valid && s.rmsSize == K.cap && s.meanSize == K.cap) {
-                // ---- steady state, state in registers
-                FixedDivisor byCap, byMOR, byScale;
-                byCap.set(capf);
-                byMOR.set(s.meanOverRMS);
-                byScale.set(K.scale);
-                FastState f;
-                f.mRA = s.meanRunAve;
-                f.rA = s.rmsRunAve;
-                f.p0 = s.params0;
-                f.bpMean = s.bpMean;
-                f.bpRms = s.bpRms;
-                f.sRMS = s.spectrumRMS;
-                f.fbc = s.fractionBadChannels;
-                f.badSpectra = s.badSpectra;
-                f.num = s.num;
-                f.meanStart = s.meanStart;
-                f.rmsStart = s.rmsStart;
-                f.mBack = s.meanBack;
-                f.mBack2 = s.meanBack2;
-                f.rBack = s.rmsBack;
-                f.rBack2 = s.rmsBack2;
-                f.mBackC = byCap.div(f.mBack);
-                f.mBack2C = byCap.div(f.mBack2);
-                f.rBackC = byCap.div(f.rBack);
-                f.rBack2C = byCap.div(f.rBack2);
-                const int cap = K.cap;
-                const bool capGe1000 = cap >= 1000;
-                int nFast = 0, reset = 0;
-                float margin = 0.0f;
-                for (;;) {
-                    if (t >= tEnd) { mode = MODE_END; break; }
-                    if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
-                    const int k = t - stageBase;
-                    // phaseA (:288-311)
-                    f.rA = f.rA - rmsFC[k];
-                    f.rA = f.rA + f.rBackC;
-                    if (K.useMeanOverRMS) f.sRMS = fabsf(byMOR.div(f.mRA));
-                    rmsBuf[f.rmsStart] = f.sRMS;
-                    f.rmsStart = f.rmsStart + 1 == cap ? 0 : f.rmsStart + 1;
-                    f.rBack2 = f.rBack;
-                    f.rBack2C = f.rBackC;
-                    f.rBack = f.sRMS;
-                    f.rBackC = byCap.div(f.sRMS);
-                    margin = K.crFactor * f.rA;
-                    const bool inBox = t - tStart < preN && f.p0 >= bx.p0lo && f.p0 <= bx.p0hi &&
-                                       margin >= bx.mlo && margin <= bx.mhi;
-                    const PreRec pr = recs[k];
-                    if (!inBox || (pr.flags & 1) || ((pr.flags & 2) && s.remainingZeros != 0) || s.ver >= kMaxVer) {
+            for (;;) {
+                if (sh_stop || t >= tEnd) { mode = MODE_END; break; }
+                if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
+                if (s.rmsSize < K.cap) {  // learning phase
+                    ++s.count[CNT_LEARN];
+                    mode = MODE_FULL;
+                    break;
+                }
+                if (fastOK && bx.valid && sh_frontsValid && s.meanSize == K.cap && s.ver < kMaxVer) {
+                    const int r = fast_run(t);
+                    if (r == FAST_BOX) {
+                        // spectrum t had its rms update and left the box
+                        const int why = slow_reason(t, sh_margin);
+                        ++s.count[why < 0 ? CNT_BOX : why];
+                        sh_dataModel = 0.0f;
                         mode = MODE_BC;
                         break;
                     }
-                    // phaseC (:345-440)
-                    f.fbc += pr.fracInc;
-                    f.mRA = f.mRA - meanFC[k];
-                    f.mRA = f.mRA + f.mBackC;
-                    meanBuf[f.meanStart] = pr.mean;
-                    f.meanStart = f.meanStart + 1 == cap ? 0 : f.meanStart + 1;
-                    f.mBack2 = f.mBack;
-                    f.mBack2C = f.mBackC;
-                    f.mBack = pr.mean;
-                    f.mBackC = meanC[k];
-                    const float tol = (float)((double)(K.srFactor * f.bpRms) / K.sqrtNBins);
-                    int good = capGe1000 ? 1 : 0;
-                    if (capGe1000 && (pr.mean - f.mRA > tol || pr.badBands >= 4)) {
-                        good = 0;
-                        ++f.badSpectra;
-                        // pop_back + push_back(back()) on both rings
-                        meanBuf[(f.meanStart == 0 ? cap : f.meanStart) - 1] = f.mBack2;
-                        f.mBack = f.mBack2;
-                        f.mBackC = f.mBack2C;
-                        rmsBuf[(f.rmsStart == 0 ? cap : f.rmsStart) - 1] = f.rBack2;
-                        f.rBack = f.rBack2;
-                        f.rBackC = f.rBack2C;
-                    }
-                    SpecRec r;
-                    r.params0 = f.p0;
-                    r.dataModel = 0.0f;
-                    r.margin = margin;
-                    r.flags = good;
-                    r.ver = s.ver;
-                    if (f.num == 0) {
-                        const float fractionBadSpectra =
-                            (float)(100.0 * (double)(float)f.badSpectra / (double)(float)(unsigned)period);
-                        if (fractionBadSpectra > 99.0) reset = 1;
-                        f.badSpectra = 0;
-                        f.fbc = 0.0f;
-                    }
-                    r.meanRunAve = f.mRA;
-                    rec[t] = r;
-                    // phaseD
-                    const float delta = f.mRA - f.bpMean;
-                    if (fabsf(delta) > 0.0f) {
-                        f.p0 += byScale.div(delta);
-                        f.bpMean = f.mRA;
-                    }
-                    f.bpRms = f.rA;
-                    f.num = f.num + 1 == period ? 0 : f.num + 1;
-                    ++nFast;
-                    ++t;
-                    if (reset) { mode = MODE_STAGE; break; }  // back to learning: restage
+                    if (r == FAST_RESET || t >= tEnd || t >= stageBase + stageN) continue;
+                    // otherwise spectrum t is blocked (undecided, fill, not
+                    // precomputed): one spectrum by the general path below
                 }
-                s.meanRunAve = f.mRA;
-                s.rmsRunAve = f.rA;
-                s.params0 = f.p0;
-                s.bpMean = f.bpMean;
-                s.bpRms = f.bpRms;
-                s.spectrumRMS = f.sRMS;
-                s.fractionBadChannels = f.fbc;
-                s.badSpectra = f.badSpectra;
-                s.num = f.num;
-                s.meanStart = f.meanStart;
-                s.rmsStart = f.rmsStart;
-                s.meanBack = f.mBack;
-                s.meanBack2 = f.mBack2;
-                s.rmsBack = f.rBack;
-                s.rmsBack2 = f.rBack2;
-                s.count[CNT_FAST] += nFast;
-                if (reset) {
-                    s.rmsSize = 0;
-                    s.meanSize = 0;
-                    s.rmsStart = 0;
-                    s.meanStart = 0;
-                }
-                if (mode == MODE_BC) {
-                    // spectrum t had its rms update; decide its reason and
-                    // hand it to the workgroup
-                    const int why = slow_reason(t, margin);
-                    ++s.count[why < 0 ? CNT_BOX : why];
-                    sh_dataModel = 0.0f;
+                float dataModel, margin;
+                phaseA_steady(t, dataModel, margin);
+                const int why = slow_reason(t, margin);
+                if (why >= 0) {
+                    ++s.count[why];
+                    sh_dataModel = dataModel;
                     sh_margin = margin;
+                    mode = MODE_BC;
+                    break;
                 }
-            } else {
-                for (;;) {
-                    if (t >= tEnd || sh_stop) { mode = MODE_END; break; }
-                    if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
-                    if (s.rmsSize < K.cap) {  // learning phase
-                        ++s.count[CNT_LEARN];
-                        mode = MODE_FULL;
-                        break;
-                    }
-                    float dataModel, margin;
-                    phaseA_steady(t, dataModel, margin);
-                    const int why = slow_reason(t, margin);
-                    if (why >= 0) {
-                        ++s.count[why];
-                        sh_dataModel = dataModel;
-                        sh_margin = margin;
-                        mode = MODE_BC;
-                        break;
-                    }
-                    const int k = t - stageBase;
-                    phaseC(t, recs[k].mean, recs[k].fracInc, recs[k].badBands, dataModel, margin, 0);
-                    phaseD();
-                    ++s.count[CNT_FAST];
-                    ++t;
-                }
+                const int k = t - stageBase;
+                phaseC(t, stg[k].mean, stg[k].fracInc, stg[k].badBands, dataModel, margin, 0);
+                phaseD();
+                ++s.count[CNT_FAST];
+                ++t;
             }
             sh_mode = mode;
             sh_t = t;
@@ -897,14 +951,41 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
             const int nPre = bx.valid ? min(stageN, preN - (t - tStart)) : 0;
             for (int k = tid; k < nPre; k += kWalkThreads) {
                 const PreRec r = pre[t - tStart + k];
-                recs[k] = r;
-                meanC[k] = r.mean / capf;
+                stg[k].mean = r.mean;
+                stg[k].fracInc = r.fracInc;
+                stg[k].badBands = r.badBands;
+                stg[k].flags = r.flags;
+                stg[k].meanC = r.mean / capf;
+            }
+            for (int k = tid; k < nPre; k += kWalkThreads)
+                if (stg[k].badBands >= 4) stg[k].flags |= 4;
+            {
+                // suffix minima over the stage (one thread per entry)
+                const int k = tid;
+                int bu = k, buf = k;
+                if (k < nPre) {
+                    bu = (stg[k].flags & 1) ? k : kStage;
+                    buf = (stg[k].flags & 3) ? k : kStage;
+                }
+                blkU[k] = bu;
+                blkUF[k] = buf;
+                __syncthreads();
+                for (int off = 1; off < kStage; off <<= 1) {
+                    if (k + off < kStage) {
+                        bu = min(bu, blkU[k + off]);
+                        buf = min(buf, blkUF[k + off]);
+                    }
+                    __syncthreads();
+                    blkU[k] = bu;
+                    blkUF[k] = buf;
+                    __syncthreads();
+                }
             }
             const int fv = s.rmsSize == K.cap && s.meanSize == K.cap && K.cap > kStage;
             if (fv) {
                 for (int k = tid; k < stageN; k += kWalkThreads) {
-                    rmsFC[k] = rmsBuf[(s.rmsStart + k) % K.cap] / capf;
-                    meanFC[k] = meanBuf[(s.meanStart + k) % K.cap] / capf;
+                    stg[k].rmsFC = rmsBuf[(s.rmsStart + k) % K.cap] / capf;
+                    stg[k].meanFC = meanBuf[(s.meanStart + k) % K.cap] / capf;
                 }
             }
             __syncthreads();
@@ -1012,17 +1093,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
                 }
             }
             __syncthreads();
-            if (tid == 0) {
-                int k = 0;
-                for (; k + 4 <= n; k += 4) {
-                    const float4 q = *reinterpret_cast<const float4*>(&rowChunk[k]);
-                    spectrumSum += q.x;
-                    spectrumSum += q.y;
-                    spectrumSum += q.z;
-                    spectrumSum += q.w;
-                }
-                for (; k < n; ++k) spectrumSum += rowChunk[k];
-            }
+            if (tid == 0) spectrumSum = fold_sum(rowChunk, n, spectrumSum);
             if (!oneChunk) __syncthreads();
         }
         if (tid == 0) {
@@ -1169,13 +1240,7 @@ __global__ __launch_bounds__(kTileThreads) void clip_stats_kernel(const float* _
         if (tile + 1 < nTiles) issue(tile + 1);
         if (wave == 0 && lane < nRows) {
             const int nb = min(kTileBins, nBins - tile * kTileBins);
-            const float* row = &vals[buf][lane][0];
-#pragma unroll 8
-            for (int k = 0; k < nb; ++k) {
-                const float v = row[k];
-                sum += v;
-                sumSq += v * v;
-            }
+            fold_sum_sq(&vals[buf][lane][0], nb, sum, sumSq);
         }
         if (tile + 1 < nTiles) flatten(tile + 1, buf ^ 1);
         __syncthreads();
