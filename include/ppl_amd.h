@@ -174,6 +174,12 @@ int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float s
                   int32_t* h_ev_ints, float* h_ev_vals, int max_events, int* count,
                   void* stream);
 
+/* u8-mode sums as dedispersed values: out = float(S - 128*nChan) * scale,
+ * the values ppl_detect_u8 compares (no reference counterpart: the
+ * reference has no quantised mode). */
+int ppl_dequantise_sums(const int32_t* d_in, float* d_out, size_t n, unsigned nChan, float scale,
+                        void* stream);
+
 /* ------------------------------------------------------------ channeliser */
 
 typedef struct ppl_ppf_plan ppl_ppf_plan;

@@ -99,6 +99,7 @@ SIGNATURES = {
     "ppl_ddbuf_fill": (u, [vp]),
     "ppl_detect": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_detect_u8": (i, [vp, i, i, i, f, i, i, f, u, vp, vp, i, vp, vp]),
+    "ppl_dequantise_sums": (i, [vp, vp, sz, u, f, vp]),
     "ppl_ppf_coefficients": (i, [u, u, i, vp]),
     "ppl_ppf_plan_create": (i, [vp, vp]),
     "ppl_ppf_plan_destroy": (i, [vp]),

@@ -199,3 +199,24 @@ extern "C" int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nCh
     return run_detect<int32_t>(d_data, tdms, nsamp, 128 * nChan, scale, nChannels, nSubbands, threshold, binPow2,
                                h_ev_ints, h_ev_vals, max_events, count, stream_of(stream));
 }
+
+namespace {
+__global__ void dequantise_kernel(const int32_t* __restrict__ in, float* __restrict__ out, long n, int offset,
+                                  float scale)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)(in[i] - offset) * scale;
+}
+}  // namespace
+
+extern "C" int ppl_dequantise_sums(const int32_t* d_in, float* d_out, size_t n, unsigned nChan, float scale,
+                                   void* stream)
+{
+    PPL_CHECK_ARG(d_in && d_out, "null argument");
+    if (n == 0) return PPL_OK;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(dequantise_kernel, dim3(blocks), dim3(256), 0, stream_of(stream), d_in, d_out, (long)n,
+                       (int)(128 * nChan), scale);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
