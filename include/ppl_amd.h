@@ -180,6 +180,23 @@ int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float s
 int ppl_dequantise_sums(const int32_t* d_in, float* d_out, size_t n, unsigned nChan, float scale,
                         void* stream);
 
+/* ---------------------------------------------------------------- ingest */
+
+/* AdapterTimeSeriesDataSet::_readData (src/lib/src/AdapterTimeSeriesDataSet.cpp:
+ * 217-293), 8- and 16-bit branches: nPackets packets, each packetBytes long
+ * with its payload after headerBytes, payload samples [sub][t][pol] complex
+ * integers (re, im) of sampleBits bits, nSamplesPerPacket samples per packet.
+ * ppl_ingest_c32 writes the TimeSeriesDataSetC32 blob [sub][pol][block][t]
+ * (complex float, nPackets*nSamplesPerPacket samples per (sub, pol)); the
+ * int16 variant writes the fused channeliser's [sub][pol][t] complex int16
+ * input (exact for both sample widths). */
+int ppl_ingest_c32(const void* d_packets, size_t packetBytes, unsigned headerBytes, unsigned nPackets,
+                   unsigned nSub, unsigned nSamplesPerPacket, unsigned nPol, int sampleBits,
+                   float* d_blob, void* stream);
+int ppl_ingest_i16(const void* d_packets, size_t packetBytes, unsigned headerBytes, unsigned nPackets,
+                   unsigned nSub, unsigned nSamplesPerPacket, unsigned nPol, int sampleBits,
+                   int16_t* d_volt, void* stream);
+
 /* ------------------------------------------------------------ channeliser */
 
 typedef struct ppl_ppf_plan ppl_ppf_plan;

@@ -100,6 +100,8 @@ SIGNATURES = {
     "ppl_detect": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_detect_u8": (i, [vp, i, i, i, f, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_dequantise_sums": (i, [vp, vp, sz, u, f, vp]),
+    "ppl_ingest_c32": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
+    "ppl_ingest_i16": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
     "ppl_ppf_coefficients": (i, [u, u, i, vp]),
     "ppl_ppf_plan_create": (i, [vp, vp]),
     "ppl_ppf_plan_destroy": (i, [vp]),
@@ -321,6 +323,25 @@ def detect_u8(d_data, n_chan, scale, n_channels, n_subbands, threshold=6.0, bin_
 
 # -------------------------------------------------------------- channeliser
 WINDOWS = {"kaiser": 0, "gaussian": 1, "blackman": 2, "hamming": 3, "hann": 4}
+
+
+def ingest(d_packets, n_packets, n_sub, n_spp, n_pol=2, sample_bits=16, header_bytes=0, packet_bytes=None,
+           out="c32", stream=None):
+    """Packet payloads -> [sub][pol][t] samples (complex float or int16 pairs)
+    on the device (AdapterTimeSeriesDataSet::_readData)."""
+    import torch
+    payload = n_sub * n_spp * n_pol * 2 * (sample_bits // 8)
+    pb = packet_bytes if packet_bytes is not None else header_bytes + payload
+    n_samp = n_spp * n_packets
+    if out == "c32":
+        o = torch.empty((n_sub, n_pol, n_samp, 2), dtype=torch.float32, device=d_packets.device)
+        check(lib().ppl_ingest_c32(_ptr(d_packets), pb, header_bytes, n_packets, n_sub, n_spp, n_pol,
+                                   sample_bits, _ptr(o), _stream(stream)))
+    else:
+        o = torch.empty((n_sub, n_pol, n_samp, 2), dtype=torch.int16, device=d_packets.device)
+        check(lib().ppl_ingest_i16(_ptr(d_packets), pb, header_bytes, n_packets, n_sub, n_spp, n_pol,
+                                   sample_bits, _ptr(o), _stream(stream)))
+    return o
 
 
 def ppf_coefficients(n_taps, n_ch, window="kaiser"):

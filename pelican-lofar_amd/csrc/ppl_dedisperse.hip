@@ -51,6 +51,19 @@ template <> struct Acc<uint8_t> { using type = int32_t; };
 // staging step also writes (offs[c][j] = delay[dm0+j][c] - delay[dm0][c] +
 // misalignment + row base).  Every LDS read of a channel for one DM trial is
 // issued before any of them is consumed.
+// Workgroups are dispatched to the eight XCDs round robin (workgroup i to
+// XCD i % 8), each XCD with its own L2.  The grid is XCD-major: XCD x gets
+// whole time tiles (x, x+8, ...), and the DM tiles of one time tile follow
+// one another on it, so their overlapping delayed input rows are fetched
+// from HBM about once per time tile instead of once per XCD.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void tile_of(int bid, int nDMTiles, int& tTile, int& dmTile)
+{
+    const int xcd = bid % kXcds, k = bid / kXcds;
+    tTile = (k / nDMTiles) * kXcds + xcd;
+    dmTile = k % nDMTiles;
+}
+
 template <typename Tin, int ND, int NT>
 __global__ __launch_bounds__(256) void dedisperse_kernel(
     const Tin* __restrict__ in, typename Acc<Tin>::type* __restrict__ out,
@@ -67,10 +80,10 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
     Tin* rows0 = reinterpret_cast<Tin*>(smem_raw);
     int* offs0 = reinterpret_cast<int*>(smem_raw + 2 * CB * rowBytes);
 
-    const int bid = blockIdx.x;
-    const int dmTile = bid % nDMTiles;
-    const int tTile = bid / nDMTiles;
+    int tTile, dmTile;
+    tile_of(blockIdx.x, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
+    if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -166,10 +179,10 @@ __global__ __launch_bounds__(256) void dedisperse_f16_kernel(
     uint32_t* lds32 = reinterpret_cast<uint32_t*>(smem_raw);
     const int bufDw = CB * (2 * rowDw + TD);
 
-    const int bid = blockIdx.x;
-    const int dmTile = bid % nDMTiles;
-    const int tTile = bid / nDMTiles;
+    int tTile, dmTile;
+    tile_of(blockIdx.x, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
+    if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -210,10 +223,9 @@ __global__ __launch_bounds__(256) void dedisperse_f16_kernel(
     // copy B: dword m holds elements 2m+1, 2m+2 of the row
     auto shift_copy = [&](int buf, int nc) {
         uint32_t* rows = rowsOf(buf);
-        for (int k = tid; k < nc * (rowDw - 1); k += 256) {
-            const int cc = k / (rowDw - 1), m = k % (rowDw - 1);
+        for (int cc = wave; cc < nc; cc += 4) {
             uint32_t* A = rows + (size_t)cc * 2 * rowDw;
-            A[rowDw + m] = __builtin_amdgcn_alignbyte(A[m + 1], A[m], 2);
+            for (int m = lane; m < rowDw - 1; m += 64) A[rowDw + m] = __builtin_amdgcn_alignbyte(A[m + 1], A[m], 2);
         }
     };
 
@@ -296,10 +308,10 @@ __global__ __launch_bounds__(256) void dedisperse_u8_kernel(
     uint32_t* lds32 = reinterpret_cast<uint32_t*>(smem_raw);
     const int bufDw = CB * (4 * rowDw + TD);
 
-    const int bid = blockIdx.x;
-    const int dmTile = bid % nDMTiles;
-    const int tTile = bid / nDMTiles;
+    int tTile, dmTile;
+    tile_of(blockIdx.x, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
+    if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -355,13 +367,14 @@ __global__ __launch_bounds__(256) void dedisperse_u8_kernel(
     };
     auto shift_copies = [&](int buf, int nc) {
         uint32_t* rows = rowsOf(buf);
-        for (int k = tid; k < nc * (rowDw - 1); k += 256) {
-            const int cc = k / (rowDw - 1), m = k % (rowDw - 1);
+        for (int cc = wave; cc < nc; cc += 4) {
             uint32_t* A = rows + (size_t)cc * 4 * rowDw;
-            const uint32_t a0 = A[m], a1 = A[m + 1];
-            A[rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 1);
-            A[2 * rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 2);
-            A[3 * rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 3);
+            for (int m = lane; m < rowDw - 1; m += 64) {
+                const uint32_t a0 = A[m], a1 = A[m + 1];
+                A[rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 1);
+                A[2 * rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 2);
+                A[3 * rowDw + m] = __builtin_amdgcn_alignbyte(a1, a0, 3);
+            }
         }
     };
 
@@ -506,7 +519,8 @@ int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT
     CB = std::max(CB, 1);
     const int nDMTiles = (tdms + TD - 1) / TD;
     const int nTTiles = (nOut + TT - 1) / TT;
-    dim3 grid((unsigned)nDMTiles * nTTiles), block(256);
+    const int nTTilesPad = (nTTiles + kXcds - 1) / kXcds * kXcds;
+    dim3 grid((unsigned)nDMTiles * nTTilesPad), block(256);
     const size_t lds = (size_t)CB * perChannel;
     TA* o = static_cast<TA*>(out);
     if constexpr (std::is_same<Tin, _Float16>::value) {
