@@ -56,13 +56,14 @@ def parse():
     ap.add_argument("--dedisp-dtype", default="f32", choices=["f32", "f16", "u8"])
     ap.add_argument("--chunks", type=int, default=2, help="distinct input chunks kept in HBM")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP16 / u8 comparison chains")
     ap.add_argument("--tile", default="", help="DM,T,chanblock tile for dedispersion (default: autotune)")
     ap.add_argument("--cpu-spectra", type=int, default=8192,
                     help="integrated spectra in the timed CPU sample")
     return ap.parse_args()
 
 
-def gen_chunk(torch, ppl_coeffs, n_samp, sample0, seed, pulses, rfi):
+def gen_chunk(torch, ppl_coeffs, n_samp, sample0, seed, pulses, rfi, impulses=True):
     """int16 voltages [sub][pol][n_samp][2] on the device."""
     from tests.synth import shift_table, tone_amplitude
     g = torch.Generator(device="cuda").manual_seed(seed)
@@ -70,7 +71,7 @@ def gen_chunk(torch, ppl_coeffs, n_samp, sample0, seed, pulses, rfi):
     shift = torch.from_numpy(shift_table(N_CHAN, FCH1, FOFF)).cuda()
     spi = N_CH * INTEG  # input samples per integrated sample
     n_int = n_samp // spi
-    imp = torch.rand(n_int, generator=g, device="cuda") < 0.005  # broadband impulses
+    imp = torch.rand(n_int, generator=g, device="cuda") < (0.005 if impulses else 0.0)  # broadband impulses
     imp_idx = torch.nonzero(imp).flatten()
     for s in range(N_SUB):
         x = torch.randn((2, n_samp, 2), generator=g, device="cuda") * SIGMA
@@ -103,6 +104,50 @@ def gen_chunk(torch, ppl_coeffs, n_samp, sample0, seed, pulses, rfi):
     return out
 
 
+def traffic_from_profile(dtype):
+    """HBM bytes per dedispersion launch from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy of known size),
+    or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "dedisperse_lofar_beam_counters.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                w = ln.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == dtype:
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
+def run_steps(chain, chunks, n, timing=None, graph=False):
+    """n steps of a chain over the chunk ring; returns per-step event lists."""
+    import torch
+    evs = []
+    for k in range(n):
+        ev = None
+        if timing is not None:
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
+            ev[0].record()
+        if graph:
+            chain.replay()
+            if ev is not None:
+                for e in ev[1:6]:
+                    e.record()
+        else:
+            chain.step(chunks[k % len(chunks)], None if ev is None else ev[1:6])
+        ints, vals, _ = chain.detect()
+        if timing is not None:
+            ev[6].record()
+            timing.append(ev)
+        evs.append((ints, vals))
+    return evs
+
+
+def event_keys(ints):
+    return {tuple(r) for r in ints[1:]}  # without the dummy event
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -117,122 +162,156 @@ def main():
 
     ppl = ge.load_package()
     ppl.init(0)
-    dev = torch.device("cuda:0")
 
     coeffs = ppl.ppf_coefficients(N_TAPS, N_CH, "hann")
     e_h = float(np.sum(coeffs * coeffs))
     level = 2 * INTEG * 2.0 * SIGMA * SIGMA * e_h          # mean integrated Stokes-I
     noise_std = np.sqrt(2 * INTEG) * 2.0 * SIGMA * SIGMA * e_h
     bp = np.array([N_CHAN, FCH1, FCH1 + FOFF * (N_CHAN - 1), FOFF, noise_std, level], np.float64)
+    inv_scale = 127.0 / 10.0  # u8: the clipper's normalised range (crFactor = 10 sigma) over 127 steps
+
+    def make_chain(dtype):
+        tile = [int(v) for v in args.tile.split(",")] if args.tile and dtype == args.dedisp_dtype else None
+        return ppl.BeamChain(N_SUB, N_CH, N_TAPS, coeffs, INTEG, bp, NSAMP_BUF, TDMS, 0.0, DM_STEP, TSAMP,
+                             FCH1, FOFF, dtype=dtype, inv_scale=inv_scale, tile=tile)
 
     dd_dtype = args.dedisp_dtype
-    inv_scale = 127.0 / 10.0  # u8: clipper's normalised range (crFactor = 10 sigma) over 127 steps
-    plan = ppl.DedispPlan(N_CHAN, NSAMP_BUF, TDMS, 0.0, DM_STEP, TSAMP, FCH1, FOFF, dtype=dd_dtype)
-    nout, overlap = plan.nout, plan.maxshift + plan.rem
-    ppf = ppl.PPF(N_SUB, N_CH, N_TAPS, coeffs.astype(np.float32), integrate=INTEG)
-    clip = ppl.Clipper(bp, 10.0, 4.0, 10000, True)
-    ddbuf = ppl.DDBuffer(N_CHAN, NSAMP_BUF, overlap, dd_dtype, inv_scale)
+    chain = make_chain(dd_dtype)
+    nout, overlap = chain.nout, chain.overlap
+    n_samp = chain.samples_per_step()
+    def make_chunks(seed, impulses):
+        rng = np.random.default_rng(seed)
+        out = []
+        for ci in range(args.chunks):
+            pulses = []
+            for _ in range(5):
+                dm = rng.uniform(50, 450)
+                width = rng.uniform(1e-3, 8e-3) / TSAMP
+                amp_sig = rng.uniform(6, 12)
+                t0 = rng.uniform(0.05, 0.8) * nout
+                pulses.append((dm, t0, width, tone_amplitude(coeffs, N_CH, SIGMA, amp_sig, INTEG)))
+            rfi = [(int(rng.integers(N_SUB)), int(rng.integers(N_CH)),
+                    tone_amplitude(coeffs, N_CH, SIGMA, 50.0, INTEG)) for _ in range(10)]
+            out.append(gen_chunk(torch, coeffs, n_samp, ci * n_samp, seed + ci, pulses, rfi, impulses))
+        return out
 
-    blk_step = nout * INTEG
-    n_samp = blk_step * N_CH
-    rng = np.random.default_rng(2024)
-    chunks = []
     t_gen = time.time()
-    for ci in range(args.chunks):
-        pulses = []
-        for _ in range(5):
-            dm = rng.uniform(50, 450)
-            width = rng.uniform(1e-3, 8e-3) / TSAMP
-            amp_sig = rng.uniform(6, 12)
-            t0 = rng.uniform(0.05, 0.8) * nout
-            pulses.append((dm, t0, width, tone_amplitude(coeffs, N_CH, SIGMA, amp_sig, INTEG)))
-        rfi = [(int(rng.integers(N_SUB)), int(rng.integers(N_CH)),
-                tone_amplitude(coeffs, N_CH, SIGMA, 50.0, INTEG)) for _ in range(10)]
-        chunks.append(gen_chunk(torch, coeffs, n_samp, ci * n_samp, 100 + ci, pulses, rfi))
-    prime_samples = overlap * INTEG * N_CH
-    prime = chunks[-1][:, :, :prime_samples].contiguous()
+    chunks = make_chunks(2024, True)
+    prime = chunks[-1][:, :, :overlap * INTEG * N_CH].contiguous()
     torch.cuda.synchronize()
     t_gen = time.time() - t_gen
 
-    stokes = torch.empty((nout, N_SUB, 1, N_CH), dtype=torch.float32, device=dev)
-    dd_out = torch.empty((TDMS, nout), dtype=plan.out_dtype(), device=dev)
-    ev_cap = 1 << 22
-
-    # pick the DM x time tile for this shape (the project's autotuner)
-    tune_in = torch.zeros((N_CHAN, NSAMP_BUF), dtype={"f32": torch.float32, "f16": torch.float16,
-                                                      "u8": torch.uint8}[dd_dtype], device=dev)
-    if args.tile:
-        plan.set_tile(*[int(v) for v in args.tile.split(",")])
-        tune_ms = None
-    else:
-        tune_ms = plan.autotune(tune_in, dd_out, reps=1)
-    del tune_in
-
-    # prime the buffer with the overlap so that every step consumes nOut spectra
-    st = ppf.stokes_i16(prime, overlap * INTEG)
-    clip.run(st)
-    consumed, full = ddbuf.add(st.view(overlap, N_CHAN))
-    assert consumed == overlap and not full
-
-    stage_names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "detect"]
-
-    def step(chunk, timing):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stage_names) + 1)]
-        ev[0].record()
-        ppf.stokes_i16(chunk, blk_step, d_out=stokes)
-        ev[1].record()
-        clip.run(stokes)
-        ev[2].record()
-        c, full = ddbuf.add(stokes.view(nout, N_CHAN))
-        assert c == nout and full
-        ev[3].record()
-        plan.run(ddbuf.tensor(), d_out=dd_out)
-        ev[4].record()
-        if dd_dtype == "u8":
-            _, _, n_ev = ppl.detect_u8(dd_out, N_CHAN, 1.0 / inv_scale, N_CH, N_SUB, 6.0, 6, ev_cap)
-        else:
-            _, _, n_ev = ppl.detect(dd_out, N_CH, N_SUB, 6.0, 6, ev_cap)
-        ev[5].record()
-        ddbuf.rotate()
-        if timing is not None:
-            timing.append((ev, n_ev))
-        return n_ev
-
-    for w in range(args.warmup):
-        step(chunks[w % len(chunks)], None)
+    tune_ms = None if args.tile else chain.autotune()
+    chain.prime(prime)
+    run_steps(chain, chunks, args.warmup)
     torch.cuda.synchronize()
+
+    # ---- timed: the plain (FP32) chain, eager
     timing = []
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(chunks[k % len(chunks)], timing)
+    plain_events = run_steps(chain, chunks, args.steps, timing)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-
-    stages = {n: 0.0 for n in stage_names}
-    for ev, _ in timing:
-        for j, n in enumerate(stage_names):
-            stages[n] += ev[j].elapsed_time(ev[j + 1])
-    stages = {n: v / args.steps for n, v in stages.items()}
-    events = [n for _, n in timing]
+    last_plain = chain.out.clone() if dd_dtype == "f32" else None
+    stage_names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "carry", "detect"]
+    stages = {n: sum(ev[j].elapsed_time(ev[j + 1]) for ev in timing) / args.steps
+              for j, n in enumerate(stage_names)}
 
     ms_per_step = 1e3 * wall / args.steps
     value = TDMS * nout * args.steps / wall
     rtf = nout * TSAMP * args.steps / wall
-    # roofline of the dedispersion kernel: algorithmic adds / kernel time
+
+    # ---- the same steps replayed from HIP graphs (detection stays eager)
+    chain.capture_steps([chunks[(args.warmup + args.steps + k) % len(chunks)] for k in range(2)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_steps(chain, chunks, args.steps, graph=True)
+    torch.cuda.synchronize()
+    graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+
+    # ---- roofline of the dedispersion kernel: algorithmic adds / kernel time
     adds = float(TDMS) * N_CHAN * nout
     elem = {"f32": 4, "f16": 2, "u8": 1}[dd_dtype]
     alg_bytes = N_CHAN * NSAMP_BUF * elem + TDMS * nout * 4
     t_dd = stages["dedisperse"] * 1e-3
     achieved = adds / t_dd / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS_FP32,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS_FP32, 4), "traffic": None,
-                "kernel": "dedisperse_kernel", "ops_per_launch": adds,
-                "alg_bytes_per_launch": alg_bytes,
-                "hbm_gbs_algorithmic": round(alg_bytes / t_dd / 1e9, 1)}
+    kname = {"f32": "dedisperse_kernel", "f16": "dedisperse_f16_kernel", "u8": "dedisperse_u8_kernel"}[dd_dtype]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_TFLOPS_FP32, 4), "traffic": traffic_from_profile(dd_dtype),
+                "kernel": kname, "ops_per_launch": adds, "alg_bytes_per_launch": alg_bytes,
+                "hbm_gbs_algorithmic": round(alg_bytes / t_dd / 1e9, 1),
+                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32): issue peak 39.3 T/s, "
+                        "LDS-read bound 19.7 T/s for FP32 input"}
+
+    # ---- mixed precision: the same inputs through chains with FP16 / u8
+    # dedispersion input (FP32 / int32 sums), against a plain FP32 chain.
+    # On the workload above the rejected impulse spectra (-0.2 minus the
+    # bandpass, never normalised, as in the reference) pull every
+    # dedispersed series far below the threshold, so the comparison runs on
+    # the same stream without the broadband impulses, where the pulses are
+    # found.
+    mixed = {}
+    if dd_dtype == "f32" and not args.no_mixed:
+        pchunks = make_chunks(4048, False)
+        pprime = pchunks[-1][:, :, :overlap * INTEG * N_CH].contiguous()
+        nrm = float(np.sqrt(N_CHAN))  # series noise in the clipper's normalised units
+        ref_out, ref_events = None, None
+        for mdt in ("f32", "f16", "u8"):
+            mc = make_chain(mdt)
+            if mdt == "f32":
+                mc.plan.set_tile(*chain.plan.tile)
+                m_tune = None
+            else:
+                m_tune = mc.autotune()
+            mc.prime(pprime)
+            run_steps(mc, pchunks, args.warmup)
+            torch.cuda.synchronize()
+            mt = []
+            t0 = time.perf_counter()
+            mev = run_steps(mc, pchunks, args.steps, mt)
+            torch.cuda.synchronize()
+            mwall = time.perf_counter() - t0
+            if mdt == "u8":
+                series = (mc.out.double() - 128.0 * N_CHAN) / inv_scale
+            else:
+                series = mc.out.double()
+            rec = {"value": TDMS * nout * args.steps / mwall, "ms_per_step": 1e3 * mwall / args.steps,
+                   "dedisperse_ms": round(sum(ev[3].elapsed_time(ev[4]) for ev in mt) / args.steps, 3),
+                   "tile_dm_t_chanblock": list(mc.plan.tile),
+                   "autotune_ms": None if m_tune is None else round(m_tune, 3),
+                   "events": sum(len(e[0]) - 1 for e in mev)}
+            if mdt == "f32":
+                ref_out, ref_events = series.clone(), mev
+            else:
+                err = (series - ref_out).abs()
+                n_ref = n_got = inter = 0
+                for (pe, _), (me, _) in zip(ref_events, mev):
+                    rk, gk = event_keys(pe), event_keys(me)
+                    n_ref += len(rk)
+                    n_got += len(gk)
+                    inter += len(rk & gk)
+                rec.update({"series_max_abs_err_sigma": float(err.max()) / nrm,
+                            "series_rms_err_sigma": float(err.pow(2).mean().sqrt()) / nrm,
+                            "event_recall": inter / max(n_ref, 1), "event_precision": inter / max(n_got, 1)})
+            mixed[mdt] = rec
+            del mc, series
+            torch.cuda.empty_cache()
+        mixed["stream"] = "the workload without broadband impulses (pulses and narrowband RFI only)"
+        mixed["error_unit"] = "sigma of a dedispersed sample (sqrt(nChan) in normalised units)"
+        mixed["note"] = ("f16 rounds each clipped sample to 11 bits; u8 quantises to 1/12.7 over +-10 sigma, "
+                         "and the spectra the clipper rejects (-0.2 minus the bandpass, thousands of raw "
+                         "units, as in the reference) saturate it, which is where its events differ")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_spectra, coeffs, bp, nout)
+
+    # a device copy of known size for scaling the HBM byte counters
+    a = torch.empty(1 << 28, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
 
     line = {
         "metric": METRIC,
@@ -250,12 +329,14 @@ def main():
         "config": {"workload": "lofar-beam: 62 sub x 2 pol x 64 ch (3968 ch), 8-tap PPF, Stokes-I x4, "
                                "RFI clipper, 2048 DMs 0-500 on 65536-sample buffers, 6-sigma detection",
                    "nOut_per_step": nout, "input_samples_per_stream_per_step": n_samp,
-                   "dedisp_input": dd_dtype, "dft": "fp32", "tile_dm_t_chanblock": list(plan.tile),
+                   "dedisp_input": dd_dtype, "dft": "fp32", "tile_dm_t_chanblock": list(chain.plan.tile),
                    "autotune_ms": None if tune_ms is None else round(tune_ms, 3)},
         "real_time_factor": rtf,
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-        "events_per_step": events,
-        "clipper_walk": clip.counters(),
+        "graph_ms_per_step": round(graph_ms, 3),
+        "events_per_step": [len(e[0]) for e in plain_events],
+        "clipper_walk": chain.clip.counters(),
+        "mixed": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gen_seconds": round(t_gen, 1),

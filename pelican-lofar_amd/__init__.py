@@ -469,3 +469,107 @@ class Clipper:
         out = np.zeros(n_t, np.uint8)
         check(lib().ppl_clipper_flags(self.h, out.ctypes.data_as(C.c_void_p), n_t))
         return out
+
+
+# ------------------------------------------------------------- beam chain
+class BeamChain:
+    """One beam's device-resident chain, stepped one dedispersion buffer at a
+    time: fused PPF -> Stokes-I -> integration, RFI clipper, transposition
+    into the dedispersion buffer, dedispersion, overlap carry (the
+    DedispersionPipeline stage order).  A step consumes exactly nOut new
+    integrated spectra, i.e. nOut*integrate*nCh voltage samples per subband
+    and polarisation, after prime() has filled the buffer's overlap.
+
+    Every launch goes to the current stream, so a step can be captured in a
+    HIP graph (capture_steps): the channeliser's FIR history and the
+    dedispersion buffer alternate between two device copies, so two graphs
+    (even and odd steps) replay the chain.  Detection synchronises and stays
+    outside the graphs."""
+
+    def __init__(self, n_sub, n_ch, n_taps, coeffs, integrate, bp_values, nsamp_buf, tdms, dm_low,
+                 dm_step, tsamp, fch1, foff, dtype="f32", inv_scale=127.0 / 10.0, cr=10.0, sr=4.0,
+                 history=10000, zero_dm=True, tile=None):
+        import torch
+        self.n_sub, self.n_ch, self.integrate = n_sub, n_ch, integrate
+        self.n_chan = n_sub * n_ch
+        self.dtype, self.inv_scale = dtype, inv_scale
+        self.plan = DedispPlan(self.n_chan, nsamp_buf, tdms, dm_low, dm_step, tsamp, fch1, foff, dtype=dtype)
+        self.nout, self.overlap = self.plan.nout, self.plan.maxshift + self.plan.rem
+        self.ppf = PPF(n_sub, n_ch, n_taps, np.asarray(coeffs, np.float32), integrate=integrate)
+        self.clip = Clipper(bp_values, cr, sr, history, zero_dm)
+        self.buf = DDBuffer(self.n_chan, nsamp_buf, self.overlap, dtype, inv_scale)
+        self.stokes = torch.empty((self.nout, n_sub, 1, n_ch), dtype=torch.float32, device="cuda")
+        self.out = torch.empty((tdms, self.nout), dtype=self.plan.out_dtype(), device="cuda")
+        self.graphs = None
+        self.parity = 0
+        if tile is not None:
+            self.plan.set_tile(*tile)
+
+    def samples_per_step(self):
+        return self.nout * self.integrate * self.n_ch
+
+    def autotune(self, reps=1):
+        import torch
+        tdt = {"f32": torch.float32, "f16": torch.float16, "u8": torch.uint8}[self.dtype]
+        probe = torch.zeros((self.n_chan, self.plan.cfg.nsampBuf), dtype=tdt, device="cuda")
+        ms = self.plan.autotune(probe, self.out, reps=reps)
+        del probe
+        return ms
+
+    def prime(self, d_volt):
+        """Fill the buffer's overlap from overlap*integrate*nCh samples."""
+        st = self.ppf.stokes_i16(d_volt, self.overlap * self.integrate)
+        self.clip.run(st)
+        used, full = self.buf.add(st.view(self.overlap, self.n_chan))
+        if used != self.overlap or full:
+            raise PPLError("prime: expected %d spectra to fill the overlap" % self.overlap)
+
+    def step(self, d_volt, events=None):
+        """One buffer: the dedispersed series land in self.out.  events: five
+        torch events recorded after each stage (PPF+Stokes, clipper, buffer,
+        dedispersion, carry), or None."""
+        def mark(i):
+            if events is not None:
+                events[i].record()
+        self.ppf.stokes_i16(d_volt, self.nout * self.integrate, d_out=self.stokes)
+        mark(0)
+        self.clip.run(self.stokes)
+        mark(1)
+        used, full = self.buf.add(self.stokes.view(self.nout, self.n_chan))
+        if used != self.nout or not full:
+            raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
+        mark(2)
+        self.plan.run(self.buf.tensor(), d_out=self.out)
+        mark(3)
+        self.buf.rotate()
+        mark(4)
+        self.parity ^= 1
+
+    def capture_steps(self, d_volt):
+        """Capture two consecutive steps as two graphs (even and odd steps);
+        d_volt is one input tensor, or a pair read by the graph of the
+        current parity and the next.  The chain's device state is not
+        advanced by the capture."""
+        import torch
+        if self.graphs is not None:
+            return
+        ins = list(d_volt) if isinstance(d_volt, (list, tuple)) else [d_volt, d_volt]
+        gs = {}
+        for k in range(2):
+            g = torch.cuda.CUDAGraph()
+            p = self.parity
+            with torch.cuda.graph(g):
+                self.step(ins[k])
+            gs[p] = g
+        self.graphs = gs  # host state went round twice: back to this parity
+
+    def replay(self):
+        """One step by graph, reading the input captured for this parity."""
+        self.graphs[self.parity].replay()
+        self.parity ^= 1
+
+    def detect(self, threshold=6.0, bin_pow2=6, max_events=1 << 22):
+        if self.dtype == "u8":
+            return detect_u8(self.out, self.n_chan, 1.0 / self.inv_scale, self.n_ch, self.n_sub, threshold,
+                             bin_pow2, max_events)
+        return detect(self.out, self.n_ch, self.n_sub, threshold, bin_pow2, max_events)
