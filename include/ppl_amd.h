@@ -269,6 +269,20 @@ int ppl_clipper_destroy(ppl_clipper* clip);
  * NULL) receives the weights (always 1, :143,332). */
 int ppl_clipper_run(ppl_clipper* clip, float* d_I, float* d_W, unsigned nT, unsigned nSub,
                     unsigned nPol, unsigned nCh, void* stream);
+/* One beam's part of a batched run: spectra (and weights, may be NULL) of
+ * nT spectra [nT][nSub][nPol][nCh]. */
+typedef struct {
+    float* d_I;
+    float* d_W;
+    unsigned nT, nSub, nPol, nCh;
+} ppl_clip_job;
+enum { PPL_CLIP_MAX_BEAMS = 48 };
+/* RFI_Clipper::run on several independent clippers (beams) at once, each
+ * on its own spectra and shape: every stage is one launch for all beams and
+ * the sequential walk runs one workgroup per beam, so the beams' recurrences
+ * proceed side by side.  Results equal nBeams separate ppl_clipper_run calls.
+ * At most PPL_CLIP_MAX_BEAMS beams; a clipper may appear once per batch. */
+int ppl_clipper_run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int nBeams, void* stream);
 /* Same, and also writes the clipped Stokes-I spectra transposed into a
  * dedispersion buffer (fusing DedispersionBuffer::addSamples). */
 int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,

@@ -118,6 +118,7 @@ SIGNATURES = {
     "ppl_clipper_stats": (i, [vp, vp, vp]),
     "ppl_clipper_flags": (i, [vp, vp, u]),
     "ppl_clipper_counters": (i, [vp, vp, i]),
+    "ppl_clipper_run_batch": (i, [vp, vp, i, vp]),
 }
 
 
@@ -421,6 +422,24 @@ def integrate(d_in, window, bin_ch=1, d_out=None, stream=None):
 
 
 # ------------------------------------------------------------------ clipper
+class ClipJob(C.Structure):
+    _fields_ = [("d_I", C.c_void_p), ("d_W", C.c_void_p), ("nT", C.c_uint), ("nSub", C.c_uint),
+                ("nPol", C.c_uint), ("nCh", C.c_uint)]
+
+
+def clipper_run_batch(clippers, d_spectra, d_weights=None, stream=None):
+    """RFI_Clipper::run on several beams at once (one launch per stage for all
+    beams); d_spectra[k] is beam k's [nT][nSub][nPol][nCh] float tensor."""
+    n = len(clippers)
+    handles = (C.c_void_p * n)(*[c.h for c in clippers])
+    jobs = (ClipJob * n)()
+    for k, x in enumerate(d_spectra):
+        n_t, n_sub, n_pol, n_ch = x.shape
+        w = None if d_weights is None else d_weights[k]
+        jobs[k] = ClipJob(x.data_ptr(), None if w is None else w.data_ptr(), n_t, n_sub, n_pol, n_ch)
+    check(lib().ppl_clipper_run_batch(handles, jobs, n, _stream(stream)))
+
+
 class Clipper:
     """RFI_Clipper with its running state on the device."""
 

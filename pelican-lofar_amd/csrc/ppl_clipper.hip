@@ -416,10 +416,40 @@ __device__ __forceinline__ void fold_sum_sq(const float* row, int n, float& sum,
 // 0.5 / cpb exceeds the 2^-21 error of the product (cpb < 2^19).
 __device__ __forceinline__ int band_of(int i, float invCpb) { return (int)(((float)i + 0.5f) * invCpb); }
 
+// Device-resident state of one clipper (one beam), fixed between runs.
+struct BeamState {
+    ClipConst K;
+    ClipScalars* S;
+    float *meanBuf, *rmsBuf, *lastGood, *lastGoodVer, *bpLo, *bpHi;
+    double *meanPrefix, *rmsPrefix;
+    const double* terms;
+    Box* box;
+    PreRec* pre;
+    SpecRec* rec;
+    float2* stats;
+};
+
+// What changes per run, for every beam of a batch (kernel argument): the
+// beams' states, their spectra and weights, spectrum counts, sub-chunk
+// lengths and access modes.  blockIdx.z selects the beam.
+constexpr int kMaxBeams = PPL_CLIP_MAX_BEAMS;
+struct BatchIO {
+    const BeamState* st[kMaxBeams];
+    float* I[kMaxBeams];
+    float* W[kMaxBeams];
+    int nT[kMaxBeams];
+    int subLen[kMaxBeams];
+    int ldMode[kMaxBeams];
+};
+
 // Start of a run: version 0 of the last-good spectrum is its current state.
-__global__ void clip_begin_kernel(ClipScalars* __restrict__ S, const float* __restrict__ lastGood,
-                                  float* __restrict__ lastGoodVer, int nBins)
+__global__ void clip_begin_kernel(BatchIO io)
 {
+    const BeamState& bs = *io.st[blockIdx.z];
+    ClipScalars* __restrict__ S = bs.S;
+    const float* __restrict__ lastGood = bs.lastGood;
+    float* __restrict__ lastGoodVer = bs.lastGoodVer;
+    const int nBins = bs.K.nBins;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         S->tNext = 0;
@@ -429,9 +459,17 @@ __global__ void clip_begin_kernel(ClipScalars* __restrict__ S, const float* __re
 }
 
 // The box around the sub-chunk-start state and the bandpass at its two offsets.
-__global__ void clip_bounds_kernel(const ClipScalars* __restrict__ S, ClipConst K, const double* __restrict__ terms,
-                                   Box* __restrict__ box, float* __restrict__ bpLo, float* __restrict__ bpHi, int nT)
+__global__ void clip_bounds_kernel(BatchIO io)
 {
+    const int beam = blockIdx.z;
+    const BeamState& bs = *io.st[beam];
+    const ClipScalars* __restrict__ S = bs.S;
+    const ClipConst K = bs.K;
+    const double* __restrict__ terms = bs.terms;
+    Box* __restrict__ box = bs.box;
+    float* __restrict__ bpLo = bs.bpLo;
+    float* __restrict__ bpHi = bs.bpHi;
+    const int nT = io.nT[beam];
     const ClipScalars s = *S;
     if (s.tNext >= nT) return;
     const bool steady = s.rmsSize == K.cap && s.meanSize == K.cap;
@@ -468,15 +506,19 @@ __global__ void clip_bounds_kernel(const ClipScalars* __restrict__ S, ClipConst 
 // (fl(I - bp) falls as bp rises; bp is monotone in the offset).  Clipped
 // channels enter the fold as +0.0, which leaves a sum started at +0.0
 // unchanged.
-__global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(const float* __restrict__ I, ClipConst K,
-                                                                const ClipScalars* __restrict__ S,
-                                                                const Box* __restrict__ box,
-                                                                const float* __restrict__ bpLo,
-                                                                const float* __restrict__ bpHi,
-                                                                const float* __restrict__ lastGood,
-                                                                PreRec* __restrict__ pre, int nT, int subLen,
-                                                                int ldMode)
+__global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
 {
+    const int beam = blockIdx.z;
+    const BeamState& bs = *io.st[beam];
+    const float* __restrict__ I = io.I[beam];
+    const ClipConst K = bs.K;
+    const ClipScalars* __restrict__ S = bs.S;
+    const Box* __restrict__ box = bs.box;
+    const float* __restrict__ bpLo = bs.bpLo;
+    const float* __restrict__ bpHi = bs.bpHi;
+    const float* __restrict__ lastGood = bs.lastGood;
+    PreRec* __restrict__ pre = bs.pre;
+    const int nT = io.nT[beam], subLen = io.subLen[beam], ldMode = io.ldMode[beam];
     __shared__ float vals[2][kTileRows][kTileBins + 1];
     __shared__ int cnt[kTileRows][8];
     __shared__ int rowFlags[kTileRows];
@@ -597,13 +639,24 @@ struct Staged {
 // covers subLen spectra (all that remain when lastIter is set); a state
 // that leaves the box ends it after that spectrum, so that the next
 // sub-chunk speculates around the new state.
-__global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(
-    float* __restrict__ I, ClipConst K, ClipScalars* __restrict__ S, float* __restrict__ meanBuf,
-    double* __restrict__ meanPrefix, float* __restrict__ rmsBuf, double* __restrict__ rmsPrefix,
-    float* __restrict__ lastGood, float* __restrict__ lastGoodVer, const double* __restrict__ terms,
-    const Box* __restrict__ box, const PreRec* __restrict__ pre, SpecRec* __restrict__ rec, int nT, int subLen,
-    int lastIter)
+__global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int lastIter)
 {
+    const int beam = blockIdx.z;
+    const BeamState& bst = *io.st[beam];
+    float* __restrict__ I = io.I[beam];
+    const ClipConst K = bst.K;
+    ClipScalars* __restrict__ S = bst.S;
+    float* __restrict__ meanBuf = bst.meanBuf;
+    double* __restrict__ meanPrefix = bst.meanPrefix;
+    float* __restrict__ rmsBuf = bst.rmsBuf;
+    double* __restrict__ rmsPrefix = bst.rmsPrefix;
+    float* __restrict__ lastGood = bst.lastGood;
+    float* __restrict__ lastGoodVer = bst.lastGoodVer;
+    const double* __restrict__ terms = bst.terms;
+    const Box* __restrict__ box = bst.box;
+    const PreRec* __restrict__ pre = bst.pre;
+    SpecRec* __restrict__ rec = bst.rec;
+    const int nT = io.nT[beam], subLen = io.subLen[beam];
     __shared__ ClipScalars s;
     __shared__ float red[kWalkThreads / 64 + 1];
     __shared__ int goodCnt[8];
@@ -1182,16 +1235,22 @@ __device__ __forceinline__ float flatten_bin(float v, float bpv, const SpecRec& 
 // from its ordered sums (RFI_Clipper.cpp:486-501), row-tiled as the
 // precompute.  Writes stats[t] = (mean, rms).
 template <int NT>
-__global__ __launch_bounds__(kTileThreads) void clip_stats_kernel(const float* __restrict__ I, ClipConst K,
-                                                                  const double* __restrict__ terms,
-                                                                  const float* __restrict__ lastGoodVer,
-                                                                  const SpecRec* __restrict__ rec,
-                                                                  float2* __restrict__ stats, int nT, int ldMode)
+__global__ __launch_bounds__(kTileThreads) void clip_stats_kernel(BatchIO io)
 {
     __shared__ float vals[2][kTileRows][kTileBins + 1];
     __shared__ SpecRec recs[kTileRows];
+    const int beam = blockIdx.z;
+    const BeamState& bs = *io.st[beam];
+    const float* __restrict__ I = io.I[beam];
+    const ClipConst K = bs.K;
+    const double* __restrict__ terms = bs.terms;
+    const float* __restrict__ lastGoodVer = bs.lastGoodVer;
+    const SpecRec* __restrict__ rec = bs.rec;
+    float2* __restrict__ stats = bs.stats;
+    const int nT = io.nT[beam], ldMode = io.ldMode[beam];
     const int t0 = blockIdx.x * kTileRows;
     const int nRows = min(kTileRows, nT - t0);
+    if (nRows <= 0) return;  // a shorter beam of the batch
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
     const long rowStride = (long)nBins * nPol;
@@ -1259,17 +1318,24 @@ constexpr int kWriteRows = 16;
 // flattened polarisation 0 normalised for kept spectra (:502-532), weights.
 // Threads own 4 consecutive bins; a workgroup walks kWriteRows spectra.
 template <int NT>
-__global__ __launch_bounds__(256) void clip_write_kernel(float* __restrict__ I, float* __restrict__ W, ClipConst K,
-                                                         const double* __restrict__ terms,
-                                                         const float* __restrict__ lastGoodVer,
-                                                         const SpecRec* __restrict__ rec,
-                                                         const float2* __restrict__ stats, int nT, int ldMode)
+__global__ __launch_bounds__(256) void clip_write_kernel(BatchIO io)
 {
     __shared__ SpecRec recs[kWriteRows];
     __shared__ float2 st[kWriteRows];
+    const int beam = blockIdx.z;
+    const BeamState& bs = *io.st[beam];
+    float* __restrict__ I = io.I[beam];
+    float* __restrict__ W = io.W[beam];
+    const ClipConst K = bs.K;
+    const double* __restrict__ terms = bs.terms;
+    const float* __restrict__ lastGoodVer = bs.lastGoodVer;
+    const SpecRec* __restrict__ rec = bs.rec;
+    const float2* __restrict__ stats = bs.stats;
+    const int nT = io.nT[beam], ldMode = io.ldMode[beam];
     const int tid = threadIdx.x;
     const int t0 = blockIdx.y * kWriteRows;
     const int nRows = min(kWriteRows, nT - t0);
+    if (nRows <= 0) return;  // a shorter beam of the batch
     if (tid < nRows) {
         recs[tid] = rec[t0 + tid];
         st[tid] = stats[t0 + tid];
@@ -1332,6 +1398,7 @@ struct ppl_clipper {
     PreRec* d_pre = nullptr;
     Box* d_box = nullptr;
     float *d_bpLo = nullptr, *d_bpHi = nullptr, *d_lastGoodVer = nullptr;
+    BeamState* d_bs = nullptr;  // device descriptor of this clipper for the batched kernels
     int recCap = 0;
     std::vector<uint8_t> lastFlags;
     int lastNT = 0;
@@ -1355,6 +1422,8 @@ void free_state(ppl_clipper* c)
     (void)hipFree(c->d_bpLo);
     (void)hipFree(c->d_bpHi);
     (void)hipFree(c->d_lastGoodVer);
+    (void)hipFree(c->d_bs);
+    c->d_bs = nullptr;
     c->d_lastGoodVer = nullptr;
     c->d_pre = nullptr;
     c->d_box = nullptr;
@@ -1478,87 +1547,154 @@ extern "C" int ppl_clipper_destroy(ppl_clipper* c)
 // The two apply passes: statistics of the flattened spectra, then the
 // elementwise replacement / flatten / normalise.
 template <int NT>
-static int launch_apply(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, int ldMode, hipStream_t st)
+static int launch_apply(const BatchIO& io, int nBeams, int maxBins, int maxNT, hipStream_t st)
 {
-    hipLaunchKernelGGL(clip_stats_kernel<NT>, dim3(cdiv(nT, kTileRows)), dim3(kTileThreads), 0, st, d_I, c->K,
-                       c->d_terms, c->d_lastGoodVer, c->d_rec, c->d_stats, (int)nT, ldMode);
+    hipLaunchKernelGGL(clip_stats_kernel<NT>, dim3(cdiv(maxNT, kTileRows), 1, nBeams), dim3(kTileThreads), 0, st,
+                       io);
     PPL_LAUNCHED();
-    hipLaunchKernelGGL(clip_write_kernel<NT>, dim3(cdiv((unsigned)c->nBins / 4, 256), cdiv(nT, kWriteRows)), dim3(256),
-                       0, st, d_I, d_W, c->K, c->d_terms, c->d_lastGoodVer, c->d_rec, c->d_stats, (int)nT, ldMode);
+    hipLaunchKernelGGL(clip_write_kernel<NT>, dim3(cdiv((unsigned)maxBins / 4, 256), cdiv(maxNT, kWriteRows), nBeams),
+                       dim3(256), 0, st, io);
     PPL_LAUNCHED();
     return PPL_OK;
 }
 
-static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
-                       unsigned nCh, hipStream_t st)
+// Brings one clipper to the run's shape: the map on its first run, the
+// per-spectrum records' capacity, the polarisation count; uploads its
+// device descriptor when any of these changed.
+static int prepare(ppl_clipper* c, const ppl_clip_job& j)
 {
-    PPL_CHECK_ARG(c && d_I, "null argument");
-    const unsigned nBins = nSub * nCh;
+    PPL_CHECK_ARG(c && j.d_I, "null argument");
+    const unsigned nBins = j.nSub * j.nCh;
     PPL_CHECK_ARG(nBins % 8 == 0 && nBins > 0, "RFI_Clipper: nBins (%u) must be a positive multiple of 8", nBins);
-    PPL_CHECK_ARG(nPol >= 1, "no polarisations");
+    PPL_CHECK_ARG(j.nPol >= 1, "no polarisations");
+    PPL_CHECK_ARG(nBins <= 4u * 65535u * 256u, "RFI_Clipper: %u bins exceed the launch grid", nBins);
+    bool dirty = false;
     if (c->nBins == 0) {
-        int rc = setup_map(c, nSub, nPol, nCh);
+        int rc = setup_map(c, j.nSub, j.nPol, j.nCh);
         if (rc != PPL_OK) return rc;
+        dirty = true;
     }
-    PPL_CHECK_ARG((unsigned)c->nBins == nBins && (unsigned)c->K.nCh == nCh,
+    PPL_CHECK_ARG((unsigned)c->nBins == nBins && (unsigned)c->K.nCh == j.nCh,
                   "RFI_Clipper: spectrum shape changed (%u bins, was %d)", nBins, c->nBins);
-    c->K.nPol = (int)nPol;
-    if (nT == 0) return PPL_OK;
-    if ((int)nT > c->recCap) {
+    if (c->K.nPol != (int)j.nPol) {
+        c->K.nPol = (int)j.nPol;
+        dirty = true;
+    }
+    if ((int)j.nT > c->recCap) {
         (void)hipFree(c->d_rec);
         (void)hipFree(c->d_pre);
         (void)hipFree(c->d_stats);
         c->d_rec = nullptr;
         c->d_pre = nullptr;
         c->d_stats = nullptr;
-        PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * nT));
-        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * std::min((int)nT, kSubLen)));
-        PPL_HIP(hipMalloc(&c->d_stats, sizeof(float2) * nT));
-        c->recCap = (int)nT;
+        PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * j.nT));
+        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * std::min((int)j.nT, kSubLen)));
+        PPL_HIP(hipMalloc(&c->d_stats, sizeof(float2) * j.nT));
+        c->recCap = (int)j.nT;
+        dirty = true;
     }
-    // access mode of the row-tiled passes (see load_quad)
+    if (dirty || !c->d_bs) {
+        BeamState h{};
+        h.K = c->K;
+        h.S = c->d_S;
+        h.meanBuf = c->d_meanBuf;
+        h.rmsBuf = c->d_rmsBuf;
+        h.lastGood = c->d_lastGood;
+        h.lastGoodVer = c->d_lastGoodVer;
+        h.bpLo = c->d_bpLo;
+        h.bpHi = c->d_bpHi;
+        h.meanPrefix = c->d_meanPrefix;
+        h.rmsPrefix = c->d_rmsPrefix;
+        h.terms = c->d_terms;
+        h.box = c->d_box;
+        h.pre = c->d_pre;
+        h.rec = c->d_rec;
+        h.stats = c->d_stats;
+        if (!c->d_bs) PPL_HIP(hipMalloc(&c->d_bs, sizeof(BeamState)));
+        PPL_HIP(hipMemcpy(c->d_bs, &h, sizeof h, hipMemcpyHostToDevice));
+    }
+    return PPL_OK;
+}
+
+// One run over a batch of clippers (beams): each stage is one launch for
+// all beams (blockIdx.z), the walk one workgroup per beam.  Beams keep
+// their own shapes and spectrum counts; shorter beams' blocks return early.
+static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n, hipStream_t st)
+{
+    PPL_CHECK_ARG(clips && jobs && n >= 0, "null argument");
+    PPL_CHECK_ARG(n <= kMaxBeams, "at most %d beams per clipper batch (got %d)", kMaxBeams, n);
+    BatchIO io{};
+    int maxBins = 0, maxNT = 0, maxIter = 0, nb = 0, nTerms = -2;
+    ppl_clipper* used[kMaxBeams];
     auto aligned = [](const void* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-    const int ldMode = !(aligned(d_I) && aligned(d_W)) ? 2 : nPol == 1 ? 0 : nCh % 4 == 0 ? 1 : 2;
+    for (int k = 0; k < n; ++k) {
+        for (int q = 0; q < k; ++q)
+            PPL_CHECK_ARG(clips[q] != clips[k], "clipper %d appears twice in one batch", k);
+        int rc = prepare(clips[k], jobs[k]);
+        if (rc != PPL_OK) return rc;
+        const ppl_clip_job& j = jobs[k];
+        if (j.nT == 0) continue;
+        ppl_clipper* c = clips[k];
+        io.st[nb] = c->d_bs;
+        io.I[nb] = j.d_I;
+        io.W[nb] = j.d_W;
+        io.nT[nb] = (int)j.nT;
+        io.subLen[nb] = std::min((int)j.nT, kSubLen);
+        // access mode of the row-tiled passes (see load_quad)
+        io.ldMode[nb] = !(aligned(j.d_I) && aligned(j.d_W)) ? 2 : j.nPol == 1 ? 0 : j.nCh % 4 == 0 ? 1 : 2;
+        maxBins = std::max(maxBins, c->nBins);
+        maxNT = std::max(maxNT, (int)j.nT);
+        maxIter = std::max(maxIter, (int)cdiv(j.nT, (unsigned)io.subLen[nb]));
+        nTerms = nTerms == -2 ? c->K.nTerms : (nTerms == c->K.nTerms ? nTerms : -1);
+        used[nb++] = c;
+    }
+    if (nb == 0) return PPL_OK;
     // Sub-chunks: box around the state reached, precompute, walk.  The walk
-    // advances S->tNext by kSubLen spectra, or less when the state leaves the
-    // box; the last launch walks whatever remains.  Launches past the end
-    // return at once.
-    const int subLen = std::min((int)nT, kSubLen);
-    const int nIter = (int)cdiv(nT, (unsigned)subLen) + kExtraIter;
-    hipLaunchKernelGGL(clip_begin_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->d_lastGood,
-                       c->d_lastGoodVer, (int)nBins);
+    // advances S->tNext by up to kSubLen spectra, or less when the state
+    // leaves the box; the last launch walks whatever remains.  Launches past
+    // a beam's end return at once.
+    const int nIter = maxIter + kExtraIter;
+    hipLaunchKernelGGL(clip_begin_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
     PPL_LAUNCHED();
+    const int maxSub = std::min(maxNT, kSubLen);
     for (int it = 0; it < nIter; ++it) {
-        hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(nBins, 256)), dim3(256), 0, st, c->d_S, c->K, c->d_terms,
-                           c->d_box, c->d_bpLo, c->d_bpHi, (int)nT);
+        hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
         PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(subLen, kTileRows)), dim3(kTileThreads), 0, st, d_I, c->K,
-                           c->d_S, c->d_box, c->d_bpLo, c->d_bpHi, c->d_lastGood, c->d_pre, (int)nT, subLen,
-                           ldMode);
+        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(maxSub, kTileRows), 1, nb), dim3(kTileThreads), 0, st, io);
         PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_walk_kernel, dim3(1), dim3(kWalkThreads), 0, st, d_I, c->K, c->d_S, c->d_meanBuf,
-                           c->d_meanPrefix, c->d_rmsBuf, c->d_rmsPrefix, c->d_lastGood, c->d_lastGoodVer,
-                           c->d_terms, c->d_box, c->d_pre, c->d_rec, (int)nT, subLen, it == nIter - 1 ? 1 : 0);
+        hipLaunchKernelGGL(clip_walk_kernel, dim3(1, 1, nb), dim3(kWalkThreads), 0, st, io, it == nIter - 1 ? 1 : 0);
         PPL_LAUNCHED();
     }
     int rc = PPL_OK;
-    switch (c->K.nTerms) {
-        case 0: rc = launch_apply<0>(c, d_I, d_W, nT, ldMode, st); break;
-        case 1: rc = launch_apply<1>(c, d_I, d_W, nT, ldMode, st); break;
-        case 2: rc = launch_apply<2>(c, d_I, d_W, nT, ldMode, st); break;
-        case 3: rc = launch_apply<3>(c, d_I, d_W, nT, ldMode, st); break;
-        case 4: rc = launch_apply<4>(c, d_I, d_W, nT, ldMode, st); break;
-        default: rc = launch_apply<-1>(c, d_I, d_W, nT, ldMode, st);
+    switch (nTerms) {
+        case 0: rc = launch_apply<0>(io, nb, maxBins, maxNT, st); break;
+        case 1: rc = launch_apply<1>(io, nb, maxBins, maxNT, st); break;
+        case 2: rc = launch_apply<2>(io, nb, maxBins, maxNT, st); break;
+        case 3: rc = launch_apply<3>(io, nb, maxBins, maxNT, st); break;
+        case 4: rc = launch_apply<4>(io, nb, maxBins, maxNT, st); break;
+        default: rc = launch_apply<-1>(io, nb, maxBins, maxNT, st);
     }
     if (rc != PPL_OK) return rc;
-    c->lastNT = (int)nT;
+    for (int k = 0; k < nb; ++k) used[k]->lastNT = io.nT[k];
     return PPL_OK;
+}
+
+static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
+                       unsigned nCh, hipStream_t st)
+{
+    ppl_clip_job j{d_I, d_W, nT, nSub, nPol, nCh};
+    return run_batch(&c, &j, 1, st);
 }
 
 extern "C" int ppl_clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
                                unsigned nCh, void* stream)
 {
     return clipper_run(c, d_I, d_W, nT, nSub, nPol, nCh, stream_of(stream));
+}
+
+extern "C" int ppl_clipper_run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int nBeams, void* stream)
+{
+    return run_batch(clips, jobs, nBeams, stream_of(stream));
 }
 
 extern "C" int ppl_clipper_stats(ppl_clipper* c, float* mean, float* rms)

@@ -133,3 +133,43 @@ def test_clipper_sub_chunks_and_drift(orc, ppl):
     assert cnt["stop"] >= 1 and cnt["fast"] > n_t // 2, cnt
     st_o, _ = oc.state()
     assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
+
+
+def test_clipper_batch_equals_separate_runs(orc, ppl):
+    """Beams of different shapes and spectrum counts in one batched run give
+    the same outputs, flags and running statistics as separate runs, through
+    the learning phase into the steady state."""
+    shapes = [(8, 16, 1), (4, 16, 2), (62, 64, 1)]
+    hist = 1000
+    streams = [_stream(3600, ns, nc, seed=50 + k, n_pol=npol) for k, (ns, nc, npol) in enumerate(shapes)]
+    bps = [_bp(ns, nc) for ns, nc, _ in shapes]
+    single = [ppl.Clipper(bp, 10.0, 4.0, hist, True) for bp in bps]
+    batch = [ppl.Clipper(bp, 10.0, 4.0, hist, True) for bp in bps]
+    oracle = orc.Clipper(bps[0], 10.0, 4.0, hist, True)
+    lens = [(700, 900, 500), (1300, 700, 900), (1600, 2000, 2200)]  # spectra per call, per beam
+    pos = [0, 0, 0]
+    for call in range(3):
+        xs = []
+        for k in range(3):
+            n = lens[call][k]
+            xs.append(np.ascontiguousarray(streams[k][pos[k]:pos[k] + n]))
+            pos[k] += n
+        ds = [torch.from_numpy(x).cuda() for x in xs]
+        dw = [torch.zeros_like(d) for d in ds]
+        ppl.clipper_run_batch(batch, ds, dw)
+        for k in range(3):
+            ref = torch.from_numpy(xs[k]).cuda()
+            single[k].run(ref)
+            got = ds[k].cpu().numpy()
+            want = ref.cpu().numpy()
+            assert np.array_equal(got, want, equal_nan=True), (call, k)
+            assert np.array_equal(batch[k].flags(lens[call][k]), single[k].flags(lens[call][k])), (call, k)
+            assert batch[k].stats() == single[k].stats(), (call, k)
+            assert np.all(dw[k].cpu().numpy() == 1.0)
+        # beam 0 also against the oracle
+        o = xs[0].copy()
+        oracle.run(o)
+        got0 = ds[0].cpu().numpy()
+        assert np.array_equal(got0, o, equal_nan=True), call
+    cnt = batch[2].counters()
+    assert cnt["fast"] > 0 and cnt["learn"] == hist, cnt
