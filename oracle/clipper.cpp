@@ -414,6 +414,33 @@ extern "C" void orc_clipper_state(orc_clipper* c, float* o, int* sizes)
     sizes[1] = (int)c->rmsBuffer.size();
 }
 
+extern "C" int orc_bandpass_probe(unsigned nBins, double start, double width, const float* params,
+                                  int nParams, float rms, float median, unsigned nBins2,
+                                  double start2, double width2, float* out4)
+{
+    if (nParams < 1 || nBins == 0) return -1;
+    BandPassR bp;
+    BinMapR map;
+    map.reset(nBins);
+    map.setStart(start);
+    map.setBinWidth(width);
+    bp.setData(map, std::vector<float>(params, params + nParams));
+    bp.setRMS(rms);
+    if (!std::isnan(median)) bp.setMedian(median);
+    if (nBins2 > 0) {
+        BinMapR m2;
+        m2.reset(nBins2);
+        m2.setStart(start2);
+        m2.setBinWidth(width2);
+        bp.reBin(m2);
+    }
+    out4[0] = bp.median_[bp.currentId];
+    out4[1] = bp.rms();
+    out4[2] = bp.currentSet()[0];
+    out4[3] = bp.params[0];
+    return 0;
+}
+
 extern "C" int orc_clipper_bandpass(orc_clipper* c, float* out, unsigned nBins)
 {
     const std::vector<float>& d = c->bp.currentSet();

@@ -119,6 +119,14 @@ int orc_clipper_last_flags(orc_clipper* c, uint8_t* good, int32_t* nclip,
 /* Scalar state after the last spectrum: meanRunAve, rmsRunAve, params0,
  * bandpass mean, meanOverRMS; history sizes. */
 void orc_clipper_state(orc_clipper* c, float* out5, int* sizes2);
+/* BandPass alone (src/lib/src/BandPass.cpp:26-108): setData over the map
+ * (nBins, start, width) with params, setRMS(rms), setMedian(median) unless
+ * median is NaN, then reBin to (nBins2, start2, width2) when nBins2 > 0.
+ * out4 = median(), rms(), intensityOfBin(0), params[0] of the result.
+ * Pinned by the known answers of BandPassTest.cpp:35-101. */
+int orc_bandpass_probe(unsigned nBins, double start, double width, const float* params,
+                       int nParams, float rms, float median, unsigned nBins2,
+                       double start2, double width2, float* out4);
 /* The bandpass values of the currently-binned map (nBins floats). */
 int orc_clipper_bandpass(orc_clipper* c, float* out, unsigned nBins);
 

@@ -60,6 +60,7 @@ def _declare(L):
         "orc_clipper_last_flags": (i, [vp, vp, vp, u]),
         "orc_clipper_state": (None, [vp, vp, vp]),
         "orc_clipper_bandpass": (i, [vp, vp, u]),
+        "orc_bandpass_probe": (i, [u, d, d, vp, i, f, f, u, d, d, vp]),
         "orc_dedisp_plan": (i, [u, u, u, f, f, d, d, d, vp, vp, vp]),
         "orc_dedisp_delays": (None, [vp, u, u, f, f, vp]),
         "orc_dedisperse_f32": (None, [vp, u, u, u, vp, u, vp, i]),
@@ -214,6 +215,36 @@ class Clipper:
         if lib().orc_clipper_bandpass(self.h, _p(o), n_bins) != 0:
             raise ValueError("bin count mismatch")
         return o
+
+
+def bandpass_probe(n_bins, start, width, params, rms, median=float("nan"), rebin=None):
+    """BandPass set up on (n_bins, start, width) with params, rms and median,
+    optionally rebinned to rebin = (n_bins2, start2, width2); returns
+    (median, rms, intensityOfBin(0), params[0])."""
+    p = np.ascontiguousarray(params, np.float32)
+    o = np.zeros(4, np.float32)
+    n2, s2, w2 = rebin if rebin is not None else (0, 0.0, 0.0)
+    if lib().orc_bandpass_probe(n_bins, start, width, _p(p), p.size, rms, median, n2, s2, w2, _p(o)) != 0:
+        raise ValueError("bad bandpass")
+    return tuple(float(v) for v in o)
+
+
+def read_bp(path):
+    """Numbers of a .bp bandpass file in order (BandPassAdapter::deserialise,
+    src/lib/src/BandPassAdapter.cpp:30-100): '#' lines are comments, one
+    number per line; 'K' kill lines are not supported here."""
+    vals = []
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if not line or line.startswith("#"):
+                continue
+            if line.startswith("K"):
+                raise ValueError("kill lines are not supported")
+            vals.append(float(line))
+    if len(vals) < 6:
+        raise ValueError("expecting at least 6 parameters")
+    return np.array(vals, np.float64)
 
 
 # --------------------------------------------------------- dedispersion

@@ -173,3 +173,24 @@ def test_clipper_batch_equals_separate_runs(orc, ppl):
         assert np.array_equal(got0, o, equal_nan=True), call
     cnt = batch[2].counters()
     assert cnt["fast"] > 0 and cnt["learn"] == hist, cnt
+
+
+def test_clipper_reference_bandpass_file(orc, ppl):
+    """The reference's own band31.bp (31 x 64 bins), read as the reference
+    reads it, through learning into the steady state with narrowband spikes
+    and broadband impulses: identical outputs, flags and state."""
+    from test_oracle_clipper import _band31_stream
+    impulses = list(range(1500, 3000, 100))
+    v, x = _band31_stream(orc, 3000, seed=12, spikes=(17, 900, 1700), impulses=impulses)
+    oc = orc.Clipper(v, 10.0, 4.0, 1000, True)
+    gc = ppl.Clipper(v, 10.0, 4.0, 1000, True)
+    for a in range(0, 3000, 1000):
+        chunk = np.ascontiguousarray(x[a:a + 1000])
+        ref = chunk.copy()
+        oc.run(ref)
+        d = torch.from_numpy(chunk).cuda()
+        gc.run(d)
+        assert np.array_equal(gc.flags(1000), oc.last_flags(1000)[0]), a
+        assert np.array_equal(d.cpu().numpy(), ref, equal_nan=True), a
+    st_o, _ = oc.state()
+    assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
