@@ -117,6 +117,26 @@ int ppl_dedisp_delays(const ppl_dedisp_plan* plan, int32_t* host_out);
  * F32/F16 input, int32 for U8 input (exact integer sums). */
 int ppl_dedisperse(const ppl_dedisp_plan* plan, const void* d_in, void* d_out, void* stream);
 
+/* Many beams' dedispersion in one launch (no reference counterpart: the
+ * reference dedisperses one beam per GPU job, DedispersionModule.cpp:263-297).
+ * Each job is a plan with its own input buffer and output; plans may differ
+ * in channels, buffer length, DM count and range, but share the dtype.  All
+ * jobs run under one DM x time tile and channel block (tileDM = tileT = 0:
+ * the first plan's), each with its own LDS row length.  Results equal n
+ * separate ppl_dedisperse calls, bit for bit. */
+typedef struct {
+    const ppl_dedisp_plan* plan;
+    const void* d_in;
+    void* d_out;
+} ppl_dedisp_job;
+enum { PPL_DEDISP_MAX_JOBS = 48 };
+int ppl_dedisperse_batch(const ppl_dedisp_job* jobs, int n, int tileDM, int tileT, int chanBlock,
+                         void* stream);
+/* Autotuner for a batch: times every instantiated tile and channel block
+ * that fits every job (reps launches each) and returns the fastest. */
+int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int reps, void* stream, int* tileDM,
+                              int* tileT, int* chanBlock, float* best_ms);
+
 /* The reference's C seam itself, with the reference's arguments (device
  * pointers; maxshift includes the remainder; outbufSize in bytes), computed
  * for every tdms (the reference's launch covers all rows only when

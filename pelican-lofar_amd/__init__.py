@@ -87,6 +87,8 @@ SIGNATURES = {
     "ppl_dedisp_autotune": (i, [vp, vp, vp, i, vp, vp]),
     "ppl_dedisp_delays": (i, [vp, vp]),
     "ppl_dedisperse": (i, [vp, vp, vp, vp]),
+    "ppl_dedisperse_batch": (i, [vp, i, i, i, i, vp]),
+    "ppl_dedisp_autotune_batch": (i, [vp, i, i, vp, vp, vp, vp, vp]),
     "ppl_cache_dedisperse_loop": (i, [vp, C.c_long, vp, f, f, i, i, vp, i, i, vp]),
     "ppl_convert_f16": (i, [vp, vp, sz, vp]),
     "ppl_quantise_u8": (i, [vp, vp, sz, f, vp]),
@@ -218,6 +220,44 @@ class DedispPlan:
             d_out = torch.empty((self.cfg.tdms, self.nout), dtype=self.out_dtype(), device=d_in.device)
         check(lib().ppl_dedisperse(self.h, _ptr(d_in), _ptr(d_out), _stream(stream)))
         return d_out
+
+
+class DedispJob(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("d_in", C.c_void_p), ("d_out", C.c_void_p)]
+
+
+MAX_DEDISP_JOBS = 48
+
+
+def _dedisp_jobs(plans, d_ins, d_outs):
+    n = len(plans)
+    if not (len(d_ins) == n == len(d_outs)):
+        raise PPLError("plans, inputs and outputs differ in number")
+    arr = (DedispJob * n)()
+    for k in range(n):
+        arr[k] = DedispJob(plans[k].h, _ptr(d_ins[k]), _ptr(d_outs[k]))
+    return arr, n
+
+
+def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0), stream=None):
+    """One launch dedispersing every (plan, input) pair; outputs as separate
+    DedispPlan.run calls would give them."""
+    import torch
+    if d_outs is None:
+        d_outs = [torch.empty((p.cfg.tdms, p.nout), dtype=p.out_dtype(), device=x.device)
+                  for p, x in zip(plans, d_ins)]
+    arr, n = _dedisp_jobs(plans, d_ins, d_outs)
+    check(lib().ppl_dedisperse_batch(arr, n, tile[0], tile[1], tile[2], _stream(stream)))
+    return d_outs
+
+
+def autotune_batch(plans, d_ins, d_outs, reps=1, stream=None):
+    """Fastest (tileDM, tileT, chanBlock) for the batch, and its time in ms."""
+    arr, n = _dedisp_jobs(plans, d_ins, d_outs)
+    td, tt, cb, best = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+    check(lib().ppl_dedisp_autotune_batch(arr, n, reps, _stream(stream), C.byref(td), C.byref(tt),
+                                          C.byref(cb), C.byref(best)))
+    return (td.value, tt.value, cb.value), best.value
 
 
 def cache_dedisperse_loop(outbuff, buff, mstartdm, mdmstep, tdms, num_samples, dm_shift, maxshift,

@@ -27,11 +27,11 @@ struct ppl_dedisp_plan {
     float mstartdm = 0, mdmstep = 0;
     std::vector<float> shift;
     std::vector<int32_t> delays;  // host [tdms][nChan]
-    int32_t* d_delayT = nullptr;  // device [nChan][tdmsPad]
-    int tdmsPad = 0;
+    int32_t* d_delayT = nullptr;  // device [nChan][tdmsPad], rows padded with the last trial
+    int tdmsPad = 0;              // tdms rounded up to 64, the widest DM tile
     int nd = 8, nt = 8;           // per-thread DM x sample register tile
     int cb = 0;                   // channels per staged block (0: as many as fit)
-    int span = 0;                 // max delay spread inside a DM tile
+    int spanTD[4] = {0, 0, 0, 0}; // max delay spread inside a DM tile of 8, 16, 32, 64 trials
 };
 
 namespace {
@@ -64,13 +64,48 @@ __device__ __forceinline__ void tile_of(int bid, int nDMTiles, int& tTile, int& 
     dmTile = k % nDMTiles;
 }
 
+// One beam's part of a launch.  A launch covers several beams (jobs): job k
+// owns workgroups [blockStart, blockStart + nDMTiles*nTTilesPad), a multiple
+// of 8, so a job's own XCD-major tile order is kept.  The table travels in
+// the kernel arguments; a workgroup finds its job with a uniform scan.
+constexpr int kMaxDDJobs = PPL_DEDISP_MAX_JOBS;
+struct DDJob {
+    const void* in;
+    void* out;
+    const int32_t* delayT;
+    int nChan, nsamp, tdms, tdmsPad, nOut;
+    int rowUnits;   // LDS row length: elements (f32) or dwords per copy (f16, u8)
+    int CB;         // channels per staged block
+    int nDMTiles;
+    int blockStart;
+    int pad_;
+};
+struct DDBatch {
+    DDJob job[kMaxDDJobs];
+    int n;
+    float one;  // 1.0f, kept opaque to the compiler (f16 kernel)
+};
+
+__device__ __forceinline__ const DDJob& find_job(const DDBatch& B, int& local)
+{
+    int jb = 0;
+    for (int k = 1; k < B.n; ++k)
+        if ((int)blockIdx.x >= B.job[k].blockStart) jb = k;
+    local = (int)blockIdx.x - B.job[jb].blockStart;
+    return B.job[jb];
+}
+
 template <typename Tin, int ND, int NT>
-__global__ __launch_bounds__(256) void dedisperse_kernel(
-    const Tin* __restrict__ in, typename Acc<Tin>::type* __restrict__ out,
-    const int32_t* __restrict__ delayT, int nChan, int nsamp, int tdms, int tdmsPad, int nOut,
-    int W, int CB, int nDMTiles)
+__global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 {
     using TA = typename Acc<Tin>::type;
+    int local;
+    const DDJob& J = find_job(B, local);
+    const Tin* __restrict__ in = static_cast<const Tin*>(J.in);
+    TA* __restrict__ out = static_cast<TA*>(J.out);
+    const int32_t* __restrict__ delayT = J.delayT;
+    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, tdmsPad = J.tdmsPad, nOut = J.nOut;
+    const int W = J.rowUnits, CB = J.CB, nDMTiles = J.nDMTiles;
     constexpr int TT = 64 * NT;
     constexpr int TD = 4 * ND;
     constexpr int E = (int)sizeof(Tin);
@@ -81,7 +116,7 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
     int* offs0 = reinterpret_cast<int*>(smem_raw + 2 * CB * rowBytes);
 
     int tTile, dmTile;
-    tile_of(blockIdx.x, nDMTiles, tTile, dmTile);
+    tile_of(local, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
     if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
@@ -168,10 +203,16 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(
 // odd element is read from a second copy of the row shifted by one element,
 // built in LDS with alignbyte from the staged dwords.
 template <int ND, int NP>
-__global__ __launch_bounds__(256) void dedisperse_f16_kernel(
-    const _Float16* __restrict__ in, float* __restrict__ out, const int32_t* __restrict__ delayT, int nChan,
-    int nsamp, int tdms, int tdmsPad, int nOut, int rowDw, int CB, int nDMTiles, float one)
+__global__ __launch_bounds__(256) void dedisperse_f16_kernel(const DDBatch B)
 {
+    int local;
+    const DDJob& J = find_job(B, local);
+    const _Float16* __restrict__ in = static_cast<const _Float16*>(J.in);
+    float* __restrict__ out = static_cast<float*>(J.out);
+    const int32_t* __restrict__ delayT = J.delayT;
+    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, tdmsPad = J.tdmsPad, nOut = J.nOut;
+    const int rowDw = J.rowUnits, CB = J.CB, nDMTiles = J.nDMTiles;
+    const float one = B.one;
     constexpr int TT = 128 * NP;
     constexpr int TD = 4 * ND;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -180,7 +221,7 @@ __global__ __launch_bounds__(256) void dedisperse_f16_kernel(
     const int bufDw = CB * (2 * rowDw + TD);
 
     int tTile, dmTile;
-    tile_of(blockIdx.x, nDMTiles, tTile, dmTile);
+    tile_of(local, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
     if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
@@ -297,10 +338,15 @@ __global__ __launch_bounds__(256) void dedisperse_f16_kernel(
 // at once (add3); the halves are flushed into int32 sums every 256 channels,
 // before they can overflow.  Integer sums are exact in any order.
 template <int ND, int NQ>
-__global__ __launch_bounds__(256) void dedisperse_u8_kernel(
-    const uint8_t* __restrict__ in, int32_t* __restrict__ out, const int32_t* __restrict__ delayT, int nChan,
-    int nsamp, int tdms, int tdmsPad, int nOut, int rowDw, int CB, int nDMTiles)
+__global__ __launch_bounds__(256) void dedisperse_u8_kernel(const DDBatch B)
 {
+    int local;
+    const DDJob& J = find_job(B, local);
+    const uint8_t* __restrict__ in = static_cast<const uint8_t*>(J.in);
+    int32_t* __restrict__ out = static_cast<int32_t*>(J.out);
+    const int32_t* __restrict__ delayT = J.delayT;
+    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, tdmsPad = J.tdmsPad, nOut = J.nOut;
+    const int rowDw = J.rowUnits, CB = J.CB, nDMTiles = J.nDMTiles;
     constexpr int TT = 256 * NQ;
     constexpr int TD = 4 * ND;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -309,7 +355,7 @@ __global__ __launch_bounds__(256) void dedisperse_u8_kernel(
     const int bufDw = CB * (4 * rowDw + TD);
 
     int tTile, dmTile;
-    tile_of(blockIdx.x, nDMTiles, tTile, dmTile);
+    tile_of(local, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
     if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
@@ -502,14 +548,25 @@ void lds_geometry(const Shape& s, int span, int& rowUnits, int& perChannel)
     }
 }
 
-template <typename Tin>
-int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT, int nChan,
-                 int nsamp, int tdms, int tdmsPad, int nOut, int span, hipStream_t stream)
+int span_of(const ppl_dedisp_plan* p, int TD)
 {
-    using TA = typename Acc<Tin>::type;
+    return p->spanTD[TD <= 8 ? 0 : TD <= 16 ? 1 : TD <= 32 ? 2 : 3];
+}
+
+int elem_bytes(int dtype) { return dtype == PPL_F32 ? 4 : dtype == PPL_F16 ? 2 : 1; }
+
+// Fills a job for plan p under shape s; *ldsBytes receives the job's LDS need.
+int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out, DDJob& J,
+             size_t* ldsBytes)
+{
     const int TT = 64 * s.nt, TD = 4 * s.nd;
+    const int span = span_of(p, TD);
     int rowUnits = 0, perChannel = 0;
-    lds_geometry<Tin>(s, span, rowUnits, perChannel);
+    switch (p->cfg.dtype) {
+        case PPL_F32: lds_geometry<float>(s, span, rowUnits, perChannel); break;
+        case PPL_F16: lds_geometry<_Float16>(s, span, rowUnits, perChannel); break;
+        default: lds_geometry<uint8_t>(s, span, rowUnits, perChannel); break;
+    }
     if (perChannel > kMaxLDS) {
         set_error("dedisperse: delay spread %d too large for tile %dx%d", span, TD, TT);
         return PPL_E_CONFIG;
@@ -517,19 +574,40 @@ int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT
     int CB = std::min(16, kMaxLDS / perChannel);
     if (s.cb > 0) CB = std::min(CB, s.cb);
     CB = std::max(CB, 1);
-    const int nDMTiles = (tdms + TD - 1) / TD;
-    const int nTTiles = (nOut + TT - 1) / TT;
-    const int nTTilesPad = (nTTiles + kXcds - 1) / kXcds * kXcds;
-    dim3 grid((unsigned)nDMTiles * nTTilesPad), block(256);
-    const size_t lds = (size_t)CB * perChannel;
-    TA* o = static_cast<TA*>(out);
+    J.in = in;
+    J.out = out;
+    J.delayT = p->d_delayT;
+    J.nChan = (int)p->cfg.nChan;
+    J.nsamp = (int)p->cfg.nsampBuf;
+    J.tdms = (int)p->cfg.tdms;
+    J.tdmsPad = p->tdmsPad;
+    J.nOut = p->nOut;
+    J.rowUnits = rowUnits;
+    J.CB = CB;
+    J.nDMTiles = (J.tdms + TD - 1) / TD;
+    J.blockStart = 0;
+    J.pad_ = 0;
+    *ldsBytes = (size_t)CB * perChannel;
+    return PPL_OK;
+}
+
+int job_blocks(const DDJob& J, const Shape& s)
+{
+    const int TT = 64 * s.nt;
+    const int nTTiles = (J.nOut + TT - 1) / TT;
+    return J.nDMTiles * ((nTTiles + kXcds - 1) / kXcds * kXcds);
+}
+
+template <typename Tin>
+int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, hipStream_t stream)
+{
+    dim3 grid(blocks), block(256);
     if constexpr (std::is_same<Tin, _Float16>::value) {
-#define PPL_DD16_CASE(ND_, NT_)                                                                         \
-    if (s.nd == ND_ && s.nt == NT_) {                                                                   \
-        hipLaunchKernelGGL((dedisperse_f16_kernel<ND_, NT_ / 2>), grid, block, lds, stream, in, o, delayT, \
-                           nChan, nsamp, tdms, tdmsPad, nOut, rowUnits, CB, nDMTiles, 1.0f);          \
-        PPL_LAUNCHED();                                                                                 \
-        return PPL_OK;                                                                                  \
+#define PPL_DD16_CASE(ND_, NT_)                                                                  \
+    if (s.nd == ND_ && s.nt == NT_) {                                                            \
+        hipLaunchKernelGGL((dedisperse_f16_kernel<ND_, NT_ / 2>), grid, block, lds, stream, B); \
+        PPL_LAUNCHED();                                                                          \
+        return PPL_OK;                                                                           \
     }
         PPL_DD16_CASE(8, 8)
         PPL_DD16_CASE(8, 4)
@@ -540,12 +618,11 @@ int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT
         PPL_DD16_CASE(16, 2)
 #undef PPL_DD16_CASE
     } else if constexpr (std::is_same<Tin, uint8_t>::value) {
-#define PPL_DD8_CASE(ND_, NT_)                                                                          \
-    if (s.nd == ND_ && s.nt == NT_) {                                                                   \
-        hipLaunchKernelGGL((dedisperse_u8_kernel<ND_, NT_ / 4>), grid, block, lds, stream, in, o, delayT,  \
-                           nChan, nsamp, tdms, tdmsPad, nOut, rowUnits, CB, nDMTiles);                \
-        PPL_LAUNCHED();                                                                                 \
-        return PPL_OK;                                                                                  \
+#define PPL_DD8_CASE(ND_, NT_)                                                                  \
+    if (s.nd == ND_ && s.nt == NT_) {                                                           \
+        hipLaunchKernelGGL((dedisperse_u8_kernel<ND_, NT_ / 4>), grid, block, lds, stream, B); \
+        PPL_LAUNCHED();                                                                         \
+        return PPL_OK;                                                                          \
     }
         PPL_DD8_CASE(8, 8)
         PPL_DD8_CASE(8, 4)
@@ -555,12 +632,11 @@ int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT
         PPL_DD8_CASE(16, 4)
 #undef PPL_DD8_CASE
     } else {
-#define PPL_DD_CASE(ND_, NT_)                                                                   \
-    if (s.nd == ND_ && s.nt == NT_) {                                                           \
-        hipLaunchKernelGGL((dedisperse_kernel<Tin, ND_, NT_>), grid, block, lds, stream, in, o, \
-                           delayT, nChan, nsamp, tdms, tdmsPad, nOut, rowUnits, CB, nDMTiles);  \
-        PPL_LAUNCHED();                                                                         \
-        return PPL_OK;                                                                          \
+#define PPL_DD_CASE(ND_, NT_)                                                                       \
+    if (s.nd == ND_ && s.nt == NT_) {                                                               \
+        hipLaunchKernelGGL((dedisperse_kernel<Tin, ND_, NT_>), grid, block, lds, stream, B);       \
+        PPL_LAUNCHED();                                                                             \
+        return PPL_OK;                                                                              \
     }
         PPL_DD_CASE(8, 8)
         PPL_DD_CASE(8, 4)
@@ -572,6 +648,46 @@ int launch_shape(const Shape& s, const Tin* in, void* out, const int32_t* delayT
 #undef PPL_DD_CASE
     }
     set_error("dedisperse: unsupported tile %dx%d", s.nd, s.nt);
+    return PPL_E_ARG;
+}
+
+// One launch over n (plan, in, out) jobs of one dtype under shape s.  Jobs
+// are ordered by channel count, largest first, so the longest workgroups are
+// dispatched first and the short ones fill the tail.
+int run_jobs(const ppl_dedisp_job* jobs, int n, const Shape& s, hipStream_t st)
+{
+    PPL_CHECK_ARG(n >= 1 && n <= kMaxDDJobs, "%d dedispersion jobs (1..%d)", n, kMaxDDJobs);
+    const int dtype = jobs[0].plan->cfg.dtype;
+    int order[kMaxDDJobs];
+    for (int k = 0; k < n; ++k) {
+        PPL_CHECK_ARG(jobs[k].plan && jobs[k].d_in && jobs[k].d_out, "job %d: null argument", k);
+        PPL_CHECK_ARG(jobs[k].plan->cfg.dtype == dtype, "job %d: dtype differs from job 0", k);
+        order[k] = k;
+    }
+    std::stable_sort(order, order + n, [&](int a, int b) {
+        return jobs[a].plan->cfg.nChan > jobs[b].plan->cfg.nChan;
+    });
+    DDBatch B{};
+    B.n = n;
+    B.one = 1.0f;
+    size_t lds = 0;
+    long blocks = 0;
+    for (int k = 0; k < n; ++k) {
+        const ppl_dedisp_job& jb = jobs[order[k]];
+        size_t l = 0;
+        const int rc = make_job(jb.plan, s, jb.d_in, jb.d_out, B.job[k], &l);
+        if (rc != PPL_OK) return rc;
+        B.job[k].blockStart = (int)blocks;
+        blocks += job_blocks(B.job[k], s);
+        lds = std::max(lds, l);
+    }
+    PPL_CHECK_ARG(blocks < (1L << 31), "grid too large");
+    switch (dtype) {
+        case PPL_F32: return launch_batch<float>(s, B, (unsigned)blocks, lds, st);
+        case PPL_F16: return launch_batch<_Float16>(s, B, (unsigned)blocks, lds, st);
+        case PPL_U8: return launch_batch<uint8_t>(s, B, (unsigned)blocks, lds, st);
+    }
+    set_error("unknown dtype");
     return PPL_E_ARG;
 }
 
@@ -599,9 +715,8 @@ int tile_span(const std::vector<int32_t>& d, int tdms, int nChan, int TD)
 
 int upload_delays(ppl_dedisp_plan* p)
 {
-    const int TD = 4 * p->nd;
     const int tdms = (int)p->cfg.tdms, nChan = (int)p->cfg.nChan;
-    const int pad = ((tdms + TD - 1) / TD) * TD;
+    const int pad = ((tdms + 63) / 64) * 64;
     std::vector<int32_t> t((size_t)nChan * pad);
     for (int c = 0; c < nChan; ++c)
         for (int dm = 0; dm < pad; ++dm)
@@ -611,7 +726,7 @@ int upload_delays(ppl_dedisp_plan* p)
     PPL_HIP(hipMalloc(&p->d_delayT, t.size() * sizeof(int32_t)));
     PPL_HIP(hipMemcpy(p->d_delayT, t.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     p->tdmsPad = pad;
-    p->span = tile_span(p->delays, tdms, nChan, TD);
+    for (int k = 0; k < 4; ++k) p->spanTD[k] = tile_span(p->delays, tdms, nChan, 8 << k);
     return PPL_OK;
 }
 
@@ -729,7 +844,7 @@ extern "C" int ppl_dedisp_set_shape(ppl_dedisp_plan* p, int tileDM, int tileT, i
     p->nd = nd;
     p->nt = nt;
     p->cb = chanBlock;
-    return upload_delays(p);
+    return PPL_OK;
 }
 
 extern "C" int ppl_dedisp_autotune(ppl_dedisp_plan* p, const void* d_in, void* d_out, int reps, void* stream,
@@ -747,16 +862,9 @@ extern "C" int ppl_dedisp_autotune(ppl_dedisp_plan* p, const void* d_in, void* d
     for (const Shape& base : kShapes) {
         for (int cb : cbs) {
             if (ppl_dedisp_set_shape(p, 4 * base.nd, 64 * base.nt, cb) != PPL_OK) continue;
-            int rowUnits, perChannel;
-            const int elem = p->cfg.dtype == PPL_F32 ? 4 : p->cfg.dtype == PPL_F16 ? 2 : 1;
-            (void)elem;
-            if (p->cfg.dtype == PPL_F16)
-                lds_geometry<_Float16>(Shape{p->nd, p->nt, cb}, p->span, rowUnits, perChannel);
-            else if (p->cfg.dtype == PPL_F32)
-                lds_geometry<float>(Shape{p->nd, p->nt, cb}, p->span, rowUnits, perChannel);
-            else
-                lds_geometry<uint8_t>(Shape{p->nd, p->nt, cb}, p->span, rowUnits, perChannel);
-            if (perChannel > kMaxLDS || cb > kMaxLDS / perChannel) continue;
+            DDJob J{};
+            size_t lds = 0;
+            if (make_job(p, Shape{p->nd, p->nt, cb}, d_in, d_out, J, &lds) != PPL_OK || J.CB < cb) continue;
             if (ppl_dedisperse(p, d_in, d_out, stream) != PPL_OK) continue;  // warm
             PPL_HIP(hipEventRecord(e0, st));
             for (int r = 0; r < reps; ++r)
@@ -788,22 +896,73 @@ extern "C" int ppl_dedisp_delays(const ppl_dedisp_plan* p, int32_t* host_out)
 extern "C" int ppl_dedisperse(const ppl_dedisp_plan* p, const void* d_in, void* d_out, void* stream)
 {
     PPL_CHECK_ARG(p && d_in && d_out, "null argument");
-    const Shape s{p->nd, p->nt, p->cb};
-    const int nChan = (int)p->cfg.nChan, nsamp = (int)p->cfg.nsampBuf, tdms = (int)p->cfg.tdms;
-    hipStream_t st = stream_of(stream);
-    switch (p->cfg.dtype) {
-        case PPL_F32:
-            return launch_shape<float>(s, static_cast<const float*>(d_in), d_out, p->d_delayT, nChan,
-                                       nsamp, tdms, p->tdmsPad, p->nOut, p->span, st);
-        case PPL_F16:
-            return launch_shape<_Float16>(s, static_cast<const _Float16*>(d_in), d_out, p->d_delayT,
-                                          nChan, nsamp, tdms, p->tdmsPad, p->nOut, p->span, st);
-        case PPL_U8:
-            return launch_shape<uint8_t>(s, static_cast<const uint8_t*>(d_in), d_out, p->d_delayT,
-                                         nChan, nsamp, tdms, p->tdmsPad, p->nOut, p->span, st);
+    const ppl_dedisp_job job{p, d_in, d_out};
+    return run_jobs(&job, 1, Shape{p->nd, p->nt, p->cb}, stream_of(stream));
+}
+
+extern "C" int ppl_dedisperse_batch(const ppl_dedisp_job* jobs, int n, int tileDM, int tileT, int chanBlock,
+                                    void* stream)
+{
+    PPL_CHECK_ARG(jobs && n >= 1, "no jobs");
+    PPL_CHECK_ARG(jobs[0].plan, "job 0: null plan");
+    Shape s{jobs[0].plan->nd, jobs[0].plan->nt, jobs[0].plan->cb};
+    if (tileDM != 0 || tileT != 0) {
+        PPL_CHECK_ARG(tileDM % 4 == 0 && tileT % 64 == 0 && shape_known(tileDM / 4, tileT / 64),
+                      "tile %dx%d not instantiated", tileDM, tileT);
+        PPL_CHECK_ARG(chanBlock >= 0 && chanBlock <= 16, "channel block %d outside [0, 16]", chanBlock);
+        s = Shape{tileDM / 4, tileT / 64, chanBlock};
     }
-    set_error("unknown dtype");
-    return PPL_E_ARG;
+    return run_jobs(jobs, n, s, stream_of(stream));
+}
+
+extern "C" int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int reps, void* stream,
+                                         int* tileDM, int* tileT, int* chanBlock, float* best_ms)
+{
+    PPL_CHECK_ARG(jobs && n >= 1 && tileDM && tileT && chanBlock, "null argument");
+    if (reps < 1) reps = 1;
+    hipStream_t st = stream_of(stream);
+    hipEvent_t e0, e1;
+    PPL_HIP(hipEventCreate(&e0));
+    PPL_HIP(hipEventCreate(&e1));
+    const int cbs[] = {2, 4, 8, 16};
+    float best = 1e30f;
+    Shape bestShape{0, 0, 0};
+    for (const Shape& base : kShapes) {
+        for (int cb : cbs) {
+            const Shape s{base.nd, base.nt, cb};
+            bool fits = true;
+            for (int k = 0; k < n && fits; ++k) {
+                DDJob J{};
+                size_t lds = 0;
+                fits = jobs[k].plan && make_job(jobs[k].plan, s, jobs[k].d_in, jobs[k].d_out, J, &lds) == PPL_OK &&
+                       J.CB >= cb;
+            }
+            if (!fits || run_jobs(jobs, n, s, st) != PPL_OK) continue;  // warm
+            PPL_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; ++r)
+                if (run_jobs(jobs, n, s, st) != PPL_OK) break;
+            PPL_HIP(hipEventRecord(e1, st));
+            PPL_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PPL_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (ms < best) {
+                best = ms;
+                bestShape = s;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bestShape.nd == 0) {
+        set_error("autotune: no tile shape fits every job");
+        return PPL_E_CONFIG;
+    }
+    *tileDM = 4 * bestShape.nd;
+    *tileT = 64 * bestShape.nt;
+    *chanBlock = bestShape.cb;
+    if (best_ms) *best_ms = best;
+    return PPL_OK;
 }
 
 extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float* buff, float mstartdm,
@@ -850,8 +1009,17 @@ extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float*
     const long used = (long)tdms * nOut * (long)sizeof(float);
     if (outbufSize > used)
         PPL_HIP(hipMemsetAsync(reinterpret_cast<char*>(outbuff) + used, 0, outbufSize - used, st));
-    int rc = launch_shape<float>(s, buff, outbuff, d_delayT, i_nchans, numSamples, tdms, pad, nOut,
-                                 span, st);
+    ppl_dedisp_plan tmp;
+    tmp.cfg.nChan = (unsigned)i_nchans;
+    tmp.cfg.nsampBuf = (unsigned)numSamples;
+    tmp.cfg.tdms = (unsigned)tdms;
+    tmp.cfg.dtype = PPL_F32;
+    tmp.nOut = nOut;
+    tmp.d_delayT = d_delayT;
+    tmp.tdmsPad = pad;
+    for (int k = 0; k < 4; ++k) tmp.spanTD[k] = span;
+    const ppl_dedisp_job job{&tmp, buff, outbuff};
+    int rc = run_jobs(&job, 1, s, st);
     (void)hipFreeAsync(d_delayT, st);
     return rc;
 }

@@ -180,3 +180,55 @@ def test_autotuner_keeps_results_exact(orc, ppl):
     assert ms > 0 and plan.tile[0] % 4 == 0
     p, d = _oracle(orc, x, cfg)
     assert np.array_equal(plan.run(xd).cpu().numpy(), orc.dedisperse(x, d, p["nout"]))
+
+
+def _beam_plan(ppl, n_sub, n_ch, nsamp, tdms, dm_hi, dtype, n_int=4):
+    n_chan = n_sub * n_ch
+    foff = -0.1953125 / n_ch  # 195.3125 kHz subbands
+    tsamp = 5.12e-6 * n_ch * n_int
+    return ppl.DedispPlan(n_chan, nsamp, tdms, 0.0, dm_hi / tdms, tsamp, 150.0, foff, dtype)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16", "u8"])
+def test_batch_equals_separate_runs(ppl, dtype):
+    """Beams with different channel counts, buffer lengths, DM counts and
+    ranges in one launch give, bit for bit, the outputs of separate runs,
+    under the plans' own tile and under a forced one."""
+    import torch
+    shapes = [(8, 16, 4096, 128, 50.0), (31, 16, 8192, 512, 100.0), (12, 64, 8192, 300, 200.0),
+              (4, 256, 4096, 64, 20.0)]
+    rng = np.random.default_rng(9)
+    plans, ins = [], []
+    for n_sub, n_ch, nsamp, tdms, dm_hi in shapes:
+        p = _beam_plan(ppl, n_sub, n_ch, nsamp, tdms, dm_hi, dtype)
+        n_chan = n_sub * n_ch
+        if dtype == "u8":
+            x = torch.from_numpy(rng.integers(0, 256, size=(n_chan, nsamp), dtype=np.uint8))
+        else:
+            x = torch.from_numpy(rng.normal(size=(n_chan, nsamp)).astype(np.float32))
+            if dtype == "f16":
+                x = x.half()
+        plans.append(p)
+        ins.append(x.cuda())
+    single = [p.run(x) for p, x in zip(plans, ins)]
+    for tile in [(0, 0, 0), (16, 256, 4), (32, 512, 2)]:
+        outs = ppl.dedisperse_batch(plans, ins, tile=tile)
+        for k in range(len(plans)):
+            assert torch.equal(outs[k], single[k]), (tile, k)
+    tile, ms = ppl.autotune_batch(plans, ins, outs)
+    assert ms > 0 and tile[0] % 4 == 0 and tile[1] % 64 == 0
+    outs = ppl.dedisperse_batch(plans, ins, tile=tile)
+    for k in range(len(plans)):
+        assert torch.equal(outs[k], single[k]), ("tuned", tile, k)
+
+
+def test_batch_rejects_mixed_dtypes_and_too_many_jobs(ppl):
+    import torch
+    a = _beam_plan(ppl, 8, 16, 4096, 128, 50.0, "f32")
+    b = _beam_plan(ppl, 8, 16, 4096, 128, 50.0, "f16")
+    xa = torch.zeros((128, 4096), device="cuda")
+    xb = torch.zeros((128, 4096), device="cuda", dtype=torch.float16)
+    with pytest.raises(ppl.PPLError):
+        ppl.dedisperse_batch([a, b], [xa, xb])
+    with pytest.raises(ppl.PPLError):
+        ppl.dedisperse_batch([a] * 49, [xa] * 49)
