@@ -48,6 +48,11 @@ PEAK_TFLOPS_FP32 = 157.3   # MI355X FP32 (vector/MFMA) peak
 PEAK_HBM_GBS = 8000.0
 
 
+def log(msg):
+    """Progress on stderr (the JSON line is the only stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +65,9 @@ def parse():
     ap.add_argument("--tile", default="", help="DM,T,chanblock tile for dedispersion (default: autotune)")
     ap.add_argument("--cpu-spectra", type=int, default=8192,
                     help="integrated spectra in the timed CPU sample")
+    ap.add_argument("--beams", type=int, default=24, help="beams in the batched multi-beam run")
+    ap.add_argument("--mb-steps", type=int, default=2, help="timed steps of the multi-beam run")
+    ap.add_argument("--no-multibeam", action="store_true", help="skip the batched multi-beam run")
     return ap.parse_args()
 
 
@@ -195,18 +203,21 @@ def main():
             out.append(gen_chunk(torch, coeffs, n_samp, ci * n_samp, seed + ci, pulses, rfi, impulses))
         return out
 
+    log("generating inputs")
     t_gen = time.time()
     chunks = make_chunks(2024, True)
     prime = chunks[-1][:, :, :overlap * INTEG * N_CH].contiguous()
     torch.cuda.synchronize()
     t_gen = time.time() - t_gen
 
+    log("autotune + prime")
     tune_ms = None if args.tile else chain.autotune()
     chain.prime(prime)
     run_steps(chain, chunks, args.warmup)
     torch.cuda.synchronize()
 
     # ---- timed: the plain (FP32) chain, eager
+    log("timed steps")
     timing = []
     t0 = time.perf_counter()
     plain_events = run_steps(chain, chunks, args.steps, timing)
@@ -222,6 +233,7 @@ def main():
     rtf = nout * TSAMP * args.steps / wall
 
     # ---- the same steps replayed from HIP graphs (detection stays eager)
+    log("graph replay")
     chain.capture_steps([chunks[(args.warmup + args.steps + k) % len(chunks)] for k in range(2)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -257,6 +269,7 @@ def main():
         nrm = float(np.sqrt(N_CHAN))  # series noise in the clipper's normalised units
         ref_out, ref_events = None, None
         for mdt in ("f32", "f16", "u8"):
+            log("mixed-precision chain " + mdt)
             mc = make_chain(mdt)
             if mdt == "f32":
                 mc.plan.set_tile(*chain.plan.tile)
@@ -302,8 +315,13 @@ def main():
                          "and the spectra the clipper rejects (-0.2 minus the bandpass, thousands of raw "
                          "units, as in the reference) saturate it, which is where its events differ")
 
+    mbeam = None
+    if not args.no_multibeam:
+        mbeam = multibeam(args, torch, ppl)
+
     cpu = None
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         cpu = cpu_baseline(args.cpu_spectra, coeffs, bp, nout)
 
     # a device copy of known size for scaling the HBM byte counters
@@ -337,11 +355,140 @@ def main():
         "events_per_step": [len(e[0]) for e in plain_events],
         "clipper_walk": chain.clip.counters(),
         "mixed": mixed,
+        "multibeam": mbeam,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gen_seconds": round(t_gen, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+def draw_beams(seed, n):
+    """The heterogeneous multi-beam workload: per beam 31-62 subbands of
+    195.3125 kHz below 150 MHz, 16/64/256 channels per subband, DM 0 to
+    100-1000 pc/cm^3 over 512-4096 trials, and a 2^14-2^17-sample
+    dedispersion buffer, raised to the next power of two while it holds
+    fewer than 1.25 x maxshift samples (a shorter buffer cannot hold the
+    beam's DM sweep)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n):
+        n_sub = int(rng.integers(31, 63))
+        n_ch = int(rng.choice([16, 64, 256]))
+        dm_hi = float(rng.choice([100, 200, 300, 500, 750, 1000]))
+        tdms = int(rng.choice([512, 1024, 2048, 4096]))
+        nsamp = 1 << int(rng.integers(14, 18))
+        tsamp = TS_IN * n_ch * INTEG
+        foff = -0.1953125 / n_ch
+        fmin = FCH1 + foff * (n_sub * n_ch - 1)
+        maxshift = 4148.741601 * (1.0 / fmin ** 2 - 1.0 / FCH1 ** 2) * dm_hi / tsamp
+        while nsamp < 1.25 * maxshift:
+            nsamp *= 2
+        n_pulse = int(rng.integers(1, 4))
+        pulses = [(float(rng.uniform(0.1, 0.9) * dm_hi), float(rng.uniform(0.0, 1.0)),
+                   float(rng.uniform(1e-3, 8e-3) / tsamp), float(rng.uniform(6.0, 12.0))) for _ in range(n_pulse)]
+        out.append(dict(n_sub=n_sub, n_ch=n_ch, dm_hi=dm_hi, tdms=tdms, nsamp=nsamp, seed=seed + 17 * b,
+                        pulses=pulses))
+    return out
+
+
+def multibeam(args, torch, ppl):
+    """24 beams of different shapes through the batched chain (f32 and f16
+    dedispersion input): one clipper launch and one dedispersion launch per
+    stage for all beams, HIP-graph replayed steps, the batch tile from the
+    autotuner.  Returns the report object."""
+    from tests.synth import beam_bandpass, device_voltages, tone_amplitude
+    beams = draw_beams(2024, args.beams)
+    specs = []
+    for bm in beams:
+        h = ppl.ppf_coefficients(N_TAPS, bm["n_ch"], "hann")
+        foff = -0.1953125 / bm["n_ch"]
+        bp = beam_bandpass(h, bm["n_sub"], bm["n_ch"], INTEG, FCH1, foff, SIGMA)
+        specs.append(ppl.BeamSpec(bm["n_sub"], bm["n_ch"], N_TAPS, h, INTEG, bp, bm["nsamp"], bm["tdms"], 0.0,
+                                  bm["dm_hi"] / bm["tdms"], FCH1, foff))
+    cands = [(16, 512, 4), (32, 512, 4), (16, 256, 4), (32, 256, 8), (16, 512, 8), (8, 512, 8)]
+    report = {"beams": [{k: bm[k] for k in ("n_sub", "n_ch", "dm_hi", "tdms", "nsamp")} for bm in beams]}
+    ref_events = None
+    for dtype in ("f32", "f16"):
+        log("multi-beam %d beams, %s" % (len(specs), dtype))
+        mb = ppl.MultiBeam(specs, dtype=dtype)
+        volts = []
+        for b, (bm, sp) in enumerate(zip(beams, specs)):
+            n_samp = mb.chunk_samples(b)
+            m = mb.chunk_spectra[b]
+            pulses = [(dm, t0 * m, w, tone_amplitude(sp.coeffs, sp.n_ch, SIGMA, a, INTEG))
+                      for (dm, t0, w, a) in bm["pulses"]]
+            volts.append(device_voltages(torch, sp.n_sub, sp.n_ch, INTEG, n_samp, bm["seed"], FCH1, sp.foff,
+                                         SIGMA, pulses=pulses))
+        torch.cuda.synchronize()
+        mem_gb = torch.cuda.memory_allocated() / 1e9
+        log("multi-beam prime (%.1f GB resident)" % mem_gb)
+        mb.prime(volts)
+        log("multi-beam autotune")
+        t0 = time.perf_counter()
+        tune_ms = mb.autotune(candidates=cands)
+        tune_s = time.perf_counter() - t0
+        def run(n, graph=False, timing=None):
+            evs = []
+            for _ in range(n):
+                ev = None
+                if timing is not None:
+                    ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
+                    ev[0].record()
+                if graph:
+                    mb.replay()
+                else:
+                    mb.step(volts, None if ev is None else ev[1:6])
+                evs.append([mb.detect(b)[0] for b in range(len(specs))])
+                if ev is not None:
+                    ev[6].record()
+                    timing.append(ev)
+            return evs
+        log("multi-beam steps")
+        run(args.warmup)
+        torch.cuda.synchronize()
+        timing = []
+        t0 = time.perf_counter()
+        events = run(args.mb_steps, timing=timing)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.mb_steps
+        names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "carry", "detect"]
+        stages = {nm: round(sum(ev[j].elapsed_time(ev[j + 1]) for ev in timing) / args.mb_steps, 3)
+                  for j, nm in enumerate(names)}
+        mb.capture_steps(volts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(args.mb_steps, graph=True)
+        torch.cuda.synchronize()
+        gwall = (time.perf_counter() - t0) / args.mb_steps
+        secs = mb.data_seconds()
+        rec = {"value": mb.trials_samples() / wall, "ms_per_step": round(1e3 * wall, 3),
+               "graph_ms_per_step": round(1e3 * gwall, 3),
+               "graph_value": mb.trials_samples() / gwall,
+               "real_time_factor_aggregate": sum(secs) / wall, "real_time_factor_min_beam": min(secs) / wall,
+               "stages_ms": stages, "tile_dm_t_chanblock": list(mb.tile), "autotune_ms": round(tune_ms, 3),
+               "autotune_s": round(tune_s, 1), "resident_gb": round(mem_gb, 1),
+               "dedisperse_adds_per_step": float(sum(sp.tdms * n * sp.n_sub * sp.n_ch
+                                                     for sp, n in zip(specs, mb.nout))),
+               "events": sum(len(e) - 1 for step in events for e in step)}
+        rec["dedisperse_tadds_per_s"] = round(rec["dedisperse_adds_per_step"] / (stages["dedisperse"] * 1e-3) / 1e12, 3)
+        if dtype == "f32":
+            ref_events = events
+        else:
+            n_ref = n_got = inter = 0
+            for se_r, se_g in zip(ref_events, events):
+                for er, eg in zip(se_r, se_g):
+                    rk, gk = event_keys(er), event_keys(eg)
+                    n_ref, n_got, inter = n_ref + len(rk), n_got + len(gk), inter + len(rk & gk)
+            rec["event_recall_vs_f32"] = inter / max(n_ref, 1)
+            rec["event_precision_vs_f32"] = inter / max(n_got, 1)
+        report[dtype] = rec
+        report["data_seconds_per_step"] = round(sum(secs), 1)
+        report["trials_samples_per_step"] = mb.trials_samples()
+        del mb, volts
+        torch.cuda.empty_cache()
+    return report
 
 
 def cpu_baseline(n_spec, coeffs, bp, nout):

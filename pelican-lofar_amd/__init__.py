@@ -251,8 +251,30 @@ def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0), stream=None):
     return d_outs
 
 
-def autotune_batch(plans, d_ins, d_outs, reps=1, stream=None):
-    """Fastest (tileDM, tileT, chanBlock) for the batch, and its time in ms."""
+def autotune_batch(plans, d_ins, d_outs, reps=1, stream=None, candidates=None):
+    """Fastest (tileDM, tileT, chanBlock) for the batch, and its time in ms:
+    every instantiated tile and channel block, or only the given candidates
+    (each timed over reps launches after one untimed launch)."""
+    if candidates is not None:
+        import torch
+        best = None
+        for cand in candidates:
+            try:
+                dedisperse_batch(plans, d_ins, d_outs, tile=cand, stream=stream)
+            except PPLError:
+                continue  # does not fit every job
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                dedisperse_batch(plans, d_ins, d_outs, tile=cand, stream=stream)
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            if best is None or ms < best[1]:
+                best = (tuple(cand), ms)
+        if best is None:
+            raise PPLError("autotune: no candidate tile fits every job")
+        return best
     arr, n = _dedisp_jobs(plans, d_ins, d_outs)
     td, tt, cb, best = C.c_int(), C.c_int(), C.c_int(), C.c_float()
     check(lib().ppl_dedisp_autotune_batch(arr, n, reps, _stream(stream), C.byref(td), C.byref(tt),
@@ -632,3 +654,150 @@ class BeamChain:
             return detect_u8(self.out, self.n_chan, 1.0 / self.inv_scale, self.n_ch, self.n_sub, threshold,
                              bin_pow2, max_events)
         return detect(self.out, self.n_ch, self.n_sub, threshold, bin_pow2, max_events)
+
+
+# ------------------------------------------------------------- many beams
+class BeamSpec:
+    """One beam's shape for MultiBeam: nSub x nCh channels, the FIR, the
+    clipper's bandpass, the dedispersion buffer and DM range."""
+
+    def __init__(self, n_sub, n_ch, n_taps, coeffs, integrate, bp_values, nsamp_buf, tdms, dm_low, dm_step,
+                 fch1, foff, tsamp=None, cr=10.0, sr=4.0, history=10000, zero_dm=True):
+        self.n_sub, self.n_ch, self.n_taps, self.integrate = n_sub, n_ch, n_taps, integrate
+        self.coeffs = np.asarray(coeffs, np.float32)
+        self.bp_values = np.asarray(bp_values, np.float64)
+        self.nsamp_buf, self.tdms, self.dm_low, self.dm_step = nsamp_buf, tdms, dm_low, dm_step
+        self.fch1, self.foff = fch1, foff
+        self.tsamp = tsamp if tsamp is not None else 5.12e-6 * n_ch * integrate
+        self.cr, self.sr, self.history, self.zero_dm = cr, sr, history, zero_dm
+
+
+class MultiBeam:
+    """Many beams with different shapes stepped together on one device, one
+    dedispersion buffer of new data per beam per step: per beam the fused
+    PPF -> Stokes-I -> integration over its input chunk, then one batched
+    clipper run for every beam, the transposition into each beam's buffer,
+    one batched dedispersion launch for every beam and the overlap carries.
+
+    Each beam reads a voltage chunk of chunk_spectra[b] integrated spectra
+    (a divisor of its nOut), fed repeatedly as a continuous stream; a step
+    takes nOut[b] / chunk_spectra[b] channeliser calls per beam.  Every
+    launch goes to the current stream, so steps can be captured in two HIP
+    graphs (even and odd steps: the FIR histories and buffers alternate)."""
+
+    def __init__(self, specs, dtype="f32", inv_scale=127.0 / 10.0, max_chunk_samples=1 << 20, tile=None):
+        import torch
+        if len(specs) > min(MAX_DEDISP_JOBS, 48):
+            raise PPLError("at most 48 beams")
+        self.specs, self.dtype, self.inv_scale = list(specs), dtype, inv_scale
+        self.plans, self.ppfs, self.clips, self.bufs, self.stokes, self.outs = [], [], [], [], [], []
+        self.nout, self.overlap, self.chunk_spectra = [], [], []
+        for sp in self.specs:
+            n_chan = sp.n_sub * sp.n_ch
+            plan = DedispPlan(n_chan, sp.nsamp_buf, sp.tdms, sp.dm_low, sp.dm_step, sp.tsamp, sp.fch1, sp.foff,
+                              dtype=dtype)
+            self.plans.append(plan)
+            self.nout.append(plan.nout)
+            self.overlap.append(plan.maxshift + plan.rem)
+            self.ppfs.append(PPF(sp.n_sub, sp.n_ch, sp.n_taps, sp.coeffs, integrate=sp.integrate))
+            self.clips.append(Clipper(sp.bp_values, sp.cr, sp.sr, sp.history, sp.zero_dm))
+            self.bufs.append(DDBuffer(n_chan, sp.nsamp_buf, plan.maxshift + plan.rem, dtype, inv_scale))
+            self.stokes.append(torch.empty((plan.nout, sp.n_sub, 1, sp.n_ch), dtype=torch.float32, device="cuda"))
+            self.outs.append(torch.empty((sp.tdms, plan.nout), dtype=plan.out_dtype(), device="cuda"))
+            per = sp.integrate * sp.n_ch  # input samples per integrated spectrum
+            m = max(1, min(plan.nout, max_chunk_samples // per))
+            while plan.nout % m:
+                m -= 1
+            self.chunk_spectra.append(m)
+        self.tile = tuple(tile) if tile else (0, 0, 0)
+        self.graphs = None
+        self.parity = 0
+
+    def chunk_samples(self, b):
+        """Input samples per subband and polarisation in beam b's chunk."""
+        sp = self.specs[b]
+        return self.chunk_spectra[b] * sp.integrate * sp.n_ch
+
+    def data_seconds(self):
+        """Seconds of telescope data per step, per beam."""
+        return [n * sp.tsamp for n, sp in zip(self.nout, self.specs)]
+
+    def trials_samples(self):
+        return sum(sp.tdms * n for sp, n in zip(self.specs, self.nout))
+
+    def autotune(self, reps=1, candidates=None):
+        """Picks the batch's tile on the beams' own buffers (their contents do
+        not change the time); candidates: (tileDM, tileT, chanBlock) tuples to
+        try instead of every instantiated shape."""
+        self.tile, ms = autotune_batch(self.plans, [bf.tensor() for bf in self.bufs], self.outs, reps=reps,
+                                       candidates=candidates)
+        return ms
+
+    def _channelise(self, b, d_volt, n_spec, d_out):
+        """n_spec integrated spectra of beam b into d_out, chunk by chunk."""
+        sp, m = self.specs[b], self.chunk_spectra[b]
+        done = 0
+        while done < n_spec:
+            k = min(m, n_spec - done)
+            src = d_volt if k == m else d_volt[:, :, :k * sp.integrate * sp.n_ch].contiguous()
+            self.ppfs[b].stokes_i16(src, k * sp.integrate, d_out=d_out[done:done + k])
+            done += k
+
+    def prime(self, volts):
+        """Fill every beam's overlap (untimed set-up, beam by beam)."""
+        import torch
+        for b, sp in enumerate(self.specs):
+            ov = self.overlap[b]
+            st = torch.empty((ov, sp.n_sub, 1, sp.n_ch), dtype=torch.float32, device="cuda")
+            self._channelise(b, volts[b], ov, st)
+            self.clips[b].run(st)
+            used, full = self.bufs[b].add(st.view(ov, sp.n_sub * sp.n_ch))
+            if used != ov or full:
+                raise PPLError("prime: beam %d did not take its overlap" % b)
+
+    def step(self, volts, events=None):
+        """One buffer for every beam; events: five torch events recorded after
+        the channeliser, clipper, buffer, dedispersion and carry stages."""
+        def mark(i):
+            if events is not None:
+                events[i].record()
+        for b in range(len(self.specs)):
+            self._channelise(b, volts[b], self.nout[b], self.stokes[b])
+        mark(0)
+        clipper_run_batch(self.clips, self.stokes)
+        mark(1)
+        for b, sp in enumerate(self.specs):
+            used, full = self.bufs[b].add(self.stokes[b].view(self.nout[b], sp.n_sub * sp.n_ch))
+            if used != self.nout[b] or not full:
+                raise PPLError("step: beam %d's buffer did not fill" % b)
+        mark(2)
+        dedisperse_batch(self.plans, [bf.tensor() for bf in self.bufs], self.outs, tile=self.tile)
+        mark(3)
+        for bf in self.bufs:
+            bf.rotate()
+        mark(4)
+        self.parity ^= 1
+
+    def capture_steps(self, volts):
+        import torch
+        if self.graphs is not None:
+            return
+        gs = {}
+        for _ in range(2):
+            g = torch.cuda.CUDAGraph()
+            p = self.parity
+            with torch.cuda.graph(g):
+                self.step(volts)
+            gs[p] = g
+        self.graphs = gs
+
+    def replay(self):
+        self.graphs[self.parity].replay()
+        self.parity ^= 1
+
+    def detect(self, b, threshold=6.0, bin_pow2=6, max_events=1 << 22):
+        sp = self.specs[b]
+        if self.dtype == "u8":
+            return detect_u8(self.outs[b], sp.n_sub * sp.n_ch, 1.0 / self.inv_scale, sp.n_ch, sp.n_sub, threshold,
+                             bin_pow2, max_events)
+        return detect(self.outs[b], sp.n_ch, sp.n_sub, threshold, bin_pow2, max_events)

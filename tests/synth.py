@@ -82,3 +82,61 @@ def stokes_noise(n_t, n_sub, n_ch, level, seed, dof=16, n_pol=1):
     lv = np.broadcast_to(np.asarray(level, np.float64), (n_sub * n_ch,)).reshape(n_sub, n_ch)
     x = rng.chisquare(dof, size=(n_t, n_sub, n_pol, n_ch)) / dof
     return (x * lv[None, :, None, :]).astype(np.float32)
+
+
+def device_voltages(torch, n_sub, n_ch, integrate, n_samp, seed, fch1, foff, sigma=64.0, pulses=(),
+                    rfi=(), impulse_frac=0.0, impulse_sigma=20.0, wrap=True):
+    """int16 voltages [sub][pol][n_samp][2] generated on the device: Gaussian
+    noise, narrowband tones rfi = [(sub, ch, amp)], broadband impulses in a
+    fraction of the integrated samples, and dispersed tone-burst pulses
+    [(dm, t0, width, amp)] (t0, width in integrated samples).  With wrap the
+    chunk is periodic: a pulse's samples past the end re-enter at the start,
+    so the chunk can be fed repeatedly as a continuous stream."""
+    n_chan = n_sub * n_ch
+    tsamp = 5.12e-6 * n_ch * integrate
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    out = torch.empty((n_sub, 2, n_samp, 2), dtype=torch.int16, device="cuda")
+    shift = torch.from_numpy(shift_table(n_chan, fch1, foff)).cuda()
+    spi = n_ch * integrate
+    n_int = n_samp // spi
+    imp_idx = torch.nonzero(torch.rand(n_int, generator=g, device="cuda") < impulse_frac).flatten()
+    k = torch.arange(n_ch, device="cuda", dtype=torch.float64)
+    nu = ((k - n_ch / 2.0) / n_ch)[:, None]
+    for s in range(n_sub):
+        x = torch.randn((2, n_samp, 2), generator=g, device="cuda") * sigma
+        if imp_idx.numel():
+            seg = (imp_idx[:, None] * spi + torch.arange(spi, device="cuda")[None, :]).flatten()
+            x[:, seg, :] += torch.randn((2, seg.numel(), 2), generator=g, device="cuda") * (impulse_sigma * sigma)
+        m = torch.arange(n_samp, device="cuda", dtype=torch.float64)
+        for (rs, rk, ramp) in rfi:
+            if rs != s:
+                continue
+            ph = 2 * np.pi * ((rk - n_ch / 2.0) / n_ch) * m
+            x[:, :, 0] += (ramp * torch.cos(ph)).float()[None, :]
+            x[:, :, 1] += (ramp * torch.sin(ph)).float()[None, :]
+        for (dm, t0, width, amp) in pulses:
+            d = shift[s * n_ch:(s + 1) * n_ch] * dm / tsamp
+            a = torch.round((t0 + d) * spi).long()
+            L = max(1, int(round(width * spi)))
+            idx = a[:, None] + torch.arange(L, device="cuda")[None, :]
+            ph = 2 * np.pi * nu * idx.double()
+            if wrap:
+                idx = idx % n_samp
+                ok = torch.ones_like(idx, dtype=torch.bool)
+            else:
+                ok = (idx >= 0) & (idx < n_samp)
+            for p in range(2):
+                x[p, :, 0].index_add_(0, idx[ok], (amp * torch.cos(ph + p)).float()[ok])
+                x[p, :, 1].index_add_(0, idx[ok], (amp * torch.sin(ph + p)).float()[ok])
+        out[s] = torch.clamp(torch.round(x), -32768, 32767).to(torch.int16)
+    return out
+
+
+def beam_bandpass(coeffs, n_sub, n_ch, integrate, fch1, foff, sigma=64.0):
+    """.bp values of a flat bandpass at the mean integrated Stokes-I of noise
+    of this sigma through this filter, with its standard deviation as RMS."""
+    e_h = float(np.sum(np.asarray(coeffs, np.float64) ** 2))
+    level = 2 * integrate * 2.0 * sigma * sigma * e_h
+    noise_std = np.sqrt(2 * integrate) * 2.0 * sigma * sigma * e_h
+    n_chan = n_sub * n_ch
+    return np.array([n_chan, fch1, fch1 + foff * (n_chan - 1), foff, noise_std, level], np.float64)

@@ -1,0 +1,83 @@
+"""Many beams of different shapes stepped together (batched clipper and
+dedispersion launches) give, bit for bit, what independent single-beam chains
+give on the same voltage streams, eagerly and replayed from HIP graphs."""
+import numpy as np
+import pytest
+import torch
+
+from synth import beam_bandpass, device_voltages
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 16, 4096, 128, 50.0), (4, 64, 8192, 64, 100.0), (2, 256, 2048, 32, 200.0)]
+
+
+def _specs(ppl, dtype):
+    specs = []
+    for n_sub, n_ch, nsamp, tdms, dm_hi in SHAPES:
+        h = ppl.ppf_coefficients(8, n_ch, "hann")
+        foff = -0.1953125 / n_ch
+        bp = beam_bandpass(h, n_sub, n_ch, 4, 150.0, foff)
+        specs.append(ppl.BeamSpec(n_sub, n_ch, 8, h, 4, bp, nsamp, tdms, 0.0, dm_hi / tdms, 150.0, foff,
+                                  history=1000))
+    return specs
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_multibeam_equals_single_beam_chains(ppl, dtype):
+    specs = _specs(ppl, dtype)
+    mb = ppl.MultiBeam(specs, dtype=dtype, max_chunk_samples=1 << 15)
+    volts = []
+    for b, sp in enumerate(specs):
+        pulses = [(0.6 * sp.dm_step * sp.tdms, 100.0, 3.0, 40.0)]
+        volts.append(device_voltages(torch, sp.n_sub, sp.n_ch, 4, mb.chunk_samples(b), 70 + b, sp.fch1, sp.foff,
+                                     pulses=pulses, impulse_frac=0.002))
+    singles = []
+    for b, sp in enumerate(specs):
+        ch = ppl.BeamChain(sp.n_sub, sp.n_ch, 8, sp.coeffs, 4, sp.bp_values, sp.nsamp_buf, sp.tdms, 0.0,
+                           sp.dm_step, sp.tsamp, sp.fch1, sp.foff, dtype=dtype, history=1000)
+        singles.append(ch)
+    # the single chains read the same periodic stream: the chunk repeated
+    reps = [mb.nout[b] // mb.chunk_spectra[b] for b in range(len(specs))]
+    step_in = [volts[b].repeat(1, 1, reps[b], 1).contiguous() for b in range(len(specs))]
+    prime_in = []
+    for b, sp in enumerate(specs):
+        n = mb.overlap[b] * 4 * sp.n_ch
+        whole = volts[b].repeat(1, 1, n // mb.chunk_samples(b) + 1, 1)
+        prime_in.append(whole[:, :, :n].contiguous())
+    mb.prime(volts)
+    for b, ch in enumerate(singles):
+        ch.prime(prime_in[b])
+    # MultiBeam's steps read each chunk from its start, after the prime's
+    # partial chunk: the single chains' step input is the chunk repeated
+    mb_in = volts
+    for k in range(2):
+        mb.step(mb_in)
+        for b, ch in enumerate(singles):
+            ch.step(step_in[b])
+            assert torch.equal(mb.outs[b], ch.out), (k, b)
+            ev_m, val_m, _ = mb.detect(b)
+            ev_s, val_s, _ = ch.detect()
+            assert np.array_equal(ev_m, ev_s) and np.array_equal(val_m, val_s), (k, b)
+    # graph replays continue the same streams
+    mb.capture_steps(mb_in)
+    for b, ch in enumerate(singles):
+        ch.capture_steps(step_in[b])
+    for k in range(3):
+        mb.replay()
+        for b, ch in enumerate(singles):
+            ch.replay()
+        torch.cuda.synchronize()
+        for b, ch in enumerate(singles):
+            assert torch.equal(mb.outs[b], ch.out), ("graph", k, b)
+
+
+def test_multibeam_autotune_keeps_results(ppl):
+    specs = _specs(ppl, "f32")
+    mb = ppl.MultiBeam(specs, max_chunk_samples=1 << 15)
+    ms = mb.autotune()
+    assert ms > 0 and mb.tile[0] % 4 == 0
+    outs = [torch.randn((sp.n_sub * sp.n_ch, sp.nsamp_buf), device="cuda") for sp in specs]
+    got = ppl.dedisperse_batch(mb.plans, outs, tile=mb.tile)
+    for p, x, g in zip(mb.plans, outs, got):
+        assert torch.equal(p.run(x), g)
