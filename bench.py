@@ -68,6 +68,11 @@ def parse():
     ap.add_argument("--beams", type=int, default=24, help="beams in the batched multi-beam run")
     ap.add_argument("--mb-steps", type=int, default=2, help="timed steps of the multi-beam run")
     ap.add_argument("--no-multibeam", action="store_true", help="skip the batched multi-beam run")
+    ap.add_argument("--no-service", action="store_true", help="skip the streaming-service run")
+    ap.add_argument("--svc-seconds", type=float, default=30.0, help="wall seconds of the streaming run")
+    ap.add_argument("--svc-accel", type=float, default=10.0,
+                    help="data seconds per wall second in the streaming run (arrival schedule speed-up)")
+    ap.add_argument("--svc-dtype", default="f16", choices=["f32", "f16", "u8"])
     return ap.parse_args()
 
 
@@ -319,6 +324,10 @@ def main():
     if not args.no_multibeam:
         mbeam = multibeam(args, torch, ppl)
 
+    svc = None
+    if not args.no_service:
+        svc = service(args, torch, ppl, coeffs, bp)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -356,6 +365,7 @@ def main():
         "clipper_walk": chain.clip.counters(),
         "mixed": mixed,
         "multibeam": mbeam,
+        "service": svc,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gen_seconds": round(t_gen, 1),
@@ -445,9 +455,14 @@ def multibeam(args, torch, ppl):
                     ev[6].record()
                     timing.append(ev)
             return evs
-        log("multi-beam steps")
-        run(args.warmup)
+        # warm up until every beam's clipper has left its learning phase
+        # (history 10000 plus the 999 rejected warm-up spectra)
+        need = max(-(-max(0, 11000 - ov) // n) for ov, n in zip(mb.overlap, mb.nout))
+        warm = max(args.warmup, need)
+        log("multi-beam steps (%d warm-up)" % warm)
+        run(warm)
         torch.cuda.synchronize()
+        cnt0 = [c.counters() for c in mb.clips]
         timing = []
         t0 = time.perf_counter()
         events = run(args.mb_steps, timing=timing)
@@ -456,6 +471,8 @@ def multibeam(args, torch, ppl):
         names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "carry", "detect"]
         stages = {nm: round(sum(ev[j].elapsed_time(ev[j + 1]) for ev in timing) / args.mb_steps, 3)
                   for j, nm in enumerate(names)}
+        cnt1 = [c.counters() for c in mb.clips]
+        walk = {k: sum(c1[k] - c0[k] for c0, c1 in zip(cnt0, cnt1)) for k in cnt0[0]}
         mb.capture_steps(volts)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -468,7 +485,8 @@ def multibeam(args, torch, ppl):
                "graph_value": mb.trials_samples() / gwall,
                "real_time_factor_aggregate": sum(secs) / wall, "real_time_factor_min_beam": min(secs) / wall,
                "stages_ms": stages, "tile_dm_t_chanblock": list(mb.tile), "autotune_ms": round(tune_ms, 3),
-               "autotune_s": round(tune_s, 1), "resident_gb": round(mem_gb, 1),
+               "autotune_s": round(tune_s, 1), "resident_gb": round(mem_gb, 1), "warmup_steps": warm,
+               "clipper_walk": walk,
                "dedisperse_adds_per_step": float(sum(sp.tdms * n * sp.n_sub * sp.n_ch
                                                      for sp, n in zip(specs, mb.nout))),
                "events": sum(len(e) - 1 for step in events for e in step)}
@@ -489,6 +507,149 @@ def multibeam(args, torch, ppl):
         del mb, volts
         torch.cuda.empty_cache()
     return report
+
+
+def service(args, torch, ppl, coeffs, bp):
+    """48 beams of the LOFAR-beam shape as a continuous chunk stream: beam
+    sessions start at Poisson times (mean 2/s) and last 30-300 s, each beam
+    admitted while its device footprint fits the HBM budget, one pulse per
+    10.7 s chunk per beam (DM 0-500, 6-15 sigma, 1-8 ms) plus narrowband RFI.
+    The schedule runs args.svc_accel times faster than real time, so the
+    deadline of a chunk is its data duration / accel.  Chunk latency is from
+    the moment its last sample is due to the end of its round (including the
+    dedispersion and detection of a buffer it completes).  Returns the report
+    object; event recall compares the service's buffers with a plain FP32
+    per-beam run over the same chunk sequence for a sample of sessions."""
+    from tests.synth import device_voltages, tone_amplitude
+    rng = np.random.default_rng(4096)
+    n_beams, chunk_spec, ring = 48, 8192, 6
+    accel = args.svc_accel
+    spec = ppl.BeamSpec(N_SUB, N_CH, N_TAPS, coeffs, INTEG, bp, NSAMP_BUF, TDMS, 0.0, DM_STEP, FCH1, FOFF)
+    chunk_s = chunk_spec * TSAMP             # data seconds per chunk
+    n_samp = chunk_spec * INTEG * N_CH
+    log("service: %d-chunk input ring" % ring)
+    chunks = []
+    for c in range(ring):
+        pulses = [(float(rng.uniform(0, 500)), float(rng.uniform(0.05, 0.95) * chunk_spec),
+                   float(rng.uniform(1e-3, 8e-3) / TSAMP),
+                   tone_amplitude(coeffs, N_CH, SIGMA, float(rng.uniform(6, 15)), INTEG))]
+        rfi = [(int(rng.integers(N_SUB)), int(rng.integers(N_CH)), tone_amplitude(coeffs, N_CH, SIGMA, 50.0, INTEG))
+               for _ in range(10)]
+        chunks.append(device_voltages(torch, N_SUB, N_CH, INTEG, n_samp, 600 + c, FCH1, FOFF, SIGMA,
+                                      pulses=pulses, rfi=rfi, wrap=False))
+    torch.cuda.synchronize()
+    free, total = torch.cuda.mem_get_info()
+    budget = int(0.85 * free)
+    svc = ppl.StreamService(spec, chunk_spec, budget, dtype=args.svc_dtype, max_beams=n_beams)
+    fp_gb = svc.footprint() / 1e9
+    # arrival schedule over the run, in data seconds
+    horizon = args.svc_seconds * accel
+    t, arrivals = 0.0, []
+    while t < horizon:
+        t += rng.exponential(0.5)
+        arrivals.append((t, float(rng.uniform(30.0, 300.0)), int(rng.integers(1 << 30))))
+    idle = list(range(n_beams))
+    sessions = {}          # beam -> dict(start, end, next chunk index, sequence, key)
+    log_seq = {}           # session key -> list of ring indices fed
+    events = {}            # session key -> {buffer index: event set}
+    lat, missed, rounds, n_chunks = [], 0, 0, 0
+    rejected = dropped = admitted = 0
+    max_active = 0
+    ai = 0
+    log("service: %.0f s of data at x%.0f, budget %.1f GB, footprint %.2f GB per beam" %
+        (horizon, accel, budget / 1e9, svc.footprint() / 1e9))
+    w0 = time.perf_counter()
+    last_log = w0
+    while True:
+        now = (time.perf_counter() - w0) * accel
+        if now >= horizon:
+            break
+        while ai < len(arrivals) and arrivals[ai][0] <= now:
+            t_a, dur, seed = arrivals[ai]
+            ai += 1
+            if not idle:
+                dropped += 1
+                continue
+            b = idle.pop(int(seed % len(idle)))
+            key = (b, t_a)
+            if not svc.admit(key):
+                rejected += 1
+                idle.append(b)
+                continue
+            admitted += 1
+            sessions[b] = {"start": t_a, "end": t_a + dur, "k": 0, "key": key, "seed": seed}
+            log_seq[key] = []
+            events[key] = {}
+        max_active = max(max_active, len(sessions))
+        ready, due = [], []
+        for b, ss in list(sessions.items()):
+            t_due = ss["start"] + (ss["k"] + 1) * chunk_s
+            if t_due > ss["end"]:
+                svc.release(ss["key"])
+                del sessions[b]
+                idle.append(b)
+                continue
+            if t_due <= now:
+                ci = (ss["seed"] + ss["k"]) % ring
+                ready.append((ss["key"], chunks[ci]))
+                due.append(t_due)
+                log_seq[ss["key"]].append(ci)
+                ss["k"] += 1
+        if not ready:
+            nxt = min([ss["start"] + (ss["k"] + 1) * chunk_s for ss in sessions.values()] +
+                      ([arrivals[ai][0]] if ai < len(arrivals) else [horizon]))
+            time.sleep(max(0.0, min(0.05, (nxt - now) / accel)))
+            continue
+        for key, idx, ints, vals in svc.process(ready):
+            events[key][idx] = event_keys(ints)
+        torch.cuda.synchronize()
+        done = (time.perf_counter() - w0) * accel
+        for t_due in due:
+            l_wall = (done - t_due) / accel
+            lat.append(l_wall)
+            missed += l_wall > chunk_s / accel
+        rounds += 1
+        n_chunks += len(ready)
+        if time.perf_counter() - last_log > 10.0:
+            last_log = time.perf_counter()
+            log("service: %.0f s data, %d active, %d chunks" % (now, len(sessions), n_chunks))
+    wall = time.perf_counter() - w0
+    for b, ss in list(sessions.items()):
+        svc.release(ss["key"])
+    del svc
+    torch.cuda.empty_cache()
+    lat_ms = np.array(lat) * 1e3
+    # recall against plain FP32 per-beam runs of the same chunk sequences
+    log("service: recall check")
+    keys = sorted([k for k in events if events[k]], key=lambda k: -len(events[k]))[:3]
+    n_ref = n_got = inter = 0
+    for key in keys:
+        plain = ppl.StreamService(spec, chunk_spec, 1 << 40, dtype="f32", max_beams=1)
+        plain.admit(key)
+        ref = {}
+        for ci in log_seq[key]:
+            for _, idx, ints, _ in plain.process([(key, chunks[ci])]):
+                ref[idx] = event_keys(ints)
+        for idx, ev in events[key].items():
+            r = ref.get(idx, set())
+            n_ref, n_got, inter = n_ref + len(r), n_got + len(ev), inter + len(r & ev)
+        del plain
+    torch.cuda.empty_cache()
+    data_s = n_chunks * chunk_s
+    return {"beams": n_beams, "arrival_rate_per_s": 2.0, "session_s": [30, 300], "accel": accel,
+            "dtype": args.svc_dtype, "chunk_spectra": chunk_spec, "chunk_data_s": round(chunk_s, 3),
+            "deadline_ms": round(1e3 * chunk_s / accel, 1), "deadline_ms_real_time": round(1e3 * chunk_s, 1),
+            "latency_ms_p50": round(float(np.percentile(lat_ms, 50)), 2) if lat else None,
+            "latency_ms_p99": round(float(np.percentile(lat_ms, 99)), 2) if lat else None,
+            "latency_ms_max": round(float(lat_ms.max()), 2) if lat else None,
+            "missed_deadline": int(missed), "chunks": n_chunks, "rounds": rounds, "wall_s": round(wall, 1),
+            "sessions_admitted": admitted, "rejected_by_hbm": rejected, "dropped_no_idle_beam": dropped,
+            "max_active": max_active, "footprint_gb_per_beam": round(fp_gb, 3),
+            "budget_gb": round(budget / 1e9, 1),
+            "throughput_data_s_per_s": round(data_s / wall, 1),
+            "throughput_dm_trials_samples_per_s": data_s / TSAMP * TDMS / wall,
+            "event_recall_vs_plain_f32": inter / max(n_ref, 1), "event_precision_vs_plain_f32": inter / max(n_got, 1),
+            "recall_sessions": len(keys), "recall_events_ref": n_ref}
 
 
 def cpu_baseline(n_spec, coeffs, bp, nout):

@@ -801,3 +801,115 @@ class MultiBeam:
             return detect_u8(self.outs[b], sp.n_sub * sp.n_ch, 1.0 / self.inv_scale, sp.n_ch, sp.n_sub, threshold,
                              bin_pow2, max_events)
         return detect(self.outs[b], sp.n_ch, sp.n_sub, threshold, bin_pow2, max_events)
+
+
+# ------------------------------------------------------------ streaming
+class StreamService:
+    """A streaming loop over many beams of one shape, each arriving and
+    leaving on its own: beams are admitted while their device footprint fits
+    the HBM budget, and their FIR history, clipper state and dedispersion
+    overlap stay resident across chunks.  Chunks from all beams that are
+    ready are processed in rounds: per beam the fused channeliser over its
+    chunk, one batched clipper run for every beam in the round, the
+    transposition into each beam's buffer, one batched dedispersion launch
+    for every beam whose buffer filled, its detection, then the overlap carry
+    and the rest of the chunk into the next buffer (DedispersionBuffer's
+    split, src/lib/src/DedispersionBuffer.cpp:98-179).  All beams share one
+    dedispersion plan (one delay table)."""
+
+    class Beam:
+        def __init__(self, svc, key):
+            import torch
+            sp = svc.spec
+            self.key = key
+            self.ppf = PPF(sp.n_sub, sp.n_ch, sp.n_taps, sp.coeffs, integrate=sp.integrate)
+            self.clip = Clipper(sp.bp_values, sp.cr, sp.sr, sp.history, sp.zero_dm)
+            self.buf = DDBuffer(svc.n_chan, sp.nsamp_buf, svc.overlap, svc.dtype, svc.inv_scale)
+            self.stokes = torch.empty((svc.chunk_spectra, sp.n_sub, 1, sp.n_ch), dtype=torch.float32,
+                                      device="cuda")
+            self.out = torch.empty((sp.tdms, svc.plan.nout), dtype=svc.plan.out_dtype(), device="cuda")
+            self.buffers = 0  # dedispersion buffers completed
+
+    def __init__(self, spec, chunk_spectra, budget_bytes, dtype="f32", inv_scale=127.0 / 10.0, tile=None,
+                 max_beams=48):
+        self.spec, self.dtype, self.inv_scale = spec, dtype, inv_scale
+        self.n_chan = spec.n_sub * spec.n_ch
+        self.plan = DedispPlan(self.n_chan, spec.nsamp_buf, spec.tdms, spec.dm_low, spec.dm_step, spec.tsamp,
+                               spec.fch1, spec.foff, dtype=dtype)
+        if tile is not None:
+            self.plan.set_tile(*tile)
+        self.overlap = self.plan.maxshift + self.plan.rem
+        self.chunk_spectra = int(chunk_spectra)
+        self.budget = int(budget_bytes)
+        self.max_beams = min(int(max_beams), MAX_DEDISP_JOBS)
+        self.beams = {}
+        self.used = 0
+
+    def footprint(self):
+        """Device bytes one admitted beam holds: the two alternating
+        dedispersion buffers, the chunk's spectra, the dedispersed output, the
+        channeliser history and the clipper's state and work areas."""
+        sp = self.spec
+        elem = {"f32": 4, "f16": 2, "u8": 1}[self.dtype]
+        buf = 2 * self.n_chan * sp.nsamp_buf * elem
+        stokes = self.chunk_spectra * self.n_chan * 4
+        out = sp.tdms * self.plan.nout * 4
+        ppf = 2 * sp.n_sub * 2 * (sp.n_taps - 1) * sp.n_ch * 8
+        clip = 64 * self.n_chan * 4 + self.chunk_spectra * 256 + (16 << 20)
+        return buf + stokes + out + ppf + clip
+
+    def admit(self, key):
+        """Admits beam `key` if its footprint fits the budget; returns whether
+        it did."""
+        if key in self.beams:
+            raise PPLError("beam %r already admitted" % (key,))
+        fp = self.footprint()
+        if len(self.beams) >= self.max_beams or self.used + fp > self.budget:
+            return False
+        self.beams[key] = StreamService.Beam(self, key)
+        self.used += fp
+        return True
+
+    def release(self, key):
+        b = self.beams.pop(key)
+        self.used -= self.footprint()
+        del b
+
+    def process(self, ready, threshold=6.0, bin_pow2=6):
+        """One round over ready = [(key, int16 voltages [sub][pol][chunk*integrate*nCh][2])],
+        at most one chunk per beam.  Returns [(key, buffer index, event ints,
+        event values)] for the buffers completed in this round."""
+        sp = self.spec
+        m = self.chunk_spectra
+        beams = [self.beams[k] for k, _ in ready]
+        if len({id(b) for b in beams}) != len(beams):
+            raise PPLError("one chunk per beam per round")
+        for b, (_, v) in zip(beams, ready):
+            b.ppf.stokes_i16(v, m * sp.integrate, d_out=b.stokes)
+        if beams:
+            clipper_run_batch([b.clip for b in beams], [b.stokes for b in beams])
+        used = []
+        full = []
+        for b in beams:
+            u, f = b.buf.add(b.stokes.view(m, self.n_chan))
+            used.append(u)
+            if f:
+                full.append(b)
+        results = []
+        if full:
+            dedisperse_batch([self.plan] * len(full), [b.buf.tensor() for b in full], [b.out for b in full])
+            for b in full:
+                if self.dtype == "u8":
+                    ints, vals, _ = detect_u8(b.out, self.n_chan, 1.0 / self.inv_scale, sp.n_ch, sp.n_sub,
+                                              threshold, bin_pow2)
+                else:
+                    ints, vals, _ = detect(b.out, sp.n_ch, sp.n_sub, threshold, bin_pow2)
+                results.append((b.key, b.buffers, ints, vals))
+                b.buffers += 1
+                b.buf.rotate()
+        for b, u in zip(beams, used):
+            if u < m:
+                u2, f2 = b.buf.add(b.stokes.view(m, self.n_chan), first=u)
+                if u + u2 != m or f2:
+                    raise PPLError("chunk larger than one buffer of new samples")
+        return results
