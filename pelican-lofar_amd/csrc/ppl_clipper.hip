@@ -662,7 +662,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     __shared__ int goodCnt[8];
     __shared__ int fillCnt;
     __shared__ float sh_dataModel, sh_margin;
-    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_stop;
+    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_stop, sh_span;
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
     __shared__ Staged stg[kStage];
@@ -807,16 +807,17 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     // division itself.  The spectrum tolerance RN_f32(RN_f64(a / sqrtN))
     // lies within one float step of RN_f32(RN_f64(a * RN_f64(1/sqrtN))),
     // so the division is needed only when the comparison falls in that step.
-    enum { FAST_LIMIT = 0, FAST_BOX = 1, FAST_RESET = 2 };
+    enum { FAST_LIMIT = 0, FAST_BOX = 1, FAST_RESET = 2, FAST_MAXN = 3 };
     const double rcpCap = 1.0 / (double)capf;
     const double rcpScale = 1.0 / (double)K.scale;
     const double rcpSqrtN = 1.0 / K.sqrtNBins;
     const bool fastOK = K.cap >= 1000 && K.cap > kStage && isfinite(K.scale) && fabsf(K.scale) >= 0x1p-126f;
-    auto fast_run = [&](int& t) -> int {
+    auto fast_run = [&](int& t, int maxN) -> int {
         const int k0 = t - stageBase;
         const bool zerosLeft = s.remainingZeros != 0;
-        const int kLim = min(min(stageN, tEnd - stageBase),
+        const int kBlk = min(min(stageN, tEnd - stageBase),
                              min(preN - (stageBase - tStart), zerosLeft ? blkUF[k0] : blkU[k0]));
+        const int kLim = min(kBlk, k0 + maxN);
         if (k0 >= kLim) return FAST_LIMIT;
         const float mOR = s.meanOverRMS;
         const bool useMOR = K.useMeanOverRMS != 0;
@@ -923,6 +924,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 break;
             }
         }
+        if (status == FAST_LIMIT && kLim < kBlk) status = FAST_MAXN;
         s.count[CNT_FAST] += (t - stageBase) - k0;
         s.meanRunAve = mRA;
         s.rmsRunAve = rA;
@@ -948,8 +950,176 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         return status;
     };
 
-    enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3 };
+    // ---- speculative steady-state block (whole workgroup).  Assumes every
+    // spectrum from t on is kept and inside the box.  The four recurrences
+    // (running mean, running rms, bandpass offset, bad-channel fraction) are
+    // then folds over values known in advance; each is run by one lane with
+    // exactly the operations of the scalar run, in the same order, so every
+    // value is bit-identical to it.  The decisions are then checked in
+    // parallel and the block is committed up to the first spectrum that is
+    // not kept or leaves the box; that spectrum goes to the scalar run.
+    // Returns the spectra committed.
+    auto spec_span = [&](int t) -> int {
+        const int k0 = t - stageBase;
+        const bool zerosLeft = s.remainingZeros != 0;
+        int kLim = min(min(stageN, tEnd - stageBase),
+                       min(preN - (stageBase - tStart), zerosLeft ? blkUF[k0] : blkU[k0]));
+        const float mOR = s.meanOverRMS;
+        if (K.useMeanOverRMS && !(isfinite(mOR) && fabsf(mOR) >= 0x1p-126f)) return 0;
+        if (s.num == 0) return 0;  // the statistics-reset check falls on spectrum t
+        kLim = min(kLim, k0 + (period - s.num));
+        return max(0, kLim - k0);
+    };
+    float* const spM = rowChunk;                                   // running mean after spectrum k
+    float* const spR = rowChunk + kStage;                          // running rms after k
+    float* const spS = rowChunk + 2 * kStage;                      // spectrumRMS pushed by k
+    float* const spB = rowChunk + 3 * kStage;                      // that value / history length
+    float* const spQ = rowChunk + 4 * kStage;                      // offset step of k
+    int* const spD = reinterpret_cast<int*>(rowChunk + 5 * kStage); // k moves the offset
+    float* const spP = rowChunk + 6 * kStage;                      // offset before k
+    float* const spPA = rowChunk + 7 * kStage;                     // offset after k
+    float* const spBM = rowChunk + 8 * kStage;                     // bandpass mean after k
+    float* const spF = rowChunk + 9 * kStage;                      // bad-channel fraction after k
+    static_assert(10 * kStage <= kRowChunk, "speculation arrays alias the row chunk");
+    __shared__ int sh_fail;
+    auto spec_block = [&](int t, int span) -> int {
+        const int k0 = t - stageBase, kEnd = k0 + span;
+        const bool useMOR = K.useMeanOverRMS != 0;
+        const float mOR = s.meanOverRMS;
+        const double rcpMOR = 1.0 / (double)mOR;
+        if (tid == 0) {
+            float v = s.meanRunAve, bc = s.meanBack / capf;
+            for (int k = k0; k < kEnd; ++k) {
+                v = (v - stg[k].meanFC) + bc;
+                spM[k] = v;
+                bc = stg[k].meanC;
+            }
+        } else if (tid == 64) {
+            float f = s.fractionBadChannels;
+            for (int k = k0; k < kEnd; ++k) {
+                f += stg[k].fracInc;
+                spF[k] = f;
+            }
+        }
+        __syncthreads();
+        {
+            const int k = tid;
+            if (k >= k0 && k < kEnd) {
+                const float mPrev = k == k0 ? s.meanRunAve : spM[k - 1];
+                float sR = s.spectrumRMS;
+                if (useMOR) sR = fabsf((float)((double)mPrev * rcpMOR));
+                float rb = (float)((double)sR * rcpCap);
+                if (!(fabsf(rb) >= 0x1p-125f) || (useMOR && !(sR >= 0x1p-125f))) {
+                    if (useMOR) sR = fabsf(mPrev / mOR);
+                    rb = sR / capf;
+                }
+                spS[k] = sR;
+                spB[k] = rb;
+                // the bandpass mean before k is the running mean after k-1 (an
+                // unchanged one differs from it at most in the sign of a zero,
+                // which neither the test nor the step below can see)
+                const float bmPrev = k == k0 ? s.bpMean : spM[k - 1];
+                const float delta = spM[k] - bmPrev;
+                int d = 0;
+                float qd = 0.0f;
+                if (fabsf(delta) > 0.0f) {
+                    d = 1;
+                    qd = (float)((double)delta * rcpScale);
+                    if (!(fabsf(qd) >= 0x1p-125f)) qd = delta / K.scale;
+                }
+                spQ[k] = qd;
+                spD[k] = d;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float r = s.rmsRunAve, bc = s.rmsBack / capf;
+            for (int k = k0; k < kEnd; ++k) {
+                r = (r - stg[k].rmsFC) + bc;
+                spR[k] = r;
+                bc = spB[k];
+            }
+        } else if (tid == 64) {
+            float p = s.params0, bm = s.bpMean;
+            for (int k = k0; k < kEnd; ++k) {
+                spP[k] = p;
+                if (spD[k]) {
+                    p += spQ[k];
+                    bm = spM[k];
+                }
+                spPA[k] = p;
+                spBM[k] = bm;
+            }
+        } else if (tid == 128) {
+            sh_fail = kEnd;
+        }
+        __syncthreads();
+        {
+            const int k = tid;
+            if (k >= k0 && k < kEnd) {
+                const float margin = K.crFactor * spR[k];
+                const float p0 = spP[k];
+                bool ok = p0 >= bx.p0lo && p0 <= bx.p0hi && margin >= bx.mlo && margin <= bx.mhi;
+                const float x = stg[k].mean - spM[k];
+                const float a32 = K.srFactor * (k == k0 ? s.bpRms : spR[k - 1]);
+                const float tolA = (float)((double)a32 * rcpSqrtN);
+                bool gt;
+                if (tolA >= 0x1p-126f && tolA < 0x1p127f) {
+                    const bool gtHi = x > __int_as_float(__float_as_int(tolA) + 1);
+                    const bool gtLo = x > __int_as_float(__float_as_int(tolA) - 1);
+                    gt = gtHi;
+                    if (gtLo != gtHi) gt = x > (float)((double)a32 / K.sqrtNBins);
+                } else {
+                    gt = x > (float)((double)a32 / K.sqrtNBins);
+                }
+                ok = ok && !(gt || (stg[k].flags & 4));
+                if (!ok) atomicMin(&sh_fail, k);
+            }
+        }
+        __syncthreads();
+        const int kf = sh_fail;
+        {
+            const int k = tid;
+            if (k >= k0 && k < kf) {
+                SpecRec rr;
+                rr.params0 = spP[k];
+                rr.dataModel = 0.0f;
+                rr.margin = K.crFactor * spR[k];
+                rr.meanRunAve = spM[k];
+                rr.flags = 1;
+                rr.ver = s.ver;
+                rec[stageBase + k] = rr;
+                rmsBuf[(s.rmsStart + (k - k0)) % K.cap] = spS[k];
+                meanBuf[(s.meanStart + (k - k0)) % K.cap] = stg[k].mean;
+            }
+        }
+        __syncthreads();
+        const int n = kf - k0;
+        if (tid == 0 && n > 0) {
+            const int kl = kf - 1;
+            s.meanRunAve = spM[kl];
+            s.rmsRunAve = spR[kl];
+            s.params0 = spPA[kl];
+            s.bpMean = spBM[kl];
+            s.bpRms = spR[kl];
+            s.spectrumRMS = spS[kl];
+            s.fractionBadChannels = spF[kl];
+            s.num = (s.num + n) % period;
+            s.meanStart = (s.meanStart + n) % K.cap;
+            s.rmsStart = (s.rmsStart + n) % K.cap;
+            s.meanBack2 = n >= 2 ? stg[kl - 1].mean : s.meanBack;
+            s.meanBack = stg[kl].mean;
+            s.rmsBack2 = n >= 2 ? spS[kl - 1] : s.rmsBack;
+            s.rmsBack = spS[kl];
+            s.count[CNT_FAST] += n;
+        }
+        return n;
+    };
+    constexpr int kSpecMin = 24;  // shorter runs go to the scalar run
+
+    enum { MODE_END = 0, MODE_STAGE = 1, MODE_FULL = 2, MODE_BC = 3, MODE_SPEC = 4 };
     int t = tStart;
+    int scalarNext = 0;  // thread 0: spectra the scalar run takes before the next speculation
     for (;;) {
         if (tid == 0) {
             int mode = MODE_END;
@@ -962,7 +1132,17 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                     break;
                 }
                 if (fastOK && bx.valid && sh_frontsValid && s.meanSize == K.cap && s.ver < kMaxVer) {
-                    const int r = fast_run(t);
+                    if (scalarNext == 0) {
+                        const int span = spec_span(t);
+                        if (span >= kSpecMin) {
+                            sh_span = span;
+                            mode = MODE_SPEC;
+                            break;
+                        }
+                    }
+                    const int r = fast_run(t, scalarNext > 0 ? scalarNext : kStage);
+                    scalarNext = 0;
+                    if (r == FAST_MAXN) continue;
                     if (r == FAST_BOX) {
                         // spectrum t had its rms update and left the box
                         const int why = slow_reason(t, sh_margin);
@@ -998,6 +1178,17 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         const int mode = sh_mode;
         t = sh_t;
         if (mode == MODE_END) break;
+        if (mode == MODE_SPEC) {
+            const int span = sh_span;
+            const int n = spec_block(t, span);
+            t += n;
+            if (tid == 0) {
+                if (n < span) scalarNext = 1;  // the spectrum that failed
+                sh_t = t;
+            }
+            __syncthreads();
+            continue;
+        }
         if (mode == MODE_STAGE) {
             stageBase = t;
             stageN = min(kStage, tEnd - t);
