@@ -213,6 +213,7 @@ __device__ __forceinline__ float ring_front(RingView r, int start) { return r.bu
 constexpr int kWalkThreads = 512;
 constexpr int kStage = 512;      // spectra whose precomputed records are staged in LDS at a time
 constexpr int kRowChunk = 8192;  // bins of one spectrum staged in LDS for the ordered sums
+constexpr int kMaxBinsWalk = 32768;  // bins per spectrum the walk supports (kept-bin mask in LDS)
 static_assert(kStage == kWalkThreads, "stage scans use one thread per staged spectrum");
 
 // Per-sub-chunk speculation box: every spectrum of the sub-chunk whose
@@ -386,6 +387,39 @@ __device__ __forceinline__ float fold_sum(const float* row, int n, float sum)
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) sum += a[j];
+    return sum;
+}
+
+// The same fold for long rows (n % 32 == 0, 16-byte aligned): 32 values
+// (eight 16-byte reads) in flight while the previous 32 are added, so the
+// LDS latency hides behind the dependent adds.
+__device__ __forceinline__ float fold_sum_deep(const float* row, int n, float sum)
+{
+    const float4* r4 = reinterpret_cast<const float4*>(row);
+    float4 a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = r4[j];
+    for (int k = 8; k < n / 4; k += 8) {
+        float4 b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = r4[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            sum += a[j].x;
+            sum += a[j].y;
+            sum += a[j].z;
+            sum += a[j].w;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        sum += a[j].x;
+        sum += a[j].y;
+        sum += a[j].z;
+        sum += a[j].w;
+    }
     return sum;
 }
 __device__ __forceinline__ void fold_sum_sq(const float* row, int n, float& sum, float& sumSq)
@@ -656,13 +690,19 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     const Box* __restrict__ box = bst.box;
     const PreRec* __restrict__ pre = bst.pre;
     SpecRec* __restrict__ rec = bst.rec;
+    const float* __restrict__ bpLo = bst.bpLo;
     const int nT = io.nT[beam], subLen = io.subLen[beam];
     __shared__ ClipScalars s;
     __shared__ float red[kWalkThreads / 64 + 1];
     __shared__ int goodCnt[8];
     __shared__ int fillCnt;
     __shared__ float sh_dataModel, sh_margin;
-    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_stop, sh_span;
+    __shared__ int sh_mode, sh_t, sh_fill, sh_frontsValid, sh_stop, sh_span, sh_why;
+    // slow path: kept-bin mask, sums, and how the kept mean is obtained
+    __shared__ unsigned long long keepMask[kMaxBinsWalk / 64];
+    __shared__ double redD[2][kWalkThreads / 64];
+    __shared__ int sh_mism, sh_how, sh_total, sh_badBands;
+    __shared__ float sh_fracInc, sh_meanArg;
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
     __shared__ Staged stg[kStage];
@@ -987,16 +1027,49 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         const bool useMOR = K.useMeanOverRMS != 0;
         const float mOR = s.meanOverRMS;
         const double rcpMOR = 1.0 / (double)mOR;
+        // The folds read their inputs sixteen spectra at a time, ahead of the
+        // dependent adds, so that LDS latency is paid once per sixteen.
+        constexpr int B = 16;
         if (tid == 0) {
             float v = s.meanRunAve, bc = s.meanBack / capf;
-            for (int k = k0; k < kEnd; ++k) {
+            int k = k0;
+            for (; k + B <= kEnd; k += B) {
+                float fc[B], mc[B], o[B];
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    fc[j] = stg[k + j].meanFC;
+                    mc[j] = stg[k + j].meanC;
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    v = (v - fc[j]) + bc;
+                    o[j] = v;
+                    bc = mc[j];
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) spM[k + j] = o[j];
+            }
+            for (; k < kEnd; ++k) {
                 v = (v - stg[k].meanFC) + bc;
                 spM[k] = v;
                 bc = stg[k].meanC;
             }
         } else if (tid == 64) {
             float f = s.fractionBadChannels;
-            for (int k = k0; k < kEnd; ++k) {
+            int k = k0;
+            for (; k + B <= kEnd; k += B) {
+                float fi[B], o[B];
+#pragma unroll
+                for (int j = 0; j < B; ++j) fi[j] = stg[k + j].fracInc;
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    f += fi[j];
+                    o[j] = f;
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) spF[k + j] = o[j];
+            }
+            for (; k < kEnd; ++k) {
                 f += stg[k].fracInc;
                 spF[k] = f;
             }
@@ -1034,14 +1107,58 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         __syncthreads();
         if (tid == 0) {
             float r = s.rmsRunAve, bc = s.rmsBack / capf;
-            for (int k = k0; k < kEnd; ++k) {
+            int k = k0;
+            for (; k + B <= kEnd; k += B) {
+                float fc[B], bb[B], o[B];
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    fc[j] = stg[k + j].rmsFC;
+                    bb[j] = spB[k + j];
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    r = (r - fc[j]) + bc;
+                    o[j] = r;
+                    bc = bb[j];
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) spR[k + j] = o[j];
+            }
+            for (; k < kEnd; ++k) {
                 r = (r - stg[k].rmsFC) + bc;
                 spR[k] = r;
                 bc = spB[k];
             }
         } else if (tid == 64) {
             float p = s.params0, bm = s.bpMean;
-            for (int k = k0; k < kEnd; ++k) {
+            int k = k0;
+            for (; k + B <= kEnd; k += B) {
+                float q[B], m[B], op[B], opa[B], obm[B];
+                int d[B];
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    q[j] = spQ[k + j];
+                    m[j] = spM[k + j];
+                    d[j] = spD[k + j];
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    op[j] = p;
+                    if (d[j]) {
+                        p += q[j];
+                        bm = m[j];
+                    }
+                    opa[j] = p;
+                    obm[j] = bm;
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    spP[k + j] = op[j];
+                    spPA[k + j] = opa[j];
+                    spBM[k + j] = obm[j];
+                }
+            }
+            for (; k < kEnd; ++k) {
                 spP[k] = p;
                 if (spD[k]) {
                     p += spQ[k];
@@ -1128,6 +1245,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 if (t >= stageBase + stageN) { mode = MODE_STAGE; break; }
                 if (s.rmsSize < K.cap) {  // learning phase
                     ++s.count[CNT_LEARN];
+                    sh_why = CNT_LEARN;
                     mode = MODE_FULL;
                     break;
                 }
@@ -1147,6 +1265,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                         // spectrum t had its rms update and left the box
                         const int why = slow_reason(t, sh_margin);
                         ++s.count[why < 0 ? CNT_BOX : why];
+                        sh_why = why < 0 ? CNT_BOX : why;
                         sh_dataModel = 0.0f;
                         mode = MODE_BC;
                         break;
@@ -1160,6 +1279,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 const int why = slow_reason(t, margin);
                 if (why >= 0) {
                     ++s.count[why];
+                    sh_why = why;
                     sh_dataModel = dataModel;
                     sh_margin = margin;
                     mode = MODE_BC;
@@ -1315,38 +1435,155 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         }
         __syncthreads();
         const float dataModel = sh_dataModel, margin = sh_margin;
-        // channel clipping (:315-342), kept values staged in LDS in bin order
-        // (clipped bins as +0.0, which leaves a sum that starts at +0.0
-        // unchanged) and summed in that order by one lane
+        // channel clipping (:315-342): the decisions (clipped bins take their
+        // last-good value), the kept counts per band, a mask of the kept bins
+        // and double-precision bounds of their sum.  The ordered sum of the
+        // kept values (clipped bins as +0.0, which leaves a sum that starts at
+        // +0.0 unchanged), taken by one lane, is needed only when
+        //  * the spectrum may be kept (its mean enters the running mean), and
+        //  * the mask differs from the precomputed one (whose sum is the
+        //    precomputed mean): a spectrum that is rejected for certain
+        //    leaves no trace of its mean, since the ring entry it pushes is
+        //    overwritten by the rejection.
         float spectrumSum = 0.0f;
         const bool oneChunk = nBins <= kRowChunk;
-        for (int x0 = 0; x0 < nBins; x0 += kRowChunk) {
-            const int n = min(kRowChunk, nBins - x0);
-            for (int k = tid; k < n; k += kWalkThreads) {
-                const int i = x0 + k, sb = i / nCh, c = i % nCh;
-                const long i0 = idx(sb, 0, t) + c;
-                const float v = I[i0];
-                const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
-                if (v - dataModel - bpv > margin) {
-                    const float g = lastGood[i];
-                    for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t) + c] = g;
-                    rowChunk[k] = 0.0f;
-                } else {
-                    atomicAdd(&goodCnt[i / cpb], 1);
-                    rowChunk[k] = v;
+        const float invCpbW = 1.0f / (float)cpb;
+        const bool chkPre = sh_why == CNT_UNDECIDED && bx.valid && s.remainingZeros == 0;
+        double sd = 0.0, ad = 0.0;
+        int mism = 0;
+        if (tid == 0) sh_mism = 0;
+        __syncthreads();
+        // each thread's bins are read kG at a time, all in flight together,
+        // before any of them is judged (the replacements write I)
+        constexpr int kG = 8;
+        for (int kb0 = (tid & ~63); kb0 < nBins; kb0 += kWalkThreads * kG) {
+            float vv[kG], bl[kG];
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+                const int i = kb0 + j * kWalkThreads + (tid & 63);
+                vv[j] = 0.0f;
+                bl[j] = 0.0f;
+                if (i < nBins) {
+                    const int sb = i / nCh, c = i % nCh;
+                    vv[j] = I[idx(sb, 0, t) + c];
+                    if (chkPre) bl[j] = bpLo[i];
                 }
             }
-            __syncthreads();
-            if (tid == 0) spectrumSum = fold_sum(rowChunk, n, spectrumSum);
-            if (!oneChunk) __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+                const int k0 = kb0 + j * kWalkThreads;  // uniform over the wave
+                if (k0 >= nBins) break;
+                const int i = k0 + (tid & 63);
+                bool kept = false;
+                int band = 0;
+                if (i < nBins) {
+                    const float v = vv[j];
+                    const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+                    band = band_of(i, invCpbW);
+                    if (v - dataModel - bpv > margin) {
+                        const int sb = i / nCh, c = i % nCh;
+                        const float g = lastGood[i];
+                        for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t) + c] = g;
+                    } else {
+                        kept = true;
+                        sd += (double)v;
+                        ad += fabs((double)v);
+                        // the precompute kept a bin only when v - bpLo <= mlo
+                        if (chkPre && !((v - bl[j]) <= bx.mlo)) mism = 1;
+                    }
+                }
+                const unsigned long long kb = __ballot(kept);
+                if ((tid & 63) == 0) keepMask[k0 >> 6] = kb;
+                const int bLo = band_of(k0, invCpbW);
+                const int bHi = band_of(min(k0 + 63, nBins - 1), invCpbW);
+                for (int b = bLo; b <= bHi; ++b) {
+                    const int c = __popcll(__ballot(kept && band == b));
+                    if ((tid & 63) == 0 && c) atomicAdd(&goodCnt[b], c);
+                }
+            }
         }
+        if (__ballot(mism) != 0 && (tid & 63) == 0) atomicOr(&sh_mism, 1);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            sd += __shfl_xor(sd, off, 64);
+            ad += __shfl_xor(ad, off, 64);
+        }
+        if ((tid & 63) == 0) {
+            redD[0][tid >> 6] = sd;
+            redD[1][tid >> 6] = ad;
+        }
+        __syncthreads();
         if (tid == 0) {
             int cnt[8];
             for (int b = 0; b < 8; ++b) cnt[b] = goodCnt[b];
             int total, badBands;
             float fracInc;
             band_stats(cnt, nBins, cpb, total, badBands, fracInc);
-            sh_fill = phaseC(t, spectrumSum / (float)(unsigned)total, fracInc, badBands, dataModel, margin, 1);
+            sh_total = total;
+            sh_badBands = badBands;
+            sh_fracInc = fracInc;
+            int how = 0;  // 0: fold; 1: the precomputed mean; 2: rejected for certain
+            if (chkPre && !sh_mism) how = 1;
+            if (s.meanSize == K.cap && K.cap >= 1000) {
+                if (badBands >= 4) {
+                    how = 2;
+                    sh_meanArg = 0.0f;
+                } else if (how == 0 && total > 0) {
+                    // the running mean and tolerance the decision will use (:394-425)
+                    const float mPred = (s.meanRunAve - meanFrontDiv(t)) + s.meanBack / (float)s.meanSize;
+                    const float tol = (float)((double)(K.srFactor * s.bpRms) / K.sqrtNBins);
+                    double S = 0.0, A = 0.0;
+                    for (int w = 0; w < kWalkThreads / 64; ++w) {
+                        S += redD[0][w];
+                        A += redD[1][w];
+                    }
+                    if (isfinite(mPred) && isfinite(tol) && tol > 0.0f && isfinite(S) && isfinite(A)) {
+                        // |float fold - S| <= gamma_n * A (n additions, unit
+                        // roundoff 2^-24), plus the double sums' own error;
+                        // then the float division and subtraction, each
+                        // within 2^-24 relative
+                        const double n = (double)total;
+                        const double err = A * (1.02 * n * 0x1p-24 + n * 0x1p-52) + 0x1p-1000;
+                        double meanLo = (S - err) / n;
+                        meanLo -= fabs(meanLo) * 0x1p-21;
+                        double yLo = meanLo - (double)mPred;
+                        yLo -= fabs(yLo) * 0x1p-21;
+                        if (yLo > (double)tol * (1.0 + 0x1p-19)) {
+                            how = 2;
+                            sh_meanArg = INFINITY;  // the mean exceeds the tolerance for certain
+                        }
+                    }
+                }
+            }
+            if (how == 1) sh_meanArg = stg[t - stageBase].mean;
+            sh_how = how;
+        }
+        __syncthreads();
+        if (sh_how == 0) {
+            for (int x0 = 0; x0 < nBins; x0 += kRowChunk) {
+                const int n = min(kRowChunk, nBins - x0);
+                for (int k = tid; k < n; k += kWalkThreads) {
+                    const int i = x0 + k;
+                    const bool kept = (keepMask[i >> 6] >> (i & 63)) & 1ull;
+                    float v = 0.0f;
+                    if (kept) {
+                        const int sb = i / nCh, c = i % nCh;
+                        v = I[idx(sb, 0, t) + c];
+                    }
+                    rowChunk[k] = v;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    // a row of n % 32 == 0 takes the deep fold (n % 8 == 0 always: nBins is)
+                    if ((n & 31) == 0) spectrumSum = fold_sum_deep(rowChunk, n, spectrumSum);
+                    else spectrumSum = fold_sum(rowChunk, n, spectrumSum);
+                }
+                if (!oneChunk) __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            const float meanArg = sh_how == 0 ? spectrumSum / (float)(unsigned)sh_total : sh_meanArg;
+            sh_fill = phaseC(t, meanArg, sh_fracInc, sh_badBands, dataModel, margin, 1);
         }
         __syncthreads();
         // last-good spectrum (:441-469): a clipped bin now holds its last-good
@@ -1758,7 +1995,8 @@ static int prepare(ppl_clipper* c, const ppl_clip_job& j)
     const unsigned nBins = j.nSub * j.nCh;
     PPL_CHECK_ARG(nBins % 8 == 0 && nBins > 0, "RFI_Clipper: nBins (%u) must be a positive multiple of 8", nBins);
     PPL_CHECK_ARG(j.nPol >= 1, "no polarisations");
-    PPL_CHECK_ARG(nBins <= 4u * 65535u * 256u, "RFI_Clipper: %u bins exceed the launch grid", nBins);
+    PPL_CHECK_ARG(nBins <= (unsigned)kMaxBinsWalk, "RFI_Clipper: %u bins exceed the supported %d", nBins,
+                  kMaxBinsWalk);
     bool dirty = false;
     if (c->nBins == 0) {
         int rc = setup_map(c, j.nSub, j.nPol, j.nCh);
