@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--mb-steps", type=int, default=2, help="timed steps of the multi-beam run")
     ap.add_argument("--no-multibeam", action="store_true", help="skip the batched multi-beam run")
     ap.add_argument("--no-service", action="store_true", help="skip the streaming-service run")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 256-channel split-precision run")
+    ap.add_argument("--prec-steps", type=int, default=2, help="timed steps per chain of the 256-channel run")
     ap.add_argument("--svc-seconds", type=float, default=30.0, help="wall seconds of the streaming run")
     ap.add_argument("--svc-accel", type=float, default=10.0,
                     help="data seconds per wall second in the streaming run (arrival schedule speed-up)")
@@ -324,6 +327,10 @@ def main():
     if not args.no_multibeam:
         mbeam = multibeam(args, torch, ppl)
 
+    prec = None
+    if not args.no_precision:
+        prec = precision(args, torch, ppl)
+
     svc = None
     if not args.no_service:
         svc = service(args, torch, ppl, coeffs, bp)
@@ -364,6 +371,7 @@ def main():
         "events_per_step": [len(e[0]) for e in plain_events],
         "clipper_walk": chain.clip.counters(),
         "mixed": mixed,
+        "precision_256ch": prec,
         "multibeam": mbeam,
         "service": svc,
         "roofline": roofline,
@@ -371,6 +379,121 @@ def main():
         "gen_seconds": round(t_gen, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+def ppf_fp64(volt, coeffs, n_ch, n_blk):
+    """FP64 polyphase channeliser of int16 voltages [sub][pol][n][2]: FIR with
+    the oldest block on tap 0 and zero history, then the unnormalised forward
+    DFT; returns [n_blk][sub][pol][n_ch] complex."""
+    x = volt[..., 0].astype(np.float64) + 1j * volt[..., 1].astype(np.float64)
+    n_sub, n_pol = x.shape[:2]
+    taps = coeffs.size // n_ch
+    h = np.asarray(coeffs, np.float64).reshape(taps, n_ch)
+    blocks = x[:, :, :n_blk * n_ch].reshape(n_sub, n_pol, n_blk, n_ch)
+    pad = np.concatenate([np.zeros((n_sub, n_pol, taps - 1, n_ch)), blocks], axis=2)
+    f = np.zeros_like(blocks)
+    for t in range(taps):
+        f += h[t][None, None, None, :] * pad[:, :, t:t + n_blk, :]
+    return np.fft.fft(f, axis=-1).transpose(2, 0, 1, 3)
+
+
+def precision(args, torch, ppl):
+    """The LOFAR beam at 256 channels per subband (15872 channels, tsamp
+    5.24 ms): the plain chain (FP32 DFT, FP32 dedispersion input) against
+    split-precision DFTs on the matrix cores (BF16x3, FP16x2) with FP16 and
+    8-bit dedispersion input, on the same voltages (pulses and narrowband
+    RFI, no broadband impulses).  Reports per chain the step time and stage
+    times, the channeliser's error against an FP64 channeliser on a sample,
+    the dedispersed series' error against the plain chain and the event
+    recall/precision against the plain chain's events."""
+    from tests.synth import beam_bandpass, device_voltages, tone_amplitude
+    n_ch = 256
+    tsamp = TS_IN * n_ch * INTEG
+    foff = -0.1953125 / n_ch
+    n_chan = N_SUB * n_ch
+    coeffs = ppl.ppf_coefficients(N_TAPS, n_ch, "hann")
+    bp = beam_bandpass(coeffs, N_SUB, n_ch, INTEG, FCH1, foff, SIGMA)
+    inv_scale = 127.0 / 10.0
+    chains = [("fp32", "f32"), ("bf16x3", "f16"), ("fp16x2", "f16"), ("bf16x3", "u8")]
+    tiles = {"f32": (16, 512, 4), "f16": (32, 512, 4), "u8": (32, 512, 4)}
+    # channeliser error on a sample: 2 subbands x 96 blocks against FP64
+    rng = np.random.default_rng(77)
+    sample = np.clip(np.rint(rng.normal(0, SIGMA, size=(2, 2, 96 * n_ch, 2))), -32768, 32767).astype(np.int16)
+    z64 = ppf_fp64(sample, coeffs, n_ch, 96)
+    rms64 = float(np.sqrt(np.mean(np.abs(z64) ** 2)))
+    dft_err = {}
+    for prec in ("fp32", "bf16x3", "fp16x2"):
+        pp = ppl.PPF(2, n_ch, N_TAPS, coeffs.astype(np.float32), precision=prec)
+        o = pp.spectra_i16(torch.from_numpy(sample).cuda(), 96).cpu().numpy().astype(np.float64)
+        e = np.abs(o[..., 0] + 1j * o[..., 1] - z64)
+        dft_err[prec] = {"max_err_rms": float(e.max() / rms64), "rms_err_rms": float(np.sqrt(np.mean(e ** 2)) / rms64)}
+        del pp
+    log("precision: 256-channel beam, generating one step of voltages")
+    probe = ppl.DedispPlan(n_chan, NSAMP_BUF, TDMS, 0.0, DM_STEP, tsamp, FCH1, foff)
+    nout, overlap = probe.nout, probe.maxshift + probe.rem
+    del probe
+    n_samp = nout * INTEG * n_ch
+    pulses = []
+    for _ in range(5):
+        pulses.append((float(rng.uniform(50, 450)), float(rng.uniform(0.05, 0.8) * nout),
+                       float(rng.uniform(1e-3, 8e-3) / tsamp), tone_amplitude(coeffs, n_ch, SIGMA, float(rng.uniform(6, 12)), INTEG)))
+    rfi = [(int(rng.integers(N_SUB)), int(rng.integers(n_ch)), tone_amplitude(coeffs, n_ch, SIGMA, 50.0, INTEG))
+           for _ in range(10)]
+    volt = device_voltages(torch, N_SUB, n_ch, INTEG, n_samp, 909, FCH1, foff, SIGMA, pulses=pulses, rfi=rfi)
+    prime = volt[:, :, :overlap * INTEG * n_ch].contiguous()
+    torch.cuda.synchronize()
+    out = {"workload": "62 sub x 256 ch (15872 ch), 8-tap PPF, Stokes-I x4 (tsamp %.3f ms), clipper, 2048 DMs 0-500 "
+                       "on 65536-sample buffers, pulses + narrowband RFI" % (tsamp * 1e3),
+           "nOut_per_step": nout, "data_s_per_step": round(nout * tsamp, 1), "dft_error_vs_fp64": dft_err}
+    ref_series, ref_events = None, None
+    nrm = float(np.sqrt(n_chan))
+    for dft, dd in chains:
+        log("precision: chain dft=%s dedisp=%s" % (dft, dd))
+        ch = ppl.BeamChain(N_SUB, n_ch, N_TAPS, coeffs, INTEG, bp, NSAMP_BUF, TDMS, 0.0, DM_STEP, tsamp, FCH1, foff,
+                           dtype=dd, inv_scale=inv_scale, tile=tiles[dd], dft=dft)
+        ch.prime(prime)
+        for _ in range(args.warmup):
+            ch.step(volt)
+            ch.detect()
+        torch.cuda.synchronize()
+        timing, events = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.prec_steps):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
+            ev[0].record()
+            ch.step(volt, ev[1:6])
+            ints, _, _ = ch.detect()
+            ev[6].record()
+            timing.append(ev)
+            events.append(ints)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.prec_steps
+        names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "carry", "detect"]
+        stages = {nm: round(sum(e[j].elapsed_time(e[j + 1]) for e in timing) / args.prec_steps, 3)
+                  for j, nm in enumerate(names)}
+        series = (ch.out.double() - 128.0 * n_chan) / inv_scale if dd == "u8" else ch.out.double()
+        rec = {"dft": dft, "dedisp_input": dd, "ms_per_step": round(1e3 * wall, 3),
+               "value": TDMS * nout / wall, "real_time_factor": nout * tsamp / wall, "stages_ms": stages,
+               "events": sum(len(e) - 1 for e in events)}
+        if ref_series is None:
+            ref_series, ref_events = series.clone(), events
+        else:
+            err = (series - ref_series).abs()
+            n_ref = n_got = inter = 0
+            for er, eg in zip(ref_events, events):
+                rk, gk = event_keys(er), event_keys(eg)
+                n_ref, n_got, inter = n_ref + len(rk), n_got + len(gk), inter + len(rk & gk)
+            rec.update({"series_max_abs_err_sigma": float(err.max()) / nrm,
+                        "series_rms_err_sigma": float(err.pow(2).mean().sqrt()) / nrm,
+                        "event_recall_vs_plain": inter / max(n_ref, 1),
+                        "event_precision_vs_plain": inter / max(n_got, 1)})
+        out["%s+%s" % (dft, dd)] = rec
+        del ch, series
+        torch.cuda.empty_cache()
+    del volt, prime
+    torch.cuda.empty_cache()
+    out["error_unit"] = "channeliser: max / RMS error over the spectrum's RMS; series: sigma of a dedispersed sample"
+    return out
 
 
 def draw_beams(seed, n):

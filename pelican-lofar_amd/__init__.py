@@ -569,14 +569,14 @@ class BeamChain:
 
     def __init__(self, n_sub, n_ch, n_taps, coeffs, integrate, bp_values, nsamp_buf, tdms, dm_low,
                  dm_step, tsamp, fch1, foff, dtype="f32", inv_scale=127.0 / 10.0, cr=10.0, sr=4.0,
-                 history=10000, zero_dm=True, tile=None):
+                 history=10000, zero_dm=True, tile=None, dft="fp32"):
         import torch
         self.n_sub, self.n_ch, self.integrate = n_sub, n_ch, integrate
         self.n_chan = n_sub * n_ch
         self.dtype, self.inv_scale = dtype, inv_scale
         self.plan = DedispPlan(self.n_chan, nsamp_buf, tdms, dm_low, dm_step, tsamp, fch1, foff, dtype=dtype)
         self.nout, self.overlap = self.plan.nout, self.plan.maxshift + self.plan.rem
-        self.ppf = PPF(n_sub, n_ch, n_taps, np.asarray(coeffs, np.float32), integrate=integrate)
+        self.ppf = PPF(n_sub, n_ch, n_taps, np.asarray(coeffs, np.float32), precision=dft, integrate=integrate)
         self.clip = Clipper(bp_values, cr, sr, history, zero_dm)
         self.buf = DDBuffer(self.n_chan, nsamp_buf, self.overlap, dtype, inv_scale)
         self.stokes = torch.empty((self.nout, n_sub, 1, n_ch), dtype=torch.float32, device="cuda")

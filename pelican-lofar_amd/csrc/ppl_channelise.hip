@@ -530,6 +530,7 @@ extern "C" int ppl_ppf_stokes_i16(ppl_ppf_plan* p, const int16_t* d_in, unsigned
 uint32_t* ppl_ppf_hist_next(ppl_ppf_plan* p) { return p->d_hist[p->cur ^ 1]; }
 const ppl_ppf_config& ppl_ppf_cfg(const ppl_ppf_plan* p) { return p->cfg; }
 const float* ppl_ppf_coeffs(const ppl_ppf_plan* p) { return p->d_coeffs; }
+const float* ppl_ppf_twiddle(const ppl_ppf_plan* p) { return p->d_twiddle; }
 void*& ppl_ppf_mfma_slot(ppl_ppf_plan* p) { return p->mfma; }
 int ppl_ppf_save_history(ppl_ppf_plan* p, const int16_t* d_in, unsigned nBlk, hipStream_t st)
 {
