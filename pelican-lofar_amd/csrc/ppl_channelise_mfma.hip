@@ -46,8 +46,10 @@ typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTB = 16;      // blocks per tile
 constexpr int kThreads = 256;
+// blocks per tile: 16, and 8 at 256 channels so that four workgroups share a
+// CU's LDS (the filtered tile and the per-block Stokes-I take 40 KB)
+template <int N> constexpr int tile_blocks() { return N == 256 ? 8 : 16; }
 
 struct C2 {
     float re, im;
@@ -131,7 +133,7 @@ __device__ __forceinline__ C2 load_c(const uint32_t* __restrict__ p, long idx)
 
 // OUT: 0 = integrated Stokes-I [nBlk/integ][sub][1][N]; 1 = complex spectra
 // [nBlk][sub][2][N].  One workgroup runs the blocks [run*runLen, +runLen) of
-// one subband, both polarisations, in tiles of kTB blocks.
+// one subband, both polarisations, in tiles of tile_blocks<N>() blocks.
 template <int N, int TAPS, int PREC, int OUT>
 __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __restrict__ in,
                                                             const uint32_t* __restrict__ hist,
@@ -142,6 +144,7 @@ __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __re
 {
     constexpr int R = N / 16;
     static_assert(R == 4 || R == 16, "split-precision DFT: 64 or 256 channels");
+    constexpr int kTB = tile_blocks<N>();
     constexpr int H = TAPS - 1;
     constexpr int ITEMS = (2 * N + kThreads - 1) / kThreads;  // (pol, channel) streams per thread
     constexpr int ROWT = kTB * R / 16;                        // 16-row tiles per polarisation
@@ -226,10 +229,20 @@ __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __re
 
     for (int s0 = 0; s0 < len; s0 += kTB) {
         const int nb = min(kTB, len - s0);
+        // ---- the tile's input, all loads in flight together
+        uint32_t xin[ITEMS][kTB];
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q) {
+            const int it = tid + kThreads * q;
+            const bool on = it < 2 * N;
+            const int pol = on ? it / N : 0, n = on ? it % N : 0;
+            const uint32_t* x = in + (long)(2 * sub + pol) * T + (b0 + s0) * N + n;
+#pragma unroll
+            for (int s = 0; s < kTB; ++s) xin[q][s] = (on && s < nb) ? x[(long)s * N] : 0u;
+        }
         // ---- FIR of the tile's blocks into LDS
+#pragma unroll
         for (int s = 0; s < kTB; ++s) {
-            const bool live = s < nb;
-            const long b = b0 + s0 + s;
 #pragma unroll
             for (int q = 0; q < ITEMS; ++q) {
                 const int it = tid + kThreads * q;
@@ -237,7 +250,8 @@ __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __re
                 const int pol = on ? it / N : 0, n = on ? it % N : 0;
 #pragma unroll
                 for (int t = 0; t < TAPS - 1; ++t) w[q][t] = w[q][t + 1];
-                w[q][TAPS - 1] = (on && live) ? load_c(in + (long)(2 * sub + pol) * T, b * N + n) : C2{0.f, 0.f};
+                const uint32_t xv = xin[q][s];
+                w[q][TAPS - 1] = {(float)(int16_t)(xv & 0xffffu), (float)(int16_t)(xv >> 16)};
                 C2 f = {0.f, 0.f};
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) {
@@ -391,8 +405,9 @@ int ppl_mfma_create(ppl_ppf_plan* p)
         set_error("split-precision DFT channeliser: built for 64 and 256 channels per subband (got %u)", c.nCh);
         return PPL_E_CONFIG;
     }
-    if (c.integrate == 0 || 16 % c.integrate != 0) {
-        set_error("split-precision DFT channeliser: integrateTimeBins must divide 16 (got %u)", c.integrate);
+    const unsigned tb = c.nCh == 256 ? tile_blocks<256>() : tile_blocks<64>();
+    if (c.integrate == 0 || tb % c.integrate != 0) {
+        set_error("split-precision DFT channeliser: integrateTimeBins must divide %u (got %u)", tb, c.integrate);
         return PPL_E_CONFIG;
     }
     return PPL_OK;
