@@ -527,10 +527,12 @@ def draw_beams(seed, n):
 
 
 def multibeam(args, torch, ppl):
-    """24 beams of different shapes through the batched chain (f32 and f16
-    dedispersion input): one clipper launch and one dedispersion launch per
-    stage for all beams, HIP-graph replayed steps, the batch tile from the
-    autotuner.  Returns the report object."""
+    """24 beams of different shapes through the batched chain, plain (FP32
+    DFT and dedispersion input) and mixed (BF16x3 DFT on the matrix cores for
+    the 64- and 256-channel beams, FP16 dedispersion input): one clipper
+    launch per stage for all beams, one dedispersion launch per group of
+    beams sharing an autotuned tile, HIP-graph replayed steps.  Returns the
+    report object."""
     from tests.synth import beam_bandpass, device_voltages, tone_amplitude
     beams = draw_beams(2024, args.beams)
     specs = []
@@ -543,9 +545,11 @@ def multibeam(args, torch, ppl):
     cands = [(16, 512, 4), (32, 512, 4), (16, 256, 4), (32, 256, 8), (16, 512, 8), (8, 512, 8)]
     report = {"beams": [{k: bm[k] for k in ("n_sub", "n_ch", "dm_hi", "tdms", "nsamp")} for bm in beams]}
     ref_events = None
+    # the plain form (FP32 DFT, FP32 input) and the mixed form (BF16x3 DFT
+    # where built, FP16 input), both with a tile per beam
     for dtype in ("f32", "f16"):
         log("multi-beam %d beams, %s" % (len(specs), dtype))
-        mb = ppl.MultiBeam(specs, dtype=dtype)
+        mb = ppl.MultiBeam(specs, dtype=dtype, dft="fp32" if dtype == "f32" else "bf16x3")
         volts = []
         for b, (bm, sp) in enumerate(zip(beams, specs)):
             n_samp = mb.chunk_samples(b)
@@ -560,7 +564,7 @@ def multibeam(args, torch, ppl):
         mb.prime(volts)
         log("multi-beam autotune")
         t0 = time.perf_counter()
-        tune_ms = mb.autotune(candidates=cands)
+        tune_ms = mb.autotune(candidates=cands, per_beam=True)
         tune_s = time.perf_counter() - t0
         def run(n, graph=False, timing=None):
             evs = []
@@ -607,7 +611,10 @@ def multibeam(args, torch, ppl):
                "graph_ms_per_step": round(1e3 * gwall, 3),
                "graph_value": mb.trials_samples() / gwall,
                "real_time_factor_aggregate": sum(secs) / wall, "real_time_factor_min_beam": min(secs) / wall,
-               "stages_ms": stages, "tile_dm_t_chanblock": list(mb.tile), "autotune_ms": round(tune_ms, 3),
+               "stages_ms": stages,
+               "tile_dm_t_chanblock": list(mb.tile) if mb.tile != "own" else [list(p.tile) for p in mb.plans],
+               "dft": "fp32" if dtype == "f32" else "bf16x3 (64, 256 ch), fp32 (16 ch)",
+               "autotune_ms": round(tune_ms, 3),
                "autotune_s": round(tune_s, 1), "resident_gb": round(mem_gb, 1), "warmup_steps": warm,
                "clipper_walk": walk,
                "dedisperse_adds_per_step": float(sum(sp.tdms * n * sp.n_sub * sp.n_ch

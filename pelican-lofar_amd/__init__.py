@@ -240,12 +240,23 @@ def _dedisp_jobs(plans, d_ins, d_outs):
 
 
 def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0), stream=None):
-    """One launch dedispersing every (plan, input) pair; outputs as separate
-    DedispPlan.run calls would give them."""
+    """One launch dedispersing every (plan, input) pair under one tile; outputs
+    as separate DedispPlan.run calls would give them.  tile="own": one launch
+    per group of plans that share their own (autotuned) tile."""
     import torch
     if d_outs is None:
         d_outs = [torch.empty((p.cfg.tdms, p.nout), dtype=p.out_dtype(), device=x.device)
                   for p, x in zip(plans, d_ins)]
+    if isinstance(tile, str):
+        if tile != "own":
+            raise PPLError("tile must be a (tileDM, tileT, chanBlock) tuple or 'own'")
+        groups = {}
+        for k, p in enumerate(plans):
+            groups.setdefault(tuple(p.tile), []).append(k)
+        for t, ks in groups.items():
+            dedisperse_batch([plans[k] for k in ks], [d_ins[k] for k in ks], [d_outs[k] for k in ks], tile=t,
+                             stream=stream)
+        return d_outs
     arr, n = _dedisp_jobs(plans, d_ins, d_outs)
     check(lib().ppl_dedisperse_batch(arr, n, tile[0], tile[1], tile[2], _stream(stream)))
     return d_outs
@@ -685,7 +696,8 @@ class MultiBeam:
     launch goes to the current stream, so steps can be captured in two HIP
     graphs (even and odd steps: the FIR histories and buffers alternate)."""
 
-    def __init__(self, specs, dtype="f32", inv_scale=127.0 / 10.0, max_chunk_samples=1 << 20, tile=None):
+    def __init__(self, specs, dtype="f32", inv_scale=127.0 / 10.0, max_chunk_samples=1 << 20, tile=None,
+                 dft="fp32"):
         import torch
         if len(specs) > min(MAX_DEDISP_JOBS, 48):
             raise PPLError("at most 48 beams")
@@ -699,7 +711,9 @@ class MultiBeam:
             self.plans.append(plan)
             self.nout.append(plan.nout)
             self.overlap.append(plan.maxshift + plan.rem)
-            self.ppfs.append(PPF(sp.n_sub, sp.n_ch, sp.n_taps, sp.coeffs, integrate=sp.integrate))
+            # the split-precision DFT where it is built (64 and 256 channels)
+            prec = dft if (dft != "fp32" and sp.n_ch in (64, 256)) else "fp32"
+            self.ppfs.append(PPF(sp.n_sub, sp.n_ch, sp.n_taps, sp.coeffs, precision=prec, integrate=sp.integrate))
             self.clips.append(Clipper(sp.bp_values, sp.cr, sp.sr, sp.history, sp.zero_dm))
             self.bufs.append(DDBuffer(n_chan, sp.nsamp_buf, plan.maxshift + plan.rem, dtype, inv_scale))
             self.stokes.append(torch.empty((plan.nout, sp.n_sub, 1, sp.n_ch), dtype=torch.float32, device="cuda"))
@@ -725,13 +739,24 @@ class MultiBeam:
     def trials_samples(self):
         return sum(sp.tdms * n for sp, n in zip(self.specs, self.nout))
 
-    def autotune(self, reps=1, candidates=None):
-        """Picks the batch's tile on the beams' own buffers (their contents do
-        not change the time); candidates: (tileDM, tileT, chanBlock) tuples to
-        try instead of every instantiated shape."""
-        self.tile, ms = autotune_batch(self.plans, [bf.tensor() for bf in self.bufs], self.outs, reps=reps,
-                                       candidates=candidates)
-        return ms
+    def autotune(self, reps=1, candidates=None, per_beam=False):
+        """Picks the tile on the beams' own buffers (their contents do not
+        change the time); candidates: (tileDM, tileT, chanBlock) tuples to try
+        instead of every instantiated shape.  per_beam: each beam keeps its
+        fastest tile and the dedispersion runs one launch per group of beams
+        sharing a tile; otherwise one tile for the whole batch.  Returns the
+        tuned time per step in ms."""
+        if not per_beam:
+            self.tile, ms = autotune_batch(self.plans, [bf.tensor() for bf in self.bufs], self.outs, reps=reps,
+                                           candidates=candidates)
+            return ms
+        total = 0.0
+        for p, bf, o in zip(self.plans, self.bufs, self.outs):
+            t, ms = autotune_batch([p], [bf.tensor()], [o], reps=reps, candidates=candidates)
+            p.set_tile(*t)
+            total += ms
+        self.tile = "own"
+        return total
 
     def _channelise(self, b, d_volt, n_spec, d_out):
         """n_spec integrated spectra of beam b into d_out, chunk by chunk."""

@@ -23,10 +23,10 @@ def _specs(ppl, dtype):
     return specs
 
 
-@pytest.mark.parametrize("dtype", ["f32", "f16"])
-def test_multibeam_equals_single_beam_chains(ppl, dtype):
+@pytest.mark.parametrize("dtype,dft", [("f32", "fp32"), ("f16", "fp32"), ("f16", "bf16x3")])
+def test_multibeam_equals_single_beam_chains(ppl, dtype, dft):
     specs = _specs(ppl, dtype)
-    mb = ppl.MultiBeam(specs, dtype=dtype, max_chunk_samples=1 << 15)
+    mb = ppl.MultiBeam(specs, dtype=dtype, max_chunk_samples=1 << 15, dft=dft)
     volts = []
     for b, sp in enumerate(specs):
         pulses = [(0.6 * sp.dm_step * sp.tdms, 100.0, 3.0, 40.0)]
@@ -34,8 +34,9 @@ def test_multibeam_equals_single_beam_chains(ppl, dtype):
                                      pulses=pulses, impulse_frac=0.002))
     singles = []
     for b, sp in enumerate(specs):
+        prec = dft if sp.n_ch in (64, 256) else "fp32"
         ch = ppl.BeamChain(sp.n_sub, sp.n_ch, 8, sp.coeffs, 4, sp.bp_values, sp.nsamp_buf, sp.tdms, 0.0,
-                           sp.dm_step, sp.tsamp, sp.fch1, sp.foff, dtype=dtype, history=1000)
+                           sp.dm_step, sp.tsamp, sp.fch1, sp.foff, dtype=dtype, history=1000, dft=prec)
         singles.append(ch)
     # the single chains read the same periodic stream: the chunk repeated
     reps = [mb.nout[b] // mb.chunk_spectra[b] for b in range(len(specs))]
@@ -72,11 +73,12 @@ def test_multibeam_equals_single_beam_chains(ppl, dtype):
             assert torch.equal(mb.outs[b], ch.out), ("graph", k, b)
 
 
-def test_multibeam_autotune_keeps_results(ppl):
+@pytest.mark.parametrize("per_beam", [False, True])
+def test_multibeam_autotune_keeps_results(ppl, per_beam):
     specs = _specs(ppl, "f32")
     mb = ppl.MultiBeam(specs, max_chunk_samples=1 << 15)
-    ms = mb.autotune()
-    assert ms > 0 and mb.tile[0] % 4 == 0
+    ms = mb.autotune(per_beam=per_beam, candidates=[(16, 512, 4), (32, 256, 8), (8, 512, 8)] if per_beam else None)
+    assert ms > 0 and (mb.tile == "own" if per_beam else mb.tile[0] % 4 == 0)
     outs = [torch.randn((sp.n_sub * sp.n_ch, sp.nsamp_buf), device="cuda") for sp in specs]
     got = ppl.dedisperse_batch(mb.plans, outs, tile=mb.tile)
     for p, x, g in zip(mb.plans, outs, got):
