@@ -450,6 +450,16 @@ __device__ __forceinline__ void fold_sum_sq(const float* row, int n, float& sum,
 // 0.5 / cpb exceeds the 2^-21 error of the product (cpb < 2^19).
 __device__ __forceinline__ int band_of(int i, float invCpb) { return (int)(((float)i + 0.5f) * invCpb); }
 
+// Data-only statistics of one learning-phase spectrum (RFI_Clipper.cpp:214-242
+// and the kept-channel sum of :315-342), taken before the walk reaches it.
+struct LearnRec {
+    float bmin[8], bsum[8], bsq[8];  // per band: minimum, ordered sum and sum of squares
+    float fullSum;   // ordered sum of every bin: the kept sum when nothing is clipped
+    float predSum;   // ordered sum of the bins <= thr (the others as +0.0)
+    float thr;       // predicted clipping level: mean + 10 standard deviations of the spectrum
+    int predKept;    // bins <= thr
+};
+
 // Device-resident state of one clipper (one beam), fixed between runs.
 struct BeamState {
     ClipConst K;
@@ -459,6 +469,7 @@ struct BeamState {
     const double* terms;
     Box* box;
     PreRec* pre;
+    LearnRec* learn;
     SpecRec* rec;
     float2* stats;
 };
@@ -657,6 +668,100 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
     }
 }
 
+// Learning-phase records (LearnRec) of the sub-chunk's spectra, one row per
+// lane of wave 0 in bin order, tiles staged in LDS as in the precompute.
+// Runs only while the clipper is learning at the sub-chunk's start.
+__global__ __launch_bounds__(kTileThreads) void clip_learn_kernel(BatchIO io)
+{
+    const int beam = blockIdx.z;
+    const BeamState& bs = *io.st[beam];
+    const float* __restrict__ I = io.I[beam];
+    const ClipConst K = bs.K;
+    const ClipScalars* __restrict__ S = bs.S;
+    LearnRec* __restrict__ lr = bs.learn;
+    const int nT = io.nT[beam], subLen = io.subLen[beam], ldMode = io.ldMode[beam];
+    const int tStart = S->tNext;
+    if (tStart >= nT || S->rmsSize >= K.cap) return;
+    const int k0 = blockIdx.x * kTileRows;
+    const int t0 = tStart + k0;
+    const int nRows = min(min(kTileRows, nT - t0), subLen - k0);
+    if (nRows <= 0) return;
+    __shared__ float vals[kTileRows][kTileBins + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
+    const int cpb = nBins / 8;
+    const long rowStride = (long)nBins * nPol;
+    const int nTiles = (nBins + kTileBins - 1) / kTileBins;
+    const bool rowLane = wave == 0 && lane < nRows;
+    LearnRec* out = lr + k0 + lane;
+    float cmin = 1e6f, csum = 0.0f, csq = 0.0f, full = 0.0f, pred = 0.0f, thr = 0.0f;
+    double ds = 0.0, dq = 0.0;
+    int band = 0, bandEnd = cpb, kept = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int tile = 0; tile < nTiles; ++tile) {
+            const int tb0 = tile * kTileBins;
+            const int i = tb0 + lane * 4;
+            if (i < nBins) {
+#pragma unroll
+                for (int q = 0; q < kRowsPerWave; ++q) {
+                    const int r = wave + 4 * q;
+                    if (r < nRows) {
+                        const float4 v = load_quad(I, (long)(t0 + r) * rowStride, i, nCh, nPol, ldMode);
+                        vals[r][lane * 4 + 0] = v.x;
+                        vals[r][lane * 4 + 1] = v.y;
+                        vals[r][lane * 4 + 2] = v.z;
+                        vals[r][lane * 4 + 3] = v.w;
+                    }
+                }
+            }
+            __syncthreads();
+            if (rowLane) {
+                const int nb = min(kTileBins, nBins - tb0);
+                if (pass == 0) {
+                    for (int k = 0; k < nb; ++k) {
+                        const float v = vals[lane][k];
+                        cmin = (v < cmin) ? v : cmin;
+                        csum += v;
+                        csq += (v * v);
+                        full += v;
+                        ds += (double)v;
+                        dq += (double)v * (double)v;
+                        if (tb0 + k + 1 == bandEnd) {
+                            out->bmin[band] = cmin;
+                            out->bsum[band] = csum;
+                            out->bsq[band] = csq;
+                            cmin = 1e6f;
+                            csum = 0.0f;
+                            csq = 0.0f;
+                            ++band;
+                            bandEnd += cpb;
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < nb; ++k) {
+                        const float v = vals[lane][k];
+                        const bool keep = v <= thr;
+                        pred += keep ? v : 0.0f;
+                        kept += keep ? 1 : 0;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (rowLane && pass == 0) {
+            const double m = ds / nBins;
+            const double var = dq / nBins - m * m;
+            thr = (float)(m + 10.0 * sqrt(var > 0.0 ? var : 0.0));
+        }
+    }
+    if (rowLane) {
+        out->fullSum = full;
+        out->predSum = pred;
+        out->thr = thr;
+        out->predKept = kept;
+    }
+}
+
 // Staged per spectrum of the walk: the precomputed record, its mean and the
 // two ring fronts, each divided by the (full) history length.
 struct Staged {
@@ -691,6 +796,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     const PreRec* __restrict__ pre = bst.pre;
     SpecRec* __restrict__ rec = bst.rec;
     const float* __restrict__ bpLo = bst.bpLo;
+    const LearnRec* __restrict__ learn = bst.learn;
     const int nT = io.nT[beam], subLen = io.subLen[beam];
     __shared__ ClipScalars s;
     __shared__ float red[kWalkThreads / 64 + 1];
@@ -701,7 +807,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     // slow path: kept-bin mask, sums, and how the kept mean is obtained
     __shared__ unsigned long long keepMask[kMaxBinsWalk / 64];
     __shared__ double redD[2][kWalkThreads / 64];
-    __shared__ int sh_mism, sh_how, sh_total, sh_badBands;
+    __shared__ int sh_mism, sh_mismL, sh_how, sh_total, sh_badBands;
     __shared__ float sh_fracInc, sh_meanArg;
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
@@ -725,6 +831,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     if (tStart >= nT) return;
     const int tEnd = lastIter ? nT : min(nT, tStart + subLen);
     const int preN = min(subLen, nT - tStart);  // spectra the precompute covered
+    const bool learnRun = s.rmsSize < K.cap;    // the learning records cover this sub-chunk
     const Box bx = *box;
     const float capf = (float)K.cap;
 
@@ -1360,7 +1467,30 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         // ---- slow spectrum t, whole workgroup
         if (tid < 8) goodCnt[tid] = 0;
         if (tid == 0) fillCnt = 0;
-        if (mode == MODE_FULL) {
+        const bool useRec = mode == MODE_FULL && learnRun && (t - tStart) < preN;
+        if (useRec) {
+            // learning-phase band statistics (RFI_Clipper.cpp:214-242): the
+            // data's from the record, the model's minimum per band by wave b
+            const int b = tid >> 6;
+            float mm = 1e6f;
+            for (int i = b * cpb + (tid & 63); i < (b + 1) * cpb; i += 64) {
+                const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+                mm = (bpv < mm) ? bpv : mm;
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float o = __shfl_xor(mm, off, 64);
+                mm = (o < mm) ? o : mm;
+            }
+            if ((tid & 63) == 0) {
+                const LearnRec& R = learn[t - tStart];
+                modelMin[b] = mm;
+                bandMin[b] = R.bmin[b];
+                bandMean[b] = R.bsum[b];
+                bandMeanSq[b] = R.bsq[b];
+            }
+        }
+        if (mode == MODE_FULL && !useRec) {
             // learning-phase band statistics (RFI_Clipper.cpp:214-242)
             for (int b = 0; b < 8; ++b) {
                 float md = 1e6f, mm = 1e6f;
@@ -1401,6 +1531,8 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 bandMean[tid] = bsum;
                 bandMeanSq[tid] = bsq;
             }
+        }
+        if (mode == MODE_FULL) {
             __syncthreads();
             if (tid == 0) {
                 float dmm[8], sig[8];
@@ -1448,10 +1580,17 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         float spectrumSum = 0.0f;
         const bool oneChunk = nBins <= kRowChunk;
         const float invCpbW = 1.0f / (float)cpb;
-        const bool chkPre = sh_why == CNT_UNDECIDED && bx.valid && s.remainingZeros == 0;
+        const bool chkPre = sh_why == CNT_UNDECIDED && bx.valid;
+        // learning phase: the recorded sums stand for the ordered sum when
+        // nothing is clipped or the mask is the predicted one
+        const bool chkLearn = useRec;
+        const float lthr = chkLearn ? learn[t - tStart].thr : 0.0f;
         double sd = 0.0, ad = 0.0;
-        int mism = 0;
-        if (tid == 0) sh_mism = 0;
+        int mism = 0, mismL = 0;
+        if (tid == 0) {
+            sh_mism = 0;
+            sh_mismL = 0;
+        }
         __syncthreads();
         // each thread's bins are read kG at a time, all in flight together,
         // before any of them is judged (the replacements write I)
@@ -1491,6 +1630,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                         // the precompute kept a bin only when v - bpLo <= mlo
                         if (chkPre && !((v - bl[j]) <= bx.mlo)) mism = 1;
                     }
+                    if (chkLearn && kept != (v <= lthr)) mismL = 1;
                 }
                 const unsigned long long kb = __ballot(kept);
                 if ((tid & 63) == 0) keepMask[k0 >> 6] = kb;
@@ -1503,6 +1643,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             }
         }
         if (__ballot(mism) != 0 && (tid & 63) == 0) atomicOr(&sh_mism, 1);
+        if (__ballot(mismL) != 0 && (tid & 63) == 0) atomicOr(&sh_mismL, 1);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             sd += __shfl_xor(sd, off, 64);
@@ -1522,8 +1663,18 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             sh_total = total;
             sh_badBands = badBands;
             sh_fracInc = fracInc;
-            int how = 0;  // 0: fold; 1: the precomputed mean; 2: rejected for certain
+            int how = 0;  // 0: fold; 1: the precomputed mean; 2: rejected for certain; 3: a recorded sum
             if (chkPre && !sh_mism) how = 1;
+            if (chkLearn && total > 0) {
+                const LearnRec& R = learn[t - tStart];
+                if (total == nBins) {
+                    how = 3;
+                    sh_meanArg = R.fullSum / (float)(unsigned)total;
+                } else if (!sh_mismL) {
+                    how = 3;
+                    sh_meanArg = R.predSum / (float)(unsigned)total;
+                }
+            }
             if (s.meanSize == K.cap && K.cap >= 1000) {
                 if (badBands >= 4) {
                     how = 2;
@@ -1586,19 +1737,15 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             sh_fill = phaseC(t, meanArg, sh_fracInc, sh_badBands, dataModel, margin, 1);
         }
         __syncthreads();
-        // last-good spectrum (:441-469): a clipped bin now holds its last-good
-        // value, so only kept non-zero bins can fill an empty slot; every
+        // last-good spectrum (:441-469), read from the spectrum itself: a
+        // clipped bin now holds its last-good value, so only kept non-zero
+        // bins can fill an empty slot; every
         // change starts a new version for the spectra that follow (the apply
         // pass replaces from it)
         if (sh_fill) {
             for (int i = tid; i < nBins; i += kWalkThreads) {
-                float v;
-                if (oneChunk) {
-                    v = rowChunk[i];
-                } else {
-                    const int sb = i / nCh, c = i % nCh;
-                    v = I[idx(sb, 0, t) + c];
-                }
+                const int sb = i / nCh, c = i % nCh;
+                const float v = I[idx(sb, 0, t) + c];
                 if (lastGood[i] == 0.0f && v != 0.0f) {
                     lastGood[i] = v;
                     atomicAdd(&fillCnt, 1);
@@ -1824,6 +1971,7 @@ struct ppl_clipper {
     float2* d_stats = nullptr;
     SpecRec* d_rec = nullptr;
     PreRec* d_pre = nullptr;
+    LearnRec* d_learn = nullptr;
     Box* d_box = nullptr;
     float *d_bpLo = nullptr, *d_bpHi = nullptr, *d_lastGoodVer = nullptr;
     BeamState* d_bs = nullptr;  // device descriptor of this clipper for the batched kernels
@@ -1846,6 +1994,7 @@ void free_state(ppl_clipper* c)
     (void)hipFree(c->d_stats);
     (void)hipFree(c->d_rec);
     (void)hipFree(c->d_pre);
+    (void)hipFree(c->d_learn);
     (void)hipFree(c->d_box);
     (void)hipFree(c->d_bpLo);
     (void)hipFree(c->d_bpHi);
@@ -1854,6 +2003,7 @@ void free_state(ppl_clipper* c)
     c->d_bs = nullptr;
     c->d_lastGoodVer = nullptr;
     c->d_pre = nullptr;
+    c->d_learn = nullptr;
     c->d_box = nullptr;
     c->d_bpLo = c->d_bpHi = nullptr;
     c->d_S = nullptr;
@@ -2012,12 +2162,15 @@ static int prepare(ppl_clipper* c, const ppl_clip_job& j)
     if ((int)j.nT > c->recCap) {
         (void)hipFree(c->d_rec);
         (void)hipFree(c->d_pre);
+        (void)hipFree(c->d_learn);
         (void)hipFree(c->d_stats);
         c->d_rec = nullptr;
         c->d_pre = nullptr;
+        c->d_learn = nullptr;
         c->d_stats = nullptr;
         PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * j.nT));
         PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * std::min((int)j.nT, kSubLen)));
+        PPL_HIP(hipMalloc(&c->d_learn, sizeof(LearnRec) * std::min((int)j.nT, kSubLen)));
         PPL_HIP(hipMalloc(&c->d_stats, sizeof(float2) * j.nT));
         c->recCap = (int)j.nT;
         dirty = true;
@@ -2037,6 +2190,7 @@ static int prepare(ppl_clipper* c, const ppl_clip_job& j)
         h.terms = c->d_terms;
         h.box = c->d_box;
         h.pre = c->d_pre;
+        h.learn = c->d_learn;
         h.rec = c->d_rec;
         h.stats = c->d_stats;
         if (!c->d_bs) PPL_HIP(hipMalloc(&c->d_bs, sizeof(BeamState)));
@@ -2090,6 +2244,8 @@ static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n,
         hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
         PPL_LAUNCHED();
         hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(maxSub, kTileRows), 1, nb), dim3(kTileThreads), 0, st, io);
+        PPL_LAUNCHED();
+        hipLaunchKernelGGL(clip_learn_kernel, dim3(cdiv(maxSub, kTileRows), 1, nb), dim3(kTileThreads), 0, st, io);
         PPL_LAUNCHED();
         hipLaunchKernelGGL(clip_walk_kernel, dim3(1, 1, nb), dim3(kWalkThreads), 0, st, io, it == nIter - 1 ? 1 : 0);
         PPL_LAUNCHED();
