@@ -73,7 +73,7 @@ def parse():
                     help="skip the 256-channel split-precision run")
     ap.add_argument("--prec-steps", type=int, default=2, help="timed steps per chain of the 256-channel run")
     ap.add_argument("--svc-seconds", type=float, default=30.0, help="wall seconds of the streaming run")
-    ap.add_argument("--svc-accel", type=float, default=5.0,
+    ap.add_argument("--svc-accel", type=float, default=10.0,
                     help="data seconds per wall second in the streaming run (arrival schedule speed-up)")
     ap.add_argument("--svc-dtype", default="f16", choices=["f32", "f16", "u8"])
     return ap.parse_args()
