@@ -530,9 +530,9 @@ def multibeam(args, torch, ppl):
     """24 beams of different shapes through the batched chain, plain (FP32
     DFT and dedispersion input) and mixed (BF16x3 DFT on the matrix cores for
     the 64- and 256-channel beams, FP16 dedispersion input): one clipper
-    launch per stage for all beams, one dedispersion launch per group of
-    beams sharing an autotuned tile, HIP-graph replayed steps.  Returns the
-    report object."""
+    launch per stage for all beams, one dedispersion launch for all beams
+    (or per group of beams sharing a tile, when the autotuner finds that
+    faster), HIP-graph replayed steps.  Returns the report object."""
     from tests.synth import beam_bandpass, device_voltages, tone_amplitude
     beams = draw_beams(2024, args.beams)
     specs = []
@@ -546,7 +546,8 @@ def multibeam(args, torch, ppl):
     report = {"beams": [{k: bm[k] for k in ("n_sub", "n_ch", "dm_hi", "tdms", "nsamp")} for bm in beams]}
     ref_events = None
     # the plain form (FP32 DFT, FP32 input) and the mixed form (BF16x3 DFT
-    # where built, FP16 input), both with a tile per beam
+    # where built, FP16 input); the autotuner keeps one batch tile or per-beam
+    # tiles, whichever is faster on the whole batch
     for dtype in ("f32", "f16"):
         log("multi-beam %d beams, %s" % (len(specs), dtype))
         mb = ppl.MultiBeam(specs, dtype=dtype, dft="fp32" if dtype == "f32" else "bf16x3")

@@ -742,21 +742,36 @@ class MultiBeam:
     def autotune(self, reps=1, candidates=None, per_beam=False):
         """Picks the tile on the beams' own buffers (their contents do not
         change the time); candidates: (tileDM, tileT, chanBlock) tuples to try
-        instead of every instantiated shape.  per_beam: each beam keeps its
-        fastest tile and the dedispersion runs one launch per group of beams
-        sharing a tile; otherwise one tile for the whole batch.  Returns the
-        tuned time per step in ms."""
+        instead of every instantiated shape.  One tile for the whole batch,
+        or with per_beam each beam's own fastest tile (one launch per group
+        of beams sharing a tile) when that is faster on the whole batch.
+        Returns the tuned dedispersion time per step in ms."""
+        import torch
+        ins = [bf.tensor() for bf in self.bufs]
+        tile, ms = autotune_batch(self.plans, ins, self.outs, reps=reps, candidates=candidates)
+        self.tile = tile
         if not per_beam:
-            self.tile, ms = autotune_batch(self.plans, [bf.tensor() for bf in self.bufs], self.outs, reps=reps,
-                                           candidates=candidates)
             return ms
-        total = 0.0
-        for p, bf, o in zip(self.plans, self.bufs, self.outs):
-            t, ms = autotune_batch([p], [bf.tensor()], [o], reps=reps, candidates=candidates)
+        # per-beam tiles (each tuned alone), kept only if the grouped launches
+        # beat the single batch tile on the whole batch
+        old = [p.tile for p in self.plans]
+        for p, x, o in zip(self.plans, ins, self.outs):
+            t, _ = autotune_batch([p], [x], [o], reps=reps, candidates=candidates)
             p.set_tile(*t)
-            total += ms
-        self.tile = "own"
-        return total
+        dedisperse_batch(self.plans, ins, self.outs, tile="own")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            dedisperse_batch(self.plans, ins, self.outs, tile="own")
+        e1.record()
+        e1.synchronize()
+        own_ms = e0.elapsed_time(e1) / reps
+        if own_ms < ms:
+            self.tile = "own"
+            return own_ms
+        for p, t in zip(self.plans, old):
+            p.set_tile(*t)
+        return ms
 
     def _channelise(self, b, d_volt, n_spec, d_out):
         """n_spec integrated spectra of beam b into d_out, chunk by chunk."""
