@@ -140,3 +140,20 @@ def beam_bandpass(coeffs, n_sub, n_ch, integrate, fch1, foff, sigma=64.0):
     noise_std = np.sqrt(2 * integrate) * 2.0 * sigma * sigma * e_h
     n_chan = n_sub * n_ch
     return np.array([n_chan, fch1, fch1 + foff * (n_chan - 1), foff, noise_std, level], np.float64)
+
+
+def generator_blob(dm, n_samples=6400, n_sub=32, n_ch=64, start_bin=0, width=10, fch1=150.0, tsamp=0.00032768):
+    """The reference's DedispersionDataGenerator::generate for one blob
+    (src/lib/test/src/DedispersionDataGenerator.cpp:29-37,47-81): every
+    channel is 1.0 for `width` samples from its FP64 dispersion index,
+    else 0.  Returns the dedispersion-buffer layout [channel][sample]."""
+    n = n_sub * n_ch
+    foff = -6.0 / n
+    out = np.zeros((n, n_samples), np.float32)
+    for c in range(n):
+        f = fch1 + foff * c
+        idx = int(dm * (4148.741601 * ((1.0 / f / f) - (1.0 / fch1 / fch1)) / tsamp)) + start_bin
+        a, b = max(idx, 0), min(idx + width, n_samples)
+        if a < b:
+            out[c, a:b] = 1.0
+    return out

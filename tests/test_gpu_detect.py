@@ -50,3 +50,20 @@ def test_u8_detection_dequantises(orc, ppl):
     ev_o, val_o = orc.detect(deq, 16, 8)
     ev_g, val_g, n = ppl.detect_u8(torch.from_numpy(s).cuda(), n_chan, float(scale), 16, 8)
     assert np.array_equal(ev_g, ev_o) and np.array_equal(val_g, val_o)
+
+
+def test_reference_single_event(orc, ppl):
+    """DedispersionAnalyserTest::test_singleEvent (DedispersionAnalyserTest.cpp:50-77)
+    through the device plan, dedispersion and detection: more events than DM
+    bins, the same events and values as the oracle."""
+    from synth import generator_blob
+    n_sub, n_ch, nsamp, tdms = 32, 64, 6400, 200
+    buf = generator_blob(10.0)
+    plan = ppl.DedispPlan(n_sub * n_ch, nsamp, tdms, 0.0, 0.1, 0.00032768, 150.0, -6.0 / (n_sub * n_ch))
+    out = plan.run(torch.from_numpy(buf).cuda())
+    ev, vals, n = ppl.detect(out, n_ch, n_sub)
+    assert n - 1 > tdms
+    p = orc.dedisp_plan(n_sub * n_ch, nsamp, tdms, 0.0, 0.1, 0.00032768, 150.0, -6.0 / (n_sub * n_ch))
+    d = orc.dedisp_delays(p["shift"], tdms, p["mstartdm"], p["mdmstep"])
+    ev_o, vals_o = orc.detect(orc.dedisperse(buf, d, p["nout"]), n_ch, n_sub, 6.0, 6)
+    assert np.array_equal(ev, ev_o) and np.array_equal(vals, vals_o)

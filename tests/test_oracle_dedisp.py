@@ -176,3 +176,26 @@ def test_detection_invariants(orc):
     found = {(int(dm), int(bf)) for dm, t, bf in ev}
     for bf in (1, 2, 4, 8):
         assert (2, bf) in found
+
+
+def test_reference_single_event(orc):
+    """DedispersionAnalyserTest::test_singleEvent (DedispersionAnalyserTest.cpp:50-77)
+    with DedispersionDataGenerator::dedispersionData(10) (:86-120): a DM-10
+    signal, 2 x 10 / 0.1 = 200 trials of step 0.1 over a 6400-sample buffer,
+    default analyser (threshold 6, 2^6 segments): more events than DM bins."""
+    from synth import generator_blob
+    n_sub, n_ch, nsamp = 32, 64, 6400
+    tdms = int(2 * 10 / 0.1)
+    buf = generator_blob(10.0)
+    p = orc.dedisp_plan(n_sub * n_ch, nsamp, tdms, 0.0, 0.1, 0.00032768, 150.0, -6.0 / (n_sub * n_ch))
+    assert p["rc"] == 0
+    d = orc.dedisp_delays(p["shift"], tdms, p["mstartdm"], p["mdmstep"])
+    out = orc.dedisperse(buf, d, p["nout"])
+    ev, vals = orc.detect(out, n_ch, n_sub, 6.0, 6)
+    assert len(ev) - 1 > tdms
+    # among single-sample events (binFactor 1) the strongest is at DM index
+    # 100 (DM 10), where the whole band sums to 2048 in one sample
+    ev = np.asarray(ev)
+    one = np.nonzero(ev[1:, 2] == 1)[0] + 1
+    k = one[int(np.argmax(vals[one]))]
+    assert ev[k][0] == 100 and vals[k] == 2048.0
