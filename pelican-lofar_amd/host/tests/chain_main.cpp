@@ -2,31 +2,38 @@
 // (DedispersionPipeline over the Pelican-style modules) and writes every
 // analysed buffer's events.
 //
-//   chain_main <pipeline.xml> <voltages.i16> <nSub> <nSamples> <chunk> <sampleTime> <events.txt>
+//   chain_main <pipeline.xml> <voltages.i16> <nSub> <nSamples> <chunk> <sampleTime> <events.txt> [outputs.xml]
 //
 // voltages: complex int16 [sub][pol=2][t][re,im], nSamples per (sub, pol);
 // each chunk of `chunk` samples per (sub, pol) becomes one
 // TimeSeriesDataSetC32 of chunk/nCh blocks.  Output: one line per buffer
 // "buffer <k> <n>" followed by n lines "dm t binFactor value" (the dummy
-// event first), then "pool <free> <history>".
+// event first), then "pool <free> <history>".  Chunk k carries the time
+// stamp k * chunk * sampleTime.  outputs.xml, if given, holds
+// <DedispersionDataAnalysisOutput> and/or <TriggerOutput> sections: those
+// writers receive every analysed buffer too, as the pipeline's
+// "DedispersionDataAnalysis" output stream.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "../lofar/DedispersionDataAnalysisOutput.h"
 #include "../lofar/DedispersionPipeline.h"
+#include "../lofar/TriggerOutput.h"
 
 using namespace pelican;
 using namespace pelican::ampp;
 
 int main(int argc, char** argv)
 {
-    if (argc != 8) {
+    if (argc != 8 && argc != 9) {
         std::cerr << "usage: chain_main <pipeline.xml> <voltages.i16> <nSub> <nSamples> <chunk> <sampleTime> "
-                     "<events.txt>\n";
+                     "<events.txt> [outputs.xml]\n";
         return 2;
     }
     try {
@@ -44,10 +51,25 @@ int main(int argc, char** argv)
         vf.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(int16_t)));
         if (!vf) throw std::runtime_error("short voltage file");
 
+        std::unique_ptr<DedispersionDataAnalysisOutput> datOut;
+        std::unique_ptr<TriggerOutput> trigOut;
+        if (argc == 9) {
+            std::ifstream of(argv[8]);
+            std::stringstream os;
+            os << of.rdbuf();
+            ConfigNode outputs(os.str());
+            if (!outputs.getNode("DedispersionDataAnalysisOutput").tag().empty())
+                datOut.reset(new DedispersionDataAnalysisOutput(outputs.getNode("DedispersionDataAnalysisOutput")));
+            if (!outputs.getNode("TriggerOutput").tag().empty())
+                trigOut.reset(new TriggerOutput(outputs.getNode("TriggerOutput")));
+        }
+
         std::ofstream out(argv[7]);
         int buffers = 0;
         DedispersionPipeline pipe(config);
         pipe.setEventCallback([&](const DedispersionDataAnalysis& r) {
+            if (datOut) datOut->send("DedispersionDataAnalysis", &r);
+            if (trigOut) trigOut->send("DedispersionDataAnalysis", &r);
             out << "buffer " << buffers++ << " " << r.eventsFound() << "\n";
             for (const auto& e : r.events()) {
                 char v[32];
@@ -60,6 +82,7 @@ int main(int argc, char** argv)
         for (unsigned c0 = 0; c0 + chunk <= nSamp; c0 += chunk) {
             ts.resize(nBlk, nSub, 2, nCh);
             ts.setBlockRate(sampleTime);
+            ts.setLofarTimestamp((double)c0 * sampleTime);
             std::complex<float>* d = ts.data();
             for (unsigned s = 0; s < nSub; ++s)
                 for (unsigned p = 0; p < 2; ++p) {
