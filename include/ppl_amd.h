@@ -217,6 +217,16 @@ int ppl_ingest_i16(const void* d_packets, size_t packetBytes, unsigned headerByt
                    unsigned nSub, unsigned nSamplesPerPacket, unsigned nPol, int sampleBits,
                    int16_t* d_volt, void* stream);
 
+/* --------------------------------------------------------------- bandpass */
+
+/* BandPassRecorder::run, the integrated spectrum (src/lib/src/BandPassRecorder.cpp:
+ * 85-97): for each bin b = s*nChannels + c of a Stokes blob [t][s][pol][c],
+ * d_sum[b] += I[t][s][0][c] over t = 0..nSamples-1 in order, in float (the
+ * reference's sequential sums, bit for bit, across any split into blobs).
+ * d_sum holds nSubbands*nChannels floats; clear it before the first blob. */
+int ppl_bandpass_accumulate(const float* d_stokes, int nSamples, int nSubbands, int nPolarisations,
+                            int nChannels, float* d_sum, void* stream);
+
 /* ------------------------------------------------------------ channeliser */
 
 typedef struct ppl_ppf_plan ppl_ppf_plan;

@@ -104,6 +104,7 @@ SIGNATURES = {
     "ppl_dequantise_sums": (i, [vp, vp, sz, u, f, vp]),
     "ppl_ingest_c32": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
     "ppl_ingest_i16": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
+    "ppl_bandpass_accumulate": (i, [vp, i, i, i, i, vp, vp]),
     "ppl_ppf_coefficients": (i, [u, u, i, vp]),
     "ppl_ppf_plan_create": (i, [vp, vp]),
     "ppl_ppf_plan_destroy": (i, [vp]),
@@ -416,6 +417,20 @@ def ingest(d_packets, n_packets, n_sub, n_spp, n_pol=2, sample_bits=16, header_b
         check(lib().ppl_ingest_i16(_ptr(d_packets), pb, header_bytes, n_packets, n_sub, n_spp, n_pol,
                                    sample_bits, _ptr(o), _stream(stream)))
     return o
+
+
+def bandpass_accumulate(stokes, d_sum, stream=None):
+    """d_sum[s*nCh + c] += stokes[t, s, 0, c] over t in order, float
+    (BandPassRecorder::run's integrated spectrum).  stokes is a float32
+    [t][s][pol][c] device tensor, d_sum a float32 device tensor of s*nCh."""
+    import torch
+    n_t, n_sub, n_pol, n_ch = stokes.shape
+    if stokes.dtype != torch.float32 or not stokes.is_contiguous():
+        raise ValueError("stokes must be a contiguous float32 tensor [t][s][pol][c]")
+    if d_sum.numel() != n_sub * n_ch or d_sum.dtype != stokes.dtype:
+        raise ValueError("d_sum must hold nSubbands*nChannels float32 values")
+    check(lib().ppl_bandpass_accumulate(_ptr(stokes), n_t, n_sub, n_pol, n_ch, _ptr(d_sum), _stream(stream)))
+    return d_sum
 
 
 def ppf_coefficients(n_taps, n_ch, window="kaiser"):

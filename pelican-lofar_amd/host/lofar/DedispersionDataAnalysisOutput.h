@@ -1,5 +1,5 @@
 // DedispersionDataAnalysisOutput.h -- the event text file writer
-// (src/lib/DedispersionDataAnalysisOutput.h:26-42,
+// (src/lib/DedispersionDataAnalysisOutput.h:27-45,
 //  src/lib/src/DedispersionDataAnalysisOutput.cpp:18-112).
 //
 // Configuration (as the reference's):  <file name="PREFIX"/>.  The

@@ -1,5 +1,5 @@
 // TriggerOutput.h -- the FRATS trigger message for the best event of a
-// buffer (src/lib/TriggerOutput.h:38-80, src/lib/src/TriggerOutput.cpp:32-203).
+// buffer (src/lib/TriggerOutput.h:38-75, src/lib/src/TriggerOutput.cpp:32-203).
 //
 // Configuration (the reference's tags):
 //   <Receiver host="IP" port="N"/>        UDP receiver (numeric address)

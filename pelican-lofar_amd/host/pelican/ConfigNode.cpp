@@ -125,6 +125,7 @@ class Parser {
 }  // namespace
 
 unsigned ConfigValue::toUInt() const { return (unsigned)std::strtoul(trim(_s).c_str(), nullptr, 10); }
+unsigned long ConfigValue::toULong() const { return std::strtoul(trim(_s).c_str(), nullptr, 10); }
 int ConfigValue::toInt() const { return (int)std::strtol(trim(_s).c_str(), nullptr, 10); }
 float ConfigValue::toFloat() const { return std::strtof(trim(_s).c_str(), nullptr); }
 double ConfigValue::toDouble() const { return std::strtod(trim(_s).c_str(), nullptr); }

@@ -2,7 +2,7 @@
 // pulsar-search modules use (ConfigNode::getOption / hasAttribute /
 // getAttribute / setFromString / searchFile), over a small XML reader.
 // Values are strings with the Qt-style conversions the modules call
-// (toUInt, toInt, toFloat, toDouble, toLower).
+// (toUInt, toULong, toInt, toFloat, toDouble, toLower).
 #pragma once
 
 #include <map>
@@ -19,6 +19,7 @@ class ConfigValue {
     ConfigValue(std::string s) : _s(std::move(s)) {}
     const std::string& str() const { return _s; }
     unsigned toUInt() const;
+    unsigned long toULong() const;
     int toInt() const;
     float toFloat() const;
     double toDouble() const;
