@@ -223,9 +223,24 @@ int ppl_ingest_i16(const void* d_packets, size_t packetBytes, unsigned headerByt
  * 85-97): for each bin b = s*nChannels + c of a Stokes blob [t][s][pol][c],
  * d_sum[b] += I[t][s][0][c] over t = 0..nSamples-1 in order, in float (the
  * reference's sequential sums, bit for bit, across any split into blobs).
- * d_sum holds nSubbands*nChannels floats; clear it before the first blob. */
+ * d_sum holds nSubbands*nChannels floats; clear it before the first blob.
+ * An empty blob (nSamples 0, d_stokes may be null) adds nothing. */
 int ppl_bandpass_accumulate(const float* d_stokes, int nSamples, int nSubbands, int nPolarisations,
                             int nChannels, float* d_sum, void* stream);
+
+/* ---------------------------------------------------------------- sigproc */
+
+/* SigprocStokesWriter::sendStream, the data written per Stokes blob
+ * (src/lib/src/SigprocStokesWriter.cpp:184-240, _float2int :285-290): d_out
+ * receives nSamples*nPolsToWrite*nSubbands*nChannels values of nBits (32:
+ * float, 8: byte) in file order -- t, p < nPolsToWrite, subbands last to
+ * first, channels last to first.  8 bits: x = min(v, cropMax), byte =
+ * x < cropMin ? 0 : low byte of (int)rint((x - cropMin) * 255 / (cropMax -
+ * cropMin)), float arithmetic.  Other nBits are rejected, as the reference
+ * throws for them. */
+int ppl_sigproc_pack(const float* d_stokes, unsigned nSamples, unsigned nSubbands, unsigned nPolarisations,
+                     unsigned nChannels, unsigned nPolsToWrite, int nBits, float cropMin, float cropMax,
+                     void* d_out, void* stream);
 
 /* ------------------------------------------------------------ channeliser */
 

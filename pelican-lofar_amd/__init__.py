@@ -105,6 +105,7 @@ SIGNATURES = {
     "ppl_ingest_c32": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
     "ppl_ingest_i16": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
     "ppl_bandpass_accumulate": (i, [vp, i, i, i, i, vp, vp]),
+    "ppl_sigproc_pack": (i, [vp, u, u, u, u, u, i, f, f, vp, vp]),
     "ppl_ppf_coefficients": (i, [u, u, i, vp]),
     "ppl_ppf_plan_create": (i, [vp, vp]),
     "ppl_ppf_plan_destroy": (i, [vp]),
@@ -431,6 +432,22 @@ def bandpass_accumulate(stokes, d_sum, stream=None):
         raise ValueError("d_sum must hold nSubbands*nChannels float32 values")
     check(lib().ppl_bandpass_accumulate(_ptr(stokes), n_t, n_sub, n_pol, n_ch, _ptr(d_sum), _stream(stream)))
     return d_sum
+
+
+def sigproc_pack(stokes, n_pols_write=1, n_bits=32, crop_min=0.0, crop_max=255.0, stream=None):
+    """The filterbank bytes of a Stokes blob [t][s][pol][c] as
+    SigprocStokesWriter writes them (subbands and channels reversed; 32-bit
+    floats or 8-bit scaled bytes); returns a device tensor (float32 or uint8)
+    of shape [t][n_pols_write][s][c]."""
+    import torch
+    n_t, n_sub, n_pol, n_ch = stokes.shape
+    if stokes.dtype != torch.float32 or not stokes.is_contiguous():
+        raise ValueError("stokes must be a contiguous float32 tensor [t][s][pol][c]")
+    dt = torch.float32 if n_bits == 32 else torch.uint8
+    out = torch.empty((n_t, n_pols_write, n_sub, n_ch), dtype=dt, device=stokes.device)
+    check(lib().ppl_sigproc_pack(_ptr(stokes), n_t, n_sub, n_pol, n_ch, n_pols_write, n_bits, crop_min, crop_max,
+                                 _ptr(out), _stream(stream)))
+    return out
 
 
 def ppf_coefficients(n_taps, n_ch, window="kaiser"):

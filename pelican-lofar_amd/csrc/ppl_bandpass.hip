@@ -4,18 +4,20 @@
 // sums equal the reference's sequential loop bit for bit whatever the
 // number and size of the blobs fed in.
 //
-// One thread per bin keeps each bin's additions in time order.  Threads of a
-// wave hold consecutive channels, so every time step is one coalesced row
-// read; 16 time steps are loaded before they are added, which keeps 16 loads
-// per thread in flight (the sum is HBM-bound: 4 B read per bin and sample).
+// One thread per bin keeps each bin's additions in time order, so the
+// parallelism is the number of bins (15872 at 62 x 256: 248 waves).  Each
+// workgroup is one wave, so those waves land on as many CUs; lanes hold
+// consecutive channels, so every time step is one coalesced 256-byte row
+// read; 32 time steps are loaded before they are added, which keeps 32 loads
+// per lane in flight (the sum is HBM-bound: 4 B read per bin and sample).
 #include "ppl_internal.h"
 
 namespace {
 
 using namespace ppl;
 
-constexpr int kThreads = 256;
-constexpr int kUnroll = 16;
+constexpr int kThreads = 64;
+constexpr int kUnroll = 32;
 
 __global__ __launch_bounds__(kThreads) void bp_accumulate_kernel(const float* __restrict__ I, float* __restrict__ sum,
                                                                    int nT, int nSub, int nPol, int nCh)
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(kThreads) void bp_accumulate_kernel(const float* __
 extern "C" int ppl_bandpass_accumulate(const float* d_stokes, int nSamples, int nSubbands, int nPolarisations,
                                        int nChannels, float* d_sum, void* stream)
 {
-    PPL_CHECK_ARG(d_stokes && d_sum, "null argument");
+    PPL_CHECK_ARG(d_sum && (d_stokes || nSamples == 0), "null argument");
     PPL_CHECK_ARG(nSamples >= 0 && nSubbands > 0 && nPolarisations > 0 && nChannels > 0,
                   "bad blob shape (%d samples, %d subbands, %d polarisations, %d channels)", nSamples, nSubbands,
                   nPolarisations, nChannels);

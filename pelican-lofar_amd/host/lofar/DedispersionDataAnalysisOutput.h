@@ -60,5 +60,7 @@ class DedispersionDataAnalysisOutput : public AbstractOutputStream {
     int _indexOfDump = 0;
 };
 
+PELICAN_DECLARE(AbstractOutputStream, DedispersionDataAnalysisOutput)
+
 }  // namespace ampp
 }  // namespace pelican

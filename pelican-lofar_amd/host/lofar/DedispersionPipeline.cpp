@@ -55,10 +55,16 @@ void DedispersionPipeline::dedispersionAnalysis(DataBlob* blob)
     const unsigned found = (unsigned)_dedispersionAnalyser->analyse(data, &result);
     ++_analysed;
     if (!found) return;
+    if (_outputs) _outputs->send(&result, "TriggerInput");
     bool write;
     if (_minEventsFound >= _maxEventsFound) write = found >= _minEventsFound;
     else write = found >= _minEventsFound && found <= _maxEventsFound;
-    if (write && _onEvents) _onEvents(result);
+    if (!write) return;
+    if (_onEvents) _onEvents(result);
+    if (_outputs) {
+        _outputs->send(&result, "DedispersionDataAnalysis");
+        for (const SpectrumDataSetStokes* d : data->inputDataBlobs()) _outputs->send(d, "SignalFoundSpectrum");
+    }
 }
 
 void DedispersionPipeline::updateBufferLock(const std::vector<DataBlob*>& freeData)

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <vector>
 
+#include "../pelican/OutputStreamManager.h"
 #include "DataSets.h"
 #include "DedispersionModule.h"
 #include "PPFChanneliser.h"
@@ -70,6 +71,12 @@ class DedispersionPipeline {
     // Called for every analysed dedispersion buffer whose event count passes
     // the events min/max rule (the "DedispersionDataAnalysis" output).
     void setEventCallback(EventCallback cb) { _onEvents = std::move(cb); }
+    // The output streams the pipeline publishes to (dataOutput,
+    // DedispersionPipeline.cpp:175-206): "TriggerInput" for every analysed
+    // buffer with events, then, under the events min/max rule,
+    // "DedispersionDataAnalysis" and "SignalFoundSpectrum" (each Stokes blob
+    // that fed the buffer).  Not owned; null publishes nothing.
+    void setOutputStreams(OutputStreamManager* outputs) { _outputs = outputs; }
     // Waits until every dedispersion buffer submitted so far is analysed.
     void waitForCompletion() { _dedispersionModule->waitForJobCompletion(); }
     DedispersionModule& dedispersionModule() { return *_dedispersionModule; }
@@ -95,6 +102,7 @@ class DedispersionPipeline {
     std::unique_ptr<LockingPtrContainer<SpectrumDataSetStokes>> _stokesBuffer;
     WeightedSpectrumDataSet _weightedIntStokes;
     EventCallback _onEvents;
+    OutputStreamManager* _outputs = nullptr;
     int _analysed = 0;
 };
 

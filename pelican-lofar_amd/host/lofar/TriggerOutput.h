@@ -70,5 +70,7 @@ class TriggerOutput : public AbstractOutputStream {
     int _messageCounter = 0;
 };
 
+PELICAN_DECLARE(AbstractOutputStream, TriggerOutput)
+
 }  // namespace ampp
 }  // namespace pelican

@@ -7,7 +7,11 @@
 // (DedispersionPipeline's "DedispersionDataAnalysis" output).
 #pragma once
 
+#include <functional>
+#include <map>
+#include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "ConfigNode.h"
 #include "DataBlob.h"
@@ -31,4 +35,37 @@ class AbstractOutputStream {
     ConfigNode _config;
 };
 
+// Name -> constructor registry of output streams (PELICAN_DECLARE).
+class OutputStreamFactory {
+  public:
+    using Maker = std::function<AbstractOutputStream*(const ConfigNode&)>;
+    static OutputStreamFactory& instance()
+    {
+        static OutputStreamFactory f;
+        return f;
+    }
+    bool add(const std::string& name, Maker m)
+    {
+        _makers[name] = std::move(m);
+        return true;
+    }
+    AbstractOutputStream* create(const std::string& name, const ConfigNode& config) const
+    {
+        auto it = _makers.find(name);
+        if (it == _makers.end()) throw std::runtime_error("OutputStreamFactory: unknown output stream " + name);
+        return it->second(config);
+    }
+
+  private:
+    std::map<std::string, Maker> _makers;
+};
+
 }  // namespace pelican
+
+// PELICAN_DECLARE(AbstractOutputStream, TYPE): registers an output stream by
+// its class name, as the reference's writers declare themselves.
+#define PELICAN_DECLARE(BASE, TYPE)                                                                        \
+    static_assert(std::is_same<BASE, ::pelican::AbstractOutputStream>::value,                               \
+                  "only output streams are declared this way");                                              \
+    static const bool pelican_output_registered_##TYPE = ::pelican::OutputStreamFactory::instance().add(      \
+        #TYPE, [](const ::pelican::ConfigNode& c) -> ::pelican::AbstractOutputStream* { return new TYPE(c); });

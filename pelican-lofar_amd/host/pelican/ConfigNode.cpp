@@ -179,6 +179,19 @@ ConfigNode ConfigNode::getNode(const std::string& tag) const
     return n;
 }
 
+std::vector<ConfigNode> ConfigNode::getNodes(const std::string& tag) const
+{
+    std::vector<ConfigNode> out;
+    if (_root)
+        for (const auto& c : _root->children)
+            if (tag.empty() || c->tag == tag) {
+                ConfigNode n;
+                n._root = c;
+                out.push_back(n);
+            }
+    return out;
+}
+
 std::string ConfigNode::searchFile(const std::string& file) const
 {
     auto exists = [](const std::string& f) { return std::ifstream(f).good(); };

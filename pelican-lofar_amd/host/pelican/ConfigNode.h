@@ -55,6 +55,8 @@ class ConfigNode {
     ConfigValue getAttribute(const std::string& attribute) const;
     // A child element as its own node (e.g. a module's section of a pipeline file).
     ConfigNode getNode(const std::string& tag) const;
+    // Every child element `tag` in document order (every child if tag is empty).
+    std::vector<ConfigNode> getNodes(const std::string& tag = "") const;
     std::string tag() const { return _root ? _root->tag : std::string(); }
     // Resolves a file name: absolute, or relative to the directories listed
     // in the PPL_CONFIG_PATH environment variable; throws if missing.

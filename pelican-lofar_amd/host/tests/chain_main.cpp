@@ -9,10 +9,10 @@
 // TimeSeriesDataSetC32 of chunk/nCh blocks.  Output: one line per buffer
 // "buffer <k> <n>" followed by n lines "dm t binFactor value" (the dummy
 // event first), then "pool <free> <history>".  Chunk k carries the time
-// stamp k * chunk * sampleTime.  outputs.xml, if given, holds
-// <DedispersionDataAnalysisOutput> and/or <TriggerOutput> sections: those
-// writers receive every analysed buffer too, as the pipeline's
-// "DedispersionDataAnalysis" output stream.
+// stamp k * chunk * sampleTime.  outputs.xml, if given, is a Pelican <output>
+// element (dataStreams + streamers): the pipeline publishes its
+// "TriggerInput", "DedispersionDataAnalysis" and "SignalFoundSpectrum"
+// streams to the writers it names.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -22,8 +22,10 @@
 #include <string>
 #include <vector>
 
+#include "../lofar/BandPassOutput.h"
 #include "../lofar/DedispersionDataAnalysisOutput.h"
 #include "../lofar/DedispersionPipeline.h"
+#include "../lofar/SigprocStokesWriter.h"
 #include "../lofar/TriggerOutput.h"
 
 using namespace pelican;
@@ -51,25 +53,19 @@ int main(int argc, char** argv)
         vf.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(int16_t)));
         if (!vf) throw std::runtime_error("short voltage file");
 
-        std::unique_ptr<DedispersionDataAnalysisOutput> datOut;
-        std::unique_ptr<TriggerOutput> trigOut;
+        std::unique_ptr<OutputStreamManager> outputs;
         if (argc == 9) {
             std::ifstream of(argv[8]);
             std::stringstream os;
             os << of.rdbuf();
-            ConfigNode outputs(os.str());
-            if (!outputs.getNode("DedispersionDataAnalysisOutput").tag().empty())
-                datOut.reset(new DedispersionDataAnalysisOutput(outputs.getNode("DedispersionDataAnalysisOutput")));
-            if (!outputs.getNode("TriggerOutput").tag().empty())
-                trigOut.reset(new TriggerOutput(outputs.getNode("TriggerOutput")));
+            outputs.reset(new OutputStreamManager(ConfigNode(os.str())));
         }
 
         std::ofstream out(argv[7]);
         int buffers = 0;
         DedispersionPipeline pipe(config);
+        pipe.setOutputStreams(outputs.get());
         pipe.setEventCallback([&](const DedispersionDataAnalysis& r) {
-            if (datOut) datOut->send("DedispersionDataAnalysis", &r);
-            if (trigOut) trigOut->send("DedispersionDataAnalysis", &r);
             out << "buffer " << buffers++ << " " << r.eventsFound() << "\n";
             for (const auto& e : r.events()) {
                 char v[32];

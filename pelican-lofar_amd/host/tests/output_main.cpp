@@ -1,19 +1,22 @@
-// output_main.cpp -- drives the two event writers (DedispersionDataAnalysisOutput,
-// TriggerOutput) from a small script, on the host only.
+// output_main.cpp -- drives the output streams (DedispersionDataAnalysisOutput,
+// TriggerOutput, SigprocStokesWriter, ...) through an OutputStreamManager from
+// a small script.
 //
-//   output_main <outputs.xml>  < script
+//   output_main <output.xml>  < script
 //
-// outputs.xml: a root element holding <DedispersionDataAnalysisOutput> and/or
-// <TriggerOutput> sections in the writers' own configuration format.
+// output.xml: a Pelican <output> element (dataStreams + streamers).
 // Script lines (whitespace separated):
 //   spectra <timeBins> <dmBins> <dmStart> <dmStep> <startTime> <blockRate> <firstSample>
 //   rms <rms>
 //   event <dmIndex> <timeIndex> <binFactor> <value>
-//   send                 hand the analysis built so far to every writer, then
-//                        start a new one (time-stamp event first, as the
-//                        analyser does)
-// stdout: "dat <file>" for the analysis writer's file, and the trigger
-// writer's "Datagram written" for each message it sends.
+//   send <stream>        publish the analysis built so far on <stream>
+//   clear                start a new analysis (time-stamp event first, as
+//                        the analyser does)
+//   stokes <file.f32> <nT> <nSub> <nPol> <nCh> <timeStamp> <blockRate>
+//                        load a Stokes blob [t][sub][pol][ch] (nT may be 0:
+//                        no file is read)
+//   sendstokes <stream>  publish the Stokes blob on <stream>
+// The Stokes blob's data reach the device only when a writer asks for them.
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
@@ -23,7 +26,9 @@
 
 #include "../lofar/DataSets.h"
 #include "../lofar/DedispersionDataAnalysisOutput.h"
+#include "../lofar/SigprocStokesWriter.h"
 #include "../lofar/TriggerOutput.h"
+#include "../pelican/OutputStreamManager.h"
 
 using namespace pelican;
 using namespace pelican::ampp;
@@ -31,26 +36,22 @@ using namespace pelican::ampp;
 int main(int argc, char** argv)
 {
     if (argc != 2) {
-        std::cerr << "usage: output_main <outputs.xml> < script\n";
+        std::cerr << "usage: output_main <output.xml> < script\n";
         return 2;
     }
     try {
         std::ifstream xf(argv[1]);
         std::stringstream xs;
         xs << xf.rdbuf();
-        ConfigNode config(xs.str());
-        std::unique_ptr<DedispersionDataAnalysisOutput> dat;
-        std::unique_ptr<TriggerOutput> trig;
-        if (!config.getNode("DedispersionDataAnalysisOutput").tag().empty()) {
-            dat.reset(new DedispersionDataAnalysisOutput(config.getNode("DedispersionDataAnalysisOutput")));
+        OutputStreamManager outputs{ConfigNode(xs.str())};
+        if (auto* dat = dynamic_cast<DedispersionDataAnalysisOutput*>(outputs.streamer("DedispersionDataAnalysisOutput")))
             std::cout << "dat " << dat->fileName() << std::endl;
-        }
-        if (!config.getNode("TriggerOutput").tag().empty())
-            trig.reset(new TriggerOutput(config.getNode("TriggerOutput")));
+        if (auto* fil = dynamic_cast<SigprocStokesWriter*>(outputs.streamer("SigprocStokesWriter")))
+            std::cout << "fil " << fil->fileName() << std::endl;
 
-        SpectrumDataSetStokes stokes;
+        SpectrumDataSetStokes stokes, timing;
         DedispersionSpectra spectra;
-        spectra.setInputDataBlobs({&stokes});
+        spectra.setInputDataBlobs({&timing});
         DedispersionDataAnalysis analysis;
         analysis.reset(&spectra);
         analysis.addEvent(0, 0, 1, 0.0f);
@@ -67,8 +68,8 @@ int main(int argc, char** argv)
                 if (!ls) throw std::runtime_error("bad spectra line: " + line);
                 spectra.resize(tb, nd, (float)start, (float)width);
                 spectra.setFirstSample(first);
-                stokes.setLofarTimestamp(t0);
-                stokes.setBlockRate(rate);
+                timing.setLofarTimestamp(t0);
+                timing.setBlockRate(rate);
             } else if (op == "rms") {
                 double r;
                 ls >> r;
@@ -80,12 +81,32 @@ int main(int argc, char** argv)
                 if (!ls) throw std::runtime_error("bad event line: " + line);
                 analysis.addEvent(dm, t, (float)bf, (float)v);
             } else if (op == "send") {
-                if (dat) dat->send("DedispersionDataAnalysis", &analysis);
-                if (trig) trig->send("DedispersionDataAnalysis", &analysis);
+                std::string stream;
+                if (!(ls >> stream)) throw std::runtime_error("send needs a stream name");
+                outputs.send(&analysis, stream);
+            } else if (op == "clear") {
                 const float rms = analysis.getRMS();
                 analysis.reset(&spectra);
                 analysis.setRMS(rms);
                 analysis.addEvent(0, 0, 1, 0.0f);
+            } else if (op == "stokes") {
+                std::string file;
+                unsigned nT, nSub, nPol, nCh;
+                double t0, rate;
+                ls >> file >> nT >> nSub >> nPol >> nCh >> t0 >> rate;
+                if (!ls) throw std::runtime_error("bad stokes line: " + line);
+                stokes.resize(nT, nSub, nPol, nCh);
+                stokes.setLofarTimestamp(t0);
+                stokes.setBlockRate(rate);
+                if (nT) {
+                    std::ifstream f(file, std::ios::binary);
+                    f.read(reinterpret_cast<char*>(stokes.data()), (std::streamsize)stokes.bytes());
+                    if (!f) throw std::runtime_error("short Stokes file " + file);
+                }
+            } else if (op == "sendstokes") {
+                std::string stream;
+                if (!(ls >> stream)) throw std::runtime_error("sendstokes needs a stream name");
+                outputs.send(&stokes, stream);
             } else {
                 throw std::runtime_error("unknown script line: " + line);
             }

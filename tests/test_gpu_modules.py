@@ -80,15 +80,51 @@ def _read_events(path):
     return bufs, pool
 
 
-OUTPUTS = """<outputs>
-  <DedispersionDataAnalysisOutput><file name="{prefix}"/></DedispersionDataAnalysisOutput>
-  <TriggerOutput>
-    <Trigger format="FRATS"/><Logfile name="{log}"/>
-    <Telescope stationID="CS302" beamID="0"/><RADec rajd="1" decjd="2"/>
-    <frequencyChannel1 MHz="{fch1}"/><Threshold snr="0" events="2"/>
-  </TriggerOutput>
-</outputs>
+OUTPUTS = """<output>
+  <dataStreams>
+    <stream name="DedispersionDataAnalysis" listeners="DedispersionDataAnalysisOutput"/>
+    <stream name="TriggerInput" listeners="TriggerOutput"/>
+    <stream name="SignalFoundSpectrum" listeners="SigprocStokesWriter"/>
+  </dataStreams>
+  <streamers>
+    <DedispersionDataAnalysisOutput><file name="{prefix}"/></DedispersionDataAnalysisOutput>
+    <TriggerOutput>
+      <Trigger format="FRATS"/><Logfile name="{log}"/>
+      <Telescope stationID="CS302" beamID="0"/><RADec rajd="1" decjd="2"/>
+      <frequencyChannel1 MHz="{fch1}"/><Threshold snr="0" events="2"/>
+    </TriggerOutput>
+    <SigprocStokesWriter writeHeader="true">
+      <file filepath="{fil}"/>
+      <subbandsPerPacket value="{nsub}"/><outputChannelsPerSubband value="{nch}"/>
+      <fch1 value="{fch1}"/><foff value="{foff}"/><tsamp value="{tsamp}"/>
+      <RAJ value="053431.9"/><DecJ value="-220052.1"/>
+    </SigprocStokesWriter>
+  </streamers>
+</output>
 """
+
+
+def _check_filterbank(got, plan, clipped, fil_prefix):
+    """The filterbank of every input blob of every written buffer, in order:
+    the header once, then each blob's spectra with subbands and channels
+    reversed (the clipped Stokes-I the C-ABI chain produced)."""
+    import time
+    import sigproc
+    fil = glob.glob(fil_prefix + "_D*.fil")
+    assert len(fil) == 1, fil
+    data = open(fil[0], "rb").read()
+    epoch = time.mktime((2011, 1, 1, 0, 0, 0, 0, 0, -1))
+    hdr = sigproc.header(9, 0, "0534-2200", 53431.9, -220052.1, FCH1, FOFF, N_SUB * N_CH,
+                         5.12e-6 * N_CH * 4, 32, (0.0 - epoch) / 86400 + 55562.0, 1)
+    assert data[:len(hdr)] == hdr
+    per_blob = CHUNK // N_CH // 4
+    body = b""
+    for b in range(len(got)):
+        first, last = b * plan.nout // per_blob, (b * plan.nout + 4095) // per_blob
+        for k in range(first, last + 1):
+            body += sigproc.pack(clipped[k].reshape(per_blob, N_SUB, 1, N_CH))
+    assert len(data) == len(hdr) + len(body)
+    assert data[len(hdr):] == body
 
 
 def _check_writers(got, plan, prefix, log):
@@ -152,8 +188,10 @@ def test_module_chain_matches_c_abi_chain(orc, ppl, tmp_path):
                            fch1=repr(FCH1), foff=repr(FOFF)))
     np.ascontiguousarray(x).tofile(volt_path)
     out_path, prefix, log = str(tmp_path / "outputs.xml"), str(tmp_path / "events"), str(tmp_path / "trigger.log")
+    fil = str(tmp_path / "found")
     with open(out_path, "w") as f:
-        f.write(OUTPUTS.format(prefix=prefix, log=log, fch1=repr(FCH1)))
+        f.write(OUTPUTS.format(prefix=prefix, log=log, fch1=repr(FCH1), fil=fil, nsub=N_SUB, nch=N_CH,
+                               foff=repr(FOFF), tsamp=repr(5.12e-6 * N_CH * 4)))
     r = subprocess.run([CHAIN, xml_path, volt_path, str(N_SUB), str(CHUNK * N_CHUNKS), str(CHUNK),
                         repr(5.12e-6), ev_path, out_path], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
@@ -173,3 +211,4 @@ def test_module_chain_matches_c_abi_chain(orc, ppl, tmp_path):
                 for b, (ev, _) in enumerate(got) for e in ev[1:])
     assert found
     _check_writers(got, plan, prefix, log)
+    _check_filterbank(got, plan, clipped, fil)
