@@ -1,7 +1,7 @@
 // OutputStreamManager.h -- routes a pipeline's published blobs to output
 // streams by stream name (Pelican's OutputStreamManager, configured by the
 // <output> element of a pipeline configuration, e.g.
-// src/pipelines/udpstream/data/stream1.xml:68-82):
+// src/pipelines/udpstream/data/stream1.xml:69-107):
 //
 //   <output>
 //     <dataStreams>
