@@ -692,11 +692,70 @@ class BeamChain:
         self.graphs[self.parity].replay()
         self.parity ^= 1
 
-    def detect(self, threshold=6.0, bin_pow2=6, max_events=1 << 22):
+    def detect(self, threshold=6.0, bin_pow2=6, max_events=1 << 22, d_out=None, stream=None):
+        d = self.out if d_out is None else d_out
         if self.dtype == "u8":
-            return detect_u8(self.out, self.n_chan, 1.0 / self.inv_scale, self.n_ch, self.n_sub, threshold,
-                             bin_pow2, max_events)
-        return detect(self.out, self.n_ch, self.n_sub, threshold, bin_pow2, max_events)
+            return detect_u8(d, self.n_chan, 1.0 / self.inv_scale, self.n_ch, self.n_sub, threshold,
+                             bin_pow2, max_events, stream=stream)
+        return detect(d, self.n_ch, self.n_sub, threshold, bin_pow2, max_events, stream=stream)
+
+    def run_pipelined(self, volts, threshold=6.0, bin_pow2=6, max_events=1 << 22):
+        """Steps over the voltage chunks in `volts` on two streams, yielding
+        each step's (ints, vals, count) detection in order.
+
+        The front stream runs PPF+Stokes, the clipper, the buffer fill and
+        the overlap carry; the back stream runs dedispersion and detection.
+        Step k's front work overlaps step k-1's dedispersion, as the
+        reference's asynchronous DedispersionModule overlaps the next blobs
+        with the GPU job.  Ordering: dedispersion k waits for buffer k to be
+        filled; the carry of step k (which writes the buffer copy
+        dedispersion k-1 reads) waits for dedispersion k-1; dedispersion k
+        writes one of two output tensors, the other holding step k-1's series
+        until its detection has run.  The results equal step()'s."""
+        import torch
+        front, back = torch.cuda.Stream(), torch.cuda.Stream()
+        outs = [self.out, torch.empty_like(self.out)]
+        cur = torch.cuda.current_stream()
+        front.wait_stream(cur)
+        back.wait_stream(cur)
+        done = None
+        k = 0
+        pending = None
+        for d_volt in volts:
+            with torch.cuda.stream(front):
+                self.ppf.stokes_i16(d_volt, self.nout * self.integrate, d_out=self.stokes)
+                self.clip.run(self.stokes)
+                used, full = self.buf.add(self.stokes.view(self.nout, self.n_chan))
+                if used != self.nout or not full:
+                    raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
+                d_in = self.buf.tensor()
+                filled = torch.cuda.Event()
+                filled.record(front)
+                if done is not None:
+                    front.wait_event(done)
+                self.buf.rotate()
+            result = None
+            with torch.cuda.stream(back):
+                if pending is not None:
+                    result = self.detect(threshold, bin_pow2, max_events, d_out=pending, stream=back)
+                out = outs[k & 1]
+                back.wait_event(filled)
+                self.plan.run(d_in, d_out=out)
+                done = torch.cuda.Event()
+                done.record(back)
+            if result is not None:
+                yield result
+            pending = out
+            self.parity ^= 1
+            k += 1
+        if pending is not None:
+            with torch.cuda.stream(back):
+                result = self.detect(threshold, bin_pow2, max_events, d_out=pending, stream=back)
+            yield result
+        cur.wait_stream(front)
+        cur.wait_stream(back)
+        if k and not k & 1:  # leave the last series in self.out, as step() does
+            self.out.copy_(outs[1])
 
 
 # ------------------------------------------------------------- many beams
