@@ -255,6 +255,11 @@ constexpr int kTileRows = 32;
 constexpr int kTileBins = 256;
 constexpr int kTileThreads = 256;
 constexpr int kRowsPerWave = kTileRows / 4;
+// The precompute and learning records: spectra per workgroup.  Each row's
+// ordered sum is one lane's dependent chain, so fewer rows per workgroup
+// spread a sub-chunk's rows over more CUs.
+constexpr int kPreRows = 8;
+constexpr int kPreRowsPerWave = kPreRows / 4;
 
 // Loads of 4 consecutive bins (i % 4 == 0) of polarisation 0 of one spectrum.
 // mode 0: rows of contiguous bins (nPol == 1), 16-byte aligned; mode 1:
@@ -564,16 +569,16 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
     const float* __restrict__ lastGood = bs.lastGood;
     PreRec* __restrict__ pre = bs.pre;
     const int nT = io.nT[beam], subLen = io.subLen[beam], ldMode = io.ldMode[beam];
-    __shared__ float vals[2][kTileRows][kTileBins + 1];
-    __shared__ int cnt[kTileRows][8];
-    __shared__ int rowFlags[kTileRows];
+    __shared__ float vals[2][kPreRows][kTileBins + 1];
+    __shared__ int cnt[kPreRows][8];
+    __shared__ int rowFlags[kPreRows];
     const int tStart = S->tNext;
     if (tStart >= nT) return;
     const Box bx = *box;
     if (!bx.valid) return;
-    const int k0 = blockIdx.x * kTileRows;
+    const int k0 = blockIdx.x * kPreRows;
     const int t0 = tStart + k0;
-    const int nRows = min(min(kTileRows, nT - t0), subLen - k0);
+    const int nRows = min(min(kPreRows, nT - t0), subLen - k0);
     if (nRows <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
@@ -581,16 +586,16 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
     const float invCpb = 1.0f / (float)cpb;
     const long rowStride = (long)nBins * nPol;
     const int nTiles = (nBins + kTileBins - 1) / kTileBins;
-    for (int i = tid; i < kTileRows * 8; i += kTileThreads) (&cnt[0][0])[i] = 0;
-    if (tid < kTileRows) rowFlags[tid] = 0;
+    for (int i = tid; i < kPreRows * 8; i += kTileThreads) (&cnt[0][0])[i] = 0;
+    if (tid < kPreRows) rowFlags[tid] = 0;
     __syncthreads();
 
-    float4 raw[kRowsPerWave];
+    float4 raw[kPreRowsPerWave];
     auto issue = [&](int tile) {
         const int i = tile * kTileBins + lane * 4;
         if (i < nBins) {
 #pragma unroll
-            for (int q = 0; q < kRowsPerWave; ++q) {
+            for (int q = 0; q < kPreRowsPerWave; ++q) {
                 const int r = wave + 4 * q;
                 if (r < nRows) raw[q] = load_quad(I, (long)(t0 + r) * rowStride, i, nCh, nPol, ldMode);
             }
@@ -612,7 +617,7 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
         const int bandLo = band_of(tb0, invCpb);
         const int bandHi = band_of(min(tb0 + kTileBins, nBins) - 1, invCpb);
 #pragma unroll
-        for (int q = 0; q < kRowsPerWave; ++q) {
+        for (int q = 0; q < kPreRowsPerWave; ++q) {
             const int r = wave + 4 * q;
             if (r >= nRows) break;  // uniform over the wave
             bool keep[4];
@@ -682,11 +687,11 @@ __global__ __launch_bounds__(kTileThreads) void clip_learn_kernel(BatchIO io)
     const int nT = io.nT[beam], subLen = io.subLen[beam], ldMode = io.ldMode[beam];
     const int tStart = S->tNext;
     if (tStart >= nT || S->rmsSize >= K.cap) return;
-    const int k0 = blockIdx.x * kTileRows;
+    const int k0 = blockIdx.x * kPreRows;
     const int t0 = tStart + k0;
-    const int nRows = min(min(kTileRows, nT - t0), subLen - k0);
+    const int nRows = min(min(kPreRows, nT - t0), subLen - k0);
     if (nRows <= 0) return;
-    __shared__ float vals[kTileRows][kTileBins + 1];
+    __shared__ float vals[kPreRows][kTileBins + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nBins = K.nBins, nCh = K.nCh, nPol = K.nPol;
     const int cpb = nBins / 8;
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(kTileThreads) void clip_learn_kernel(BatchIO io)
             const int i = tb0 + lane * 4;
             if (i < nBins) {
 #pragma unroll
-                for (int q = 0; q < kRowsPerWave; ++q) {
+                for (int q = 0; q < kPreRowsPerWave; ++q) {
                     const int r = wave + 4 * q;
                     if (r < nRows) {
                         const float4 v = load_quad(I, (long)(t0 + r) * rowStride, i, nCh, nPol, ldMode);
@@ -812,6 +817,9 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     __shared__ float bandMin[8], modelMin[8];
     __shared__ float bandMean[8], bandMeanSq[8];
     __shared__ Staged stg[kStage];
+    // the fold inputs of the staged spectra again, one array per field, so
+    // that the single-lane folds read four spectra per LDS access
+    __shared__ __align__(16) float soaMeanFC[kStage], soaMeanC[kStage], soaRmsFC[kStage], soaFracInc[kStage];
     // first spectrum at or after k the fast run must not take: undecided
     // channels (blkU), or undecided channels or a last-good fill (blkUF)
     __shared__ int blkU[kStage], blkUF[kStage];
@@ -1135,17 +1143,28 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         const float mOR = s.meanOverRMS;
         const double rcpMOR = 1.0 / (double)mOR;
         // The folds read their inputs sixteen spectra at a time, ahead of the
-        // dependent adds, so that LDS latency is paid once per sixteen.
+        // dependent adds, so that LDS latency is paid once per sixteen; from
+        // the first multiple of four on, four spectra per LDS access.
         constexpr int B = 16;
+        auto ld4 = [](const float* a, int k) { return *reinterpret_cast<const float4*>(a + k); };
+        auto st4 = [](float* a, int k, float x, float y, float z, float w) {
+            *reinterpret_cast<float4*>(a + k) = make_float4(x, y, z, w);
+        };
         if (tid == 0) {
             float v = s.meanRunAve, bc = s.meanBack / capf;
             int k = k0;
+            for (; k < kEnd && (k & 3); ++k) {
+                v = (v - soaMeanFC[k]) + bc;
+                spM[k] = v;
+                bc = soaMeanC[k];
+            }
             for (; k + B <= kEnd; k += B) {
                 float fc[B], mc[B], o[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) {
-                    fc[j] = stg[k + j].meanFC;
-                    mc[j] = stg[k + j].meanC;
+                for (int j = 0; j < B; j += 4) {
+                    const float4 a = ld4(soaMeanFC, k + j), c = ld4(soaMeanC, k + j);
+                    fc[j] = a.x; fc[j + 1] = a.y; fc[j + 2] = a.z; fc[j + 3] = a.w;
+                    mc[j] = c.x; mc[j + 1] = c.y; mc[j + 2] = c.z; mc[j + 3] = c.w;
                 }
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
@@ -1154,30 +1173,37 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                     bc = mc[j];
                 }
 #pragma unroll
-                for (int j = 0; j < B; ++j) spM[k + j] = o[j];
+                for (int j = 0; j < B; j += 4) st4(spM, k + j, o[j], o[j + 1], o[j + 2], o[j + 3]);
             }
             for (; k < kEnd; ++k) {
-                v = (v - stg[k].meanFC) + bc;
+                v = (v - soaMeanFC[k]) + bc;
                 spM[k] = v;
-                bc = stg[k].meanC;
+                bc = soaMeanC[k];
             }
         } else if (tid == 64) {
             float f = s.fractionBadChannels;
             int k = k0;
+            for (; k < kEnd && (k & 3); ++k) {
+                f += soaFracInc[k];
+                spF[k] = f;
+            }
             for (; k + B <= kEnd; k += B) {
                 float fi[B], o[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) fi[j] = stg[k + j].fracInc;
+                for (int j = 0; j < B; j += 4) {
+                    const float4 a = ld4(soaFracInc, k + j);
+                    fi[j] = a.x; fi[j + 1] = a.y; fi[j + 2] = a.z; fi[j + 3] = a.w;
+                }
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     f += fi[j];
                     o[j] = f;
                 }
 #pragma unroll
-                for (int j = 0; j < B; ++j) spF[k + j] = o[j];
+                for (int j = 0; j < B; j += 4) st4(spF, k + j, o[j], o[j + 1], o[j + 2], o[j + 3]);
             }
             for (; k < kEnd; ++k) {
-                f += stg[k].fracInc;
+                f += soaFracInc[k];
                 spF[k] = f;
             }
         }
@@ -1215,12 +1241,18 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         if (tid == 0) {
             float r = s.rmsRunAve, bc = s.rmsBack / capf;
             int k = k0;
+            for (; k < kEnd && (k & 3); ++k) {
+                r = (r - soaRmsFC[k]) + bc;
+                spR[k] = r;
+                bc = spB[k];
+            }
             for (; k + B <= kEnd; k += B) {
                 float fc[B], bb[B], o[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) {
-                    fc[j] = stg[k + j].rmsFC;
-                    bb[j] = spB[k + j];
+                for (int j = 0; j < B; j += 4) {
+                    const float4 a = ld4(soaRmsFC, k + j), c = ld4(spB, k + j);
+                    fc[j] = a.x; fc[j + 1] = a.y; fc[j + 2] = a.z; fc[j + 3] = a.w;
+                    bb[j] = c.x; bb[j + 1] = c.y; bb[j + 2] = c.z; bb[j + 3] = c.w;
                 }
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
@@ -1229,45 +1261,58 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                     bc = bb[j];
                 }
 #pragma unroll
-                for (int j = 0; j < B; ++j) spR[k + j] = o[j];
+                for (int j = 0; j < B; j += 4) st4(spR, k + j, o[j], o[j + 1], o[j + 2], o[j + 3]);
             }
             for (; k < kEnd; ++k) {
-                r = (r - stg[k].rmsFC) + bc;
+                r = (r - soaRmsFC[k]) + bc;
                 spR[k] = r;
                 bc = spB[k];
             }
         } else if (tid == 64) {
             float p = s.params0, bm = s.bpMean;
+            const int* const spDi = spD;
             int k = k0;
+            for (; k < kEnd && (k & 3); ++k) {
+                spP[k] = p;
+                if (spDi[k]) {
+                    p += spQ[k];
+                    bm = spM[k];
+                }
+                spPA[k] = p;
+                spBM[k] = bm;
+            }
             for (; k + B <= kEnd; k += B) {
                 float q[B], m[B], op[B], opa[B], obm[B];
                 int d[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) {
-                    q[j] = spQ[k + j];
-                    m[j] = spM[k + j];
-                    d[j] = spD[k + j];
+                for (int j = 0; j < B; j += 4) {
+                    const float4 a = ld4(spQ, k + j), c = ld4(spM, k + j);
+                    const int4 e = *reinterpret_cast<const int4*>(spDi + k + j);
+                    q[j] = a.x; q[j + 1] = a.y; q[j + 2] = a.z; q[j + 3] = a.w;
+                    m[j] = c.x; m[j + 1] = c.y; m[j + 2] = c.z; m[j + 3] = c.w;
+                    d[j] = e.x; d[j + 1] = e.y; d[j + 2] = e.z; d[j + 3] = e.w;
                 }
+                // selects, not branches: the chain is one add and one
+                // select per spectrum (p + q is taken only where it moves)
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     op[j] = p;
-                    if (d[j]) {
-                        p += q[j];
-                        bm = m[j];
-                    }
+                    const float pn = p + q[j];
+                    p = d[j] ? pn : p;
+                    bm = d[j] ? m[j] : bm;
                     opa[j] = p;
                     obm[j] = bm;
                 }
 #pragma unroll
-                for (int j = 0; j < B; ++j) {
-                    spP[k + j] = op[j];
-                    spPA[k + j] = opa[j];
-                    spBM[k + j] = obm[j];
+                for (int j = 0; j < B; j += 4) {
+                    st4(spP, k + j, op[j], op[j + 1], op[j + 2], op[j + 3]);
+                    st4(spPA, k + j, opa[j], opa[j + 1], opa[j + 2], opa[j + 3]);
+                    st4(spBM, k + j, obm[j], obm[j + 1], obm[j + 2], obm[j + 3]);
                 }
             }
             for (; k < kEnd; ++k) {
                 spP[k] = p;
-                if (spD[k]) {
+                if (spDi[k]) {
                     p += spQ[k];
                     bm = spM[k];
                 }
@@ -1427,6 +1472,8 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 stg[k].badBands = r.badBands;
                 stg[k].flags = r.flags;
                 stg[k].meanC = r.mean / capf;
+                soaMeanC[k] = stg[k].meanC;
+                soaFracInc[k] = r.fracInc;
             }
             for (int k = tid; k < nPre; k += kWalkThreads)
                 if (stg[k].badBands >= 4) stg[k].flags |= 4;
@@ -1457,6 +1504,8 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 for (int k = tid; k < stageN; k += kWalkThreads) {
                     stg[k].rmsFC = rmsBuf[(s.rmsStart + k) % K.cap] / capf;
                     stg[k].meanFC = meanBuf[(s.meanStart + k) % K.cap] / capf;
+                    soaRmsFC[k] = stg[k].rmsFC;
+                    soaMeanFC[k] = stg[k].meanFC;
                 }
             }
             __syncthreads();
@@ -1593,18 +1642,22 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         }
         __syncthreads();
         // each thread's bins are read kG at a time, all in flight together,
-        // before any of them is judged (the replacements write I)
+        // before any of them is judged (the replacements write I); so are
+        // their last-good values, which this pass only reads, so that the
+        // replacements do not wait for one load after another
         constexpr int kG = 8;
         for (int kb0 = (tid & ~63); kb0 < nBins; kb0 += kWalkThreads * kG) {
-            float vv[kG], bl[kG];
+            float vv[kG], bl[kG], lg[kG];
 #pragma unroll
             for (int j = 0; j < kG; ++j) {
                 const int i = kb0 + j * kWalkThreads + (tid & 63);
                 vv[j] = 0.0f;
                 bl[j] = 0.0f;
+                lg[j] = 0.0f;
                 if (i < nBins) {
                     const int sb = i / nCh, c = i % nCh;
                     vv[j] = I[idx(sb, 0, t) + c];
+                    lg[j] = lastGood[i];
                     if (chkPre) bl[j] = bpLo[i];
                 }
             }
@@ -1621,7 +1674,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                     band = band_of(i, invCpbW);
                     if (v - dataModel - bpv > margin) {
                         const int sb = i / nCh, c = i % nCh;
-                        const float g = lastGood[i];
+                        const float g = lg[j];
                         for (int pol = 0; pol < nPol; ++pol) I[idx(sb, pol, t) + c] = g;
                     } else {
                         kept = true;
@@ -2243,9 +2296,9 @@ static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n,
     for (int it = 0; it < nIter; ++it) {
         hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
         PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(maxSub, kTileRows), 1, nb), dim3(kTileThreads), 0, st, io);
+        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(maxSub, kPreRows), 1, nb), dim3(kTileThreads), 0, st, io);
         PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_learn_kernel, dim3(cdiv(maxSub, kTileRows), 1, nb), dim3(kTileThreads), 0, st, io);
+        hipLaunchKernelGGL(clip_learn_kernel, dim3(cdiv(maxSub, kPreRows), 1, nb), dim3(kTileThreads), 0, st, io);
         PPL_LAUNCHED();
         hipLaunchKernelGGL(clip_walk_kernel, dim3(1, 1, nb), dim3(kWalkThreads), 0, st, io, it == nIter - 1 ? 1 : 0);
         PPL_LAUNCHED();
