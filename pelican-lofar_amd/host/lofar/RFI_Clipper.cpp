@@ -57,6 +57,7 @@ RFI_Clipper::RFI_Clipper(const ConfigNode& config) : AbstractModule(config)
         throw std::runtime_error(std::string("RFI_Clipper: ") + e.what());
     }
     _bp = readBandPassFile(path);
+    _bandPass = BandPass::fromFileValues(_bp);
     ppl_clipper_config cfg{};
     cfg.bp_values = _bp.data();
     cfg.n_values = (int)_bp.size();

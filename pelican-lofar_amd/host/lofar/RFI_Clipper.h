@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../pelican/DataBlob.h"
+#include "BandPass.h"
 #include "DataSets.h"
 
 struct ppl_clipper;
@@ -34,10 +35,15 @@ class RFI_Clipper : public AbstractModule {
     // Per-spectrum flags of the last run (1 = kept).
     std::vector<unsigned char> lastFlags() const;
     const std::vector<double>& bandPassValues() const { return _bp; }
+    // The bandpass read from the file (RFI_Clipper.h:38): BandPassAdapter's
+    // model on the file's map, its median and RMS.  The running model the
+    // clipper updates per spectrum stays on the device.
+    const BandPass& bandPass() const { return _bandPass; }
 
   private:
     bool _active = true;
     std::vector<double> _bp;
+    BandPass _bandPass;
     ppl_clipper* _clip = nullptr;
     unsigned _lastNT = 0;
 };

@@ -3,6 +3,7 @@
 //
 //   bandpass_main <config.xml> stream <stokes.f32> <nT> <nSub> <nPol> <nCh> <chunk>
 //   bandpass_main <config.xml> spectrum <spectrum.f32> <nBins>
+//   bandpass_main - bp <file.bp> [<nBins2> <start2> <width2>]
 //
 // config.xml: a root element holding <BandPassRecorder> and, optionally,
 // <BandPassOutput> sections.  "stream" feeds the float32 [t][sub][pol][ch]
@@ -11,7 +12,10 @@
 // averaged spectrum to fitSpectrum (host only).  stdout: one JSON object
 // with the blobs used, spectra integrated, fits made, the bandpass RMS,
 // median, mean and parameters (%.9g) and, for "spectrum", the cleaned
-// spectrum's file (<spectrum.f32>.clean).
+// spectrum's file (<spectrum.f32>.clean).  "bp" reads a .bp file as the
+// clipper does (BandPass::fromFileValues), optionally rebins it, and prints
+// the current map's median, RMS, mean, first and last bin, the constant
+// coefficient and the map's start and end frequencies.
 #include <cstdio>
 #include <algorithm>
 #include <cstdlib>
@@ -25,6 +29,7 @@
 #include "../lofar/BandPass.h"
 #include "../lofar/BandPassOutput.h"
 #include "../lofar/BandPassRecorder.h"
+#include "../lofar/RFI_Clipper.h"
 
 using namespace pelican;
 using namespace pelican::ampp;
@@ -54,6 +59,23 @@ int main(int argc, char** argv)
         return 2;
     }
     try {
+        if (std::string(argv[2]) == "bp") {
+            if (argc != 4 && argc != 7) throw std::runtime_error("bp needs a file and optionally a map");
+            BandPass b = BandPass::fromFileValues(readBandPassFile(argv[3]));
+            if (argc == 7) {
+                BinMap m((unsigned)std::atoi(argv[4]));
+                m.setStart(std::atof(argv[5]));
+                m.setBinWidth(std::atof(argv[6]));
+                b.reBin(m);
+            }
+            const size_t n = b.currentSet().size();
+            std::cout << "{\"median\": " << g9(b.median()) << ", \"rms\": " << g9(b.rms())
+                      << ", \"mean\": " << g9(b.mean()) << ", \"first\": " << g9(b.intensityOfBin(0))
+                      << ", \"last\": " << g9(b.intensityOfBin((unsigned)n - 1)) << ", \"p0\": "
+                      << g9(b.params()[0]) << ", \"start\": " << g9(b.startFrequency())
+                      << ", \"end\": " << g9(b.endFrequency()) << ", \"bins\": " << n << "}" << std::endl;
+            return 0;
+        }
         std::ifstream xf(argv[1]);
         std::stringstream xs;
         xs << xf.rdbuf();
