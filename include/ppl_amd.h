@@ -217,6 +217,39 @@ int ppl_ingest_i16(const void* d_packets, size_t packetBytes, unsigned headerByt
                    unsigned nSub, unsigned nSamplesPerPacket, unsigned nPol, int sampleBits,
                    int16_t* d_volt, void* stream);
 
+/* LofarDataSplittingChunker's packet sequencing (src/lib/src/
+ * LofarDataSplittingChunker.cpp:186-325), host code: packets received in
+ * arrival order, each starting with the 16-byte LOFAR UDP header
+ * (src/lib/LofarUdpHeader.h:25-36), are placed in the slots of a chunk.
+ * h_slotSource[s] receives the packet index for slot s, or -1 for a lost
+ * packet, which the chunker writes as an empty (zero) packet.  As the
+ * chunker does: a packet whose timestamp is 0xFFFFFFFF or more than 10 s
+ * ahead is rejected, as is one less than nSamplesPerPacket blocks on
+ * (duplicates; the very first packet of a stream is always one); a gap of
+ * whole packets becomes empty slots, with blocks per second 156250 (160 MHz
+ * clock) or alternately 195313 and 195312 (200 MHz); a packet after gap
+ * slots that filled the chunk is dropped.  Stops when nSlots slots are
+ * filled or the packets run out (*slotsFilled, *packetsUsed); the state
+ * carries the stream position to the next call.  Zero-initialise it for a
+ * new stream. */
+typedef struct ppl_packet_seq {
+    unsigned startTime;    /* timestamp of the last slot filled (0 before any) */
+    unsigned startBlockid; /* its block sequence number */
+    unsigned long long accepted, rejected, lost;
+} ppl_packet_seq;
+int ppl_packet_sequence(ppl_packet_seq* state, const void* h_packets, size_t packetBytes, unsigned nPackets,
+                        unsigned nSamplesPerPacket, int clockMHz, int* h_slotSource, unsigned nSlots,
+                        unsigned* packetsUsed, unsigned* slotsFilled);
+/* ppl_ingest_c32 / _i16 over the slots of a chunk: slot s takes packet
+ * d_slotSource[s] (device int array) of the nPackets given, or zeros where
+ * it is negative (or not below nPackets). */
+int ppl_ingest_c32_slots(const void* d_packets, size_t packetBytes, unsigned headerBytes, unsigned nPackets,
+                         const int* d_slotSource, unsigned nSlots, unsigned nSub, unsigned nSamplesPerPacket,
+                         unsigned nPol, int sampleBits, float* d_blob, void* stream);
+int ppl_ingest_i16_slots(const void* d_packets, size_t packetBytes, unsigned headerBytes, unsigned nPackets,
+                         const int* d_slotSource, unsigned nSlots, unsigned nSub, unsigned nSamplesPerPacket,
+                         unsigned nPol, int sampleBits, int16_t* d_volt, void* stream);
+
 /* --------------------------------------------------------------- bandpass */
 
 /* BandPassRecorder::run, the integrated spectrum (src/lib/src/BandPassRecorder.cpp:

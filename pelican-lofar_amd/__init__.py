@@ -55,6 +55,12 @@ class PPFConfig(C.Structure):
                 ("coeffs", C.c_void_p), ("precision", C.c_int), ("integrate", C.c_uint)]
 
 
+class PacketSeq(C.Structure):
+    """Stream position and counts of ppl_packet_sequence (zero for a new stream)."""
+    _fields_ = [("startTime", C.c_uint), ("startBlockid", C.c_uint), ("accepted", C.c_ulonglong),
+                ("rejected", C.c_ulonglong), ("lost", C.c_ulonglong)]
+
+
 class ClipperConfig(C.Structure):
     _fields_ = [("bp_values", C.c_void_p), ("n_values", C.c_int), ("crFactor", C.c_float),
                 ("srFactor", C.c_float), ("maxHistory", C.c_int), ("zeroDM", C.c_int),
@@ -104,6 +110,9 @@ SIGNATURES = {
     "ppl_dequantise_sums": (i, [vp, vp, sz, u, f, vp]),
     "ppl_ingest_c32": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
     "ppl_ingest_i16": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
+    "ppl_packet_sequence": (i, [vp, vp, sz, u, u, i, vp, u, vp, vp]),
+    "ppl_ingest_c32_slots": (i, [vp, sz, u, u, vp, u, u, u, u, i, vp, vp]),
+    "ppl_ingest_i16_slots": (i, [vp, sz, u, u, vp, u, u, u, u, i, vp, vp]),
     "ppl_bandpass_accumulate": (i, [vp, i, i, i, i, vp, vp]),
     "ppl_sigproc_pack": (i, [vp, u, u, u, u, u, i, f, f, vp, vp]),
     "ppl_ppf_coefficients": (i, [u, u, i, vp]),
@@ -401,13 +410,41 @@ def detect_u8(d_data, n_chan, scale, n_channels, n_subbands, threshold=6.0, bin_
 WINDOWS = {"kaiser": 0, "gaussian": 1, "blackman": 2, "hamming": 3, "hann": 4}
 
 
+def sequence_packets(state, h_packets, n_spp, n_slots, clock_mhz=200):
+    """Place received packets (a host uint8 array [packet][bytes], LOFAR
+    headers first) in n_slots chunk slots as LofarDataSplittingChunker does;
+    returns (slot sources: int32 array, -1 = lost packet, packets used).
+    state: a PacketSeq, updated."""
+    raw = np.ascontiguousarray(h_packets, np.uint8)
+    slots = np.full(n_slots, -2, np.int32)
+    used, filled = C.c_uint(0), C.c_uint(0)
+    check(lib().ppl_packet_sequence(C.byref(state), raw.ctypes.data_as(C.c_void_p), raw.shape[1], raw.shape[0],
+                                    n_spp, clock_mhz, slots.ctypes.data_as(C.c_void_p), n_slots, C.byref(used),
+                                    C.byref(filled)))
+    return slots[:filled.value], used.value
+
+
 def ingest(d_packets, n_packets, n_sub, n_spp, n_pol=2, sample_bits=16, header_bytes=0, packet_bytes=None,
-           out="c32", stream=None):
+           out="c32", stream=None, slots=None):
     """Packet payloads -> [sub][pol][t] samples (complex float or int16 pairs)
-    on the device (AdapterTimeSeriesDataSet::_readData)."""
+    on the device (AdapterTimeSeriesDataSet::_readData); with `slots`, chunk
+    slot s holds packet slots[s], or zeros where it is -1."""
     import torch
     payload = n_sub * n_spp * n_pol * 2 * (sample_bits // 8)
     pb = packet_bytes if packet_bytes is not None else header_bytes + payload
+    if slots is not None:
+        # slots: a device int32 tensor of packet indices, -1 for a lost packet
+        n_slots = slots.numel()
+        n_samp = n_spp * n_slots
+        if out == "c32":
+            o = torch.empty((n_sub, n_pol, n_samp, 2), dtype=torch.float32, device=d_packets.device)
+            check(lib().ppl_ingest_c32_slots(_ptr(d_packets), pb, header_bytes, n_packets, _ptr(slots), n_slots,
+                                             n_sub, n_spp, n_pol, sample_bits, _ptr(o), _stream(stream)))
+        else:
+            o = torch.empty((n_sub, n_pol, n_samp, 2), dtype=torch.int16, device=d_packets.device)
+            check(lib().ppl_ingest_i16_slots(_ptr(d_packets), pb, header_bytes, n_packets, _ptr(slots), n_slots,
+                                             n_sub, n_spp, n_pol, sample_bits, _ptr(o), _stream(stream)))
+        return o
     n_samp = n_spp * n_packets
     if out == "c32":
         o = torch.empty((n_sub, n_pol, n_samp, 2), dtype=torch.float32, device=d_packets.device)

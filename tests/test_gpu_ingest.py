@@ -57,3 +57,36 @@ def test_ingest_rejects_bad_bits(ppl):
     d = torch.zeros(1024, dtype=torch.uint8, device="cuda")
     with pytest.raises(ppl.PPLError):
         ppl.ingest(d, 1, 2, 4, 2, sample_bits=12)
+
+
+@pytest.mark.parametrize("bits", [16, 8])
+def test_ingest_slots_with_lost_packets(ppl, bits):
+    """A received stream with gaps, duplicates and bad timestamps, placed in
+    chunk slots by the host sequencer (LofarDataSplittingChunker), then
+    ingested slot by slot: lost slots are zeros (the chunker's empty
+    packets), the rest the packets' payloads, as the oracle lays them out."""
+    import packets as orc_packets
+    from test_packets import N_SPP, _random_stream
+    rng = np.random.default_rng(bits)
+    stream = _random_stream(rng, 120)
+    n_sub, n_pol, header = 31, 2, 16
+    raw, payload = _packets(rng, len(stream), n_sub, N_SPP, n_pol, bits, header)
+    for k, (s, b) in enumerate(stream):
+        raw[k, :16] = np.frombuffer(orc_packets.header(s, b), np.uint8)
+    st = ppl.PacketSeq()
+    slots, used = ppl.sequence_packets(st, raw, N_SPP, 100)
+    st_o = dict(startTime=0, startBlockid=0, accepted=0, rejected=0, lost=0)
+    want_slots, _ = orc_packets.sequence(st_o, stream, N_SPP, 100)
+    assert list(slots) == want_slots and (slots < 0).any()
+    d = torch.from_numpy(raw).cuda()
+    d_slots = torch.from_numpy(slots.astype(np.int32)).cuda()
+    want = orc_packets.chunk(payload, want_slots)
+    c32 = ppl.ingest(d, len(stream), n_sub, N_SPP, n_pol, bits, header, out="c32", slots=d_slots).cpu().numpy()
+    assert np.array_equal(c32, want.astype(np.float32))
+    i16 = ppl.ingest(d, len(stream), n_sub, N_SPP, n_pol, bits, header, out="i16", slots=d_slots).cpu().numpy()
+    assert np.array_equal(i16, want.astype(np.int16))
+    # an index past the packets given reads as a lost packet
+    bad = torch.tensor([0, len(stream), -1], dtype=torch.int32, device="cuda")
+    out = ppl.ingest(d, len(stream), n_sub, N_SPP, n_pol, bits, header, out="i16", slots=bad).cpu().numpy()
+    assert np.array_equal(out[:, :, :N_SPP], payload[0].transpose(0, 2, 1, 3).astype(np.int16))
+    assert not out[:, :, N_SPP:].any()
