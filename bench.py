@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--chunks", type=int, default=2, help="distinct input chunks kept in HBM")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP16 / u8 comparison chains")
-    ap.add_argument("--tile", default="", help="DM,T,chanblock tile for dedispersion (default: autotune)")
+    ap.add_argument("--tile", default="", help="DM,T,chanblock[,readbytes] tile for dedispersion (default: autotune)")
     ap.add_argument("--cpu-spectra", type=int, default=8192,
                     help="integrated spectra in the timed CPU sample")
     ap.add_argument("--beams", type=int, default=24, help="beams in the batched multi-beam run")
@@ -325,7 +325,7 @@ def main():
                 series = mc.out.double()
             rec = {"value": TDMS * nout * args.steps / mwall, "ms_per_step": 1e3 * mwall / args.steps,
                    "dedisperse_ms": round(sum(ev[3].elapsed_time(ev[4]) for ev in mt) / args.steps, 3),
-                   "tile_dm_t_chanblock": list(mc.plan.tile),
+                   "tile_dm_t_chanblock_readbytes": list(mc.plan.tile),
                    "autotune_ms": None if m_tune is None else round(m_tune, 3),
                    "events": sum(len(e[0]) - 1 for e in mev)}
             if mdt == "f32":
@@ -390,7 +390,7 @@ def main():
         "config": {"workload": "lofar-beam: 62 sub x 2 pol x 64 ch (3968 ch), 8-tap PPF, Stokes-I x4, "
                                "RFI clipper, 2048 DMs 0-500 on 65536-sample buffers, 6-sigma detection",
                    "nOut_per_step": nout, "input_samples_per_stream_per_step": n_samp,
-                   "dedisp_input": dd_dtype, "dft": "fp32", "tile_dm_t_chanblock": list(chain.plan.tile),
+                   "dedisp_input": dd_dtype, "dft": "fp32", "tile_dm_t_chanblock_readbytes": list(chain.plan.tile),
                    "autotune_ms": None if tune_ms is None else round(tune_ms, 3)},
         "real_time_factor": rtf,
         "schedule": "two-stream" if pipe is not None and value == pipe["value"] else "one-stream",
@@ -643,7 +643,7 @@ def multibeam(args, torch, ppl):
                "graph_value": mb.trials_samples() / gwall,
                "real_time_factor_aggregate": sum(secs) / wall, "real_time_factor_min_beam": min(secs) / wall,
                "stages_ms": stages,
-               "tile_dm_t_chanblock": list(mb.tile) if mb.tile != "own" else [list(p.tile) for p in mb.plans],
+               "tile_dm_t_chanblock_readbytes": list(mb.tile) if mb.tile != "own" else [list(p.tile) for p in mb.plans],
                "dft": "fp32" if dtype == "f32" else "bf16x3 (64, 256 ch), fp32 (16 ch)",
                "autotune_ms": round(tune_ms, 3),
                "autotune_s": round(tune_s, 1), "resident_gb": round(mem_gb, 1), "warmup_steps": warm,

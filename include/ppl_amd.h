@@ -87,6 +87,7 @@ typedef struct {
     float mdmstep;          /* dmStep / tsamp (float) */
     int tileDM, tileT;      /* DM x time tile chosen for this plan */
     int chanBlock;          /* channels staged per block (0: as many as fit in LDS) */
+    int readBytes;          /* LDS read width per lane of the chosen kernel: 4 or 16 */
 } ppl_dedisp_info;
 
 /* DedispersionModule::resize (src/lib/src/DedispersionModule.cpp:109-190):
@@ -102,7 +103,11 @@ int ppl_dedisp_plan_info(const ppl_dedisp_plan* plan, ppl_dedisp_info* info);
 int ppl_dedisp_set_tile(ppl_dedisp_plan* plan, int tileDM, int tileT);
 /* Same with the number of channels staged per LDS block (0: as many as fit). */
 int ppl_dedisp_set_shape(ppl_dedisp_plan* plan, int tileDM, int tileT, int chanBlock);
-/* Autotuner: times every instantiated DM x time tile and channel block on
+/* Same with the kernel's LDS read width per lane: 4 bytes (one sample), or 16
+ * (four consecutive samples per read, FP32 input only; tileT a multiple of
+ * 256).  Both give the same sums bit for bit. */
+int ppl_dedisp_set_kernel(ppl_dedisp_plan* plan, int tileDM, int tileT, int chanBlock, int readBytes);
+/* Autotuner: times every instantiated DM x time tile, channel block and read width on
  * the given buffers (reps launches each, HIP events on the stream), keeps the
  * fastest for the plan and returns its time per launch in *best_ms. */
 int ppl_dedisp_autotune(ppl_dedisp_plan* plan, const void* d_in, void* d_out, int reps, void* stream,
@@ -121,9 +126,9 @@ int ppl_dedisperse(const ppl_dedisp_plan* plan, const void* d_in, void* d_out, v
  * reference dedisperses one beam per GPU job, DedispersionModule.cpp:263-297).
  * Each job is a plan with its own input buffer and output; plans may differ
  * in channels, buffer length, DM count and range, but share the dtype.  All
- * jobs run under one DM x time tile and channel block (tileDM = tileT = 0:
- * the first plan's), each with its own LDS row length.  Results equal n
- * separate ppl_dedisperse calls, bit for bit. */
+ * jobs run under one DM x time tile, channel block and read width (tileDM =
+ * tileT = 0: the first plan's), each with its own LDS row length.  Results
+ * equal n separate ppl_dedisperse calls, bit for bit. */
 typedef struct {
     const ppl_dedisp_plan* plan;
     const void* d_in;
@@ -131,11 +136,11 @@ typedef struct {
 } ppl_dedisp_job;
 enum { PPL_DEDISP_MAX_JOBS = 48 };
 int ppl_dedisperse_batch(const ppl_dedisp_job* jobs, int n, int tileDM, int tileT, int chanBlock,
-                         void* stream);
-/* Autotuner for a batch: times every instantiated tile and channel block
- * that fits every job (reps launches each) and returns the fastest. */
+                         int readBytes, void* stream);
+/* Autotuner for a batch: times every instantiated tile, channel block and
+ * read width that fits every job (reps launches each) and returns the fastest. */
 int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int reps, void* stream, int* tileDM,
-                              int* tileT, int* chanBlock, float* best_ms);
+                              int* tileT, int* chanBlock, int* readBytes, float* best_ms);
 
 /* The reference's C seam itself, with the reference's arguments (device
  * pointers; maxshift includes the remainder; outbufSize in bytes), computed

@@ -47,7 +47,7 @@ class DedispConfig(C.Structure):
 class DedispInfo(C.Structure):
     _fields_ = [("maxshift", C.c_int), ("remainingSamples", C.c_int), ("nOut", C.c_int),
                 ("mstartdm", C.c_float), ("mdmstep", C.c_float), ("tileDM", C.c_int),
-                ("tileT", C.c_int), ("chanBlock", C.c_int)]
+                ("tileT", C.c_int), ("chanBlock", C.c_int), ("readBytes", C.c_int)]
 
 
 class PPFConfig(C.Structure):
@@ -90,11 +90,12 @@ SIGNATURES = {
     "ppl_dedisp_plan_info": (i, [vp, vp]),
     "ppl_dedisp_set_tile": (i, [vp, i, i]),
     "ppl_dedisp_set_shape": (i, [vp, i, i, i]),
+    "ppl_dedisp_set_kernel": (i, [vp, i, i, i, i]),
     "ppl_dedisp_autotune": (i, [vp, vp, vp, i, vp, vp]),
     "ppl_dedisp_delays": (i, [vp, vp]),
     "ppl_dedisperse": (i, [vp, vp, vp, vp]),
-    "ppl_dedisperse_batch": (i, [vp, i, i, i, i, vp]),
-    "ppl_dedisp_autotune_batch": (i, [vp, i, i, vp, vp, vp, vp, vp]),
+    "ppl_dedisperse_batch": (i, [vp, i, i, i, i, i, vp]),
+    "ppl_dedisp_autotune_batch": (i, [vp, i, i, vp, vp, vp, vp, vp, vp]),
     "ppl_cache_dedisperse_loop": (i, [vp, C.c_long, vp, f, f, i, i, vp, i, i, vp]),
     "ppl_convert_f16": (i, [vp, vp, sz, vp]),
     "ppl_quantise_u8": (i, [vp, vp, sz, f, vp]),
@@ -192,7 +193,7 @@ class DedispPlan:
         check(lib().ppl_dedisp_plan_info(h, C.byref(info)))
         self.maxshift, self.rem, self.nout = info.maxshift, info.remainingSamples, info.nOut
         self.mstartdm, self.mdmstep = info.mstartdm, info.mdmstep
-        self.tile = (info.tileDM, info.tileT, info.chanBlock)
+        self.tile = (info.tileDM, info.tileT, info.chanBlock, info.readBytes)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -202,10 +203,12 @@ class DedispPlan:
     def _refresh(self):
         info = DedispInfo()
         check(lib().ppl_dedisp_plan_info(self.h, C.byref(info)))
-        self.tile = (info.tileDM, info.tileT, info.chanBlock)
+        self.tile = (info.tileDM, info.tileT, info.chanBlock, info.readBytes)
 
-    def set_tile(self, tile_dm, tile_t, chan_block=0):
-        check(lib().ppl_dedisp_set_shape(self.h, tile_dm, tile_t, chan_block))
+    def set_tile(self, tile_dm, tile_t, chan_block=0, read_bytes=4):
+        """Tile (DM trials x samples), channels staged per block (0: as many
+        as fit) and LDS read width per lane (4, or 16 for FP32 input)."""
+        check(lib().ppl_dedisp_set_kernel(self.h, tile_dm, tile_t, chan_block, read_bytes))
         self._refresh()
 
     def autotune(self, d_in, d_out, reps=2, stream=None):
@@ -250,7 +253,17 @@ def _dedisp_jobs(plans, d_ins, d_outs):
     return arr, n
 
 
-def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0), stream=None):
+def _tile4(tile):
+    """(tileDM, tileT, chanBlock[, readBytes]) with the read width filled in (4 bytes)."""
+    t = tuple(int(v) for v in tile)
+    if len(t) == 3:
+        t = t + (4,)
+    if len(t) != 4:
+        raise PPLError("tile must be (tileDM, tileT, chanBlock[, readBytes])")
+    return t
+
+
+def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0, 4), stream=None):
     """One launch dedispersing every (plan, input) pair under one tile; outputs
     as separate DedispPlan.run calls would give them.  tile="own": one launch
     per group of plans that share their own (autotuned) tile."""
@@ -260,7 +273,7 @@ def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0), stream=None):
                   for p, x in zip(plans, d_ins)]
     if isinstance(tile, str):
         if tile != "own":
-            raise PPLError("tile must be a (tileDM, tileT, chanBlock) tuple or 'own'")
+            raise PPLError("tile must be a (tileDM, tileT, chanBlock[, readBytes]) tuple or 'own'")
         groups = {}
         for k, p in enumerate(plans):
             groups.setdefault(tuple(p.tile), []).append(k)
@@ -269,13 +282,14 @@ def dedisperse_batch(plans, d_ins, d_outs=None, tile=(0, 0, 0), stream=None):
                              stream=stream)
         return d_outs
     arr, n = _dedisp_jobs(plans, d_ins, d_outs)
-    check(lib().ppl_dedisperse_batch(arr, n, tile[0], tile[1], tile[2], _stream(stream)))
+    t = _tile4(tile)
+    check(lib().ppl_dedisperse_batch(arr, n, t[0], t[1], t[2], t[3], _stream(stream)))
     return d_outs
 
 
 def autotune_batch(plans, d_ins, d_outs, reps=1, stream=None, candidates=None):
-    """Fastest (tileDM, tileT, chanBlock) for the batch, and its time in ms:
-    every instantiated tile and channel block, or only the given candidates
+    """Fastest (tileDM, tileT, chanBlock, readBytes) for the batch, and its time
+    in ms: every instantiated tile, channel block and read width, or only the given candidates
     (each timed over reps launches after one untimed launch)."""
     if candidates is not None:
         import torch
@@ -293,15 +307,15 @@ def autotune_batch(plans, d_ins, d_outs, reps=1, stream=None, candidates=None):
             e1.synchronize()
             ms = e0.elapsed_time(e1) / reps
             if best is None or ms < best[1]:
-                best = (tuple(cand), ms)
+                best = (_tile4(cand), ms)
         if best is None:
             raise PPLError("autotune: no candidate tile fits every job")
         return best
     arr, n = _dedisp_jobs(plans, d_ins, d_outs)
-    td, tt, cb, best = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+    td, tt, cb, rb, best = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_float()
     check(lib().ppl_dedisp_autotune_batch(arr, n, reps, _stream(stream), C.byref(td), C.byref(tt),
-                                          C.byref(cb), C.byref(best)))
-    return (td.value, tt.value, cb.value), best.value
+                                          C.byref(cb), C.byref(rb), C.byref(best)))
+    return (td.value, tt.value, cb.value, rb.value), best.value
 
 
 def cache_dedisperse_loop(outbuff, buff, mstartdm, mdmstep, tdms, num_samples, dm_shift, maxshift,
@@ -851,7 +865,7 @@ class MultiBeam:
             while plan.nout % m:
                 m -= 1
             self.chunk_spectra.append(m)
-        self.tile = tuple(tile) if tile else (0, 0, 0)
+        self.tile = _tile4(tile) if tile else (0, 0, 0, 4)
         self.graphs = None
         self.parity = 0
 
@@ -869,7 +883,7 @@ class MultiBeam:
 
     def autotune(self, reps=1, candidates=None, per_beam=False):
         """Picks the tile on the beams' own buffers (their contents do not
-        change the time); candidates: (tileDM, tileT, chanBlock) tuples to try
+        change the time); candidates: (tileDM, tileT, chanBlock[, readBytes]) tuples to try
         instead of every instantiated shape.  One tile for the whole batch,
         or with per_beam each beam's own fastest tile (one launch per group
         of beams sharing a tile) when that is faster on the whole batch.

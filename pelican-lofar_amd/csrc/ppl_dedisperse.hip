@@ -31,6 +31,7 @@ struct ppl_dedisp_plan {
     int tdmsPad = 0;              // tdms rounded up to 64, the widest DM tile
     int nd = 8, nt = 8;           // per-thread DM x sample register tile
     int cb = 0;                   // channels per staged block (0: as many as fit)
+    int rw = 4;                   // LDS read width per lane, bytes (4, or 16 for FP32 input)
     int spanTD[4] = {0, 0, 0, 0}; // max delay spread inside a DM tile of 8, 16, 32, 64 trials
 };
 
@@ -191,6 +192,212 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
             for (int i = 0; i < NT; ++i) {
                 const int t = t0 + lane + 64 * i;
                 if (t < nOut) out[(size_t)dm * nOut + t] = acc[j][i];
+            }
+        }
+    }
+}
+
+// FP32 input, 16-byte LDS reads.  gfx950 serves ds_read_b128 at about twice
+// the bytes per clock of ds_read_b32 (228 against 114 B/clk/CU measured), so
+// lane l accumulates four consecutive samples t0 + 4l + 256q + {0,1,2,3}
+// (q < NQ) from one read.  A read must be 16-byte aligned while the delays
+// are arbitrary, so each staged channel row is held in four copies shifted
+// by 0..3 elements: the read for a delay offset o is the aligned float4 at
+// element o - (o & 3) of copy (o & 3).  The sums are still taken in channel
+// order, one FP32 add per sample, so the result is bit-identical to the
+// one-sample kernel above.
+// Only copy 0 comes from global memory (LDS DMA); copies 1..3 are built from
+// it in LDS with aligned 16-byte reads and writes.  Loading all four by DMA
+// moves four times the bytes through the load path, which then bounds the
+// kernel (about 9 TB/s of LDS DMA over the chip).
+// The two staging buffers are separate __shared__ arrays: the compiler then
+// knows that reads of the current buffer do not alias the LDS DMA into the
+// other one, and does not hold them behind it (with one dynamic LDS array it
+// waits for every DMA in flight before the first read of each block).
+// Buffer layout: rows[CB][4 copies][W4 float4] then offs[CB][TD] (byte
+// offset of each (channel, DM) read base inside the buffer).  W4 is a
+// multiple of 16; copies 1..3 hold W4 - 1 valid float4 (their last one would
+// need the element after copy 0).
+constexpr int kX4BufBytes = 26 * 1024;  // two buffers: 52 KB, three workgroups per CU
+
+template <int ND, int NQ>
+__global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
+{
+    __shared__ __align__(16) unsigned char bufA[kX4BufBytes];
+    __shared__ __align__(16) unsigned char bufB[kX4BufBytes];
+    int local;
+    const DDJob& J = find_job(B, local);
+    const float* __restrict__ in = static_cast<const float*>(J.in);
+    float* __restrict__ out = static_cast<float*>(J.out);
+    const int32_t* __restrict__ delayT = J.delayT;
+    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, tdmsPad = J.tdmsPad, nOut = J.nOut;
+    const int W4 = J.rowUnits, CB = J.CB, nDMTiles = J.nDMTiles;
+    constexpr int TT = 256 * NQ;
+    constexpr int TD = 4 * ND;
+    const int copyBytes = W4 * 16;
+    const int rowBytes = 4 * copyBytes;  // one channel, four copies
+    const int nChunks = W4 / 16;         // 64-dword DMA rows in copy 0
+
+    int tTile, dmTile;
+    tile_of(local, nDMTiles, tTile, dmTile);
+    const int t0 = tTile * TT;
+    if (t0 >= nOut) return;  // padding of the XCD-major grid
+    const int dm0 = dmTile * TD;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t totalElems = (size_t)nChan * nsamp;
+
+    float acc[ND][NQ][4];
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[j][q][k] = 0.0f;
+
+    const int nBlocks = (nChan + CB - 1) / CB;
+    auto count = [&](int blk) { return min(CB, nChan - blk * CB); };
+    // Delays of the next block to stage, loaded one block ahead: they are
+    // issued right after a block's DMA and land with it, so a block start
+    // never waits on a delay-table round trip.  Addresses are clamped, so
+    // the loads need no branch.  Lane l of dminLane holds the first trial's
+    // delay of channel l of the block; dj[i] the delay of trial dm0 + lane
+    // of channel wave + 4i (CB <= 16: at most four channels per wave).
+    int dminLane = 0;
+    int dj[4] = {0, 0, 0, 0};
+    auto load_delays = [&](int blk) {
+        const int cb0 = blk * CB;
+        const int cl = min(cb0 + min(lane, CB - 1), nChan - 1);
+        dminLane = delayT[(size_t)cl * tdmsPad + dm0];
+        const int dml = dm0 + min(lane, TD - 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = min(cb0 + min(wave + 4 * i, CB - 1), nChan - 1);
+            dj[i] = delayT[(size_t)c * tdmsPad + dml];
+        }
+    };
+    // copy 0 of block blk's channels into buf by LDS DMA (its delays loaded
+    // and landed): wave w loads the 64-dword rows w, w+4, ... of every
+    // channel and writes the read offsets of channels w, w+4, ...
+    auto stage = [&](unsigned char* buf, int blk) {
+        const int cb0 = blk * CB, nc = count(blk);
+        int* offs = reinterpret_cast<int*>(buf + CB * rowBytes);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc < nc) {
+                const int o = dj[i] - __builtin_amdgcn_readlane(dminLane, cc);
+                if (lane < TD) offs[cc * TD + lane] = cc * rowBytes + (o & 3) * copyBytes + (o >> 2) * 16;
+            }
+        }
+        for (int cc = 0; cc < nc; ++cc) {
+            const int dmin = __builtin_amdgcn_readlane(dminLane, cc);
+            const size_t e0 = (size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin;
+            unsigned char* dst = buf + cc * rowBytes;
+            if (e0 + (size_t)(4 * W4) <= totalElems) {
+                const float* src = in + e0 + lane;
+                for (int k = wave; k < nChunks; k += 4)
+                    __builtin_amdgcn_global_load_lds(src + 64 * k, dst + 256 * k, 4, 0, 0);
+            } else {  // the end of the input: words past it are not loaded
+                for (int k = wave; k < nChunks; k += 4)
+                    if (e0 + 64 * k + lane < totalElems)
+                        __builtin_amdgcn_global_load_lds(in + e0 + 64 * k + lane, dst + 256 * k, 4, 0, 0);
+            }
+        }
+    };
+    // copies 1..3 from copy 0: copy s float4 m = elements 4m+s .. 4m+s+3
+    auto shift_copies = [&](unsigned char* buf, int nc) {
+        for (int cc = 0; cc < nc; ++cc) {
+            const float4* c0 = reinterpret_cast<const float4*>(buf + cc * rowBytes);
+            float4* c1 = reinterpret_cast<float4*>(buf + cc * rowBytes + copyBytes);
+            float4* c2 = reinterpret_cast<float4*>(buf + cc * rowBytes + 2 * copyBytes);
+            float4* c3 = reinterpret_cast<float4*>(buf + cc * rowBytes + 3 * copyBytes);
+            for (int m = tid; m < W4 - 1; m += 256) {
+                const float4 a = c0[m], b = c0[m + 1];
+                // a.x and b.w are not needed; without this empty use the
+                // compiler narrows the two 16-byte reads to six dword reads at
+                // a 16-byte lane stride, a four-way LDS bank conflict
+                asm volatile("" ::"v"(a.x), "v"(b.w));
+                c1[m] = make_float4(a.y, a.z, a.w, b.x);
+                c2[m] = make_float4(a.z, a.w, b.x, b.y);
+                c3[m] = make_float4(a.w, b.x, b.y, b.z);
+            }
+        }
+    };
+    auto compute = [&](const unsigned char* buf, int nc) {
+        const int* offs = reinterpret_cast<const int*>(buf + CB * rowBytes) + wave * ND;
+        for (int cc = 0; cc < nc; ++cc) {
+            int o[ND];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) o[j] = offs[cc * TD + j];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                const float4* p = reinterpret_cast<const float4*>(buf + o[j]) + lane;
+                float4 v[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) v[q] = p[64 * q];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    acc[j][q][0] = acc[j][q][0] + v[q].x;
+                    acc[j][q][1] = acc[j][q][1] + v[q].y;
+                    acc[j][q][2] = acc[j][q][2] + v[q].z;
+                    acc[j][q][3] = acc[j][q][3] + v[q].w;
+                }
+            }
+        }
+    };
+
+    load_delays(0);
+    __builtin_amdgcn_s_waitcnt(0);
+    stage(bufA, 0);
+    if (nBlocks > 1) load_delays(1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    shift_copies(bufA, count(0));
+    __syncthreads();
+    for (int blk = 0; blk < nBlocks; blk += 2) {
+        const bool more1 = blk + 1 < nBlocks;
+        if (more1) {
+            stage(bufB, blk + 1);
+            if (blk + 2 < nBlocks) load_delays(blk + 2);
+        }
+        compute(bufA, count(blk));
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (!more1) break;
+        shift_copies(bufB, count(blk + 1));
+        __syncthreads();
+        const bool more2 = blk + 2 < nBlocks;
+        if (more2) {
+            stage(bufA, blk + 2);
+            if (blk + 3 < nBlocks) load_delays(blk + 3);
+        }
+        compute(bufB, count(blk + 1));
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (!more2) break;
+        shift_copies(bufA, count(blk + 2));
+        __syncthreads();
+    }
+
+    const int dmw = dm0 + wave * ND;
+    const bool vec = (nOut & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        const int dm = dmw + j;
+        if (dm < tdms) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int t = t0 + 4 * lane + 256 * q;
+                float* o = out + (size_t)dm * nOut + t;
+                if (vec && t + 3 < nOut) {
+                    *reinterpret_cast<float4*>(o) = make_float4(acc[j][q][0], acc[j][q][1], acc[j][q][2], acc[j][q][3]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (t + k < nOut) o[k] = acc[j][q][k];
+                }
             }
         }
     }
@@ -521,6 +728,7 @@ __global__ void build_delays_kernel(const float* __restrict__ shift, int nChan, 
 struct Shape {
     int nd, nt;  // DM trials per wave, 64-sample groups per lane
     int cb;      // channels per staged block (0: as many as fit)
+    int rw = 4;  // LDS read width per lane: 4 bytes, or 16 (FP32 input, four samples)
 };
 
 constexpr int kMaxLDS = 64 * 1024;
@@ -562,16 +770,30 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
     const int TT = 64 * s.nt, TD = 4 * s.nd;
     const int span = span_of(p, TD);
     int rowUnits = 0, perChannel = 0;
-    switch (p->cfg.dtype) {
-        case PPL_F32: lds_geometry<float>(s, span, rowUnits, perChannel); break;
-        case PPL_F16: lds_geometry<_Float16>(s, span, rowUnits, perChannel); break;
-        default: lds_geometry<uint8_t>(s, span, rowUnits, perChannel); break;
+    int budget = kMaxLDS;
+    if (s.rw == 16) {
+        if (p->cfg.dtype != PPL_F32 || s.nt % 4 != 0) {
+            set_error("dedisperse: 16-byte reads need FP32 input and a tile of 256k samples");
+            return PPL_E_ARG;
+        }
+        // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1,
+        // and copies 1..3 hold W4 - 1 float4; a multiple of 16, so that copy
+        // 0 is a whole number of 64-dword DMA rows
+        rowUnits = (span / 4 + TT / 4 + 2 + 15) & ~15;
+        perChannel = 4 * rowUnits * 16 + TD * 4;  // one buffer; the kernel holds two
+        budget = kX4BufBytes;
+    } else {
+        switch (p->cfg.dtype) {
+            case PPL_F32: lds_geometry<float>(s, span, rowUnits, perChannel); break;
+            case PPL_F16: lds_geometry<_Float16>(s, span, rowUnits, perChannel); break;
+            default: lds_geometry<uint8_t>(s, span, rowUnits, perChannel); break;
+        }
     }
-    if (perChannel > kMaxLDS) {
+    if (perChannel > budget) {
         set_error("dedisperse: delay spread %d too large for tile %dx%d", span, TD, TT);
         return PPL_E_CONFIG;
     }
-    int CB = std::min(16, kMaxLDS / perChannel);
+    int CB = std::min(16, budget / perChannel);
     if (s.cb > 0) CB = std::min(CB, s.cb);
     CB = std::max(CB, 1);
     J.in = in;
@@ -587,7 +809,7 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
     J.nDMTiles = (J.tdms + TD - 1) / TD;
     J.blockStart = 0;
     J.pad_ = 0;
-    *ldsBytes = (size_t)CB * perChannel;
+    *ldsBytes = s.rw == 16 ? 0 : (size_t)CB * perChannel;  // the 16-byte kernel's LDS is static
     return PPL_OK;
 }
 
@@ -632,6 +854,24 @@ int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, 
         PPL_DD8_CASE(16, 4)
 #undef PPL_DD8_CASE
     } else {
+        if (s.rw == 16) {
+#define PPL_DDX4_CASE(ND_, NT_)                                                                  \
+    if (s.nd == ND_ && s.nt == NT_) {                                                            \
+        hipLaunchKernelGGL((dedisperse_f32x4_kernel<ND_, NT_ / 4>), grid, block, 0, stream, B); \
+        PPL_LAUNCHED();                                                                          \
+        return PPL_OK;                                                                           \
+    }
+            PPL_DDX4_CASE(4, 8)
+            PPL_DDX4_CASE(8, 8)
+            PPL_DDX4_CASE(16, 8)
+            PPL_DDX4_CASE(4, 16)
+            PPL_DDX4_CASE(8, 16)
+            PPL_DDX4_CASE(8, 4)
+            PPL_DDX4_CASE(16, 4)
+#undef PPL_DDX4_CASE
+            set_error("dedisperse: unsupported 16-byte tile %dx%d", 4 * s.nd, 64 * s.nt);
+            return PPL_E_ARG;
+        }
 #define PPL_DD_CASE(ND_, NT_)                                                                       \
     if (s.nd == ND_ && s.nt == NT_) {                                                               \
         hipLaunchKernelGGL((dedisperse_kernel<Tin, ND_, NT_>), grid, block, lds, stream, B);       \
@@ -691,14 +931,19 @@ int run_jobs(const ppl_dedisp_job* jobs, int n, const Shape& s, hipStream_t st)
     return PPL_E_ARG;
 }
 
-const Shape kShapes[] = {{8, 8, 0}, {8, 4, 0}, {4, 8, 0}, {4, 4, 0}, {2, 8, 0}, {16, 4, 0}, {16, 2, 0}};
+const Shape kShapes[] = {{8, 8, 0}, {8, 4, 0}, {4, 8, 0}, {4, 4, 0}, {2, 8, 0}, {16, 4, 0}, {16, 2, 0},
+                         {4, 8, 0, 16}, {8, 8, 0, 16}, {16, 8, 0, 16}, {4, 16, 0, 16}, {8, 16, 0, 16},
+                         {8, 4, 0, 16}, {16, 4, 0, 16}};
 
-bool shape_known(int nd, int nt)
+bool shape_known(int nd, int nt, int rw)
 {
     for (const Shape& s : kShapes)
-        if (s.nd == nd && s.nt == nt) return true;
+        if (s.nd == nd && s.nt == nt && s.rw == rw) return true;
     return false;
 }
+
+// The shapes the autotuner tries for a dtype: 16-byte reads only for FP32.
+bool shape_for(const Shape& s, int dtype) { return s.rw == 4 || dtype == PPL_F32; }
 
 // Largest delay spread inside any DM tile of TD trials.
 int tile_span(const std::vector<int32_t>& d, int tdms, int nChan, int TD)
@@ -824,26 +1069,46 @@ extern "C" int ppl_dedisp_plan_info(const ppl_dedisp_plan* p, ppl_dedisp_info* i
     info->tileDM = 4 * p->nd;
     info->tileT = 64 * p->nt;
     info->chanBlock = p->cb;
+    info->readBytes = p->rw;
     return PPL_OK;
 }
 
 extern "C" int ppl_dedisp_set_tile(ppl_dedisp_plan* p, int tileDM, int tileT)
 {
-    return ppl_dedisp_set_shape(p, tileDM, tileT, 0);
+    return ppl_dedisp_set_kernel(p, tileDM, tileT, 0, 4);
 }
 
 extern "C" int ppl_dedisp_set_shape(ppl_dedisp_plan* p, int tileDM, int tileT, int chanBlock)
 {
+    return ppl_dedisp_set_kernel(p, tileDM, tileT, chanBlock, 4);
+}
+
+namespace {
+// Checks an explicit (tileDM, tileT, chanBlock, readBytes) for a dtype.
+int parse_shape(int tileDM, int tileT, int chanBlock, int readBytes, int dtype, Shape& s)
+{
+    PPL_CHECK_ARG(tileDM % 4 == 0 && tileT % 64 == 0, "tile %dx%d not a multiple of 4x64", tileDM, tileT);
+    PPL_CHECK_ARG(chanBlock >= 0 && chanBlock <= 16, "channel block %d outside [0, 16]", chanBlock);
+    PPL_CHECK_ARG(readBytes == 4 || readBytes == 16, "read width %d bytes (4 or 16)", readBytes);
+    PPL_CHECK_ARG(readBytes == 4 || dtype == PPL_F32, "16-byte reads need FP32 input");
+    PPL_CHECK_ARG(shape_known(tileDM / 4, tileT / 64, readBytes), "tile %dx%d with %d-byte reads not instantiated",
+                  tileDM, tileT, readBytes);
+    s = Shape{tileDM / 4, tileT / 64, chanBlock, readBytes};
+    return PPL_OK;
+}
+}  // namespace
+
+extern "C" int ppl_dedisp_set_kernel(ppl_dedisp_plan* p, int tileDM, int tileT, int chanBlock, int readBytes)
+{
     PPL_CHECK_ARG(p, "null plan");
     if (tileDM == 0 && tileT == 0) return PPL_OK;
-    PPL_CHECK_ARG(tileDM % 4 == 0 && tileT % 64 == 0, "tile %dx%d not a multiple of 4x64", tileDM,
-                  tileT);
-    PPL_CHECK_ARG(chanBlock >= 0 && chanBlock <= 16, "channel block %d outside [0, 16]", chanBlock);
-    const int nd = tileDM / 4, nt = tileT / 64;
-    PPL_CHECK_ARG(shape_known(nd, nt), "tile %dx%d not instantiated", tileDM, tileT);
-    p->nd = nd;
-    p->nt = nt;
-    p->cb = chanBlock;
+    Shape s;
+    const int rc = parse_shape(tileDM, tileT, chanBlock, readBytes, p->cfg.dtype, s);
+    if (rc != PPL_OK) return rc;
+    p->nd = s.nd;
+    p->nt = s.nt;
+    p->cb = s.cb;
+    p->rw = s.rw;
     return PPL_OK;
 }
 
@@ -856,19 +1121,21 @@ extern "C" int ppl_dedisp_autotune(ppl_dedisp_plan* p, const void* d_in, void* d
     hipEvent_t e0, e1;
     PPL_HIP(hipEventCreate(&e0));
     PPL_HIP(hipEventCreate(&e1));
-    const int cbs[] = {2, 4, 8, 16};
+    const int cbs[] = {1, 2, 4, 8, 16};
     float best = 1e30f;
-    Shape bestShape{p->nd, p->nt, p->cb};
+    Shape bestShape{p->nd, p->nt, p->cb, p->rw};
     for (const Shape& base : kShapes) {
+        if (!shape_for(base, p->cfg.dtype)) continue;
         for (int cb : cbs) {
-            if (ppl_dedisp_set_shape(p, 4 * base.nd, 64 * base.nt, cb) != PPL_OK) continue;
+            const Shape s{base.nd, base.nt, cb, base.rw};
             DDJob J{};
             size_t lds = 0;
-            if (make_job(p, Shape{p->nd, p->nt, cb}, d_in, d_out, J, &lds) != PPL_OK || J.CB < cb) continue;
-            if (ppl_dedisperse(p, d_in, d_out, stream) != PPL_OK) continue;  // warm
+            if (make_job(p, s, d_in, d_out, J, &lds) != PPL_OK || J.CB < cb) continue;
+            const ppl_dedisp_job job{p, d_in, d_out};
+            if (run_jobs(&job, 1, s, st) != PPL_OK) continue;  // warm
             PPL_HIP(hipEventRecord(e0, st));
             for (int r = 0; r < reps; ++r)
-                if (ppl_dedisperse(p, d_in, d_out, stream) != PPL_OK) break;
+                if (run_jobs(&job, 1, s, st) != PPL_OK) break;
             PPL_HIP(hipEventRecord(e1, st));
             PPL_HIP(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -876,14 +1143,14 @@ extern "C" int ppl_dedisp_autotune(ppl_dedisp_plan* p, const void* d_in, void* d
             ms /= reps;
             if (ms < best) {
                 best = ms;
-                bestShape = Shape{p->nd, p->nt, cb};
+                bestShape = s;
             }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (best_ms) *best_ms = best;
-    return ppl_dedisp_set_shape(p, 4 * bestShape.nd, 64 * bestShape.nt, bestShape.cb);
+    return ppl_dedisp_set_kernel(p, 4 * bestShape.nd, 64 * bestShape.nt, bestShape.cb, bestShape.rw);
 }
 
 extern "C" int ppl_dedisp_delays(const ppl_dedisp_plan* p, int32_t* host_out)
@@ -897,39 +1164,39 @@ extern "C" int ppl_dedisperse(const ppl_dedisp_plan* p, const void* d_in, void* 
 {
     PPL_CHECK_ARG(p && d_in && d_out, "null argument");
     const ppl_dedisp_job job{p, d_in, d_out};
-    return run_jobs(&job, 1, Shape{p->nd, p->nt, p->cb}, stream_of(stream));
+    return run_jobs(&job, 1, Shape{p->nd, p->nt, p->cb, p->rw}, stream_of(stream));
 }
 
 extern "C" int ppl_dedisperse_batch(const ppl_dedisp_job* jobs, int n, int tileDM, int tileT, int chanBlock,
-                                    void* stream)
+                                    int readBytes, void* stream)
 {
     PPL_CHECK_ARG(jobs && n >= 1, "no jobs");
     PPL_CHECK_ARG(jobs[0].plan, "job 0: null plan");
-    Shape s{jobs[0].plan->nd, jobs[0].plan->nt, jobs[0].plan->cb};
+    Shape s{jobs[0].plan->nd, jobs[0].plan->nt, jobs[0].plan->cb, jobs[0].plan->rw};
     if (tileDM != 0 || tileT != 0) {
-        PPL_CHECK_ARG(tileDM % 4 == 0 && tileT % 64 == 0 && shape_known(tileDM / 4, tileT / 64),
-                      "tile %dx%d not instantiated", tileDM, tileT);
-        PPL_CHECK_ARG(chanBlock >= 0 && chanBlock <= 16, "channel block %d outside [0, 16]", chanBlock);
-        s = Shape{tileDM / 4, tileT / 64, chanBlock};
+        const int rc = parse_shape(tileDM, tileT, chanBlock, readBytes, jobs[0].plan->cfg.dtype, s);
+        if (rc != PPL_OK) return rc;
     }
     return run_jobs(jobs, n, s, stream_of(stream));
 }
 
 extern "C" int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int reps, void* stream,
-                                         int* tileDM, int* tileT, int* chanBlock, float* best_ms)
+                                         int* tileDM, int* tileT, int* chanBlock, int* readBytes, float* best_ms)
 {
-    PPL_CHECK_ARG(jobs && n >= 1 && tileDM && tileT && chanBlock, "null argument");
+    PPL_CHECK_ARG(jobs && n >= 1 && tileDM && tileT && chanBlock && readBytes, "null argument");
+    PPL_CHECK_ARG(jobs[0].plan, "job 0: null plan");
     if (reps < 1) reps = 1;
     hipStream_t st = stream_of(stream);
     hipEvent_t e0, e1;
     PPL_HIP(hipEventCreate(&e0));
     PPL_HIP(hipEventCreate(&e1));
-    const int cbs[] = {2, 4, 8, 16};
+    const int cbs[] = {1, 2, 4, 8, 16};
     float best = 1e30f;
     Shape bestShape{0, 0, 0};
     for (const Shape& base : kShapes) {
+        if (!shape_for(base, jobs[0].plan->cfg.dtype)) continue;
         for (int cb : cbs) {
-            const Shape s{base.nd, base.nt, cb};
+            const Shape s{base.nd, base.nt, cb, base.rw};
             bool fits = true;
             for (int k = 0; k < n && fits; ++k) {
                 DDJob J{};
@@ -961,6 +1228,7 @@ extern "C" int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int 
     *tileDM = 4 * bestShape.nd;
     *tileT = 64 * bestShape.nt;
     *chanBlock = bestShape.cb;
+    *readBytes = bestShape.rw;
     if (best_ms) *best_ms = best;
     return PPL_OK;
 }

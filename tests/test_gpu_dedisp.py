@@ -25,7 +25,9 @@ def test_delay_table_matches_oracle(orc, ppl):
     assert np.array_equal(plan.delays(), d)
 
 
-@pytest.mark.parametrize("tile", [None, (32, 256), (16, 512), (8, 512), (16, 256), (64, 256)])
+@pytest.mark.parametrize("tile", [None, (32, 256), (16, 512), (8, 512), (16, 256), (64, 256),
+                                  (16, 512, 0, 16), (32, 512, 0, 16), (64, 512, 0, 16), (16, 1024, 0, 16),
+                                  (32, 1024, 1, 16), (32, 256, 0, 16), (64, 256, 2, 16)])
 def test_f32_bit_exact_small(orc, ppl, tile):
     rng = np.random.default_rng(11)
     x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
@@ -39,15 +41,21 @@ def test_f32_bit_exact_small(orc, ppl, tile):
     assert np.array_equal(out, exp)
 
 
+@pytest.mark.parametrize("reads", [4, 16])
 @pytest.mark.parametrize("shape", [(37, 2048, 100, 0.7), (300, 8192, 200, 0.1), (1, 1024, 3, 2.0),
                                    (129, 4096, 41, 0.3)])
-def test_f32_bit_exact_ragged(orc, ppl, shape):
+def test_f32_bit_exact_ragged(orc, ppl, shape, reads):
+    """Ragged shapes with one-sample (4-byte) and four-sample (16-byte) LDS
+    reads; the last channel's staged rows run past the end of the input."""
     n_chan, ns, tdms, step = shape
     cfg = dict(n_chan=n_chan, nsamp_buf=ns, tdms=tdms, dm_low=0.0, dm_step=step,
                tsamp=0.00032768, fch1=150.0, foff=-6.0 / 2048)
     rng = np.random.default_rng(n_chan)
     x = (rng.normal(size=(n_chan, ns)) * 3 + 1).astype(np.float32)
     plan = ppl.DedispPlan(**cfg)
+    if reads == 16:
+        plan.set_tile(32, 512, 0, 16)
+        assert plan.tile == (32, 512, 0, 16)
     out = plan.run(torch.from_numpy(x).cuda()).cpu().numpy()
     p, d = _oracle(orc, x, cfg)
     assert np.array_equal(out, orc.dedisperse(x, d, p["nout"]))
@@ -142,10 +150,12 @@ def test_linearity_full_size(ppl):
 
 @pytest.mark.parametrize("dtype", ["f32", "f16", "u8"])
 @pytest.mark.parametrize("tile", [(32, 512, 2), (64, 256, 4), (16, 512, 8), (32, 256, 16), (8, 512, 0),
-                                  (64, 128, 3)])
+                                  (64, 128, 3), (16, 512, 3, 16), (32, 512, 1, 16), (64, 256, 0, 16),
+                                  (16, 1024, 2, 16), (32, 1024, 0, 16), (32, 256, 5, 16), (64, 256, 1, 16)])
 def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
-    """Every instantiated tile and channel block, odd channel counts and odd
-    delay parities (the FP16 form reads pairs from a shifted copy)."""
+    """Every instantiated tile, channel block and read width, odd channel
+    counts and odd delay parities (the FP16 form reads pairs from a shifted
+    copy, the 16-byte FP32 form from four copies shifted by 0..3)."""
     cfg = dict(n_chan=53, nsamp_buf=3000, tdms=77, dm_low=0.0, dm_step=0.9, tsamp=0.00032768,
                fch1=150.0, foff=-0.05)
     rng = np.random.default_rng(hash(tile) % 1000)
@@ -160,6 +170,10 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
         host = orc.quantise_u8(x, np.float32(12.7))
         dev = torch.from_numpy(host).cuda()
     plan = ppl.DedispPlan(**cfg, dtype=dtype)
+    if len(tile) == 4 and dtype != "f32":
+        with pytest.raises(ppl.PPLError):  # 16-byte reads are built for FP32 input only
+            plan.set_tile(*tile)
+        return
     plan.set_tile(*tile)
     if dtype == "u8" and tile[1] % 256:
         with pytest.raises(ppl.PPLError):  # the u8 form needs 4 samples per lane
@@ -167,6 +181,17 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
         return
     out = plan.run(dev).cpu().numpy()
     assert np.array_equal(out, orc.dedisperse(host, d, p["nout"]))
+
+
+def test_wide_reads_reject_a_spread_too_wide(ppl):
+    """A DM tile whose delay spread needs a staged row longer than the
+    16-byte kernel's static LDS buffer is refused, not truncated."""
+    cfg = dict(n_chan=64, nsamp_buf=16384, tdms=64, dm_low=0.0, dm_step=4.0, tsamp=0.00032768,
+               fch1=150.0, foff=-0.05)
+    plan = ppl.DedispPlan(**cfg)
+    plan.set_tile(64, 512, 0, 16)
+    with pytest.raises(ppl.PPLError):
+        plan.run(torch.zeros((64, 16384), device="cuda"))
 
 
 def test_autotuner_keeps_results_exact(orc, ppl):
@@ -211,7 +236,13 @@ def test_batch_equals_separate_runs(ppl, dtype):
         plans.append(p)
         ins.append(x.cuda())
     single = [p.run(x) for p, x in zip(plans, ins)]
-    for tile in [(0, 0, 0), (16, 256, 4), (32, 512, 2)]:
+    tiles = [(0, 0, 0), (16, 256, 4), (32, 512, 2)]
+    if dtype == "f32":
+        tiles += [(32, 512, 2, 16), (64, 256, 0, 16)]
+    else:
+        with pytest.raises(ppl.PPLError):
+            ppl.dedisperse_batch(plans, ins, tile=(32, 512, 2, 16))
+    for tile in tiles:
         outs = ppl.dedisperse_batch(plans, ins, tile=tile)
         for k in range(len(plans)):
             assert torch.equal(outs[k], single[k]), (tile, k)
