@@ -133,25 +133,50 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 #pragma unroll
         for (int i = 0; i < NT; ++i) acc[j][i] = TA(0);
 
-    // stage channels [cb0, cb0+nc) into buffer `buf`: wave w takes channels w, w+4, ...
+    // stage channels [cb0, cb0+nc) into buffer `buf`: wave w takes channels
+    // w, w+4, ... (CB <= 16: at most four).  Loads complete in issue order,
+    // so the delays of all of a wave's channels are loaded and used before
+    // its first DMA: a delay load issued after a DMA would hold its user
+    // behind that DMA.  Rows are whole 64-dword DMA rows, so only a row that
+    // reaches the end of the input needs per-lane guards.
     auto stage = [&](int buf, int cb0, int nc) {
         Tin* rows = rows0 + (size_t)buf * CB * W;
         int* offs = offs0 + buf * CB * TD;
         const int rowDwords = rowBytes / 4;
-        for (int cc = wave; cc < nc; cc += 4) {
-            const int c = cb0 + cc;
+        int dmin[4], dj[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = min(cb0 + min(wave + 4 * i, CB - 1), nChan - 1);
             const int32_t* dl = delayT + (size_t)c * tdmsPad + dm0;
-            const int dmin = dl[0];
-            const size_t byteStart = ((size_t)c * nsamp + (size_t)t0 + (size_t)dmin) * E;
+            dmin[i] = __builtin_amdgcn_readfirstlane(dl[0]);
+            dj[i] = dl[min(lane, TD - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc >= nc) break;
+            const size_t byteStart = ((size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin[i]) * E;
+            const int mis = (int)(byteStart & 3) / E;
+            if (lane < TD) offs[cc * TD + lane] = dj[i] - dmin[i] + mis + cc * W;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc >= nc) break;
+            const size_t byteStart = ((size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin[i]) * E;
             const size_t byte0 = byteStart & ~(size_t)3;
-            const int mis = (int)(byteStart - byte0) / E;
             unsigned char* dstRow = reinterpret_cast<unsigned char*>(rows + cc * W);
-            for (int k0 = 0; k0 < rowDwords; k0 += 64) {
-                const size_t g = byte0 + 4 * (size_t)(k0 + lane);
-                if (k0 + lane < rowDwords && g + 4 <= totalBytes)
-                    __builtin_amdgcn_global_load_lds(inBytes + g, dstRow + 4 * k0, 4, 0, 0);
+            if (byte0 + 4 * (size_t)rowDwords <= totalBytes) {
+                const unsigned char* g = inBytes + byte0 + 4 * lane;
+                for (int k0 = 0; k0 < rowDwords; k0 += 64)
+                    __builtin_amdgcn_global_load_lds(g + 4 * k0, dstRow + 4 * k0, 4, 0, 0);
+            } else {
+                for (int k0 = 0; k0 < rowDwords; k0 += 64) {
+                    const size_t g = byte0 + 4 * (size_t)(k0 + lane);
+                    if (k0 + lane < rowDwords && g + 4 <= totalBytes)
+                        __builtin_amdgcn_global_load_lds(inBytes + g, dstRow + 4 * k0, 4, 0, 0);
+                }
             }
-            if (lane < TD) offs[cc * TD + lane] = dl[lane] - dmin + mis + cc * W;
         }
     };
 
@@ -206,18 +231,15 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 // element o - (o & 3) of copy (o & 3).  The sums are still taken in channel
 // order, one FP32 add per sample, so the result is bit-identical to the
 // one-sample kernel above.
-// Only copy 0 comes from global memory (LDS DMA); copies 1..3 are built from
-// it in LDS with aligned 16-byte reads and writes.  Loading all four by DMA
-// moves four times the bytes through the load path, which then bounds the
-// kernel (about 9 TB/s of LDS DMA over the chip).
+// Wave s loads copy s of every staged channel by LDS DMA, from element s of
+// the row on; the four loads of a row hit the same cache lines.
 // The two staging buffers are separate __shared__ arrays: the compiler then
 // knows that reads of the current buffer do not alias the LDS DMA into the
 // other one, and does not hold them behind it (with one dynamic LDS array it
 // waits for every DMA in flight before the first read of each block).
 // Buffer layout: rows[CB][4 copies][W4 float4] then offs[CB][TD] (byte
 // offset of each (channel, DM) read base inside the buffer).  W4 is a
-// multiple of 16; copies 1..3 hold W4 - 1 valid float4 (their last one would
-// need the element after copy 0).
+// multiple of 16, so a copy is a whole number of 64-dword DMA rows.
 constexpr int kX4BufBytes = 26 * 1024;  // two buffers: 52 KB, three workgroups per CU
 
 template <int ND, int NQ>
@@ -236,7 +258,7 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
     constexpr int TD = 4 * ND;
     const int copyBytes = W4 * 16;
     const int rowBytes = 4 * copyBytes;  // one channel, four copies
-    const int nChunks = W4 / 16;         // 64-dword DMA rows in copy 0
+    const int nChunks = W4 / 16;         // 64-dword DMA rows per copy
 
     int tTile, dmTile;
     tile_of(local, nDMTiles, tTile, dmTile);
@@ -277,9 +299,9 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
             dj[i] = delayT[(size_t)c * tdmsPad + dml];
         }
     };
-    // copy 0 of block blk's channels into buf by LDS DMA (its delays loaded
-    // and landed): wave w loads the 64-dword rows w, w+4, ... of every
-    // channel and writes the read offsets of channels w, w+4, ...
+    // block blk's channels into buf by LDS DMA (its delays loaded and
+    // landed): wave w loads copy w of every channel and writes the read
+    // offsets of channels w, w+4, ...
     auto stage = [&](unsigned char* buf, int blk) {
         const int cb0 = blk * CB, nc = count(blk);
         int* offs = reinterpret_cast<int*>(buf + CB * rowBytes);
@@ -293,35 +315,17 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
         }
         for (int cc = 0; cc < nc; ++cc) {
             const int dmin = __builtin_amdgcn_readlane(dminLane, cc);
-            const size_t e0 = (size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin;
-            unsigned char* dst = buf + cc * rowBytes;
+            // wave w loads copy w: the row from element w on
+            const size_t e0 = (size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin + (size_t)wave;
+            unsigned char* dst = buf + cc * rowBytes + wave * copyBytes;
             if (e0 + (size_t)(4 * W4) <= totalElems) {
                 const float* src = in + e0 + lane;
-                for (int k = wave; k < nChunks; k += 4)
+                for (int k = 0; k < nChunks; ++k)
                     __builtin_amdgcn_global_load_lds(src + 64 * k, dst + 256 * k, 4, 0, 0);
             } else {  // the end of the input: words past it are not loaded
-                for (int k = wave; k < nChunks; k += 4)
+                for (int k = 0; k < nChunks; ++k)
                     if (e0 + 64 * k + lane < totalElems)
                         __builtin_amdgcn_global_load_lds(in + e0 + 64 * k + lane, dst + 256 * k, 4, 0, 0);
-            }
-        }
-    };
-    // copies 1..3 from copy 0: copy s float4 m = elements 4m+s .. 4m+s+3
-    auto shift_copies = [&](unsigned char* buf, int nc) {
-        for (int cc = 0; cc < nc; ++cc) {
-            const float4* c0 = reinterpret_cast<const float4*>(buf + cc * rowBytes);
-            float4* c1 = reinterpret_cast<float4*>(buf + cc * rowBytes + copyBytes);
-            float4* c2 = reinterpret_cast<float4*>(buf + cc * rowBytes + 2 * copyBytes);
-            float4* c3 = reinterpret_cast<float4*>(buf + cc * rowBytes + 3 * copyBytes);
-            for (int m = tid; m < W4 - 1; m += 256) {
-                const float4 a = c0[m], b = c0[m + 1];
-                // a.x and b.w are not needed; without this empty use the
-                // compiler narrows the two 16-byte reads to six dword reads at
-                // a 16-byte lane stride, a four-way LDS bank conflict
-                asm volatile("" ::"v"(a.x), "v"(b.w));
-                c1[m] = make_float4(a.y, a.z, a.w, b.x);
-                c2[m] = make_float4(a.z, a.w, b.x, b.y);
-                c3[m] = make_float4(a.w, b.x, b.y, b.z);
             }
         }
     };
@@ -354,8 +358,6 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
     if (nBlocks > 1) load_delays(1);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    shift_copies(bufA, count(0));
-    __syncthreads();
     for (int blk = 0; blk < nBlocks; blk += 2) {
         const bool more1 = blk + 1 < nBlocks;
         if (more1) {
@@ -366,8 +368,6 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (!more1) break;
-        shift_copies(bufB, count(blk + 1));
-        __syncthreads();
         const bool more2 = blk + 2 < nBlocks;
         if (more2) {
             stage(bufA, blk + 2);
@@ -377,8 +377,6 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (!more2) break;
-        shift_copies(bufA, count(blk + 2));
-        __syncthreads();
     }
 
     const int dmw = dm0 + wave * ND;
@@ -748,9 +746,10 @@ void lds_geometry(const Shape& s, int span, int& rowUnits, int& perChannel)
         rowUnits = (TT + span + 3 + 3) / 4 + 1;
         perChannel = 2 * 4 * (4 * rowUnits + TD);
     } else {
-        // rowUnits = elements, a whole number of dwords
+        // rowUnits = elements, a whole number of 64-dword DMA rows when an
+        // element is a dword
         int W = TT + span + 4;
-        W = (W + 15) & ~15;
+        W = sizeof(Tin) == 4 ? (W + 63) & ~63 : (W + 15) & ~15;
         rowUnits = W;
         perChannel = 2 * (W * (int)sizeof(Tin) + TD * 4);
     }
@@ -776,10 +775,9 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
             set_error("dedisperse: 16-byte reads need FP32 input and a tile of 256k samples");
             return PPL_E_ARG;
         }
-        // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1,
-        // and copies 1..3 hold W4 - 1 float4; a multiple of 16, so that copy
-        // 0 is a whole number of 64-dword DMA rows
-        rowUnits = (span / 4 + TT / 4 + 2 + 15) & ~15;
+        // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1;
+        // a multiple of 16, so that a copy is a whole number of 64-dword DMA rows
+        rowUnits = (span / 4 + TT / 4 + 1 + 15) & ~15;
         perChannel = 4 * rowUnits * 16 + TD * 4;  // one buffer; the kernel holds two
         budget = kX4BufBytes;
     } else {
