@@ -54,25 +54,50 @@ template <typename Tin> struct HistT;
 template <> struct HistT<uint32_t> { using type = uint32_t; };
 template <> struct HistT<float2> { using type = float2; };
 
-// G-point radix-2 decimation-in-frequency FFT across the lanes of a lane
-// group; lane l ends with X[bitrev(l)].
-template <int G>
-__device__ __forceinline__ void lane_fft(float& re, float& im, const float* twr, const float* twi)
+// Value of lane l ^ M.  M = 1, 2 and 8 are DPP moves (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_ror:8); the rest go through ds_bpermute.
+template <int M>
+__device__ __forceinline__ float xor_lane(float v)
 {
-    const int lane = threadIdx.x & 63;
-    int s = 0;
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1, ++s) {
-        const float pr = __shfl_xor(re, m, 64);
-        const float pi = __shfl_xor(im, m, 64);
-        if ((lane & m) == 0) {
-            re = re + pr;
-            im = im + pi;
+    if constexpr (M == 1)
+        return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
+    else if constexpr (M == 2)
+        return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
+    else if constexpr (M == 8)
+        return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x128, 0xF, 0xF, true));
+    else
+        return __shfl_xor(v, M, 64);
+}
+
+// Per-lane constants of the lane FFT, stage s (m = G/2 >> s): sgn = +1 on
+// the lane that keeps the sum (lane & m == 0), -1 on its partner; the
+// partner's twiddle W_{2m}^j, j = lane & (m-1), and (1, 0) on the sum lane.
+struct LaneTw {
+    float sgn[6], twr[6], twi[6];
+};
+
+// G-point radix-2 decimation-in-frequency FFT across the lanes of a lane
+// group; lane l ends with X[bitrev(l)].  Branch-free butterflies: both lanes
+// form pr + sgn*x in one rounding (the sum, or the partner's difference),
+// then multiply by their twiddle ((1, 0) on the sum lane, an exact no-op up
+// to the sign of a zero).  The last stage's twiddle is 1 on every lane.
+template <int G, int S = 0>
+__device__ __forceinline__ void lane_fft(float& re, float& im, const LaneTw& L)
+{
+    constexpr int m = G >> (S + 1);
+    if constexpr (m >= 1) {
+        const float pr = xor_lane<m>(re);
+        const float pi = xor_lane<m>(im);
+        const float dr = __builtin_fmaf(L.sgn[S], re, pr);
+        const float di = __builtin_fmaf(L.sgn[S], im, pi);
+        if constexpr (m == 1) {
+            re = dr;
+            im = di;
         } else {
-            const float dr = pr - re, di = pi - im;
-            re = dr * twr[s] - di * twi[s];
-            im = dr * twi[s] + di * twr[s];
+            re = dr * L.twr[S] - di * L.twi[S];
+            im = dr * L.twi[S] + di * L.twr[S];
         }
+        lane_fft<G, S + 1>(re, im, L);
     }
 }
 
@@ -142,15 +167,17 @@ __global__ __launch_bounds__(256) void ppf_kernel(
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int t = 0; t < TAPS; ++t) hc[r][t] = coeffs[t * N + l + G * r];
-    float twr[6], twi[6];
+    LaneTw ltw;
     {
         int s = 0;
 #pragma unroll
         for (int m = G / 2; m >= 1; m >>= 1, ++s) {
             const int j = l & (m - 1);
             const int k = j * (N / (2 * m));  // W_{2m}^j = W_N^{j N/(2m)}
-            twr[s] = tw[2 * k];
-            twi[s] = tw[2 * k + 1];
+            const bool sumLane = (l & m) == 0;
+            ltw.sgn[s] = sumLane ? 1.0f : -1.0f;
+            ltw.twr[s] = sumLane ? 1.0f : tw[2 * k];
+            ltw.twi[s] = sumLane ? 0.0f : tw[2 * k + 1];
         }
     }
     C2 rtw[R];
@@ -160,9 +187,13 @@ __global__ __launch_bounds__(256) void ppf_kernel(
         rtw[r] = {tw[2 * k], tw[2 * k + 1]};
     }
 
-    // Sliding windows w[slot][r]: slot 0 oldest .. TAPS-1 newest.  Before the
-    // first step slots 1..TAPS-1 hold the TAPS-1 blocks preceding the run
-    // (from this chunk or from the stream's saved history).
+    // The FIR window is a ring of TAPS blocks per polarisation: step s writes
+    // its new block into slot s % TAPS, and tap t (oldest first) is slot
+    // (s + 1 + t) % TAPS.  The step loop is unrolled by TAPS, so every slot
+    // index is a compile-time constant and the window never moves between
+    // registers.  Before the first step slots 1..TAPS-1 hold the TAPS-1
+    // blocks preceding the run (from this chunk or from the stream's saved
+    // history).
     C2 w0[TAPS][R], w1[TAPS][R];
     const long b0 = (long)run * runLen;
 #pragma unroll
@@ -195,82 +226,87 @@ __global__ __launch_bounds__(256) void ppf_kernel(
     const int lastRunLen = (int)(nBlk - (long)(nRuns - 1) * runLen);
     const int len = active ? (run == nRuns - 1 ? lastRunLen : runLen) : 0;
     const int kLane = bitrev<G>(l);
-    // every lane runs every step: the lane FFT shuffles need the whole wave
-    for (int s = 0; s < runLen; ++s) {
-        const bool live = s < len;
-        const long b = b0 + s;
+    int inWindow = 0;  // blocks summed into the current integration window
+    // output row of the current window: b0 is a multiple of integ (runLen is),
+    // so row (b0 + s) / integ is counted instead of divided
+    long uo = OUT == 0 ? b0 / integ : 0;
+    // every lane runs every step: the lane FFT shuffles need the whole wave.
+    // Steps past runLen (up to the next multiple of TAPS) are not live: they
+    // store nothing.
+    for (int s0 = 0; s0 < runLen; s0 += TAPS) {
 #pragma unroll
-        for (int t = 0; t < TAPS - 1; ++t)
+        for (int u = 0; u < TAPS; ++u) {
+            const int s = s0 + u;
+            const bool live = s < len;
+            const long b = b0 + s;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                w0[t][r] = w0[t + 1][r];
-                w1[t][r] = w1[t + 1][r];
+                const int n = l + G * r;
+                w0[u][r] = live ? load_sample(x0, b * N + n) : C2{0.f, 0.f};
+                w1[u][r] = live ? load_sample(x1, b * N + n) : C2{0.f, 0.f};
             }
+            // FIR, oldest block first, coefficient index t*N + n
+            C2 f0[R], f1[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int n = l + G * r;
-            w0[TAPS - 1][r] = live ? load_sample(x0, b * N + n) : C2{0.f, 0.f};
-            w1[TAPS - 1][r] = live ? load_sample(x1, b * N + n) : C2{0.f, 0.f};
-        }
-        // FIR, oldest block first, coefficient index t*N + n
-        C2 f0[R], f1[R];
+            for (int r = 0; r < R; ++r) {
+                f0[r] = f1[r] = {0.f, 0.f};
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            f0[r] = f1[r] = {0.f, 0.f};
-#pragma unroll
-            for (int t = 0; t < TAPS; ++t) {
-                const float coeff = hc[r][t];
-                const float a_re = w0[t][r].re * coeff, a_im = w0[t][r].im * coeff;
-                const float c_re = w1[t][r].re * coeff, c_im = w1[t][r].im * coeff;
-                f0[r].re = f0[r].re + a_re;
-                f0[r].im = f0[r].im + a_im;
-                f1[r].re = f1[r].re + c_re;
-                f1[r].im = f1[r].im + c_im;
-            }
-        }
-        // DFT: registers, twiddle, lanes
-        reg_dft<R>(f0);
-        reg_dft<R>(f1);
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-            const C2 a = f0[r], c = f1[r], t = rtw[r];
-            f0[r] = {a.re * t.re - a.im * t.im, a.re * t.im + a.im * t.re};
-            f1[r] = {c.re * t.re - c.im * t.im, c.re * t.im + c.im * t.re};
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            lane_fft<G>(f0[r].re, f0[r].im, twr, twi);
-            lane_fft<G>(f1[r].re, f1[r].im, twr, twi);
-        }
-        if constexpr (OUT == 1) {
-            if (live) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int k = r + R * kLane;
-                    float2* o = reinterpret_cast<float2*>(out) + ((long)(b * nSub + sub) * 2) * N + k;
-                    o[0] = make_float2(f0[r].re, f0[r].im);
-                    o[N] = make_float2(f1[r].re, f1[r].im);
+                for (int t = 0; t < TAPS; ++t) {
+                    const int slot = (u + 1 + t) % TAPS;
+                    const float coeff = hc[r][t];
+                    const float a_re = w0[slot][r].re * coeff, a_im = w0[slot][r].im * coeff;
+                    const float c_re = w1[slot][r].re * coeff, c_im = w1[slot][r].im * coeff;
+                    f0[r].re = f0[r].re + a_re;
+                    f0[r].im = f0[r].im + a_im;
+                    f1[r].re = f1[r].re + c_re;
+                    f1[r].im = f1[r].im + c_im;
                 }
             }
-        } else {
+            // DFT: registers, twiddle, lanes
+            reg_dft<R>(f0);
+            reg_dft<R>(f1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const C2 a = f0[r], c = f1[r], t = rtw[r];
+                f0[r] = {a.re * t.re - a.im * t.im, a.re * t.im + a.im * t.re};
+                f1[r] = {c.re * t.re - c.im * t.im, c.re * t.im + c.im * t.re};
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const float powerX = f0[r].re * f0[r].re + f0[r].im * f0[r].im;
-                const float powerY = f1[r].re * f1[r].re + f1[r].im * f1[r].im;
-                const float I = powerX + powerY;
-                acc[r] = acc[r] + I;
+                lane_fft<G>(f0[r].re, f0[r].im, ltw);
+                lane_fft<G>(f1[r].re, f1[r].im, ltw);
             }
-            if ((s + 1) % integ == 0) {
+            if constexpr (OUT == 1) {
                 if (live) {
-                    const long u = b / integ;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int k = r + R * kLane;
-                        out[(u * nSub + sub) * N + k] = acc[r];
+                        float2* o = reinterpret_cast<float2*>(out) + ((long)(b * nSub + sub) * 2) * N + k;
+                        o[0] = make_float2(f0[r].re, f0[r].im);
+                        o[N] = make_float2(f1[r].re, f1[r].im);
                     }
                 }
+            } else {
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+                for (int r = 0; r < R; ++r) {
+                    const float powerX = f0[r].re * f0[r].re + f0[r].im * f0[r].im;
+                    const float powerY = f1[r].re * f1[r].re + f1[r].im * f1[r].im;
+                    const float I = powerX + powerY;
+                    acc[r] = acc[r] + I;
+                }
+                if (++inWindow == integ) {  // (s + 1) % integ == 0
+                    inWindow = 0;
+                    if (live) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int k = r + R * kLane;
+                            out[(uo * nSub + sub) * N + k] = acc[r];
+                        }
+                    }
+                    ++uo;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+                }
             }
         }
     }
