@@ -283,12 +283,15 @@ def main():
     t_dd = stages["dedisperse"] * 1e-3
     achieved = adds / t_dd / 1e12
     kname = {"f32": "dedisperse_kernel", "f16": "dedisperse_f16_kernel", "u8": "dedisperse_u8_kernel"}[dd_dtype]
+    if dd_dtype == "f32" and chain.plan.tile[3] == 16:
+        kname = "dedisperse_f32x4_kernel"
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_TFLOPS_FP32, 4), "traffic": traffic_from_profile(dd_dtype),
                 "kernel": kname, "ops_per_launch": adds, "alg_bytes_per_launch": alg_bytes,
                 "hbm_gbs_algorithmic": round(alg_bytes / t_dd / 1e9, 1),
-                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32): issue peak 39.3 T/s, "
-                        "LDS-read bound 19.7 T/s for FP32 input"}
+                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32): issue peak 39.3 T/s; "
+                        "one 4-byte LDS read per add: 19.7 T/s at 128 B/clk/CU, 17.5 T/s at the 114 B/clk/CU "
+                        "ds_read_b32 rate measured on gfx950 (ds_read_b128: 228 B/clk/CU)"}
 
     # ---- mixed precision: the same inputs through chains with FP16 / u8
     # dedispersion input (FP32 / int32 sums), against a plain FP32 chain.
