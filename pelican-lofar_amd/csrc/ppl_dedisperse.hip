@@ -444,25 +444,47 @@ __global__ __launch_bounds__(256) void dedisperse_f16_kernel(const DDBatch B)
 
     auto rowsOf = [&](int buf) { return lds32 + (size_t)buf * bufDw; };
     auto offsOf = [&](int buf) { return reinterpret_cast<int*>(lds32 + (size_t)buf * bufDw + CB * 2 * rowDw); };
+    // wave w stages channels w, w+4, ... (CB <= 16: at most four), their
+    // delays loaded and used before the first DMA (see the FP32 kernel)
     auto stage = [&](int buf, int cb0, int nc) {
         uint32_t* rows = rowsOf(buf);
         int* offs = offsOf(buf);
-        for (int cc = wave; cc < nc; cc += 4) {
-            const int c = cb0 + cc;
+        int dmin[4], dj[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = min(cb0 + min(wave + 4 * i, CB - 1), nChan - 1);
             const int32_t* dl = delayT + (size_t)c * tdmsPad + dm0;
-            const int dmin = dl[0];
-            const size_t byteStart = ((size_t)c * nsamp + (size_t)t0 + (size_t)dmin) * 2;
-            const size_t byte0 = byteStart & ~(size_t)3;
-            const int mis = (int)(byteStart - byte0) / 2;
-            unsigned char* dst = reinterpret_cast<unsigned char*>(rows + (size_t)cc * 2 * rowDw);
-            for (int k0 = 0; k0 < rowDw; k0 += 64) {
-                const size_t g = byte0 + 4 * (size_t)(k0 + lane);
-                if (k0 + lane < rowDw && g + 4 <= totalBytes)
-                    __builtin_amdgcn_global_load_lds(inBytes + g, dst + 4 * k0, 4, 0, 0);
-            }
+            dmin[i] = __builtin_amdgcn_readfirstlane(dl[0]);
+            dj[i] = dl[min(lane, TD - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc >= nc) break;
+            const size_t byteStart = ((size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin[i]) * 2;
+            const int mis = (int)(byteStart & 3) / 2;
             if (lane < TD) {
-                const int o = dl[lane] - dmin + mis;  // element offset in the row
+                const int o = dj[i] - dmin[i] + mis;  // element offset in the row
                 offs[cc * TD + lane] = cc * 2 * rowDw + ((o & 1) ? rowDw : 0) + (o >> 1);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc >= nc) break;
+            const size_t byteStart = ((size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin[i]) * 2;
+            const size_t byte0 = byteStart & ~(size_t)3;
+            unsigned char* dst = reinterpret_cast<unsigned char*>(rows + (size_t)cc * 2 * rowDw);
+            if (byte0 + 4 * (size_t)rowDw <= totalBytes) {  // whole 64-dword rows
+                const unsigned char* g = inBytes + byte0 + 4 * lane;
+                for (int k0 = 0; k0 < rowDw; k0 += 64)
+                    __builtin_amdgcn_global_load_lds(g + 4 * k0, dst + 4 * k0, 4, 0, 0);
+            } else {
+                for (int k0 = 0; k0 < rowDw; k0 += 64) {
+                    const size_t g = byte0 + 4 * (size_t)(k0 + lane);
+                    if (k0 + lane < rowDw && g + 4 <= totalBytes)
+                        __builtin_amdgcn_global_load_lds(inBytes + g, dst + 4 * k0, 4, 0, 0);
+                }
             }
         }
     };
@@ -594,25 +616,47 @@ __global__ __launch_bounds__(256) void dedisperse_u8_kernel(const DDBatch B)
 
     auto rowsOf = [&](int buf) { return lds32 + (size_t)buf * bufDw; };
     auto offsOf = [&](int buf) { return reinterpret_cast<int*>(lds32 + (size_t)buf * bufDw + CB * 4 * rowDw); };
+    // wave w stages channels w, w+4, ... (CB <= 16: at most four), their
+    // delays loaded and used before the first DMA (see the FP32 kernel)
     auto stage = [&](int buf, int cb0, int nc) {
         uint32_t* rows = rowsOf(buf);
         int* offs = offsOf(buf);
-        for (int cc = wave; cc < nc; cc += 4) {
-            const int c = cb0 + cc;
+        int dmin[4], dj[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = min(cb0 + min(wave + 4 * i, CB - 1), nChan - 1);
             const int32_t* dl = delayT + (size_t)c * tdmsPad + dm0;
-            const int dmin = dl[0];
-            const size_t byteStart = (size_t)c * nsamp + (size_t)t0 + (size_t)dmin;
-            const size_t byte0 = byteStart & ~(size_t)3;
-            const int mis = (int)(byteStart - byte0);
-            unsigned char* dst = reinterpret_cast<unsigned char*>(rows + (size_t)cc * 4 * rowDw);
-            for (int k0 = 0; k0 < rowDw; k0 += 64) {
-                const size_t g = byte0 + 4 * (size_t)(k0 + lane);
-                if (k0 + lane < rowDw && g + 4 <= totalBytes)
-                    __builtin_amdgcn_global_load_lds(in + g, dst + 4 * k0, 4, 0, 0);
-            }
+            dmin[i] = __builtin_amdgcn_readfirstlane(dl[0]);
+            dj[i] = dl[min(lane, TD - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc >= nc) break;
+            const size_t byteStart = (size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin[i];
+            const int mis = (int)(byteStart & 3);
             if (lane < TD) {
-                const int o = dl[lane] - dmin + mis;  // byte offset in the row
+                const int o = dj[i] - dmin[i] + mis;  // byte offset in the row
                 offs[cc * TD + lane] = cc * 4 * rowDw + (o & 3) * rowDw + (o >> 2);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = wave + 4 * i;
+            if (cc >= nc) break;
+            const size_t byteStart = (size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin[i];
+            const size_t byte0 = byteStart & ~(size_t)3;
+            unsigned char* dst = reinterpret_cast<unsigned char*>(rows + (size_t)cc * 4 * rowDw);
+            if (byte0 + 4 * (size_t)rowDw <= totalBytes) {  // whole 64-dword rows
+                const uint8_t* g = in + byte0 + 4 * lane;
+                for (int k0 = 0; k0 < rowDw; k0 += 64)
+                    __builtin_amdgcn_global_load_lds(g + 4 * k0, dst + 4 * k0, 4, 0, 0);
+            } else {
+                for (int k0 = 0; k0 < rowDw; k0 += 64) {
+                    const size_t g = byte0 + 4 * (size_t)(k0 + lane);
+                    if (k0 + lane < rowDw && g + 4 <= totalBytes)
+                        __builtin_amdgcn_global_load_lds(in + g, dst + 4 * k0, 4, 0, 0);
+                }
             }
         }
     };
@@ -739,11 +783,11 @@ void lds_geometry(const Shape& s, int span, int& rowUnits, int& perChannel)
     if constexpr (std::is_same<Tin, _Float16>::value) {
         // rowUnits = dwords per copy: (TT + span + misalignment) halves plus
         // the dword the shifted copy reads past the end
-        rowUnits = (TT + span + 1 + 1) / 2 + 1;
+        rowUnits = ((TT + span + 1 + 1) / 2 + 1 + 63) & ~63;  // whole 64-dword DMA rows
         perChannel = 2 * 4 * (2 * rowUnits + TD);
     } else if constexpr (std::is_same<Tin, uint8_t>::value) {
         // four shifted copies of (TT + span + misalignment) bytes, in dwords
-        rowUnits = (TT + span + 3 + 3) / 4 + 1;
+        rowUnits = ((TT + span + 3 + 3) / 4 + 1 + 63) & ~63;  // whole 64-dword DMA rows
         perChannel = 2 * 4 * (4 * rowUnits + TD);
     } else {
         // rowUnits = elements, a whole number of 64-dword DMA rows when an
