@@ -184,9 +184,10 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
 
 
 def test_wide_reads_reject_a_spread_too_wide(ppl):
-    """A DM tile whose delay spread needs a staged row longer than the
-    16-byte kernel's static LDS buffer is refused, not truncated."""
-    cfg = dict(n_chan=64, nsamp_buf=16384, tdms=64, dm_low=0.0, dm_step=4.0, tsamp=0.00032768,
+    """A DM tile whose delay spread (about 9200 samples over 64 trials) needs
+    a staged row longer than two LDS buffers of 64 KB hold is refused, not
+    truncated."""
+    cfg = dict(n_chan=64, nsamp_buf=16384, tdms=64, dm_low=0.0, dm_step=6.0, tsamp=0.00032768,
                fch1=150.0, foff=-0.05)
     plan = ppl.DedispPlan(**cfg)
     plan.set_tile(64, 512, 0, 16)
