@@ -22,8 +22,10 @@ Prints one JSON line.  value = DM trials x output samples per second of the
 whole chain (tdms * nOut * steps / wall), with the real-time factor, per-stage
 times, the dedispersion roofline and the CPU baseline alongside.  With
 --two-stream the steps are also timed on two streams (step k's channeliser,
-clipper and buffer fill beside step k-1's dedispersion); value is then the
-faster schedule's, and both are reported.
+clipper and buffer fill beside step k-1's dedispersion), and with --cu-split N
+on two streams over disjoint CU slots (the clipper, buffer fill and carry on N
+slots of every XCD, beside step k-1's dedispersion); value is then the fastest
+schedule's, and every timed schedule is reported.
 """
 import argparse
 import json
@@ -75,6 +77,10 @@ def parse():
     ap.add_argument("--two-stream", action="store_true",
                     help="also time the two-stream schedule (front stages of step k beside the "
                          "dedispersion of step k-1); off by default, since it is slower on one MI355X")
+    ap.add_argument("--cu-split", type=int, default=0,
+                    help="also time the CU-split schedule with this many CU slots per XCD for the clipper, "
+                         "buffer fill and carry (the rest run the channeliser, dedispersion and detection); "
+                         "0: off")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 256-channel split-precision run")
     ap.add_argument("--prec-steps", type=int, default=2, help="timed steps per chain of the 256-channel run")
@@ -272,9 +278,29 @@ def main():
         pipe = {"ms_per_step": 1e3 * pwall / args.steps, "value": TDMS * nout * args.steps / pwall,
                 "real_time_factor": nout * TSAMP * args.steps / pwall,
                 "events_per_step": [len(e[0]) for e in pev]}
+    # ---- the CU-split schedule: as above, with the clipper, buffer fill and
+    # carry on their own CU slots of every XCD, so that the clipper's
+    # one-workgroup walk runs beside the dedispersion grid.
+    split = None
+    if args.cu_split:
+        log("timed steps, CU-split schedule (%d CU slots per XCD)" % args.cu_split)
+        base = args.warmup + 2 * args.steps
+        list(chain.run_split([chunks[base % len(chunks)]], small_cus=args.cu_split))  # untimed
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sev = list(chain.run_split((chunks[(base + 1 + k) % len(chunks)] for k in range(args.steps)),
+                                   small_cus=args.cu_split))
+        torch.cuda.synchronize()
+        swall = time.perf_counter() - t0
+        split = {"ms_per_step": 1e3 * swall / args.steps, "value": TDMS * nout * args.steps / swall,
+                 "real_time_factor": nout * TSAMP * args.steps / swall, "cu_slots_per_xcd": args.cu_split,
+                 "events_per_step": [len(e[0]) for e in sev]}
     sequential = {"ms_per_step": ms_per_step, "value": value, "real_time_factor": rtf}
-    if pipe is not None and pipe["value"] > value:
-        ms_per_step, value, rtf = pipe["ms_per_step"], pipe["value"], pipe["real_time_factor"]
+    schedule = "one-stream"
+    for name, alt in (("two-stream", pipe), ("cu-split", split)):
+        if alt is not None and alt["value"] > value:
+            ms_per_step, value, rtf = alt["ms_per_step"], alt["value"], alt["real_time_factor"]
+            schedule = name
 
     # ---- roofline of the dedispersion kernel: algorithmic adds / kernel time
     adds = float(TDMS) * N_CHAN * nout
@@ -396,9 +422,10 @@ def main():
                    "dedisp_input": dd_dtype, "dft": "fp32", "tile_dm_t_chanblock_readbytes": list(chain.plan.tile),
                    "autotune_ms": None if tune_ms is None else round(tune_ms, 3)},
         "real_time_factor": rtf,
-        "schedule": "two-stream" if pipe is not None and value == pipe["value"] else "one-stream",
+        "schedule": schedule,
         "one_stream": sequential,
         "two_stream": pipe,
+        "cu_split": split,
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "graph_ms_per_step": round(graph_ms, 3),
         "events_per_step": [len(e[0]) for e in plain_events],

@@ -59,6 +59,12 @@ int ppl_memset_async(void* dst, int value, size_t bytes, void* stream);
 int ppl_stream_create(void** stream);
 int ppl_stream_destroy(void* stream);
 int ppl_stream_sync(void* stream);
+/* A stream restricted to CU slots [first, first + count) of every XCD
+ * (hipExtStreamCreateWithCUMask with the same range on each XCD); two
+ * disjoint ranges split the GPU between two streams.  No counterpart in the
+ * reference, where a GPU job holds a whole GPU resource while it runs
+ * (src/lib/src/GPU_Manager.cpp:48-86). */
+int ppl_stream_create_cu_range(int first, int count, void** stream);
 /* Enqueue a host callback after the work queued so far on the stream
  * (the GPU_Job callback of src/lib/src/GPU_Manager.cpp:83-85). */
 int ppl_stream_callback(void* stream, void (*fn)(void* user, int status), void* user);

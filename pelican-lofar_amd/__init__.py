@@ -83,6 +83,7 @@ SIGNATURES = {
     "ppl_memset_async": (i, [vp, i, sz, vp]),
     "ppl_stream_create": (i, [vp]),
     "ppl_stream_destroy": (i, [vp]),
+    "ppl_stream_create_cu_range": (i, [i, i, vp]),
     "ppl_stream_sync": (i, [vp]),
     "ppl_stream_callback": (i, [vp, vp, vp]),
     "ppl_dedisp_plan_create": (i, [vp, vp]),
@@ -168,6 +169,27 @@ def _stream(stream=None):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
     return C.c_void_p(s.cuda_stream)
+
+
+class CuRangeStream:
+    """A stream whose kernels run only on CU slots [first, first + count) of
+    every XCD (ppl_stream_create_cu_range); .stream is the torch view."""
+
+    def __init__(self, first, count):
+        import torch
+        h = C.c_void_p()
+        check(lib().ppl_stream_create_cu_range(first, count, C.byref(h)))
+        self.h = h
+        self.stream = torch.cuda.ExternalStream(h.value)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            self.stream.synchronize()
+            _lib.ppl_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def init(device=0):
@@ -807,6 +829,88 @@ class BeamChain:
         cur.wait_stream(back)
         if k and not k & 1:  # leave the last series in self.out, as step() does
             self.out.copy_(outs[1])
+
+    def run_split(self, volts, small_cus=4, threshold=6.0, bin_pow2=6, max_events=1 << 22):
+        """run_pipelined with the GPU split between the two streams by CU
+        range, the same slots on every XCD: the big stream (slots small_cus
+        and up) runs PPF+Stokes, dedispersion and detection; the small stream
+        (slots below small_cus) runs the clipper, the buffer fill and the
+        carry.  The clipper's walk is one workgroup per beam, so on its own
+        CUs it runs beside step k-1's dedispersion instead of after it, and
+        no stream's kernels wait for CU slots held by the other's.
+        Ordering: the clipper of step k waits for PPF k; dedispersion k-1
+        (issued after PPF k) waits for buffer k-1 to be filled; the carry of
+        step k waits for dedispersion k-1; PPF k waits for the clipper of
+        step k-2, whose Stokes slot it reuses (two slots).  The results
+        equal step()'s, and the chain carries on with step() afterwards."""
+        import torch
+        per_xcd = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count // 8
+        if not 0 < small_cus < per_xcd:
+            raise PPLError("small_cus must be in 1..%d" % (per_xcd - 1))
+        big_s, small_s = CuRangeStream(small_cus, per_xcd - small_cus), CuRangeStream(0, small_cus)
+        big, small = big_s.stream, small_s.stream
+        stokes = [self.stokes, torch.empty_like(self.stokes)]
+        outs = [self.out, torch.empty_like(self.out)]
+        cur = torch.cuda.current_stream()
+        big.wait_stream(cur)
+        small.wait_stream(cur)
+        filled = [None, None]  # per Stokes slot: its step's clipper and fill are done
+        prev = None            # (buffer, filled event) of step k-1
+        k = 0
+        try:
+            for d_volt in volts:
+                sk = stokes[k & 1]
+                with torch.cuda.stream(big):
+                    if filled[k & 1] is not None:
+                        big.wait_event(filled[k & 1])
+                    self.ppf.stokes_i16(d_volt, self.nout * self.integrate, d_out=sk)
+                    ppf_done = torch.cuda.Event()
+                    ppf_done.record(big)
+                with torch.cuda.stream(small):
+                    small.wait_event(ppf_done)
+                    self.clip.run(sk)
+                    used, full = self.buf.add(sk.view(self.nout, self.n_chan))
+                    if used != self.nout or not full:
+                        raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
+                    d_in = self.buf.tensor()
+                    ev = torch.cuda.Event()
+                    ev.record(small)
+                    filled[k & 1] = ev
+                result, dd_done = None, None
+                if prev is not None:
+                    with torch.cuda.stream(big):
+                        big.wait_event(prev[1])
+                        out = outs[(k - 1) & 1]
+                        self.plan.run(prev[0], d_out=out)
+                        dd_done = torch.cuda.Event()
+                        dd_done.record(big)
+                        result = self.detect(threshold, bin_pow2, max_events, d_out=out, stream=big)
+                with torch.cuda.stream(small):
+                    if dd_done is not None:
+                        small.wait_event(dd_done)
+                    self.buf.rotate()
+                if result is not None:
+                    yield result
+                prev = (d_in, ev)
+                self.parity ^= 1
+                k += 1
+            if prev is not None:
+                with torch.cuda.stream(big):
+                    big.wait_event(prev[1])
+                    out = outs[(k - 1) & 1]
+                    self.plan.run(prev[0], d_out=out)
+                    result = self.detect(threshold, bin_pow2, max_events, d_out=out, stream=big)
+                yield result
+        finally:
+            cur.wait_stream(big)
+            cur.wait_stream(small)
+            torch.cuda.synchronize()
+            # leave the last step's Stokes and series where step() leaves them
+            if k and not k & 1:
+                self.stokes.copy_(stokes[1])
+                self.out.copy_(outs[1])
+            big_s.close()
+            small_s.close()
 
 
 # ------------------------------------------------------------- many beams

@@ -3,6 +3,8 @@
 // (src/lib/src/GPU_NVidia.cpp, src/lib/src/GPU_Param.cpp) with plain HIP.
 #include "ppl_internal.h"
 
+#include <hip/hip_ext.h>
+
 #include <cstring>
 
 namespace ppl {
@@ -99,6 +101,35 @@ extern "C" int ppl_stream_create(void** stream)
     PPL_CHECK_ARG(stream, "null pointer");
     hipStream_t s;
     PPL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return PPL_OK;
+}
+
+// A stream whose kernels run only on CU slots [first, first + count) of
+// every XCD.  A CU-mask bit i selects CU i / nXcd of XCD i % nXcd, and a
+// launch's workgroups go to the XCDs round robin whatever the mask, so the
+// range is the same on every XCD.
+extern "C" int ppl_stream_create_cu_range(int first, int count, void** stream)
+{
+    PPL_CHECK_ARG(stream, "null pointer");
+    int dev = 0;
+    PPL_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    PPL_HIP(hipGetDeviceProperties(&prop, dev));
+    constexpr int kXcd = 8;
+    const int nCu = prop.multiProcessorCount;
+    PPL_CHECK_ARG(nCu % kXcd == 0 && nCu <= 1024, "%d CUs is not a whole number of %d XCDs", nCu, kXcd);
+    const int perXcd = nCu / kXcd;
+    PPL_CHECK_ARG(first >= 0 && count >= 1 && first + count <= perXcd, "CU range [%d, %d) outside 0..%d per XCD",
+                  first, first + count, perXcd);
+    uint32_t mask[32] = {0};
+    for (int u = first; u < first + count; ++u)
+        for (int x = 0; x < kXcd; ++x) {
+            const int bit = u * kXcd + x;
+            mask[bit / 32] |= 1u << (bit % 32);
+        }
+    hipStream_t s;
+    PPL_HIP(hipExtStreamCreateWithCUMask(&s, (uint32_t)((nCu + 31) / 32), mask));
     *stream = s;
     return PPL_OK;
 }

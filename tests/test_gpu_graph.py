@@ -83,3 +83,41 @@ def test_pipelined_steps_match_eager(orc, ppl, dtype):
     eager.step(tc)
     piped.step(tc)
     assert np.array_equal(piped.out.cpu().numpy(), eager.out.cpu().numpy())
+
+
+@pytest.mark.parametrize("dtype", ["f32", "u8"])
+def test_cu_split_steps_match_eager(orc, ppl, dtype):
+    """The CU-split schedule (clipper, buffer fill and carry on 4 CU slots of
+    every XCD, the rest of the chain on the others; step k's clipper beside
+    step k-1's dedispersion) gives every step's events bit for bit as step()
+    does, leaves the last series in out, and the chain carries on with
+    step() afterwards."""
+    coeffs, _, bp = _inputs(orc)
+    mk = lambda: ppl.BeamChain(N_SUB, N_CH, N_TAPS, coeffs, INTEG, bp, NSAMP_BUF, TDMS, 0.0, DM_STEP, TSAMP,
+                               FCH1, FOFF, history=1000, dtype=dtype)
+    eager, split = mk(), mk()
+    per = eager.samples_per_step()
+    n_prime = eager.overlap * INTEG * N_CH
+    n_steps = 6
+    x = noise_voltages(N_SUB, n_prime + n_steps * per, 64.0, seed=41)
+    xd = torch.from_numpy(x).cuda()
+    prime = xd[:, :, :n_prime].contiguous()
+    eager.prime(prime)
+    split.prime(prime)
+    chunks = [xd[:, :, n_prime + k * per:n_prime + (k + 1) * per].contiguous() for k in range(n_steps)]
+    want = []
+    for c in chunks:
+        eager.step(c)
+        want.append(eager.detect()[:2])
+    last = eager.out.cpu().numpy().copy()
+    got = [r[:2] for r in split.run_split(chunks, small_cus=4)]
+    torch.cuda.synchronize()
+    assert len(got) == n_steps
+    for k, ((gi, gv), (wi, wv)) in enumerate(zip(got, want)):
+        assert np.array_equal(gi, wi) and np.array_equal(gv, wv), k
+    assert np.array_equal(split.out.cpu().numpy(), last)
+    tail = noise_voltages(N_SUB, per, 64.0, seed=42)
+    tc = torch.from_numpy(tail).cuda()
+    eager.step(tc)
+    split.step(tc)
+    assert np.array_equal(split.out.cpu().numpy(), eager.out.cpu().numpy())
