@@ -235,8 +235,7 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 // the row on; the four loads of a row hit the same cache lines.
 // The two staging buffers are separate __shared__ arrays: the compiler then
 // knows that reads of the current buffer do not alias the LDS DMA into the
-// other one, and does not hold them behind it (with one dynamic LDS array it
-// waits for every DMA in flight before the first read of each block).
+// other one, and does not hold them behind it.
 // Buffer layout: rows[CB][4 copies][W4 float4] then offs[CB][TD] (byte
 // offset of each (channel, DM) read base inside the buffer).  W4 is a
 // multiple of 16, so a copy is a whole number of 64-dword DMA rows.
