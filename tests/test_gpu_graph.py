@@ -121,3 +121,18 @@ def test_cu_split_steps_match_eager(orc, ppl, dtype):
     eager.step(tc)
     split.step(tc)
     assert np.array_equal(split.out.cpu().numpy(), eager.out.cpu().numpy())
+
+
+def test_cu_range_stream_rejects_bad_ranges(ppl):
+    """CU ranges outside the slots of an XCD, or empty, are refused."""
+    per_xcd = torch.cuda.get_device_properties(0).multi_processor_count // 8
+    for first, count in [(-1, 2), (0, 0), (per_xcd - 1, 2), (0, per_xcd + 1)]:
+        with pytest.raises(ppl.PPLError):
+            ppl.CuRangeStream(first, count)
+    s = ppl.CuRangeStream(0, per_xcd)  # the whole GPU is a valid range
+    x = torch.ones(1024, device="cuda")
+    with torch.cuda.stream(s.stream):
+        y = x * 2
+    s.stream.synchronize()
+    assert float(y.sum()) == 2048.0
+    s.close()
