@@ -38,7 +38,33 @@ __device__ __forceinline__ float sample_value<int32_t>(const int32_t* data, long
     return (float)(data[idx] - offset) * scale;
 }
 
-// level hit masks of one segment; lane j of level n is valid for j < 2^(P-n)
+// Value of lane + D (within the row of 16 lanes for D <= 8: DPP row_shl;
+// across rows by ds_bpermute).
+template <int D>
+__device__ __forceinline__ float from_lane_above(float x)
+{
+    if constexpr (D <= 8)
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x100 + D, 0xf, 0xf,
+                                                                     true));
+    else
+        return __shfl_down(x, D, 64);
+}
+
+// Level hit masks of one segment.  The level-n bin j stays on lane j * 2^n
+// (valid for lanes below W that are multiples of 2^n): it is the level n-1
+// bin on that lane plus the one 2^(n-1) lanes above, the reference's pair
+// order (binned[n][j] = binned[n-1][2j] + binned[n-1][2j+1]).  A hit on
+// lane L of level n is bin L >> n, sample L of the segment.
+template <int N>
+__device__ __forceinline__ void level_up(float& cur, int lane, int W, const Thresholds& th, uint64_t* masks,
+                                         float* vals)
+{
+    constexpr int D = 1 << (N - 1);
+    cur = cur + from_lane_above<D>(cur);
+    vals[N] = cur;
+    masks[N] = __ballot(lane < W && (lane & (2 * D - 1)) == 0 && cur >= th.v[N]);
+}
+
 template <typename T>
 __device__ __forceinline__ void segment_masks(const T* data, long rowBase, int seg, int P, const Thresholds& th,
                                               int offset, float scale, uint64_t* masks, float* vals)
@@ -50,14 +76,12 @@ __device__ __forceinline__ void segment_masks(const T* data, long rowBase, int s
     vals[0] = v;
     masks[0] = __ballot(lane < W && v >= th.v[0]);
     float cur = v;
-    for (int n = 1; n <= P; ++n) {
-        const int src = (2 * lane) & 63;
-        const float a = __shfl(cur, src, 64);
-        const float b = __shfl(cur, src + 1, 64);
-        cur = a + b;
-        vals[n] = cur;
-        masks[n] = __ballot(lane < (W >> n) && cur >= th.v[n]);
-    }
+    if (P >= 1) level_up<1>(cur, lane, W, th, masks, vals);
+    if (P >= 2) level_up<2>(cur, lane, W, th, masks, vals);
+    if (P >= 3) level_up<3>(cur, lane, W, th, masks, vals);
+    if (P >= 4) level_up<4>(cur, lane, W, th, masks, vals);
+    if (P >= 5) level_up<5>(cur, lane, W, th, masks, vals);
+    if (P >= 6) level_up<6>(cur, lane, W, th, masks, vals);
 }
 
 template <typename T>
@@ -100,12 +124,12 @@ __global__ __launch_bounds__(256) void detect_write_kernel(const T* __restrict__
     for (int n = 0; n <= P; ++n) {
         const uint64_t m = masks[n];
         if ((m >> lane) & 1ull) {
+            // bins in order of j = lane >> n: the rank among the mask's lanes
             const int k = base + __popcll(m & below);
             if (k < maxEvents) {
-                const int binFactor = 1 << n;
                 evInts[3 * k + 0] = dm;
-                evInts[3 * k + 1] = seg * (1 << P) + binFactor * lane;
-                evInts[3 * k + 2] = binFactor;
+                evInts[3 * k + 1] = seg * (1 << P) + lane;  // binFactor * j
+                evInts[3 * k + 2] = 1 << n;
                 evVals[k] = vals[n];
             }
         }
