@@ -2,9 +2,10 @@
 // (DedispersionAnalyser::analyse, src/lib/src/DedispersionAnalyser.cpp:30-203).
 //
 // One wave per 64-sample segment of one DM row: lane j holds sample j of
-// the segment, level n sums pairs of level n-1 with xor shuffles
-// (binned[n][j] = binned[n-1][2j] + binned[n-1][2j+1], the reference's pair
-// order), and ballots give the per-level hit masks.  Pass 1 counts the hits
+// the segment; the level-n bin j stays on lane j * 2^n, the sum of the
+// level n-1 bins on that lane and 2^(n-1) lanes above (binned[n][j] =
+// binned[n-1][2j] + binned[n-1][2j+1], the reference's pair order), and
+// ballots give the per-level hit masks.  Pass 1 counts the hits
 // of every (DM, segment); an exclusive scan of the counts (row-major, the
 // reference's event order) gives each segment its output offset; pass 2
 // writes the events of the segments that have any.
