@@ -372,8 +372,10 @@ enum { PPL_CLIP_MAX_BEAMS = 48 };
  * proceed side by side.  Results equal nBeams separate ppl_clipper_run calls.
  * At most PPL_CLIP_MAX_BEAMS beams; a clipper may appear once per batch. */
 int ppl_clipper_run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int nBeams, void* stream);
-/* Same, and also writes the clipped Stokes-I spectra transposed into a
- * dedispersion buffer (fusing DedispersionBuffer::addSamples). */
+/* Same (Stokes-I only, nPol = 1, weights not written), then
+ * DedispersionBuffer::addSamples of the clipped spectra into a dedispersion
+ * buffer, both on the given stream: one call for a caller that feeds a
+ * buffer (the two steps are separate kernels). */
 int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,
                          ppl_ddbuf* buf, unsigned first, void* stream);
 /* Scalars after the last run: meanRunAve, rmsRunAve (WeightedSpectrumDataSet

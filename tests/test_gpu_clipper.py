@@ -228,3 +228,37 @@ def test_clipper_near_margin_channels(orc, ppl):
     assert cnt["undecided"] > 50 and cnt["fast"] > 2000, cnt
     st_o, _ = oc.state()
     assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
+
+
+@pytest.mark.parametrize("dtype", ["f32", "u8"])
+def test_run_into_equals_run_then_add(ppl, dtype):
+    """ppl_clipper_run_into (the clipper, then DedispersionBuffer::addSamples
+    of its output, in one call) leaves the same clipped spectra, flags and
+    dedispersion buffer, chunk by chunk through the learning phase and a
+    carry, as Clipper.run followed by DDBuffer.add."""
+    n_sub, n_ch = 8, 16
+    n_chan = n_sub * n_ch
+    x = _stream(2304, n_sub, n_ch, seed=33)
+    bp = _bp(n_sub, n_ch)
+    ca, cb = ppl.Clipper(bp, 10.0, 4.0, 1000, True), ppl.Clipper(bp, 10.0, 4.0, 1000, True)
+    # a 256-spectrum overlap: every fill (1024, then 768 new spectra) ends on a chunk boundary
+    ba = ppl.DDBuffer(n_chan, 1024, 256, dtype, 12.7)
+    bb = ppl.DDBuffer(n_chan, 1024, 256, dtype, 12.7)
+    for a in range(0, x.shape[0], 256):
+        chunk = np.ascontiguousarray(x[a:a + 256])
+        da, db = torch.from_numpy(chunk).cuda(), torch.from_numpy(chunk.copy()).cuda()
+        ca.run(da)
+        used, full = ba.add(da.view(da.shape[0], n_chan))
+        cb.run_into(db, bb)
+        assert torch.equal(da, db), a
+        assert np.array_equal(ca.flags(chunk.shape[0]), cb.flags(chunk.shape[0])), a
+        n = ba.fill()  # samples written so far (the rest of the buffer is not initialised)
+        assert n == bb.fill()
+        assert torch.equal(ba.tensor()[:, :n], bb.tensor()[:, :n]), a
+        assert used == chunk.shape[0]
+        if full:
+            ba.rotate()
+            bb.rotate()
+            n = ba.fill()
+            assert n == bb.fill() == 256
+            assert torch.equal(ba.tensor()[:, :n], bb.tensor()[:, :n]), a
