@@ -264,3 +264,28 @@ def test_batch_rejects_mixed_dtypes_and_too_many_jobs(ppl):
         ppl.dedisperse_batch([a, b], [xa, xb])
     with pytest.raises(ppl.PPLError):
         ppl.dedisperse_batch([a] * 49, [xa] * 49)
+
+
+def test_tile_setters_of_the_c_abi(orc, ppl):
+    """ppl_dedisp_set_tile and ppl_dedisp_set_shape (the 4-byte-read forms of
+    ppl_dedisp_set_kernel) set the tile, keep the sums exact, and refuse
+    tiles that are not instantiated."""
+    cfg = dict(SMALL)
+    rng = np.random.default_rng(23)
+    x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
+    p, d = _oracle(orc, x, cfg)
+    exp = orc.dedisperse(x, d, p["nout"])
+    plan = ppl.DedispPlan(**cfg)
+    L = ppl.lib()
+    assert L.ppl_dedisp_set_tile(plan.h, 32, 256) == 0
+    plan._refresh()
+    assert plan.tile[:2] == (32, 256) and plan.tile[3] == 4
+    assert np.array_equal(plan.run(torch.from_numpy(x).cuda()).cpu().numpy(), exp)
+    assert L.ppl_dedisp_set_shape(plan.h, 16, 512, 3) == 0
+    plan._refresh()
+    assert plan.tile == (16, 512, 3, 4)
+    assert np.array_equal(plan.run(torch.from_numpy(x).cuda()).cpu().numpy(), exp)
+    assert L.ppl_dedisp_set_tile(plan.h, 12, 512) != 0      # 12 DMs: not instantiated
+    assert L.ppl_dedisp_set_shape(plan.h, 32, 512, 17) != 0  # channel block above 16
+    plan._refresh()
+    assert plan.tile == (16, 512, 3, 4)                     # a refused call changes nothing
