@@ -94,14 +94,18 @@ typedef struct {
     int tileDM, tileT;      /* DM x time tile chosen for this plan */
     int chanBlock;          /* channels staged per block (0: as many as fit in LDS) */
     int readBytes;          /* LDS read width per lane of the chosen kernel: 4 or 16 */
+    long long fusedDelayDiffs; /* delay entries that would differ were the trial value
+                                  mstartdm + dm*mdmstep one fused multiply-add, as a
+                                  contracting build of DedispersionKernel.cu:29 may form
+                                  it; always 0 when dmLow == 0 (the sum is then exact) */
 } ppl_dedisp_info;
 
 /* DedispersionModule::resize (src/lib/src/DedispersionModule.cpp:109-190):
  * builds the FP32 delay table on the host from the FP64 shift table exactly
  * as the reference (:152-162, DedispersionKernel.cu:29,36), uploads it and
  * picks the tile shape.  Fails with PPL_E_CONFIG when nsampBuf <= maxshift
- * (the module's throw at :175-178) or when a delayed read would leave the
- * channel row. */
+ * (the module's throw at :175-178), when dmStep < 0 (delays would decrease
+ * with the trial index) or when a delayed read would leave the channel row. */
 int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_config* cfg);
 int ppl_dedisp_plan_destroy(ppl_dedisp_plan* plan);
 int ppl_dedisp_plan_info(const ppl_dedisp_plan* plan, ppl_dedisp_info* info);
@@ -151,7 +155,9 @@ int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int reps, void*
 /* The reference's C seam itself, with the reference's arguments (device
  * pointers; maxshift includes the remainder; outbufSize in bytes), computed
  * for every tdms (the reference's launch covers all rows only when
- * tdms % 40 == 0).  Builds its delay table from dmShift on the device. */
+ * tdms % 40 == 0).  Builds its delay table from dmShift on the device.
+ * PPL_E_CONFIG for a negative dmShift, mstartdm or mdmstep (delays that
+ * decrease with the trial index) or a delayed read outside the buffer. */
 int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float* buff, float mstartdm,
                               float mdmstep, int tdms, int numSamples, const float* dmShift,
                               int maxshift, int i_nchans, void* stream);
@@ -373,11 +379,15 @@ enum { PPL_CLIP_MAX_BEAMS = 48 };
  * At most PPL_CLIP_MAX_BEAMS beams; a clipper may appear once per batch. */
 int ppl_clipper_run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int nBeams, void* stream);
 /* Same (Stokes-I only, nPol = 1, weights not written), then
- * DedispersionBuffer::addSamples of the clipped spectra into a dedispersion
- * buffer, both on the given stream: one call for a caller that feeds a
- * buffer (the two steps are separate kernels). */
+ * DedispersionBuffer::addSamples of the clipped spectra [first, nT) into a
+ * dedispersion buffer, both on the given stream: one call for a caller that
+ * feeds a buffer (the two steps are separate kernels).  nSub*nCh must equal
+ * the buffer's nChan (checked before the clipper runs).  As ppl_ddbuf_add,
+ * *consumed receives how many spectra the buffer took and *full whether it
+ * is full: when it filled part way, the caller rotates it and adds the
+ * remaining clipped spectra (from first + *consumed) with ppl_ddbuf_add. */
 int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,
-                         ppl_ddbuf* buf, unsigned first, void* stream);
+                         ppl_ddbuf* buf, unsigned first, unsigned* consumed, int* full, void* stream);
 /* Scalars after the last run: meanRunAve, rmsRunAve (WeightedSpectrumDataSet
  * setMean/setRMS, :569-570); synchronises. */
 int ppl_clipper_stats(ppl_clipper* clip, float* mean, float* rms);

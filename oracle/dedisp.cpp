@@ -56,6 +56,21 @@ extern "C" void orc_dedisp_delays(const float* shift, unsigned nChan, unsigned t
     }
 }
 
+// The same table with the trial value formed as one fused multiply-add,
+// fmaf(dm, mdmstep, mstartdm): what DedispersionKernel.cu:29 computes when
+// nvcc contracts `mstartdm + (idx * mdmstep)` (its default, -fmad=true).
+extern "C" void orc_dedisp_delays_fused(const float* shift, unsigned nChan, unsigned tdms, float mstartdm,
+                                        float mdmstep, int32_t* delay)
+{
+    for (unsigned dm = 0; dm < tdms; ++dm) {
+        float dmval = std::fma((float)dm, mdmstep, mstartdm);
+        for (unsigned c = 0; c < nChan; ++c) {
+            float p = shift[c] * dmval;
+            delay[(size_t)dm * nChan + c] = (int32_t)p;
+        }
+    }
+}
+
 template <typename Tin, typename Tacc, typename Conv>
 static void brute(const Tin* in, unsigned nChan, unsigned nsampBuf, unsigned tdms, const int32_t* delay,
                   unsigned nOut, Tacc* out, int nthreads, Conv conv)

@@ -145,6 +145,8 @@ int orc_dedisp_plan(unsigned nChan, unsigned nsampBuf, unsigned tdms,
  * (src/lib/DedispersionKernel.cu:29,36). */
 void orc_dedisp_delays(const float* shift, unsigned nChan, unsigned tdms,
                        float mstartdm, float mdmstep, int32_t* delay);
+void orc_dedisp_delays_fused(const float* shift, unsigned nChan, unsigned tdms,
+                             float mstartdm, float mdmstep, int32_t* delay);
 /* Brute force, channel order, FP32 sums: out[dm][t], t < nOut
  * (DedispersionKernel.cu:15-50 arithmetic; the launch geometry is not
  * restated).  in [nChan][nsampBuf]. */

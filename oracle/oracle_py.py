@@ -63,6 +63,7 @@ def _declare(L):
         "orc_bandpass_probe": (i, [u, d, d, vp, i, f, f, u, d, d, vp]),
         "orc_dedisp_plan": (i, [u, u, u, f, f, d, d, d, vp, vp, vp]),
         "orc_dedisp_delays": (None, [vp, u, u, f, f, vp]),
+        "orc_dedisp_delays_fused": (None, [vp, u, u, f, f, vp]),
         "orc_dedisperse_f32": (None, [vp, u, u, u, vp, u, vp, i]),
         "orc_dedisperse_f16": (None, [vp, u, u, u, vp, u, vp, i]),
         "orc_dedisperse_u8": (None, [vp, u, u, u, vp, u, vp, i]),
@@ -262,6 +263,14 @@ def dedisp_delays(shift, tdms, mstartdm, mdmstep):
     shift = np.ascontiguousarray(shift, np.float32)
     out = np.zeros((tdms, shift.size), np.int32)
     lib().orc_dedisp_delays(_p(shift), shift.size, tdms, mstartdm, mdmstep, _p(out))
+    return out
+
+
+def dedisp_delays_fused(shift, tdms, mstartdm, mdmstep):
+    """The delay table with the trial value as fmaf(dm, mdmstep, mstartdm)."""
+    shift = np.ascontiguousarray(shift, np.float32)
+    out = np.zeros((tdms, shift.size), np.int32)
+    lib().orc_dedisp_delays_fused(_p(shift), shift.size, tdms, mstartdm, mdmstep, _p(out))
     return out
 
 

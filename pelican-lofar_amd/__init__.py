@@ -47,7 +47,8 @@ class DedispConfig(C.Structure):
 class DedispInfo(C.Structure):
     _fields_ = [("maxshift", C.c_int), ("remainingSamples", C.c_int), ("nOut", C.c_int),
                 ("mstartdm", C.c_float), ("mdmstep", C.c_float), ("tileDM", C.c_int),
-                ("tileT", C.c_int), ("chanBlock", C.c_int), ("readBytes", C.c_int)]
+                ("tileT", C.c_int), ("chanBlock", C.c_int), ("readBytes", C.c_int),
+                ("fusedDelayDiffs", C.c_longlong)]
 
 
 class PPFConfig(C.Structure):
@@ -129,7 +130,7 @@ SIGNATURES = {
     "ppl_clipper_create": (i, [vp, vp]),
     "ppl_clipper_destroy": (i, [vp]),
     "ppl_clipper_run": (i, [vp, vp, vp, u, u, u, u, vp]),
-    "ppl_clipper_run_into": (i, [vp, vp, u, u, u, vp, u, vp]),
+    "ppl_clipper_run_into": (i, [vp, vp, u, u, u, vp, u, vp, vp, vp]),
     "ppl_clipper_stats": (i, [vp, vp, vp]),
     "ppl_clipper_flags": (i, [vp, vp, u]),
     "ppl_clipper_counters": (i, [vp, vp, i]),
@@ -216,6 +217,8 @@ class DedispPlan:
         self.maxshift, self.rem, self.nout = info.maxshift, info.remainingSamples, info.nOut
         self.mstartdm, self.mdmstep = info.mstartdm, info.mdmstep
         self.tile = (info.tileDM, info.tileT, info.chanBlock, info.readBytes)
+        # delays that a contracted (fused multiply-add) trial value would change
+        self.fused_delay_diffs = info.fusedDelayDiffs
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -642,10 +645,15 @@ class Clipper:
         return d_I
 
     def run_into(self, d_I, ddbuf, first=0, stream=None):
+        """Clip d_I [nT][nSub][1][nCh] in place, then add spectra [first, nT)
+        to ddbuf; returns (consumed, full) as DDBuffer.add does."""
         n_t, n_sub, n_pol, n_ch = d_I.shape
-        assert n_pol == 1
+        if n_pol != 1:
+            raise PPLError("run_into takes Stokes-I spectra (nPol = 1)")
+        consumed, full = C.c_uint(0), C.c_int(0)
         check(lib().ppl_clipper_run_into(self.h, _ptr(d_I), n_t, n_sub, n_ch, ddbuf.h, first,
-                                         _stream(stream)))
+                                         C.byref(consumed), C.byref(full), _stream(stream)))
+        return consumed.value, bool(full.value)
 
     def stats(self):
         m, r = C.c_float(0), C.c_float(0)
@@ -794,41 +802,46 @@ class BeamChain:
         done = None
         k = 0
         pending = None
-        for d_volt in volts:
-            with torch.cuda.stream(front):
-                self.ppf.stokes_i16(d_volt, self.nout * self.integrate, d_out=self.stokes)
-                self.clip.run(self.stokes)
-                used, full = self.buf.add(self.stokes.view(self.nout, self.n_chan))
-                if used != self.nout or not full:
-                    raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
-                d_in = self.buf.tensor()
-                filled = torch.cuda.Event()
-                filled.record(front)
-                if done is not None:
-                    front.wait_event(done)
-                self.buf.rotate()
-            result = None
-            with torch.cuda.stream(back):
-                if pending is not None:
+        try:
+            for d_volt in volts:
+                with torch.cuda.stream(front):
+                    self.ppf.stokes_i16(d_volt, self.nout * self.integrate, d_out=self.stokes)
+                    self.clip.run(self.stokes)
+                    used, full = self.buf.add(self.stokes.view(self.nout, self.n_chan))
+                    if used != self.nout or not full:
+                        raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
+                    d_in = self.buf.tensor()
+                    filled = torch.cuda.Event()
+                    filled.record(front)
+                    if done is not None:
+                        front.wait_event(done)
+                    self.buf.rotate()
+                result = None
+                with torch.cuda.stream(back):
+                    if pending is not None:
+                        result = self.detect(threshold, bin_pow2, max_events, d_out=pending, stream=back)
+                    out = outs[k & 1]
+                    back.wait_event(filled)
+                    self.plan.run(d_in, d_out=out)
+                    done = torch.cuda.Event()
+                    done.record(back)
+                if result is not None:
+                    yield result
+                pending = out
+                self.parity ^= 1
+                k += 1
+            if pending is not None:
+                with torch.cuda.stream(back):
                     result = self.detect(threshold, bin_pow2, max_events, d_out=pending, stream=back)
-                out = outs[k & 1]
-                back.wait_event(filled)
-                self.plan.run(d_in, d_out=out)
-                done = torch.cuda.Event()
-                done.record(back)
-            if result is not None:
                 yield result
-            pending = out
-            self.parity ^= 1
-            k += 1
-        if pending is not None:
-            with torch.cuda.stream(back):
-                result = self.detect(threshold, bin_pow2, max_events, d_out=pending, stream=back)
-            yield result
-        cur.wait_stream(front)
-        cur.wait_stream(back)
-        if k and not k & 1:  # leave the last series in self.out, as step() does
-            self.out.copy_(outs[1])
+        finally:
+            # on every exit (the consumer stopping early, an error) the
+            # current stream waits for both streams' work on self.stokes,
+            # the buffer and the outputs
+            cur.wait_stream(front)
+            cur.wait_stream(back)
+            if k and not k & 1:  # leave the last series in self.out, as step() does
+                self.out.copy_(outs[1])
 
     def run_split(self, volts, small_cus=4, threshold=6.0, bin_pow2=6, max_events=1 << 22):
         """run_pipelined with the GPU split between the two streams by CU

@@ -142,6 +142,8 @@ static int add_blob(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned 
     return PPL_OK;
 }
 
+unsigned ppl_ddbuf_nchan(const ppl_ddbuf* b) { return b ? b->nChan : 0u; }
+
 int ppl_ddbuf_add_spectra(ppl_ddbuf* b, const float* d_spectra, unsigned nT, unsigned first, unsigned* consumed,
                           int* full, hipStream_t st)
 {

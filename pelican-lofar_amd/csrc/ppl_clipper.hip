@@ -2373,16 +2373,22 @@ extern "C" int ppl_clipper_flags(ppl_clipper* c, uint8_t* h_good, unsigned nT)
 }
 
 // Fused variant: clip, then transpose the clipped Stokes-I spectra into a
-// dedispersion buffer (declared in ppl_buffer.hip).
+// dedispersion buffer (ppl_buffer.hip).  The buffer's channel count is
+// checked before the clipper runs, so a mismatch leaves the clipper state
+// untouched.
+unsigned ppl_ddbuf_nchan(const ppl_ddbuf* buf);
 int ppl_ddbuf_add_spectra(ppl_ddbuf* buf, const float* d_spectra, unsigned nT, unsigned first, unsigned* consumed,
                           int* full, hipStream_t st);
 
 extern "C" int ppl_clipper_run_into(ppl_clipper* c, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,
-                                    ppl_ddbuf* buf, unsigned first, void* stream)
+                                    ppl_ddbuf* buf, unsigned first, unsigned* consumed, int* full, void* stream)
 {
+    PPL_CHECK_ARG(c && buf && consumed && full, "null argument");
+    PPL_CHECK_ARG(first <= nT, "first spectrum %u beyond the %u clipped", first, nT);
+    PPL_CHECK_ARG(nSub * nCh == ppl_ddbuf_nchan(buf),
+                  "DedispersionBuffer: input data sample size(%u) does not match buffer sample size (%u)", nSub * nCh,
+                  ppl_ddbuf_nchan(buf));
     int rc = clipper_run(c, d_I, nullptr, nT, nSub, 1, nCh, stream_of(stream));
     if (rc != PPL_OK) return rc;
-    unsigned consumed = 0;
-    int full = 0;
-    return ppl_ddbuf_add_spectra(buf, d_I, nT, first, &consumed, &full, stream_of(stream));
+    return ppl_ddbuf_add_spectra(buf, d_I, nT, first, consumed, full, stream_of(stream));
 }

@@ -33,6 +33,7 @@ struct ppl_dedisp_plan {
     int cb = 0;                   // channels per staged block (0: as many as fit)
     int rw = 4;                   // LDS read width per lane, bytes (4, or 16 for FP32 input)
     int spanTD[4] = {0, 0, 0, 0}; // max delay spread inside a DM tile of 8, 16, 32, 64 trials
+    long long fusedDiffs = 0;     // delays that differ when the trial value is one fused multiply-add
 };
 
 namespace {
@@ -1027,6 +1028,12 @@ extern "C" int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_c
     PPL_CHECK_ARG(cfg->fch1 != 0.0, "frequencyChannel1 must be a positive number");
     PPL_CHECK_ARG(cfg->foff < 0.0,
                   "channelBandwidth must be negative (descending channel order)");
+    // the kernels stage each DM tile's rows from its first trial's delay on,
+    // so the delays must not decrease with the trial index
+    if (!(cfg->dmStep >= 0.0f)) {
+        set_error("dedispersion: dmStep %g must not be negative", (double)cfg->dmStep);
+        return PPL_E_CONFIG;
+    }
     ppl_dedisp_plan* p = new ppl_dedisp_plan;
     p->cfg = *cfg;
     if (p->cfg.dmLow < 0.0f) p->cfg.dmLow = 0.0f;  // DedispersionModule.cpp:57
@@ -1055,14 +1062,20 @@ extern "C" int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_c
     p->mdmstep = p->cfg.dmStep / tsamp_f;
     p->delays.resize((size_t)cfg->tdms * N);
     int maxdelay = 0;
+    p->fusedDiffs = 0;
     for (unsigned dm = 0; dm < cfg->tdms; ++dm) {
         const float prod = (float)dm * p->mdmstep;
         const float dmval = p->mstartdm + prod;
+        // the same trial value with mstartdm + dm*mdmstep as one fused
+        // multiply-add, as a compiler contracting DedispersionKernel.cu:29
+        // would build it (identical when dmLow == 0: the sum is then exact)
+        const float dmvalFused = std::fma((float)dm, p->mdmstep, p->mstartdm);
         for (unsigned c = 0; c < N; ++c) {
             const float v = p->shift[c] * dmval;
             const int32_t d = (int32_t)v;
             p->delays[(size_t)dm * N + c] = d;
             maxdelay = std::max(maxdelay, d);
+            if (dmvalFused != dmval && (int32_t)(p->shift[c] * dmvalFused) != d) ++p->fusedDiffs;
         }
     }
     if (p->nOut <= 0 || p->nOut - 1 + maxdelay >= (int)cfg->nsampBuf) {
@@ -1111,6 +1124,7 @@ extern "C" int ppl_dedisp_plan_info(const ppl_dedisp_plan* p, ppl_dedisp_info* i
     info->tileT = 64 * p->nt;
     info->chanBlock = p->cb;
     info->readBytes = p->rw;
+    info->fusedDelayDiffs = p->fusedDiffs;
     return PPL_OK;
 }
 
@@ -1283,6 +1297,13 @@ extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float*
     PPL_CHECK_ARG(tdms > 0 && i_nchans > 0 && numSamples > maxshift && maxshift >= 0,
                   "bad shape (tdms %d, nchans %d, numSamples %d, maxshift %d)", tdms, i_nchans,
                   numSamples, maxshift);
+    // delays must not decrease with the trial index (the kernels stage each
+    // DM tile's rows from its first trial's delay on)
+    if (!(mstartdm >= 0.0f && mdmstep >= 0.0f)) {
+        set_error("cacheDedisperseLoop: mstartdm %g and mdmstep %g must not be negative", (double)mstartdm,
+                  (double)mdmstep);
+        return PPL_E_CONFIG;
+    }
     const int nOut = numSamples - maxshift;
     PPL_CHECK_ARG((long)tdms * nOut * (long)sizeof(float) <= outbufSize,
                   "output buffer (%ld bytes) smaller than tdms*nOut floats", outbufSize);
@@ -1303,6 +1324,12 @@ extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float*
     PPL_HIP(hipStreamSynchronize(st));
     int span = 0;
     for (int c = 0; c < i_nchans; ++c) {
+        if (!(hshift[c] >= 0.0f)) {
+            (void)hipFreeAsync(d_delayT, st);
+            set_error("cacheDedisperseLoop: dmShift[%d] = %g is negative (ascending channel frequency)", c,
+                      (double)hshift[c]);
+            return PPL_E_CONFIG;
+        }
         for (int dm0 = 0; dm0 < tdms; dm0 += TD) {
             const int dm1 = std::min(dm0 + TD - 1, tdms - 1);
             const int a = (int)(hshift[c] * (mstartdm + (float)dm0 * mdmstep));

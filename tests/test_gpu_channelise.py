@@ -1,7 +1,7 @@
 """GPU parity of the channeliser with the oracle: spectra against the FP64
 DFT of the oracle's FP32 FIR (stated tolerance), history carried across
 calls, fused Stokes-I/integration against the unfused stages, standalone
-Stokes and integration bit-exact."""
+Stokes (I, Q, U, V) and integration bit-exact."""
 import numpy as np
 import pytest
 import torch
@@ -66,8 +66,16 @@ def test_stokes_and_integrate_bit_exact(orc, ppl):
     I4 = ppl.stokes(d, 4).cpu().numpy()
     assert np.array_equal(I4[:, :, 0:1], I)
     X, Y = spec[:, :, 0], spec[:, :, 1]
-    q = (X[..., 0] * X[..., 0] + X[..., 1] * X[..., 1]) - (Y[..., 0] * Y[..., 0] + Y[..., 1] * Y[..., 1])
-    assert np.array_equal(I4[:, :, 1], q)
+    Xr, Xi, Yr, Yi = X[..., 0], X[..., 1], Y[..., 0], Y[..., 1]
+    # StokesGenerator.cpp:93-103, each product rounded to float32 before the sum
+    power_x = Xr * Xr + Xi * Xi
+    power_y = Yr * Yr + Yi * Yi
+    assert np.array_equal(I4[:, :, 1], power_x - power_y)
+    assert np.array_equal(I4[:, :, 2], np.float32(2.0) * (Xr * Yr + Xi * Yi))
+    assert np.array_equal(I4[:, :, 3], np.float32(2.0) * (Xi * Yr - Xr * Yi))
+    # a swapped sign or operand order in V, or U taken from X alone, would differ
+    assert not np.array_equal(I4[:, :, 3], np.float32(2.0) * (Xr * Yi - Xi * Yr))
+    assert not np.array_equal(I4[:, :, 2], np.float32(2.0) * (Xr * Xr + Xi * Xi))
     integ = ppl.integrate(torch.from_numpy(I).cuda(), 4).cpu().numpy()
     assert np.array_equal(integ, orc.integrate(I, 4))
     integ2 = ppl.integrate(torch.from_numpy(I).cuda(), 2, 4).cpu().numpy()

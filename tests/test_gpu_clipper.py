@@ -234,31 +234,68 @@ def test_clipper_near_margin_channels(orc, ppl):
 def test_run_into_equals_run_then_add(ppl, dtype):
     """ppl_clipper_run_into (the clipper, then DedispersionBuffer::addSamples
     of its output, in one call) leaves the same clipped spectra, flags and
-    dedispersion buffer, chunk by chunk through the learning phase and a
-    carry, as Clipper.run followed by DDBuffer.add."""
+    dedispersion buffer, chunk by chunk through the learning phase and
+    carries, as Clipper.run followed by DDBuffer.add.  The 1000-sample
+    buffer with a 200-sample overlap fills part way through 256-spectrum
+    chunks: run_into reports how many spectra it took, and the rest of the
+    chunk goes into the rotated buffer."""
     n_sub, n_ch = 8, 16
     n_chan = n_sub * n_ch
     x = _stream(2304, n_sub, n_ch, seed=33)
     bp = _bp(n_sub, n_ch)
     ca, cb = ppl.Clipper(bp, 10.0, 4.0, 1000, True), ppl.Clipper(bp, 10.0, 4.0, 1000, True)
-    # a 256-spectrum overlap: every fill (1024, then 768 new spectra) ends on a chunk boundary
-    ba = ppl.DDBuffer(n_chan, 1024, 256, dtype, 12.7)
-    bb = ppl.DDBuffer(n_chan, 1024, 256, dtype, 12.7)
+    ba = ppl.DDBuffer(n_chan, 1000, 200, dtype, 12.7)
+    bb = ppl.DDBuffer(n_chan, 1000, 200, dtype, 12.7)
+    mid_chunk_fills = 0
     for a in range(0, x.shape[0], 256):
         chunk = np.ascontiguousarray(x[a:a + 256])
+        n_t = chunk.shape[0]
         da, db = torch.from_numpy(chunk).cuda(), torch.from_numpy(chunk.copy()).cuda()
         ca.run(da)
-        used, full = ba.add(da.view(da.shape[0], n_chan))
-        cb.run_into(db, bb)
+        used_a, full_a = ba.add(da.view(n_t, n_chan))
+        used_b, full_b = cb.run_into(db, bb)
+        assert (used_a, full_a) == (used_b, full_b), a
         assert torch.equal(da, db), a
-        assert np.array_equal(ca.flags(chunk.shape[0]), cb.flags(chunk.shape[0])), a
+        assert np.array_equal(ca.flags(n_t), cb.flags(n_t)), a
         n = ba.fill()  # samples written so far (the rest of the buffer is not initialised)
         assert n == bb.fill()
         assert torch.equal(ba.tensor()[:, :n], bb.tensor()[:, :n]), a
-        assert used == chunk.shape[0]
-        if full:
+        first = used_a
+        while first < n_t or full_a:
+            assert full_a, a
+            mid_chunk_fills += first < n_t
             ba.rotate()
             bb.rotate()
+            assert ba.fill() == bb.fill() == 200
+            assert torch.equal(ba.tensor()[:, :200], bb.tensor()[:, :200]), a
+            if first == n_t:
+                break
+            used_a, full_a = ba.add(da.view(n_t, n_chan), first)
+            used_b, full_b = bb.add(db.view(n_t, n_chan), first)
+            assert (used_a, full_a) == (used_b, full_b), a
+            first += used_a
             n = ba.fill()
-            assert n == bb.fill() == 256
             assert torch.equal(ba.tensor()[:, :n], bb.tensor()[:, :n]), a
+    assert mid_chunk_fills >= 2
+
+
+def test_run_into_rejects_a_mismatched_buffer(ppl):
+    """A buffer whose channel count differs from nSub*nCh is refused before
+    the clipper runs: the clipper state is untouched, so a later run equals
+    a fresh clipper's."""
+    n_sub, n_ch = 8, 16
+    x = _stream(256, n_sub, n_ch, seed=34)
+    bp = _bp(n_sub, n_ch)
+    ca, cb = ppl.Clipper(bp, 10.0, 4.0, 1000, True), ppl.Clipper(bp, 10.0, 4.0, 1000, True)
+    for n_chan in (n_sub * n_ch + 8, n_sub * n_ch - 8):
+        bad = ppl.DDBuffer(n_chan, 1024, 0, "f32", 1.0)
+        d = torch.from_numpy(x.copy()).cuda()
+        with pytest.raises(ppl.PPLError, match="does not match buffer sample size"):
+            cb.run_into(d, bad)
+        assert torch.equal(d, torch.from_numpy(x).cuda())
+        assert bad.fill() == 0
+    da, db = torch.from_numpy(x.copy()).cuda(), torch.from_numpy(x.copy()).cuda()
+    ca.run(da)
+    cb.run(db)
+    assert torch.equal(da, db)
+    assert ca.stats() == cb.stats()

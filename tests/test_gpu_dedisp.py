@@ -117,6 +117,44 @@ def test_reference_c_seam(orc, ppl):
     assert np.all(flat[n:] == 0.0)  # the rest of outbufSize is cleared, as cudaMemset did
 
 
+def test_plan_counts_fused_delay_differences(orc, ppl):
+    """The plan reports how many delays a fused multiply-add trial value
+    (DedispersionKernel.cu:29 under contraction) would change: none with
+    dmLow = 0, the oracle's count with dmLow > 0."""
+    lofar = dict(n_chan=3968, nsamp_buf=65536, tdms=1024, dm_low=0.0, dm_step=500 / 2048,
+                 tsamp=5.12e-6 * 64 * 4, fch1=150.0, foff=-0.1953125 / 64)
+    assert ppl.DedispPlan(**lofar).fused_delay_diffs == 0
+    lofar["dm_low"] = 37.3
+    plan = ppl.DedispPlan(**lofar)
+    p, d = _oracle(orc, None, lofar)
+    assert np.array_equal(plan.delays(), d)
+    fused = orc.dedisp_delays_fused(p["shift"], lofar["tdms"], p["mstartdm"], p["mdmstep"])
+    assert plan.fused_delay_diffs == int(np.count_nonzero(fused != d)) > 0
+
+
+def test_negative_dm_steps_and_shifts_are_refused(orc, ppl):
+    """Delays that decrease with the trial index are refused (PPL_E_CONFIG):
+    a negative dmStep in the plan, a negative dmShift, mstartdm or mdmstep
+    in the C seam."""
+    cfg = dict(SMALL)
+    cfg["dm_step"] = -0.5
+    with pytest.raises(ppl.PPLError, match="dmStep"):
+        ppl.DedispPlan(**cfg)
+    x = torch.zeros((SMALL["n_chan"], SMALL["nsamp_buf"]), device="cuda")
+    p, d = _oracle(orc, None, SMALL)
+    out = torch.zeros((SMALL["tdms"], p["nout"]), device="cuda")
+    args = (SMALL["tdms"], SMALL["nsamp_buf"])
+    bad = p["shift"].copy()
+    bad[7] = -bad[7] - 1.0
+    with pytest.raises(ppl.PPLError, match="negative"):
+        ppl.cache_dedisperse_loop(out, x, float(p["mstartdm"]), float(p["mdmstep"]), *args,
+                                  torch.from_numpy(bad).cuda(), p["maxshift"] + p["rem"], SMALL["n_chan"])
+    shift = torch.from_numpy(p["shift"]).cuda()
+    with pytest.raises(ppl.PPLError, match="negative"):
+        ppl.cache_dedisperse_loop(out, x, 0.0, -float(p["mdmstep"]), *args, shift, p["maxshift"] + p["rem"],
+                                  SMALL["n_chan"])
+
+
 def test_plan_rejects_oversized_maxshift(ppl):
     with pytest.raises(ppl.PPLError):
         ppl.DedispPlan(3968, 4096, 2048, 0.0, 500 / 2048, 5.12e-6 * 64 * 4, 150.0, -0.1953125 / 64)

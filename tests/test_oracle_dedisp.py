@@ -43,6 +43,55 @@ def _exact_delay(shift_f32, dm, mdmstep_f32):
     return int(Fraction(float(v)).__trunc__())
 
 
+def _round_f32(x):
+    """Round a Fraction to the nearest float32, ties to even (exactly)."""
+    if x == 0:
+        return np.float32(0.0)
+    sign = -1 if x < 0 else 1
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    q = a / Fraction(2) ** (e - 23)  # in [2^23, 2^24)
+    n, r = divmod(q.numerator, q.denominator)
+    if 2 * r > q.denominator or (2 * r == q.denominator and n % 2 == 1):
+        n += 1
+    return np.float32(sign * float(Fraction(n) * Fraction(2) ** (e - 23)))
+
+
+def test_fused_trial_value_pinned_and_counted(orc):
+    """DedispersionKernel.cu:29 forms mstartdm + idx*mdmstep, which nvcc may
+    contract into one fused multiply-add.  The oracle's fused table is pinned
+    by exact rational rounding of dm*mdmstep + mstartdm; with dmLow == 0 it
+    equals the unfused table entry for entry (the sum is exact), and with
+    dmLow > 0 the tables differ in a few entries: those delays are parity
+    unpinned against the reference (the product reports how many)."""
+    n, ns, tdms, step, tsamp, fch1, foff = CONFIGS["lofar"][:7]
+    p0 = orc.dedisp_plan(n, ns, tdms, 0.0, step, tsamp, fch1, foff)
+    a = orc.dedisp_delays(p0["shift"], tdms, p0["mstartdm"], p0["mdmstep"])
+    b = orc.dedisp_delays_fused(p0["shift"], tdms, p0["mstartdm"], p0["mdmstep"])
+    assert np.array_equal(a, b)
+    p1 = orc.dedisp_plan(n, ns, 1024, 37.3, step, tsamp, fch1, foff)
+    ms, st = np.float32(p1["mstartdm"]), np.float32(p1["mdmstep"])
+    assert ms > 0
+    a = orc.dedisp_delays(p1["shift"], 1024, ms, st)
+    b = orc.dedisp_delays_fused(p1["shift"], 1024, ms, st)
+    diff = np.argwhere(a != b)
+    assert 0 < len(diff) < a.size // 100, len(diff)
+    # fused entries against exact rounding: every differing entry and a sample of the rest
+    rng = np.random.default_rng(5)
+    picks = [tuple(x) for x in diff[:40]] + [(int(rng.integers(1024)), int(rng.integers(n))) for _ in range(40)]
+    for dm, c in picks:
+        v = _round_f32(Fraction(int(dm)) * Fraction(float(st)) + Fraction(float(ms)))
+        assert b[dm, c] == int(Fraction(float(np.float32(p1["shift"][c] * v))).__trunc__())
+        u = np.float32(ms + np.float32(np.float32(dm) * st))
+        assert a[dm, c] == int(Fraction(float(np.float32(p1["shift"][c] * u))).__trunc__())
+    # a differing entry differs by one sample
+    assert np.max(np.abs(a - b)) == 1
+
+
 def test_delay_table_exact(orc):
     n, ns, tdms, step, tsamp, fch1, foff = CONFIGS["small"][:7]
     p = orc.dedisp_plan(n, ns, tdms, 0.0, step, tsamp, fch1, foff)
