@@ -421,8 +421,8 @@ def detect(d_data, n_channels, n_subbands, threshold=6.0, bin_pow2=6, max_events
            stream=None):
     """DedispersionAnalyser::analyse on a device [tdms][nsamp] float tensor."""
     tdms, nsamp = d_data.shape
-    ints = np.zeros((max_events, 3), np.int32)
-    vals = np.zeros(max_events, np.float32)
+    ints = np.zeros((max(max_events, 1), 3), np.int32)
+    vals = np.zeros(max(max_events, 1), np.float32)
     count = C.c_int(0)
     check(lib().ppl_detect(_ptr(d_data), tdms, nsamp, n_channels, n_subbands, threshold, bin_pow2,
                            ints.ctypes.data_as(C.c_void_p), vals.ctypes.data_as(C.c_void_p),

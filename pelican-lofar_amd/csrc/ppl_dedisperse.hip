@@ -34,6 +34,11 @@ struct ppl_dedisp_plan {
     int rw = 4;                   // LDS read width per lane, bytes (4, or 16 for FP32 input)
     int spanTD[4] = {0, 0, 0, 0}; // max delay spread inside a DM tile of 8, 16, 32, 64 trials
     long long fusedDiffs = 0;     // delays that differ when the trial value is one fused multiply-add
+    // ring tables of the 16-byte kernel, built on first use, for DM tiles
+    // of 16 and 32 trials (index 0, 1): per tile and channel the LDS byte
+    // offsets of each trial's reads, then per tile and channel the first
+    // trial's delay
+    int32_t* d_ring[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -223,52 +228,129 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
     }
 }
 
-// FP32 input, 16-byte LDS reads.  gfx950 serves ds_read_b128 at about twice
-// the bytes per clock of ds_read_b32 (228 against 114 B/clk/CU measured), so
-// lane l accumulates four consecutive samples t0 + 4l + 256q + {0,1,2,3}
-// (q < NQ) from one read.  A read must be 16-byte aligned while the delays
-// are arbitrary, so each staged channel row is held in four copies shifted
-// by 0..3 elements: the read for a delay offset o is the aligned float4 at
-// element o - (o & 3) of copy (o & 3).  The sums are still taken in channel
-// order, one FP32 add per sample, so the result is bit-identical to the
-// one-sample kernel above.
-// Wave s loads copy s of every staged channel by LDS DMA, from element s of
-// the row on; the four loads of a row hit the same cache lines.
-// The two staging buffers are separate __shared__ arrays: the compiler then
-// knows that reads of the current buffer do not alias the LDS DMA into the
-// other one, and does not hold them behind it.
-// Buffer layout: rows[CB][4 copies][W4 float4] then offs[CB][TD] (byte
-// offset of each (channel, DM) read base inside the buffer).  W4 is a
-// multiple of 16, so a copy is a whole number of 64-dword DMA rows.
-constexpr int kX4BufBytes = 26 * 1024;  // two buffers: 52 KB, three workgroups per CU
+// FP32 input, 16-byte LDS reads from a four-stage ring of staged rows.
+// gfx950 serves ds_read_b128 at about twice the bytes per clock of
+// ds_read_b32 (228 against 114 B/clk/CU measured), so lane l accumulates
+// four consecutive samples t0 + 4l + 256q + {0,1,2,3} (q < NQ) from one
+// read.  A read must be 16-byte aligned while the delays are arbitrary, so
+// each staged channel row is held in four copies shifted by 0..3 elements:
+// the read for a delay offset r (from the tile's first trial) is the
+// aligned float4 (r >> 2) + lane of copy (r & 3).  The sums are still taken
+// in channel order, one FP32 add per sample, so the result is bit-identical
+// to the one-sample kernel.
+//
+// One channel per ring stage.  Wave s loads copy s of the row (from element
+// s on: the same cache lines for all four) by LDS DMA, always kRingDw
+// dwords (kRingDma instructions), plus one gather instruction that brings
+// the wave's ND read offsets of the channel and the first-trial delay of
+// the channel three ahead, from a per-plan ring table (offsets already in
+// LDS bytes).  So every wave has exactly kRingDma + 1 DMA instructions per
+// channel in flight, and at the top of channel c it waits (explicit vmcnt)
+// only for its DMA into stage c % 4; the workgroup then meets at a bare
+// barrier (every wave's part of stage c has landed, every wave is done with
+// channel c-1), and the DMA of channel c+3 goes into the stage channel c-1
+// used: three channels of DMA lead.  No other vector memory operation is
+// issued in the channel loop, and the compiler's own fence before
+// __syncthreads (a wait for every DMA in flight) is kept out with an
+// s_barrier in inline assembly, which also keeps the compiler from moving
+// LDS accesses across it.
+constexpr int kRingQuads = 192;           // float4 per copy of a staged row
+constexpr int kRingDw = 4 * kRingQuads;   // dwords per copy
+constexpr int kRingDma = kRingDw / 64;    // DMA instructions per copy
+constexpr int kRingStage = 4 * kRingQuads * 16 + 4 * 64;  // four copies, then 64 bytes of offsets per wave
+
+// s_waitcnt bounding only the vector memory counter (gfx9 encoding:
+// vmcnt[3:0] in bits 3:0, vmcnt[5:4] in bits 15:14; expcnt and lgkmcnt at
+// their maxima, i.e. not waited for).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt out of range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+__device__ __forceinline__ void bare_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// LDS address (byte offset in the workgroup's LDS) of a __shared__ pointer.
+__device__ __forceinline__ uint32_t lds_addr(const void* p)
+{
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// Reads of the staged rows in inline assembly, each block waiting for its
+// own reads (lgkmcnt(0)) before it ends, so no read is in flight when the
+// compiler takes its registers.  The compiler's wait tracking cannot
+// follow the four stages' DMA through the loop and would wait for every
+// DMA in flight before the first read of a loop iteration.
+__device__ __forceinline__ void lds_read_x2(uint32_t a, int4& x, int4& y)
+{
+    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:16\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(x), "=&v"(y)
+                 : "v"(a)
+                 : "memory");
+}
+__device__ __forceinline__ void lds_read_4x1(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, float4 (&v)[8])
+{
+    asm volatile("ds_read_b128 %0, %4\n\t"
+                 "ds_read_b128 %1, %5\n\t"
+                 "ds_read_b128 %2, %6\n\t"
+                 "ds_read_b128 %3, %7\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3])
+                 : "v"(a0), "v"(a1), "v"(a2), "v"(a3)
+                 : "memory");
+}
+__device__ __forceinline__ void lds_read_4x2(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, float4 (&v)[8])
+{
+    asm volatile("ds_read_b128 %0, %8\n\t"
+                 "ds_read_b128 %1, %8 offset:1024\n\t"
+                 "ds_read_b128 %2, %9\n\t"
+                 "ds_read_b128 %3, %9 offset:1024\n\t"
+                 "ds_read_b128 %4, %10\n\t"
+                 "ds_read_b128 %5, %10 offset:1024\n\t"
+                 "ds_read_b128 %6, %11\n\t"
+                 "ds_read_b128 %7, %11 offset:1024\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
+                   "=&v"(v[7])
+                 : "v"(a0), "v"(a1), "v"(a2), "v"(a3)
+                 : "memory");
+}
 
 template <int ND, int NQ>
-__global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
+__global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
 {
-    __shared__ __align__(16) unsigned char bufA[kX4BufBytes];
-    __shared__ __align__(16) unsigned char bufB[kX4BufBytes];
+    static_assert(ND <= 16, "a wave's offsets fill 64 bytes");
+    __shared__ __align__(16) unsigned char ring0[kRingStage];
+    __shared__ __align__(16) unsigned char ring1[kRingStage];
+    __shared__ __align__(16) unsigned char ring2[kRingStage];
+    __shared__ __align__(16) unsigned char ring3[kRingStage];
     int local;
     const DDJob& J = find_job(B, local);
     const float* __restrict__ in = static_cast<const float*>(J.in);
     float* __restrict__ out = static_cast<float*>(J.out);
-    const int32_t* __restrict__ delayT = J.delayT;
-    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, tdmsPad = J.tdmsPad, nOut = J.nOut;
-    const int W4 = J.rowUnits, CB = J.CB, nDMTiles = J.nDMTiles;
+    const int32_t* __restrict__ tab = J.delayT;  // the plan's ring table for this DM tile width
+    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, nOut = J.nOut;
+    const int nDMTiles = J.nDMTiles;
     constexpr int TT = 256 * NQ;
     constexpr int TD = 4 * ND;
-    const int copyBytes = W4 * 16;
-    const int rowBytes = 4 * copyBytes;  // one channel, four copies
-    const int nChunks = W4 / 16;         // 64-dword DMA rows per copy
 
     int tTile, dmTile;
     tile_of(local, nDMTiles, tTile, dmTile);
     const int t0 = tTile * TT;
     if (t0 >= nOut) return;  // padding of the XCD-major grid
     const int dm0 = dmTile * TD;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const size_t totalElems = (size_t)nChan * nsamp;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lastElem = nChan * nsamp - 1;  // make_job checks that it fits an int
+    // ring table: offs[tile][c][TD] (LDS byte offset of each trial's reads in
+    // a stage), then dmin[tile][c] (the tile's first-trial delay)
+    const int32_t* __restrict__ offsTab = tab + ((size_t)dmTile * nChan) * TD + wave * ND;
+    // first-trial delays through the constant address space: scalar loads,
+    // counted by lgkmcnt, not by the DMA's vmcnt (consecutive channels share
+    // a scalar cache line)
+    using ConstI32 = const __attribute__((address_space(4))) int32_t;
+    ConstI32* dminTab = (ConstI32*)(tab + (size_t)nDMTiles * nChan * TD + (size_t)dmTile * nChan);
 
     float acc[ND][NQ][4];
 #pragma unroll
@@ -278,106 +360,84 @@ __global__ __launch_bounds__(256) void dedisperse_f32x4_kernel(const DDBatch B)
 #pragma unroll
             for (int k = 0; k < 4; ++k) acc[j][q][k] = 0.0f;
 
-    const int nBlocks = (nChan + CB - 1) / CB;
-    auto count = [&](int blk) { return min(CB, nChan - blk * CB); };
-    // Delays of the next block to stage, loaded one block ahead: they are
-    // issued right after a block's DMA and land with it, so a block start
-    // never waits on a delay-table round trip.  Addresses are clamped, so
-    // the loads need no branch.  Lane l of dminLane holds the first trial's
-    // delay of channel l of the block; dj[i] the delay of trial dm0 + lane
-    // of channel wave + 4i (CB <= 16: at most four channels per wave).
-    int dminLane = 0;
-    int dj[4] = {0, 0, 0, 0};
-    auto load_delays = [&](int blk) {
-        const int cb0 = blk * CB;
-        const int cl = min(cb0 + min(lane, CB - 1), nChan - 1);
-        dminLane = delayT[(size_t)cl * tdmsPad + dm0];
-        const int dml = dm0 + min(lane, TD - 1);
+    // channel c into a stage: copy `wave` of the row (kRingDma instructions,
+    // each lane's address clamped to the input rather than masked, so the
+    // count is fixed), then the gather of this wave's offsets of channel c
+    // and dmin of channel c + 3 (lanes 0..ND).
+    auto stage = [&](unsigned char* buf, int c, int dmin) {
+        c = min(c, nChan - 1);  // past the last channel: a dummy stage, so the count stays fixed
+        const int row0 = c * nsamp + t0 + dmin + wave;  // uniform
+        float* dst = reinterpret_cast<float*>(buf + wave * kRingQuads * 16);
+        if (row0 + kRingDw - 1 <= lastElem) {
+            const float* src = in + row0 + lane;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int c = min(cb0 + min(wave + 4 * i, CB - 1), nChan - 1);
-            dj[i] = delayT[(size_t)c * tdmsPad + dml];
+            for (int k = 0; k < kRingDma; ++k) __builtin_amdgcn_global_load_lds(src + 64 * k, dst + 64 * k, 4, 0, 0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kRingDma; ++k)
+                __builtin_amdgcn_global_load_lds(in + min(row0 + 64 * k + lane, lastElem), dst + 64 * k, 4, 0, 0);
         }
+        int32_t* small = reinterpret_cast<int32_t*>(buf + 4 * kRingQuads * 16 + wave * 64);
+        if (lane < ND) __builtin_amdgcn_global_load_lds(offsTab + c * TD + lane, small, 4, 0, 0);
     };
-    // block blk's channels into buf by LDS DMA (its delays loaded and
-    // landed): wave w loads copy w of every channel and writes the read
-    // offsets of channels w, w+4, ...
-    auto stage = [&](unsigned char* buf, int blk) {
-        const int cb0 = blk * CB, nc = count(blk);
-        int* offs = reinterpret_cast<int*>(buf + CB * rowBytes);
+    auto compute = [&](const unsigned char* buf) {
+        static_assert((NQ == 1 || NQ == 2) && ND % 4 == 0, "reads in blocks of four trials x NQ float4");
+        const uint32_t base = lds_addr(buf);
+        int4 oa, ob;
+        lds_read_x2(base + 4 * kRingQuads * 16 + wave * 64, oa, ob);
+        const int o[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
+        const uint32_t lb = base + 16 * lane;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int cc = wave + 4 * i;
-            if (cc < nc) {
-                const int o = dj[i] - __builtin_amdgcn_readlane(dminLane, cc);
-                if (lane < TD) offs[cc * TD + lane] = cc * rowBytes + (o & 3) * copyBytes + (o >> 2) * 16;
-            }
-        }
-        for (int cc = 0; cc < nc; ++cc) {
-            const int dmin = __builtin_amdgcn_readlane(dminLane, cc);
-            // wave w loads copy w: the row from element w on
-            const size_t e0 = (size_t)(cb0 + cc) * nsamp + (size_t)t0 + (size_t)dmin + (size_t)wave;
-            unsigned char* dst = buf + cc * rowBytes + wave * copyBytes;
-            if (e0 + (size_t)(4 * W4) <= totalElems) {
-                const float* src = in + e0 + lane;
-                for (int k = 0; k < nChunks; ++k)
-                    __builtin_amdgcn_global_load_lds(src + 64 * k, dst + 256 * k, 4, 0, 0);
-            } else {  // the end of the input: words past it are not loaded
-                for (int k = 0; k < nChunks; ++k)
-                    if (e0 + 64 * k + lane < totalElems)
-                        __builtin_amdgcn_global_load_lds(in + e0 + 64 * k + lane, dst + 256 * k, 4, 0, 0);
-            }
-        }
-    };
-    auto compute = [&](const unsigned char* buf, int nc) {
-        const int* offs = reinterpret_cast<const int*>(buf + CB * rowBytes) + wave * ND;
-        for (int cc = 0; cc < nc; ++cc) {
-            int o[ND];
+        for (int j0 = 0; j0 < ND; j0 += 4) {
+            float4 v[8];
+            if constexpr (NQ == 2)
+                lds_read_4x2(lb + o[j0], lb + o[j0 + 1], lb + o[j0 + 2], lb + o[j0 + 3], v);
+            else
+                lds_read_4x1(lb + o[j0], lb + o[j0 + 1], lb + o[j0 + 2], lb + o[j0 + 3], v);
 #pragma unroll
-            for (int j = 0; j < ND; ++j) o[j] = offs[cc * TD + j];
-#pragma unroll
-            for (int j = 0; j < ND; ++j) {
-                const float4* p = reinterpret_cast<const float4*>(buf + o[j]) + lane;
-                float4 v[NQ];
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) v[q] = p[64 * q];
+            for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
-                    acc[j][q][0] = acc[j][q][0] + v[q].x;
-                    acc[j][q][1] = acc[j][q][1] + v[q].y;
-                    acc[j][q][2] = acc[j][q][2] + v[q].z;
-                    acc[j][q][3] = acc[j][q][3] + v[q].w;
+                    const float4 x = v[NQ * jj + q];
+                    acc[j0 + jj][q][0] = acc[j0 + jj][q][0] + x.x;
+                    acc[j0 + jj][q][1] = acc[j0 + jj][q][1] + x.y;
+                    acc[j0 + jj][q][2] = acc[j0 + jj][q][2] + x.z;
+                    acc[j0 + jj][q][3] = acc[j0 + jj][q][3] + x.w;
                 }
-            }
         }
     };
+    // top of a channel: this wave's DMA into the channel's stage has landed
+    // (the DMA of the next two stages may be in flight), then the barrier.
+    // Every channel issues one stage, past the last channel a dummy one
+    // (clamped to the last channel), so the count is the same on every path.
+    auto top = [&]() {
+        wait_vmcnt<2 * (kRingDma + 1)>();
+        bare_barrier();
+    };
+    auto dmin_of = [&](int c) { return dminTab[min(c, nChan - 1)]; };
 
-    load_delays(0);
-    __builtin_amdgcn_s_waitcnt(0);
-    stage(bufA, 0);
-    if (nBlocks > 1) load_delays(1);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    for (int blk = 0; blk < nBlocks; blk += 2) {
-        const bool more1 = blk + 1 < nBlocks;
-        if (more1) {
-            stage(bufB, blk + 1);
-            if (blk + 2 < nBlocks) load_delays(blk + 2);
-        }
-        compute(bufA, count(blk));
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (!more1) break;
-        const bool more2 = blk + 2 < nBlocks;
-        if (more2) {
-            stage(bufA, blk + 2);
-            if (blk + 3 < nBlocks) load_delays(blk + 3);
-        }
-        compute(bufB, count(blk + 1));
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (!more2) break;
+    stage(ring0, 0, dmin_of(0));
+    stage(ring1, 1, dmin_of(1));
+    stage(ring2, 2, dmin_of(2));
+    for (int c = 0; c < nChan; c += 4) {
+        top();
+        stage(ring3, c + 3, dmin_of(c + 3));
+        compute(ring0);
+        if (c + 1 >= nChan) break;
+        top();
+        stage(ring0, c + 4, dmin_of(c + 4));
+        compute(ring1);
+        if (c + 2 >= nChan) break;
+        top();
+        stage(ring1, c + 5, dmin_of(c + 5));
+        compute(ring2);
+        if (c + 3 >= nChan) break;
+        top();
+        stage(ring2, c + 6, dmin_of(c + 6));
+        compute(ring3);
     }
+    // no LDS DMA may land after the workgroup has ended
+    wait_vmcnt<0>();
 
     const int dmw = dm0 + wave * ND;
     const bool vec = (nOut & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
@@ -806,6 +866,49 @@ int span_of(const ppl_dedisp_plan* p, int TD)
 
 int elem_bytes(int dtype) { return dtype == PPL_F32 ? 4 : dtype == PPL_F16 ? 2 : 1; }
 
+// The ring kernel's table for DM tiles of TD trials (built and uploaded on
+// first use; the plan keeps it): for tile k, channel c and trial j of the
+// tile, the LDS byte offset of the trial's first read in a ring stage,
+// (r & 3) * (one copy) + (r >> 2) * 16 with r = delay[k*TD + j][c] -
+// delay[k*TD][c]; then for tile k and channel c, delay[k*TD][c].  Trials
+// past tdms take the last trial's delays, as the padded delay table does.
+int ring_table(const ppl_dedisp_plan* cp, int TD, const int32_t** out)
+{
+    ppl_dedisp_plan* p = const_cast<ppl_dedisp_plan*>(cp);
+    const int slot = TD == 16 ? 0 : 1;
+    if (!p->d_ring[slot]) {
+        if (p->delays.empty()) {
+            set_error("dedisperse: no host delay table for the 16-byte kernel");
+            return PPL_E_ARG;
+        }
+        const int tdms = (int)p->cfg.tdms, nChan = (int)p->cfg.nChan;
+        const int nTiles = (tdms + TD - 1) / TD;
+        std::vector<int32_t> t((size_t)nTiles * nChan * (TD + 1));
+        int32_t* dminPart = t.data() + (size_t)nTiles * nChan * TD;
+        for (int k = 0; k < nTiles; ++k)
+            for (int c = 0; c < nChan; ++c) {
+                const int32_t d0 = p->delays[(size_t)(k * TD) * nChan + c];
+                dminPart[(size_t)k * nChan + c] = d0;
+                for (int j = 0; j < TD; ++j) {
+                    const int dm = std::min(k * TD + j, tdms - 1);
+                    const int r = p->delays[(size_t)dm * nChan + c] - d0;
+                    t[((size_t)k * nChan + c) * TD + j] = (r & 3) * kRingQuads * 16 + (r >> 2) * 16;
+                }
+            }
+        int32_t* d = nullptr;
+        PPL_HIP(hipMalloc(&d, t.size() * sizeof(int32_t)));
+        const hipError_t e = hipMemcpy(d, t.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            set_error("dedisperse: ring table upload: %s", hipGetErrorString(e));
+            return PPL_E_HIP;
+        }
+        p->d_ring[slot] = d;
+    }
+    *out = p->d_ring[slot];
+    return PPL_OK;
+}
+
 // Fills a job for plan p under shape s; *ldsBytes receives the job's LDS need.
 int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out, DDJob& J,
              size_t* ldsBytes)
@@ -814,16 +917,29 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
     const int span = span_of(p, TD);
     int rowUnits = 0, perChannel = 0;
     int budget = kMaxLDS;
+    const int32_t* ringTab = nullptr;
     if (s.rw == 16) {
         if (p->cfg.dtype != PPL_F32 || s.nt % 4 != 0) {
             set_error("dedisperse: 16-byte reads need FP32 input and a tile of 256k samples");
             return PPL_E_ARG;
         }
-        // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1;
-        // a multiple of 16, so that a copy is a whole number of 64-dword DMA rows
-        rowUnits = (span / 4 + TT / 4 + 1 + 15) & ~15;
-        perChannel = 4 * rowUnits * 16 + TD * 4;  // one buffer; the kernel holds two
-        budget = kX4BufBytes;
+        // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1
+        // of a copy, which holds kRingQuads
+        rowUnits = kRingQuads;
+        perChannel = span / 4 + TT / 4 <= kRingQuads ? 0 : budget + 1;
+        if ((long)p->cfg.nChan * p->cfg.nsampBuf >= (1L << 31) ||
+            (long)p->cfg.nChan * p->tdmsPad >= (1L << 31)) {
+            set_error("dedisperse: 16-byte reads need nChan*nsampBuf below 2^31");
+            return PPL_E_CONFIG;
+        }
+        if (TD != 16 && TD != 32) {
+            set_error("dedisperse: 16-byte reads take DM tiles of 16 or 32 trials");
+            return PPL_E_ARG;
+        }
+        if (perChannel == 0) {
+            const int rc = ring_table(p, TD, &ringTab);
+            if (rc != PPL_OK) return rc;
+        }
     } else {
         switch (p->cfg.dtype) {
             case PPL_F32: lds_geometry<float>(s, span, rowUnits, perChannel); break;
@@ -835,12 +951,15 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
         set_error("dedisperse: delay spread %d too large for tile %dx%d", span, TD, TT);
         return PPL_E_CONFIG;
     }
-    int CB = std::min(16, budget / perChannel);
-    if (s.cb > 0) CB = std::min(CB, s.cb);
-    CB = std::max(CB, 1);
+    int CB = 1;  // the ring kernel stages one channel per stage
+    if (s.rw != 16) {
+        CB = std::min(16, budget / perChannel);
+        if (s.cb > 0) CB = std::min(CB, s.cb);
+        CB = std::max(CB, 1);
+    }
     J.in = in;
     J.out = out;
-    J.delayT = p->d_delayT;
+    J.delayT = s.rw == 16 ? ringTab : p->d_delayT;
     J.nChan = (int)p->cfg.nChan;
     J.nsamp = (int)p->cfg.nsampBuf;
     J.tdms = (int)p->cfg.tdms;
@@ -897,19 +1016,16 @@ int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, 
 #undef PPL_DD8_CASE
     } else {
         if (s.rw == 16) {
-#define PPL_DDX4_CASE(ND_, NT_)                                                                  \
-    if (s.nd == ND_ && s.nt == NT_) {                                                            \
-        hipLaunchKernelGGL((dedisperse_f32x4_kernel<ND_, NT_ / 4>), grid, block, 0, stream, B); \
-        PPL_LAUNCHED();                                                                          \
-        return PPL_OK;                                                                           \
+#define PPL_DDX4_CASE(ND_, NT_)                                                                 \
+    if (s.nd == ND_ && s.nt == NT_) {                                                           \
+        hipLaunchKernelGGL((dedisperse_ring_kernel<ND_, NT_ / 4>), grid, block, 0, stream, B); \
+        PPL_LAUNCHED();                                                                         \
+        return PPL_OK;                                                                          \
     }
             PPL_DDX4_CASE(4, 8)
             PPL_DDX4_CASE(8, 8)
-            PPL_DDX4_CASE(16, 8)
-            PPL_DDX4_CASE(4, 16)
-            PPL_DDX4_CASE(8, 16)
+            PPL_DDX4_CASE(4, 4)
             PPL_DDX4_CASE(8, 4)
-            PPL_DDX4_CASE(16, 4)
 #undef PPL_DDX4_CASE
             set_error("dedisperse: unsupported 16-byte tile %dx%d", 4 * s.nd, 64 * s.nt);
             return PPL_E_ARG;
@@ -974,8 +1090,7 @@ int run_jobs(const ppl_dedisp_job* jobs, int n, const Shape& s, hipStream_t st)
 }
 
 const Shape kShapes[] = {{8, 8, 0}, {8, 4, 0}, {4, 8, 0}, {4, 4, 0}, {2, 8, 0}, {16, 4, 0}, {16, 2, 0},
-                         {4, 8, 0, 16}, {8, 8, 0, 16}, {16, 8, 0, 16}, {4, 16, 0, 16}, {8, 16, 0, 16},
-                         {8, 4, 0, 16}, {16, 4, 0, 16}};
+                         {4, 8, 0, 16}, {8, 8, 0, 16}, {4, 4, 0, 16}, {8, 4, 0, 16}};
 
 bool shape_known(int nd, int nt, int rw)
 {
@@ -1108,6 +1223,8 @@ extern "C" int ppl_dedisp_plan_destroy(ppl_dedisp_plan* plan)
 {
     if (!plan) return PPL_OK;
     if (plan->d_delayT) (void)hipFree(plan->d_delayT);
+    for (int32_t* t : plan->d_ring)
+        if (t) (void)hipFree(t);
     delete plan;
     return PPL_OK;
 }

@@ -5,15 +5,17 @@
 // the segment; the level-n bin j stays on lane j * 2^n, the sum of the
 // level n-1 bins on that lane and 2^(n-1) lanes above (binned[n][j] =
 // binned[n-1][2j] + binned[n-1][2j+1], the reference's pair order), and
-// ballots give the per-level hit masks.  Pass 1 counts the hits
-// of every (DM, segment); an exclusive scan of the counts (row-major, the
-// reference's event order) gives each segment its output offset; pass 2
-// writes the events of the segments that have any.
+// ballots give the per-level hit masks.  Pass 1 counts the hits of every
+// (DM, segment) and of every workgroup's 64 segments; pass 2 (one
+// workgroup) scans the workgroup counts (row-major, the reference's event
+// order); pass 3 re-reads only the segments that have hits and writes their
+// events at their offsets.  No host synchronisation separates the passes.
 #include "ppl_internal.h"
 
-#include <hipcub/hipcub.hpp>
-
+#include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -85,57 +87,244 @@ __device__ __forceinline__ void segment_masks(const T* data, long rowBase, int s
     if (P >= 6) level_up<6>(cur, lane, W, th, masks, vals);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void detect_count_kernel(const T* __restrict__ data, int tdms, int nsamp,
-                                                           int P, Thresholds th, int offset, float scale,
-                                                           int* __restrict__ counts)
+// Segments are numbered row-major (id = dm * nSeg + seg), the reference's
+// event order.  A workgroup of four waves takes 64 consecutive ids, 16 per
+// wave; the ids a wave owns are loaded before any is reduced, so 16 row
+// segments are in flight per wave.
+constexpr int kSegPerWave = 16;
+constexpr int kSegPerBlock = 4 * kSegPerWave;
+
+// Hits of one segment at every level (the row's value on this lane is v).
+__device__ __forceinline__ int segment_hits(float v, int lane, int W, int P, const Thresholds& th)
 {
-    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nSeg = nsamp >> P;
-    const long total = (long)tdms * nSeg;
-    if (waveId >= total) return;
-    const int dm = (int)(waveId / nSeg), seg = (int)(waveId % nSeg);
     uint64_t masks[kMaxLevels];
     float vals[kMaxLevels];
-    segment_masks<T>(data, (long)dm * nsamp, seg, P, th, offset, scale, masks, vals);
+    masks[0] = __ballot(lane < W && v >= th.v[0]);
+    float cur = v;
+    if (P >= 1) level_up<1>(cur, lane, W, th, masks, vals);
+    if (P >= 2) level_up<2>(cur, lane, W, th, masks, vals);
+    if (P >= 3) level_up<3>(cur, lane, W, th, masks, vals);
+    if (P >= 4) level_up<4>(cur, lane, W, th, masks, vals);
+    if (P >= 5) level_up<5>(cur, lane, W, th, masks, vals);
+    if (P >= 6) level_up<6>(cur, lane, W, th, masks, vals);
     int c = 0;
     for (int n = 0; n <= P; ++n) c += __popcll(masks[n]);
-    if ((threadIdx.x & 63) == 0) counts[waveId] = c;
+    return c;
+}
+
+// Pass 1: hits per segment (at most 127, one byte each) and per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void detect_count_kernel(const T* __restrict__ data, int nsamp, int nSeg,
+                                                           int total, int P, Thresholds th, int offset,
+                                                           float scale, uint8_t* __restrict__ counts,
+                                                           int* __restrict__ blockSums)
+{
+    __shared__ int part[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = 1 << P;
+    const int first = blockIdx.x * kSegPerBlock + wave * kSegPerWave;
+    int dm = first / nSeg, seg = first - dm * nSeg;
+    float v[kSegPerWave];
+#pragma unroll
+    for (int k = 0; k < kSegPerWave; ++k) {
+        v[k] = 0.0f;
+        if (first + k < total && lane < W)
+            v[k] = sample_value<T>(data, (long)dm * nsamp + (long)seg * W + lane, offset, scale);
+        if (++seg == nSeg) {
+            seg = 0;
+            ++dm;
+        }
+    }
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSegPerWave; ++k) {
+        if (first + k < total) {
+            const int c = segment_hits(v[k], lane, W, P, th);
+            if (lane == 0) counts[first + k] = (uint8_t)c;
+            sum += c;
+        }
+    }
+    if (lane == 0) part[wave] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Pass 2 (one workgroup): exclusive prefix of the workgroup sums, in place,
+// 4096 at a time (four per thread), with off[nBlocks] = their total;
+// *evTotal = 1 + total, and the dummy event (0, 0, 1, 0) in slot 0
+// (DedispersionAnalyser.cpp:69).
+__global__ __launch_bounds__(1024) void detect_scan_kernel(int* __restrict__ off, int nBlocks,
+                                                           int* __restrict__ evTotal, int maxEvents,
+                                                           int* __restrict__ evInts, float* __restrict__ evVals)
+{
+    __shared__ int waveTot[16];
+    __shared__ int chunkTot;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < nBlocks; base += 4096) {
+        const int i0 = base + 4 * t;
+        int x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i0 + k < nBlocks ? off[i0 + k] : 0;
+        const int s = x[0] + x[1] + x[2] + x[3];
+        int incl = s;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += y;
+        }
+        if (lane == 63) waveTot[wave] = incl;
+        __syncthreads();
+        if (wave == 0) {
+            const int w = lane < 16 ? waveTot[lane] : 0;
+            int wi = w;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const int y = __shfl_up(wi, d, 64);
+                if (lane >= d) wi += y;
+            }
+            if (lane < 16) waveTot[lane] = wi - w;
+            if (lane == 15) chunkTot = wi;
+        }
+        __syncthreads();
+        int e = carry + waveTot[wave] + incl - s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < nBlocks) off[i0 + k] = e;
+            e += x[k];
+        }
+        carry += chunkTot;
+        __syncthreads();
+    }
+    if (t == 0) {
+        off[nBlocks] = carry;
+        *evTotal = 1 + carry;
+        if (maxEvents > 0) {
+            evInts[0] = 0;
+            evInts[1] = 0;
+            evInts[2] = 1;
+            evVals[0] = 0.0f;
+        }
+    }
+}
+
+// Pass 3: the events of every segment with hits, at 1 + its offset (slot 0
+// holds the dummy event).
+template <typename T>
+__global__ __launch_bounds__(256) void detect_write_kernel(const T* __restrict__ data, int nsamp, int nSeg,
+                                                           int total, int P, Thresholds th, int offset,
+                                                           float scale, const uint8_t* __restrict__ counts,
+                                                           const int* __restrict__ off, int maxEvents,
+                                                           int* __restrict__ evInts, float* __restrict__ evVals)
+{
+    const int b = blockIdx.x;
+    const int off0 = off[b];
+    if (off[b + 1] == off0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int blockFirst = b * kSegPerBlock;
+    // lane l: hits of segment blockFirst + l and the hits before it in the workgroup
+    const int c = blockFirst + lane < total ? (int)counts[blockFirst + lane] : 0;
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int excl = incl - c;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k = 0; k < kSegPerWave; ++k) {
+        const int l = wave * kSegPerWave + k;
+        if (__shfl(c, l, 64) == 0) continue;
+        const int id = blockFirst + l;
+        const int dm = id / nSeg, seg = id - dm * nSeg;
+        uint64_t masks[kMaxLevels];
+        float vals[kMaxLevels];
+        segment_masks<T>(data, (long)dm * nsamp, seg, P, th, offset, scale, masks, vals);
+        int base = 1 + off0 + __shfl(excl, l, 64);
+        for (int n = 0; n <= P; ++n) {
+            const uint64_t m = masks[n];
+            if ((m >> lane) & 1ull) {
+                // bins in order of j = lane >> n: the rank among the mask's lanes
+                const int e = base + __popcll(m & below);
+                if (e < maxEvents) {
+                    evInts[3 * e + 0] = dm;
+                    evInts[3 * e + 1] = seg * (1 << P) + lane;  // binFactor * j
+                    evInts[3 * e + 2] = 1 << n;
+                    evVals[e] = vals[n];
+                }
+            }
+            base += __popcll(m);
+        }
+    }
+}
+
+// The reference's thresholds, formed exactly as it forms them (float products).
+Thresholds make_thresholds(int nChannels, int nSubbands, float threshold, unsigned binPow2)
+{
+    Thresholds th{};
+    const float rms = std::sqrt((float)nChannels * (float)nSubbands);
+    th.v[0] = threshold * rms;
+    for (unsigned n = 1; n <= binPow2; ++n) {
+        const int binFactor = 1 << n;
+        th.v[n] = threshold * rms * std::sqrt((float)binFactor);
+    }
+    return th;
+}
+
+// The three passes on a stream, into device-accessible event buffers of
+// maxEvents slots and a device-accessible total; scratch holds
+// detect_scratch_bytes(total) bytes.
+size_t detect_scratch_bytes(long total)
+{
+    const long nBlocks = (total + kSegPerBlock - 1) / kSegPerBlock;
+    return (size_t)((total + 255) / 256 * 256) + sizeof(int) * (size_t)(nBlocks + 1);
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void detect_write_kernel(const T* __restrict__ data, int tdms, int nsamp,
-                                                           int P, Thresholds th, int offset, float scale,
-                                                           const int* __restrict__ counts,
-                                                           const int* __restrict__ offsets, int maxEvents,
-                                                           int* __restrict__ evInts, float* __restrict__ evVals)
+int launch_detect(const T* d_data, int tdms, int nsamp, int offset, float scale, const Thresholds& th, int P,
+                  void* scratch, int32_t* evInts, float* evVals, int maxEvents, int* evTotal, hipStream_t st)
 {
-    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nSeg = nsamp >> P;
-    const long total = (long)tdms * nSeg;
-    if (waveId >= total) return;
-    if (counts[waveId] == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int dm = (int)(waveId / nSeg), seg = (int)(waveId % nSeg);
-    uint64_t masks[kMaxLevels];
-    float vals[kMaxLevels];
-    segment_masks<T>(data, (long)dm * nsamp, seg, P, th, offset, scale, masks, vals);
-    int base = offsets[waveId] + 1;  // slot 0 holds the dummy event
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int n = 0; n <= P; ++n) {
-        const uint64_t m = masks[n];
-        if ((m >> lane) & 1ull) {
-            // bins in order of j = lane >> n: the rank among the mask's lanes
-            const int k = base + __popcll(m & below);
-            if (k < maxEvents) {
-                evInts[3 * k + 0] = dm;
-                evInts[3 * k + 1] = seg * (1 << P) + lane;  // binFactor * j
-                evInts[3 * k + 2] = 1 << n;
-                evVals[k] = vals[n];
-            }
-        }
-        base += __popcll(m);
+    const int total = tdms * nSeg;
+    const int nBlocks = (total + kSegPerBlock - 1) / kSegPerBlock;
+    uint8_t* counts = static_cast<uint8_t*>(scratch);
+    int* off = reinterpret_cast<int*>(counts + (size_t)(total + 255) / 256 * 256);
+    if (total > 0) {
+        hipLaunchKernelGGL(detect_count_kernel<T>, dim3(nBlocks), dim3(256), 0, st, d_data, nsamp, nSeg, total, P,
+                           th, offset, scale, counts, off);
+        PPL_LAUNCHED();
     }
+    hipLaunchKernelGGL(detect_scan_kernel, dim3(1), dim3(1024), 0, st, off, nBlocks, evTotal, maxEvents, evInts,
+                       evVals);
+    PPL_LAUNCHED();
+    if (total == 0) return PPL_OK;  // the dummy event alone
+    hipLaunchKernelGGL(detect_write_kernel<T>, dim3(nBlocks), dim3(256), 0, st, d_data, nsamp, nSeg, total, P, th,
+                       offset, scale, counts, off, maxEvents, evInts, evVals);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
+// Pinned host staging for the events of ppl_detect / ppl_detect_u8: the
+// write pass stores straight into it, so a call synchronises once.  It grows
+// to the largest max_events asked for and is shared by the calls of a
+// process under a lock (each call is synchronous).
+struct Staging {
+    std::mutex mu;
+    void* host = nullptr;
+    size_t cap = 0;  // event slots
+};
+Staging g_staging;
+
+int staging_reserve(size_t events)
+{
+    if (events <= g_staging.cap && g_staging.host) return PPL_OK;
+    if (g_staging.host) PPL_HIP(hipHostFree(g_staging.host));
+    g_staging.host = nullptr;
+    g_staging.cap = 0;
+    const size_t bytes = 256 + events * (3 * sizeof(int32_t) + sizeof(float));
+    PPL_HIP(hipHostMalloc(&g_staging.host, bytes, hipHostMallocDefault));
+    g_staging.cap = events;
+    return PPL_OK;
 }
 
 template <typename T>
@@ -143,65 +332,38 @@ int run_detect(const T* d_data, int tdms, int nsamp, int offset, float scale, in
                float threshold, unsigned binPow2, int32_t* h_ints, float* h_vals, int max_events, int* count,
                hipStream_t st)
 {
-    PPL_CHECK_ARG(d_data && count, "null argument");
+    PPL_CHECK_ARG(count, "null argument");
     PPL_CHECK_ARG(binPow2 <= 6, "power2ForBinning %u > 6 is not supported", binPow2);
     PPL_CHECK_ARG(tdms >= 0 && nsamp >= 0, "negative shape");
     PPL_CHECK_ARG(max_events <= 0 || (h_ints && h_vals), "null event buffers");
-    // thresholds exactly as the reference forms them (float products)
-    Thresholds th;
-    const float rms = std::sqrt((float)nChannels * (float)nSubbands);
-    th.v[0] = threshold * rms;
-    for (unsigned n = 1; n <= binPow2; ++n) {
-        const int binFactor = 1 << n;
-        th.v[n] = threshold * rms * std::sqrt((float)binFactor);
-    }
     const int P = (int)binPow2;
-    const int nSeg = nsamp >> P;
-    const long total = (long)tdms * nSeg;
-    int nEvents = 1;
-    if (max_events > 0) {
-        h_ints[0] = 0;
-        h_ints[1] = 0;
-        h_ints[2] = 1;
-        h_vals[0] = 0.0f;
-    }
-    if (total > 0) {
-        int *d_counts = nullptr, *d_offsets = nullptr, *d_ints = nullptr;
-        float* d_vals = nullptr;
-        void* d_tmp = nullptr;
-        size_t tmpBytes = 0;
-        PPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, d_counts, d_offsets, (int)total, st));
-        PPL_HIP(hipMallocAsync((void**)&d_counts, sizeof(int) * (total + 1), st));
-        PPL_HIP(hipMallocAsync((void**)&d_offsets, sizeof(int) * (total + 1), st));
-        PPL_HIP(hipMallocAsync(&d_tmp, tmpBytes + 16, st));
-        const unsigned blocks = (unsigned)((total + 3) / 4);
-        hipLaunchKernelGGL(detect_count_kernel<T>, dim3(blocks), dim3(256), 0, st, d_data, tdms, nsamp, P, th,
-                           offset, scale, d_counts);
-        PPL_LAUNCHED();
-        PPL_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmpBytes, d_counts, d_offsets, (int)total, st));
-        int last[2] = {0, 0};
-        PPL_HIP(hipMemcpyAsync(&last[0], d_offsets + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        PPL_HIP(hipMemcpyAsync(&last[1], d_counts + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        PPL_HIP(hipStreamSynchronize(st));
-        const int found = last[0] + last[1];
-        nEvents = 1 + found;
-        const int cap = std::min(nEvents, std::max(max_events, 0));
-        if (found > 0 && cap > 1) {
-            PPL_HIP(hipMallocAsync((void**)&d_ints, sizeof(int32_t) * 3 * cap, st));
-            PPL_HIP(hipMallocAsync((void**)&d_vals, sizeof(float) * cap, st));
-            hipLaunchKernelGGL(detect_write_kernel<T>, dim3(blocks), dim3(256), 0, st, d_data, tdms, nsamp, P,
-                               th, offset, scale, d_counts, d_offsets, cap, d_ints, d_vals);
-            PPL_LAUNCHED();
-            PPL_HIP(hipMemcpyAsync(h_ints + 3, d_ints + 3, sizeof(int32_t) * 3 * (cap - 1), hipMemcpyDeviceToHost,
-                                   st));
-            PPL_HIP(hipMemcpyAsync(h_vals + 1, d_vals + 1, sizeof(float) * (cap - 1), hipMemcpyDeviceToHost, st));
-            PPL_HIP(hipFreeAsync(d_ints, st));
-            PPL_HIP(hipFreeAsync(d_vals, st));
-        }
-        PPL_HIP(hipFreeAsync(d_counts, st));
-        PPL_HIP(hipFreeAsync(d_offsets, st));
-        PPL_HIP(hipFreeAsync(d_tmp, st));
-        PPL_HIP(hipStreamSynchronize(st));
+    const long total = (long)tdms * (nsamp >> P);
+    PPL_CHECK_ARG(d_data || total == 0, "null argument");
+    PPL_CHECK_ARG(total < (1L << 30), "%ld segments (at most 2^30)", total);
+    const Thresholds th = make_thresholds(nChannels, nSubbands, threshold, binPow2);
+    std::lock_guard<std::mutex> lock(g_staging.mu);
+    const size_t slots = (size_t)std::max(max_events, 1);
+    int rc = staging_reserve(slots);
+    if (rc != PPL_OK) return rc;
+    int* hTotal = static_cast<int*>(g_staging.host);
+    int32_t* sInts = reinterpret_cast<int32_t*>(static_cast<char*>(g_staging.host) + 256);
+    float* sVals = reinterpret_cast<float*>(sInts + 3 * slots);
+    void *dTotal = nullptr, *dInts = nullptr, *dVals = nullptr;
+    PPL_HIP(hipHostGetDevicePointer(&dTotal, hTotal, 0));
+    PPL_HIP(hipHostGetDevicePointer(&dInts, sInts, 0));
+    PPL_HIP(hipHostGetDevicePointer(&dVals, sVals, 0));
+    void* scratch = nullptr;
+    PPL_HIP(hipMallocAsync(&scratch, detect_scratch_bytes(total), st));
+    rc = launch_detect<T>(d_data, tdms, nsamp, offset, scale, th, P, scratch, static_cast<int32_t*>(dInts),
+                          static_cast<float*>(dVals), std::max(max_events, 0), static_cast<int*>(dTotal), st);
+    PPL_HIP(hipFreeAsync(scratch, st));
+    if (rc != PPL_OK) return rc;
+    PPL_HIP(hipStreamSynchronize(st));
+    const int nEvents = *hTotal;
+    const int n = std::min(nEvents, std::max(max_events, 0));
+    if (n > 0) {
+        std::memcpy(h_ints, sInts, sizeof(int32_t) * 3 * (size_t)n);
+        std::memcpy(h_vals, sVals, sizeof(float) * (size_t)n);
     }
     *count = nEvents;
     return PPL_OK;

@@ -26,8 +26,8 @@ def test_delay_table_matches_oracle(orc, ppl):
 
 
 @pytest.mark.parametrize("tile", [None, (32, 256), (16, 512), (8, 512), (16, 256), (64, 256),
-                                  (16, 512, 0, 16), (32, 512, 0, 16), (64, 512, 0, 16), (16, 1024, 0, 16),
-                                  (32, 1024, 1, 16), (32, 256, 0, 16), (64, 256, 2, 16)])
+                                  (16, 512, 0, 16), (32, 512, 0, 16), (32, 512, 1, 16), (16, 256, 0, 16),
+                                  (32, 256, 0, 16)])
 def test_f32_bit_exact_small(orc, ppl, tile):
     rng = np.random.default_rng(11)
     x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
@@ -188,14 +188,19 @@ def test_linearity_full_size(ppl):
 
 @pytest.mark.parametrize("dtype", ["f32", "f16", "u8"])
 @pytest.mark.parametrize("tile", [(32, 512, 2), (64, 256, 4), (16, 512, 8), (32, 256, 16), (8, 512, 0),
-                                  (64, 128, 3), (16, 512, 3, 16), (32, 512, 1, 16), (64, 256, 0, 16),
-                                  (16, 1024, 2, 16), (32, 1024, 0, 16), (32, 256, 5, 16), (64, 256, 1, 16)])
+                                  (64, 128, 3), (16, 512, 3, 16), (32, 512, 1, 16), (32, 512, 0, 16),
+                                  (16, 256, 0, 16), (32, 256, 0, 16)])
 def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
     """Every instantiated tile, channel block and read width, odd channel
     counts and odd delay parities (the FP16 form reads pairs from a shifted
-    copy, the 16-byte FP32 form from four copies shifted by 0..3)."""
-    cfg = dict(n_chan=53, nsamp_buf=3000, tdms=77, dm_low=0.0, dm_step=0.9, tsamp=0.00032768,
-               fch1=150.0, foff=-0.05)
+    copy, the 16-byte FP32 form from four copies shifted by 0..3, staged
+    through its four-stage ring: 53 channels end part way through a round
+    of four stages)."""
+    # 18 samples of delay per trial in the lowest channel; a third of that for
+    # the 16-byte form, whose ring stage holds a spread of up to 256 samples
+    # at 512-sample tiles
+    cfg = dict(n_chan=53, nsamp_buf=3000, tdms=77, dm_low=0.0, dm_step=0.3 if len(tile) == 4 else 0.9,
+               tsamp=0.00032768, fch1=150.0, foff=-0.05)
     rng = np.random.default_rng(hash(tile) % 1000)
     x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
     p, d = _oracle(orc, x, cfg)
@@ -222,15 +227,19 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
 
 
 def test_wide_reads_reject_a_spread_too_wide(ppl):
-    """A DM tile whose delay spread (about 9200 samples over 64 trials) needs
-    a staged row longer than two LDS buffers of 64 KB hold is refused, not
-    truncated."""
+    """A DM tile whose delay spread (about 4600 samples over 32 trials) needs
+    a staged row longer than a ring stage holds (768 samples a copy) is
+    refused, not truncated; 16-byte tiles that are not instantiated are
+    refused when set."""
     cfg = dict(n_chan=64, nsamp_buf=16384, tdms=64, dm_low=0.0, dm_step=6.0, tsamp=0.00032768,
                fch1=150.0, foff=-0.05)
     plan = ppl.DedispPlan(**cfg)
-    plan.set_tile(64, 512, 0, 16)
+    plan.set_tile(32, 512, 0, 16)
     with pytest.raises(ppl.PPLError):
         plan.run(torch.zeros((64, 16384), device="cuda"))
+    for tile in [(64, 512), (64, 256), (16, 1024), (8, 512)]:
+        with pytest.raises(ppl.PPLError, match="not instantiated"):
+            plan.set_tile(*tile, 0, 16)
 
 
 def test_autotuner_keeps_results_exact(orc, ppl):
@@ -277,7 +286,11 @@ def test_batch_equals_separate_runs(ppl, dtype):
     single = [p.run(x) for p, x in zip(plans, ins)]
     tiles = [(0, 0, 0), (16, 256, 4), (32, 512, 2)]
     if dtype == "f32":
-        tiles += [(32, 512, 2, 16), (64, 256, 0, 16)]
+        # the 31x16-channel beam spreads 292 samples over 32 trials: more than
+        # a 512-sample tile's ring stage holds, so that tile is refused
+        tiles += [(16, 512, 0, 16), (16, 256, 0, 16), (32, 256, 0, 16)]
+        with pytest.raises(ppl.PPLError, match="too large"):
+            ppl.dedisperse_batch(plans, ins, tile=(32, 512, 2, 16))
     else:
         with pytest.raises(ppl.PPLError):
             ppl.dedisperse_batch(plans, ins, tile=(32, 512, 2, 16))

@@ -67,3 +67,31 @@ def test_reference_single_event(orc, ppl):
     d = orc.dedisp_delays(p["shift"], tdms, p["mstartdm"], p["mdmstep"])
     ev_o, vals_o = orc.detect(orc.dedisperse(buf, d, p["nout"]), n_ch, n_sub, 6.0, 6)
     assert np.array_equal(ev, ev_o) and np.array_equal(vals, vals_o)
+
+
+@pytest.mark.parametrize("threshold", [2.5, 4.0])
+def test_events_identical_many_workgroups(orc, ppl, threshold):
+    """Low thresholds on noise: hits in most segments of 512 DM rows x 600
+    segments (4800 workgroups of 64 segments, more than one 4096-entry round
+    of the workgroup scan), and a truncated list of the same events."""
+    x = _data(512, 64 * 600 + 37, 16, 8, 5)
+    ev_o, val_o = orc.detect(x, 16, 8, threshold, 6, max_events=1 << 22)
+    d = torch.from_numpy(x).cuda()
+    ev_g, val_g, n = ppl.detect(d, 16, 8, threshold, 6, max_events=1 << 22)
+    assert n == len(ev_o) > (100000 if threshold < 3 else 1000)
+    assert np.array_equal(ev_g, ev_o)
+    assert np.array_equal(val_g, val_o)
+    ev_t, val_t, n_t = ppl.detect(d, 16, 8, threshold, 6, max_events=777)
+    assert n_t == n and np.array_equal(ev_t, ev_o[:777]) and np.array_equal(val_t, val_o[:777])
+
+
+def test_count_only_and_empty_shapes(orc, ppl):
+    """max_events = 0 returns the count alone; no DM rows, or rows shorter
+    than one segment, give the dummy event alone."""
+    x = _data(16, 64 * 8, 16, 8, 2, [(3, 10, 30, 30)])
+    ev_o, _ = orc.detect(x, 16, 8)
+    _, _, n = ppl.detect(torch.from_numpy(x).cuda(), 16, 8, max_events=0)
+    assert n == len(ev_o)
+    for shape in [(0, 640), (4, 63)]:
+        ev, val, n = ppl.detect(torch.zeros(shape, device="cuda"), 16, 8)
+        assert n == 1 and tuple(ev[0]) == (0, 0, 1) and val[0] == 0.0
