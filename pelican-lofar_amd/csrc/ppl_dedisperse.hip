@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
     }
 }
 
-// FP32 input, 16-byte LDS reads from a four-stage ring of staged rows.
+// FP32 input, 16-byte LDS reads from a ring of staged rows.
 // gfx950 serves ds_read_b128 at about twice the bytes per clock of
 // ds_read_b32 (228 against 114 B/clk/CU measured), so lane l accumulates
 // four consecutive samples t0 + 4l + 256q + {0,1,2,3} (q < NQ) from one
@@ -239,25 +239,33 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 // in channel order, one FP32 add per sample, so the result is bit-identical
 // to the one-sample kernel.
 //
-// One channel per ring stage.  Wave s loads copy s of the row (from element
-// s on: the same cache lines for all four) by LDS DMA, always kRingDw
-// dwords (kRingDma instructions), plus one gather instruction that brings
-// the wave's ND read offsets of the channel and the first-trial delay of
-// the channel three ahead, from a per-plan ring table (offsets already in
-// LDS bytes).  So every wave has exactly kRingDma + 1 DMA instructions per
-// channel in flight, and at the top of channel c it waits (explicit vmcnt)
-// only for its DMA into stage c % 4; the workgroup then meets at a bare
-// barrier (every wave's part of stage c has landed, every wave is done with
-// channel c-1), and the DMA of channel c+3 goes into the stage channel c-1
-// used: three channels of DMA lead.  No other vector memory operation is
-// issued in the channel loop, and the compiler's own fence before
-// __syncthreads (a wait for every DMA in flight) is kept out with an
-// s_barrier in inline assembly, which also keeps the compiler from moving
-// LDS accesses across it.
+// One channel per ring stage, DEPTH stages.  Wave s loads copy s of the row
+// (from element s on: the same cache lines for all four) by LDS DMA, always
+// kRingDw dwords (kRingDma instructions), plus one gather instruction that
+// brings the wave's ND read offsets of the channel from a per-plan ring
+// table (offsets already in LDS bytes).  So every wave has exactly
+// kRingDma + 1 DMA instructions per channel in flight, and at the top of
+// channel c it waits (explicit vmcnt) only for its DMA into c's stage; the
+// workgroup then meets at a bare barrier (every wave's part of the stage
+// has landed, every wave is done with channel c-1), and the DMA of channel
+// c + DEPTH - 1 goes into the stage channel c-1 used.  The first-trial delay
+// that places a row comes by scalar load (lgkmcnt), a channel ahead.  No
+// other vector memory operation is issued in the channel loop.  The
+// compiler's own fence before __syncthreads (a wait for every DMA in
+// flight) is kept out with an s_barrier in inline assembly, which also
+// keeps the compiler from moving LDS accesses across it.  The instruction
+// mix per channel and wave (counters on the LOFAR beam): 64 adds, 8
+// address adds, about 6 other VALU instructions; VALU is the bound.
 constexpr int kRingQuads = 192;           // float4 per copy of a staged row
 constexpr int kRingDw = 4 * kRingQuads;   // dwords per copy
 constexpr int kRingDma = kRingDw / 64;    // DMA instructions per copy
 constexpr int kRingStage = 4 * kRingQuads * 16 + 4 * 64;  // four copies, then 64 bytes of offsets per wave
+// Three stages (two channels of DMA lead): 37 KB of LDS a workgroup, so
+// four workgroups (16 waves) share a CU; four stages (49 KB, three
+// workgroups) measured 10 % slower on the LOFAR beam.
+#ifndef PPL_RING_DEPTH
+#define PPL_RING_DEPTH 3
+#endif
 
 // s_waitcnt bounding only the vector memory counter (gfx9 encoding:
 // vmcnt[3:0] in bits 3:0, vmcnt[5:4] in bits 15:14; expcnt and lgkmcnt at
@@ -270,6 +278,38 @@ __device__ __forceinline__ void wait_vmcnt()
 }
 
 __device__ __forceinline__ void bare_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// kRingDma LDS-DMA dword loads of consecutive 64-dword rows: row k from
+// global address g + 256k to LDS address lds + 256k (+ 4 * lane each).  The
+// instruction offset applies to both addresses, so one address register and
+// one M0 serve all twelve.  M0 is restored before the block ends.  The
+// compiler's hazard checks do not look into inline assembly, so the wait
+// state between the M0 write and the first DMA is written out.
+__device__ __forceinline__ void dma_rows12(const float* g, uint32_t lds)
+{
+    static_assert(kRingDma == 12, "twelve rows");
+    uint32_t saved;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %2\n\t"
+                 "s_nop 0\n\t"  // a scalar write of M0 needs one wait state before an LDS DMA reads it
+                 "global_load_lds_dword %1, off\n\t"
+                 "global_load_lds_dword %1, off offset:256\n\t"
+                 "global_load_lds_dword %1, off offset:512\n\t"
+                 "global_load_lds_dword %1, off offset:768\n\t"
+                 "global_load_lds_dword %1, off offset:1024\n\t"
+                 "global_load_lds_dword %1, off offset:1280\n\t"
+                 "global_load_lds_dword %1, off offset:1536\n\t"
+                 "global_load_lds_dword %1, off offset:1792\n\t"
+                 "global_load_lds_dword %1, off offset:2048\n\t"
+                 "global_load_lds_dword %1, off offset:2304\n\t"
+                 "global_load_lds_dword %1, off offset:2560\n\t"
+                 "global_load_lds_dword %1, off offset:2816\n\t"
+                 "s_nop 0\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved)
+                 : "v"(g), "s"(lds)
+                 : "memory");
+}
 
 // LDS address (byte offset in the workgroup's LDS) of a __shared__ pointer.
 __device__ __forceinline__ uint32_t lds_addr(const void* p)
@@ -317,14 +357,14 @@ __device__ __forceinline__ void lds_read_4x2(uint32_t a0, uint32_t a1, uint32_t 
                  : "memory");
 }
 
-template <int ND, int NQ>
+template <int ND, int NQ, int DEPTH>
 __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
 {
     static_assert(ND <= 16, "a wave's offsets fill 64 bytes");
-    __shared__ __align__(16) unsigned char ring0[kRingStage];
-    __shared__ __align__(16) unsigned char ring1[kRingStage];
-    __shared__ __align__(16) unsigned char ring2[kRingStage];
-    __shared__ __align__(16) unsigned char ring3[kRingStage];
+    static_assert(DEPTH >= 3 && (DEPTH - 2) * (kRingDma + 1) < 64, "ring depth");
+    // The stages are read only by inline assembly and written only by DMA,
+    // so one array holds them all.
+    __shared__ __align__(16) unsigned char ring[DEPTH][kRingStage];
     int local;
     const DDJob& J = find_job(B, local);
     const float* __restrict__ in = static_cast<const float*>(J.in);
@@ -362,16 +402,15 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
 
     // channel c into a stage: copy `wave` of the row (kRingDma instructions,
     // each lane's address clamped to the input rather than masked, so the
-    // count is fixed), then the gather of this wave's offsets of channel c
-    // and dmin of channel c + 3 (lanes 0..ND).
+    // count is fixed), then the gather of this wave's read offsets of the
+    // channel (lanes 0..ND-1).  Past the last channel: a dummy stage (the
+    // last channel again), so that every channel of the loop issues one.
     auto stage = [&](unsigned char* buf, int c, int dmin) {
-        c = min(c, nChan - 1);  // past the last channel: a dummy stage, so the count stays fixed
+        c = min(c, nChan - 1);
         const int row0 = c * nsamp + t0 + dmin + wave;  // uniform
         float* dst = reinterpret_cast<float*>(buf + wave * kRingQuads * 16);
         if (row0 + kRingDw - 1 <= lastElem) {
-            const float* src = in + row0 + lane;
-#pragma unroll
-            for (int k = 0; k < kRingDma; ++k) __builtin_amdgcn_global_load_lds(src + 64 * k, dst + 64 * k, 4, 0, 0);
+            dma_rows12(in + row0 + lane, lds_addr(dst));
         } else {
 #pragma unroll
             for (int k = 0; k < kRingDma; ++k)
@@ -406,35 +445,25 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
                 }
         }
     };
-    // top of a channel: this wave's DMA into the channel's stage has landed
-    // (the DMA of the next two stages may be in flight), then the barrier.
-    // Every channel issues one stage, past the last channel a dummy one
-    // (clamped to the last channel), so the count is the same on every path.
-    auto top = [&]() {
-        wait_vmcnt<2 * (kRingDma + 1)>();
-        bare_barrier();
-    };
     auto dmin_of = [&](int c) { return dminTab[min(c, nChan - 1)]; };
 
-    stage(ring0, 0, dmin_of(0));
-    stage(ring1, 1, dmin_of(1));
-    stage(ring2, 2, dmin_of(2));
-    for (int c = 0; c < nChan; c += 4) {
-        top();
-        stage(ring3, c + 3, dmin_of(c + 3));
-        compute(ring0);
-        if (c + 1 >= nChan) break;
-        top();
-        stage(ring0, c + 4, dmin_of(c + 4));
-        compute(ring1);
-        if (c + 2 >= nChan) break;
-        top();
-        stage(ring1, c + 5, dmin_of(c + 5));
-        compute(ring2);
-        if (c + 3 >= nChan) break;
-        top();
-        stage(ring2, c + 6, dmin_of(c + 6));
-        compute(ring3);
+    // prologue: channels 0 .. DEPTH-2 into their stages
+#pragma unroll
+    for (int k = 0; k < DEPTH - 1; ++k) stage(ring[k], k, dmin_of(k));
+    // dmin of the channel staged at the next top, loaded a channel ahead
+    int nextDmin = dmin_of(DEPTH - 1);
+    int slot = 0;  // stage of channel c
+    for (int c = 0; c < nChan; ++c) {
+        // this wave's DMA into channel c's stage has landed (the DMA of the
+        // next DEPTH-2 stages may be in flight); then the barrier: every
+        // wave's part has landed and every wave is done with channel c-1
+        wait_vmcnt<(DEPTH - 2) * (kRingDma + 1)>();
+        bare_barrier();
+        const int free = slot == 0 ? DEPTH - 1 : slot - 1;  // channel c-1's stage
+        stage(ring[free], c + DEPTH - 1, nextDmin);
+        nextDmin = dmin_of(c + DEPTH);
+        compute(ring[slot]);
+        slot = slot + 1 == DEPTH ? 0 : slot + 1;
     }
     // no LDS DMA may land after the workgroup has ended
     wait_vmcnt<0>();
@@ -1018,7 +1047,7 @@ int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, 
         if (s.rw == 16) {
 #define PPL_DDX4_CASE(ND_, NT_)                                                                 \
     if (s.nd == ND_ && s.nt == NT_) {                                                           \
-        hipLaunchKernelGGL((dedisperse_ring_kernel<ND_, NT_ / 4>), grid, block, 0, stream, B); \
+        hipLaunchKernelGGL((dedisperse_ring_kernel<ND_, NT_ / 4, PPL_RING_DEPTH>), grid, block, 0, stream, B); \
         PPL_LAUNCHED();                                                                         \
         return PPL_OK;                                                                          \
     }
