@@ -205,6 +205,22 @@ unsigned ppl_ddbuf_fill(const ppl_ddbuf* buf);
 int ppl_detect(const float* d_data, int tdms, int nsamp, int nChannels, int nSubbands,
                float threshold, unsigned binPow2, int32_t* h_ev_ints, float* h_ev_vals,
                int max_events, int* count, void* stream);
+/* No counterpart in the reference: ppl_detect for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same events in the same order, written to device
+ * memory: d_ev_ints [max_events][3], d_ev_vals [max_events], and the total
+ * number of events to the device int *d_count.  The per-call scratch (a hit
+ * count per segment, the workgroup offsets) is a static device array of the
+ * library, about 8.5 MB per device, held until the process ends; nothing is
+ * allocated and nothing synchronises (the results are ready when the stream
+ * reaches them).  tdms * (nsamp >> binPow2) may not exceed
+ * PPL_DETECT_NOALLOC_MAX_SEGMENTS (PPL_E_ARG).  Every call on a device shares
+ * that device's scratch: calls that overlap on one device (other streams,
+ * other threads) race. */
+enum { PPL_DETECT_NOALLOC_MAX_SEGMENTS = 1 << 23 };
+int ppl_detect_noalloc(const float* d_data, int tdms, int nsamp, int nChannels, int nSubbands,
+                       float threshold, unsigned binPow2, int32_t* d_ev_ints, float* d_ev_vals,
+                       int max_events, int* d_count, void* stream);
 /* Same over int32 u8-mode sums, dequantised as (S - 128*nChan) * scale. */
 int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float scale,
                   int nChannels, int nSubbands, float threshold, unsigned binPow2,

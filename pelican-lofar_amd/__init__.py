@@ -109,6 +109,7 @@ SIGNATURES = {
     "ppl_ddbuf_data": (vp, [vp]),
     "ppl_ddbuf_fill": (u, [vp]),
     "ppl_detect": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
+    "ppl_detect_noalloc": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_detect_u8": (i, [vp, i, i, i, f, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_dequantise_sums": (i, [vp, vp, sz, u, f, vp]),
     "ppl_ingest_c32": (i, [vp, sz, u, u, u, u, u, i, vp, vp]),
@@ -429,6 +430,19 @@ def detect(d_data, n_channels, n_subbands, threshold=6.0, bin_pow2=6, max_events
                            max_events, C.byref(count), _stream(stream)))
     n = min(count.value, max_events)
     return ints[:n].copy(), vals[:n].copy(), count.value
+
+
+DETECT_NOALLOC_MAX_SEGMENTS = 1 << 23
+
+
+def detect_noalloc(d_data, n_channels, n_subbands, d_ints, d_vals, d_count, threshold=6.0, bin_pow2=6,
+                   stream=None):
+    """ppl_detect_noalloc: detection into device tensors d_ints [max][3]
+    int32, d_vals [max] float32 and d_count [1] int32, with the library's
+    static scratch (nothing allocated, nothing synchronised)."""
+    tdms, nsamp = d_data.shape
+    check(lib().ppl_detect_noalloc(_ptr(d_data), tdms, nsamp, n_channels, n_subbands, threshold, bin_pow2,
+                                   _ptr(d_ints), _ptr(d_vals), d_vals.shape[0], _ptr(d_count), _stream(stream)))
 
 
 def detect_u8(d_data, n_chan, scale, n_channels, n_subbands, threshold=6.0, bin_pow2=6,

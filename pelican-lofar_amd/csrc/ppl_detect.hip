@@ -304,6 +304,39 @@ int launch_detect(const T* d_data, int tdms, int nsamp, int offset, float scale,
     return PPL_OK;
 }
 
+// Scratch of ppl_detect_noalloc: the hit count of every segment (a byte
+// each) and the workgroup offsets, for up to PPL_DETECT_NOALLOC_MAX_SEGMENTS
+// segments.  Every device the library runs on holds its own copy until the
+// process ends.  Host code never passes the array's name to a kernel: it
+// uses the address hipGetSymbolAddress returns for the current device,
+// looked up once per device (at ppl_init, or at the first call on a device
+// that ppl_init did not set up).
+constexpr long kNoAllocSegments = PPL_DETECT_NOALLOC_MAX_SEGMENTS;
+constexpr size_t kNoAllocBytes =
+    (size_t)((kNoAllocSegments + 255) / 256 * 256) +
+    sizeof(int) * (size_t)((kNoAllocSegments + kSegPerBlock - 1) / kSegPerBlock + 1);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void* g_scratchPtr[kMaxDevices] = {};
+
+// The current device's copy of the scratch array.
+int scratch_of_current_device(void** out)
+{
+    int dev = 0;
+    PPL_HIP(hipGetDevice(&dev));
+    PPL_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        PPL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return PPL_OK;
+}
+
 // Pinned host staging for the events of ppl_detect / ppl_detect_u8: the
 // write pass stores straight into it, so a call synchronises once.  It grows
 // to the largest max_events asked for and is shared by the calls of a
@@ -377,6 +410,35 @@ extern "C" int ppl_detect(const float* d_data, int tdms, int nsamp, int nChannel
 {
     return run_detect<float>(d_data, tdms, nsamp, 0, 1.0f, nChannels, nSubbands, threshold, binPow2, h_ev_ints,
                              h_ev_vals, max_events, count, stream_of(stream));
+}
+
+// Set-up for the current device (ppl_init): looks up its scratch address.
+int ppl::detect_noalloc_setup()
+{
+    void* p = nullptr;
+    return scratch_of_current_device(&p);
+}
+
+extern "C" int ppl_detect_noalloc(const float* d_data, int tdms, int nsamp, int nChannels, int nSubbands,
+                                  float threshold, unsigned binPow2, int32_t* d_ev_ints, float* d_ev_vals,
+                                  int max_events, int* d_count, void* stream)
+{
+    PPL_CHECK_ARG(d_count, "null argument");
+    PPL_CHECK_ARG(binPow2 <= 6, "power2ForBinning %u > 6 is not supported", binPow2);
+    PPL_CHECK_ARG(tdms >= 0 && nsamp >= 0, "negative shape");
+    PPL_CHECK_ARG(max_events <= 0 || (d_ev_ints && d_ev_vals), "null event buffers");
+    const int P = (int)binPow2;
+    const long total = (long)tdms * (nsamp >> P);
+    PPL_CHECK_ARG(d_data || total == 0, "null argument");
+    PPL_CHECK_ARG(total <= kNoAllocSegments,
+                  "%ld segments of 2^%u samples exceed the %ld the no-allocation scratch holds", total, binPow2,
+                  kNoAllocSegments);
+    void* sc = nullptr;
+    const int rc = scratch_of_current_device(&sc);
+    if (rc != PPL_OK) return rc;
+    const Thresholds th = make_thresholds(nChannels, nSubbands, threshold, binPow2);
+    return launch_detect<float>(d_data, tdms, nsamp, 0, 1.0f, th, P, sc, d_ev_ints, d_ev_vals,
+                                std::max(max_events, 0), d_count, stream_of(stream));
 }
 
 extern "C" int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float scale, int nChannels,

@@ -50,4 +50,7 @@ inline unsigned cdiv(unsigned a, unsigned b) { return (a + b - 1) / b; }
 // Wave size on CDNA.
 constexpr int kWave = 64;
 
+// Looks up the current device's scratch of ppl_detect_noalloc (ppl_detect.hip).
+int detect_noalloc_setup();
+
 }  // namespace ppl

@@ -37,7 +37,7 @@ extern "C" int ppl_init(int device)
     }
     PPL_CHECK_ARG(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
     PPL_HIP(hipSetDevice(device));
-    return PPL_OK;
+    return detect_noalloc_setup();
 }
 
 extern "C" int ppl_shutdown(void)
