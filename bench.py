@@ -557,33 +557,9 @@ def precision(args, torch, ppl):
 
 
 def draw_beams(seed, n):
-    """The heterogeneous multi-beam workload: per beam 31-62 subbands of
-    195.3125 kHz below 150 MHz, 16/64/256 channels per subband, DM 0 to
-    100-1000 pc/cm^3 over 512-4096 trials, and a 2^14-2^17-sample
-    dedispersion buffer, raised to the next power of two while it holds
-    fewer than 1.25 x maxshift samples (a shorter buffer cannot hold the
-    beam's DM sweep)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    rng = np.random.default_rng(seed)
-    out = []
-    for b in range(n):
-        n_sub = int(rng.integers(31, 63))
-        n_ch = int(rng.choice([16, 64, 256]))
-        dm_hi = float(rng.choice([100, 200, 300, 500, 750, 1000]))
-        tdms = int(rng.choice([512, 1024, 2048, 4096]))
-        nsamp = 1 << int(rng.integers(14, 18))
-        tsamp = TS_IN * n_ch * INTEG
-        foff = -0.1953125 / n_ch
-        fmin = FCH1 + foff * (n_sub * n_ch - 1)
-        maxshift = 4148.741601 * (1.0 / fmin ** 2 - 1.0 / FCH1 ** 2) * dm_hi / tsamp
-        while nsamp < 1.25 * maxshift:
-            nsamp *= 2
-        n_pulse = int(rng.integers(1, 4))
-        pulses = [(float(rng.uniform(0.1, 0.9) * dm_hi), float(rng.uniform(0.0, 1.0)),
-                   float(rng.uniform(1e-3, 8e-3) / tsamp), float(rng.uniform(6.0, 12.0))) for _ in range(n_pulse)]
-        out.append(dict(n_sub=n_sub, n_ch=n_ch, dm_hi=dm_hi, tdms=tdms, nsamp=nsamp, seed=seed + 17 * b,
-                        pulses=pulses))
-    return out
+    """The heterogeneous multi-beam workload (tests/synth.py draw_multibeam)."""
+    from tests.synth import draw_multibeam
+    return draw_multibeam(seed, n, TS_IN, INTEG, FCH1)
 
 
 def multibeam(args, torch, ppl):

@@ -157,3 +157,33 @@ def generator_blob(dm, n_samples=6400, n_sub=32, n_ch=64, start_bin=0, width=10,
         if a < b:
             out[c, a:b] = 1.0
     return out
+
+
+def draw_multibeam(seed, n, ts_in=5.12e-6, integrate=4, fch1=150.0):
+    """The heterogeneous multi-beam workload: per beam 31-62 subbands of
+    195.3125 kHz below fch1, 16/64/256 channels per subband, DM 0 to
+    100-1000 pc/cm^3 over 512-4096 trials, a 2^14-2^17-sample dedispersion
+    buffer, raised to the next power of two while it holds fewer than
+    1.25 x maxshift samples (a shorter buffer cannot hold the beam's DM
+    sweep), and 1-3 pulses (DM, start as a fraction of a chunk, width in
+    integrated samples for 1-8 ms, 6-12 sigma)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n):
+        n_sub = int(rng.integers(31, 63))
+        n_ch = int(rng.choice([16, 64, 256]))
+        dm_hi = float(rng.choice([100, 200, 300, 500, 750, 1000]))
+        tdms = int(rng.choice([512, 1024, 2048, 4096]))
+        nsamp = 1 << int(rng.integers(14, 18))
+        tsamp = ts_in * n_ch * integrate
+        foff = -0.1953125 / n_ch
+        fmin = fch1 + foff * (n_sub * n_ch - 1)
+        maxshift = KDM * (1.0 / fmin ** 2 - 1.0 / fch1 ** 2) * dm_hi / tsamp
+        while nsamp < 1.25 * maxshift:
+            nsamp *= 2
+        n_pulse = int(rng.integers(1, 4))
+        pulses = [(float(rng.uniform(0.1, 0.9) * dm_hi), float(rng.uniform(0.0, 1.0)),
+                   float(rng.uniform(1e-3, 8e-3) / tsamp), float(rng.uniform(6.0, 12.0))) for _ in range(n_pulse)]
+        out.append(dict(n_sub=n_sub, n_ch=n_ch, dm_hi=dm_hi, tdms=tdms, nsamp=nsamp, seed=seed + 17 * b,
+                        pulses=pulses))
+    return out
