@@ -191,6 +191,15 @@ int ppl_ddbuf_add_blob(ppl_ddbuf* buf, const float* d_spectra, unsigned nT, unsi
 /* copy (:65-96): carries the overlap into the start and empties the rest. */
 int ppl_ddbuf_rotate(ppl_ddbuf* buf, void* stream);
 void* ppl_ddbuf_data(ppl_ddbuf* buf);
+/* u8 buffers only (NULL otherwise; no reference counterpart: the reference
+ * has no quantised mode): the device array [nsampBuf] of the offsets the
+ * current buffer's rows were quantised relative to.  A row (spectrum) more
+ * than half of whose samples fall outside the 8-bit range -- here the RFI
+ * clipper's rejected spectra, left at -0.2 - bandpass - dataModel, far below
+ * the normalised data -- is stored as rint((x - off) * inv_scale) with off =
+ * its middle channel's value (channel nChan/2); every other row has off = 0
+ * and is stored as before.  The offsets are carried with the overlap. */
+const float* ppl_ddbuf_row_offsets(const ppl_ddbuf* buf);
 unsigned ppl_ddbuf_fill(const ppl_ddbuf* buf);
 
 /* -------------------------------------------------------------- detection */
@@ -227,6 +236,17 @@ int ppl_detect_u8(const int32_t* d_data, int tdms, int nsamp, int nChan, float s
                   int32_t* h_ev_ints, float* h_ev_vals, int max_events, int* count,
                   void* stream);
 
+/* u8 series (no reference counterpart): the dedispersed values of a u8
+ * buffer, out[dm][t] = float(S[dm][t] - 128*nChan) * scale, plus for every
+ * row e of the buffer with a nonzero offset (ppl_ddbuf_row_offsets), taken
+ * in increasing e, off[e] times the number of channels whose delay at
+ * trial dm is e - t (product rounded, then added).  So a rejected spectrum
+ * enters the series at its own level, as in the FP32 chain, instead of at
+ * the bottom of the 8-bit range.  d_sums [tdms][nOut] int32 from
+ * ppl_dedisperse; d_rowOffsets the offsets of the buffer those sums were
+ * taken from (before its rotate); d_out [tdms][nOut] float. */
+int ppl_dedisp_u8_series(const ppl_dedisp_plan* plan, const int32_t* d_sums, const float* d_rowOffsets,
+                         float scale, float* d_out, void* stream);
 /* u8-mode sums as dedispersed values: out = float(S - 128*nChan) * scale,
  * the values ppl_detect_u8 compares (no reference counterpart: the
  * reference has no quantised mode). */

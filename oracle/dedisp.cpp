@@ -171,6 +171,63 @@ extern "C" void orc_quantise_u8(const float* in, uint8_t* out, size_t n, float i
     }
 }
 
+// The u8 buffer's quantiser with exceptional rows (the product's u8 mode;
+// the reference has no quantised mode): buffer x [nChan][nsamp]; a column
+// (spectrum) t more than half of whose samples quantise outside [-128, 127]
+// (NaN included) is quantised relative to off[t] = x[nChan/2][t]; every
+// other column has off[t] = 0.
+extern "C" void orc_quantise_u8_rows(const float* x, unsigned nChan, unsigned nsamp, float inv_scale, uint8_t* q,
+                                     float* off)
+{
+    for (unsigned t = 0; t < nsamp; ++t) {
+        unsigned n = 0;
+        for (unsigned c = 0; c < nChan; ++c) {
+            const float y = std::nearbyint(x[(size_t)c * nsamp + t] * inv_scale);
+            n += (!(y >= -128.0f) || y > 127.0f) ? 1u : 0u;
+        }
+        off[t] = 2 * n > nChan ? x[(size_t)(nChan / 2) * nsamp + t] : 0.0f;
+    }
+    for (unsigned c = 0; c < nChan; ++c)
+        for (unsigned t = 0; t < nsamp; ++t) {
+            const float v = x[(size_t)c * nsamp + t];
+            float y = std::nearbyint((off[t] != 0.0f ? v - off[t] : v) * inv_scale);
+            if (!(y >= -128.0f)) y = -128.0f;
+            if (y > 127.0f) y = 127.0f;
+            q[(size_t)c * nsamp + t] = (uint8_t)((int)y + 128);
+        }
+}
+
+// The u8 series: out[dm][t] = float(S - 128*nChan) * scale, plus, over the
+// columns e with off[e] != 0 in increasing order and 0 <= e - t <=
+// max delay of trial dm, off[e] * (number of channels with delay e - t),
+// each product rounded before it is added.
+extern "C" void orc_u8_series(const int32_t* sums, const float* off, const int32_t* delay, unsigned nChan,
+                              unsigned nsamp, unsigned tdms, unsigned nOut, float scale, float* out)
+{
+    std::vector<int> rows;
+    for (unsigned e = 0; e < nsamp; ++e)
+        if (off[e] != 0.0f) rows.push_back((int)e);
+    std::vector<int> len;
+    for (unsigned dm = 0; dm < tdms; ++dm) {
+        const int32_t* d = delay + (size_t)dm * nChan;
+        int dmax = 0;
+        for (unsigned c = 0; c < nChan; ++c) dmax = d[c] > dmax ? d[c] : dmax;
+        len.assign((size_t)dmax + 1, 0);
+        for (unsigned c = 0; c < nChan; ++c) ++len[(size_t)d[c]];
+        for (unsigned t = 0; t < nOut; ++t) {
+            float corr = 0.0f;
+            for (int e : rows) {
+                const int k = e - (int)t;
+                if (k < 0) continue;
+                if (k > dmax) break;
+                if (len[(size_t)k] != 0) corr = corr + off[e] * (float)len[(size_t)k];
+            }
+            const size_t i = (size_t)dm * nOut + t;
+            out[i] = (float)(sums[i] - (int32_t)(128 * nChan)) * scale + corr;
+        }
+    }
+}
+
 // DedispersionBuffer (src/lib/src/DedispersionBuffer.cpp:65-96,98-179),
 // invert = 0, Stokes-I: spectra [t][c] are transposed into [c][t]; when the
 // buffer holds nsampBuf samples it is emitted and its last `overlap`

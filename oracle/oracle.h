@@ -165,6 +165,10 @@ void orc_dedisperse_u8(const uint8_t* in, unsigned nChan, unsigned nsampBuf,
 void orc_f32_to_f16(const float* in, uint16_t* out, size_t n);
 /* Offset-binary quantiser: q = clamp(rint(x*inv_scale), -128, 127) + 128. */
 void orc_quantise_u8(const float* in, uint8_t* out, size_t n, float inv_scale);
+void orc_quantise_u8_rows(const float* x, unsigned nChan, unsigned nsamp, float inv_scale, uint8_t* q,
+                          float* off);
+void orc_u8_series(const int32_t* sums, const float* off, const int32_t* delay, unsigned nChan,
+                   unsigned nsamp, unsigned tdms, unsigned nOut, float scale, float* out);
 
 /* DedispersionBuffer::addSamples + copy (src/lib/src/DedispersionBuffer.cpp:
  * 65-96,98-179) for Stokes-I blobs [t][sub][0][ch] with invert = 0.

@@ -69,6 +69,8 @@ def _declare(L):
         "orc_dedisperse_u8": (None, [vp, u, u, u, vp, u, vp, i]),
         "orc_f32_to_f16": (None, [vp, vp, C.c_size_t]),
         "orc_quantise_u8": (None, [vp, vp, C.c_size_t, f]),
+        "orc_quantise_u8_rows": (None, [vp, u, u, f, vp, vp]),
+        "orc_u8_series": (None, [vp, vp, vp, u, u, u, u, f, vp]),
         "orc_ddbuf_create": (vp, [u, u, u]),
         "orc_ddbuf_destroy": (None, [vp]),
         "orc_ddbuf_add": (i, [vp, vp, u, vp, i]),
@@ -297,6 +299,27 @@ def f32_to_f16_bits(x):
     x = np.ascontiguousarray(x, np.float32)
     out = np.zeros(x.shape, np.uint16)
     lib().orc_f32_to_f16(_p(x), _p(out), x.size)
+    return out
+
+
+def quantise_u8_rows(x, inv_scale):
+    """The u8 buffer [nChan][nsamp] with exceptional rows: (codes, row offsets)."""
+    x = np.ascontiguousarray(x, np.float32)
+    n_chan, nsamp = x.shape
+    q = np.zeros(x.shape, np.uint8)
+    off = np.zeros(nsamp, np.float32)
+    lib().orc_quantise_u8_rows(_p(x), n_chan, nsamp, inv_scale, _p(q), _p(off))
+    return q, off
+
+
+def u8_series(sums, off, delays, n_out, scale):
+    """Dedispersed values of a u8 buffer's int32 sums with its row offsets."""
+    sums = np.ascontiguousarray(sums, np.int32)
+    off = np.ascontiguousarray(off, np.float32)
+    d = np.ascontiguousarray(delays, np.int32)
+    tdms, n_chan = d.shape
+    out = np.zeros((tdms, n_out), np.float32)
+    lib().orc_u8_series(_p(sums), _p(off), _p(d), n_chan, off.size, tdms, n_out, scale, _p(out))
     return out
 
 

@@ -109,6 +109,8 @@ SIGNATURES = {
     "ppl_ddbuf_data": (vp, [vp]),
     "ppl_ddbuf_fill": (u, [vp]),
     "ppl_detect": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
+    "ppl_dedisp_u8_series": (i, [vp, vp, vp, f, vp, vp]),
+    "ppl_ddbuf_row_offsets": (vp, [vp]),
     "ppl_detect_noalloc": (i, [vp, i, i, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_detect_u8": (i, [vp, i, i, i, f, i, i, f, u, vp, vp, i, vp, vp]),
     "ppl_dequantise_sums": (i, [vp, vp, sz, u, f, vp]),
@@ -261,6 +263,17 @@ class DedispPlan:
         check(lib().ppl_dedisperse(self.h, _ptr(d_in), _ptr(d_out), _stream(stream)))
         return d_out
 
+    def u8_series(self, d_sums, d_row_offsets, scale, d_out=None, stream=None):
+        """Float series of a u8 plan's int32 sums: dequantised, with the
+        buffer's exceptional rows (d_row_offsets, DDBuffer.row_offsets())
+        restored at their own level (ppl_dedisp_u8_series)."""
+        import torch
+        if d_out is None:
+            d_out = torch.empty((self.cfg.tdms, self.nout), dtype=torch.float32, device=d_sums.device)
+        check(lib().ppl_dedisp_u8_series(self.h, _ptr(d_sums), _ptr(d_row_offsets), scale, _ptr(d_out),
+                                         _stream(stream)))
+        return d_out
+
 
 class DedispJob(C.Structure):
     _fields_ = [("plan", C.c_void_p), ("d_in", C.c_void_p), ("d_out", C.c_void_p)]
@@ -402,6 +415,14 @@ class DDBuffer:
         import torch
         dt = {F32: torch.float32, F16: torch.float16, U8: torch.uint8}[self.dtype]
         return _device_view(self.data_ptr(), (self.n_chan, self.nsamp), dt)
+
+    def row_offsets(self):
+        """u8 buffers: a torch view of the current buffer's row offsets
+        ([nsampBuf] float32; nonzero for rows quantised relative to their
+        middle channel), else None."""
+        import torch
+        ptr = lib().ppl_ddbuf_row_offsets(self.h)
+        return None if not ptr else _device_view(ptr, (self.nsamp,), torch.float32)
 
 
 def _device_view(ptr, shape, dtype):
@@ -719,6 +740,10 @@ class BeamChain:
         self.buf = DDBuffer(self.n_chan, nsamp_buf, self.overlap, dtype, inv_scale)
         self.stokes = torch.empty((self.nout, n_sub, 1, n_ch), dtype=torch.float32, device="cuda")
         self.out = torch.empty((tdms, self.nout), dtype=self.plan.out_dtype(), device="cuda")
+        # u8: the float series the detector reads (dequantised sums with the
+        # buffer's exceptional rows restored); self.out holds the int32 sums
+        self.series = (torch.empty((tdms, self.nout), dtype=torch.float32, device="cuda")
+                       if dtype == "u8" else None)
         self.graphs = None
         self.parity = 0
         if tile is not None:
@@ -758,11 +783,20 @@ class BeamChain:
         if used != self.nout or not full:
             raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
         mark(2)
-        self.plan.run(self.buf.tensor(), d_out=self.out)
+        self._dedisperse(self.buf.tensor(), self.buf.row_offsets(), self.out, self.series)
         mark(3)
         self.buf.rotate()
         mark(4)
         self.parity ^= 1
+
+    def _dedisperse(self, d_in, row_offsets, out, series, stream=None):
+        """Dedispersion of a full buffer into out; for u8 also the float
+        series (before the buffer's rotate, which moves its row offsets).
+        Returns the tensor the detector reads."""
+        self.plan.run(d_in, d_out=out, stream=stream)
+        if self.dtype != "u8":
+            return out
+        return self.plan.u8_series(out, row_offsets, 1.0 / self.inv_scale, d_out=series, stream=stream)
 
     def capture_steps(self, d_volt):
         """Capture two consecutive steps as two graphs (even and odd steps);
@@ -788,10 +822,8 @@ class BeamChain:
         self.parity ^= 1
 
     def detect(self, threshold=6.0, bin_pow2=6, max_events=1 << 22, d_out=None, stream=None):
-        d = self.out if d_out is None else d_out
-        if self.dtype == "u8":
-            return detect_u8(d, self.n_chan, 1.0 / self.inv_scale, self.n_ch, self.n_sub, threshold,
-                             bin_pow2, max_events, stream=stream)
+        """Events of the last step's series (or of d_out, a float series)."""
+        d = (self.series if self.dtype == "u8" else self.out) if d_out is None else d_out
         return detect(d, self.n_ch, self.n_sub, threshold, bin_pow2, max_events, stream=stream)
 
     def run_pipelined(self, volts, threshold=6.0, bin_pow2=6, max_events=1 << 22):
@@ -810,6 +842,7 @@ class BeamChain:
         import torch
         front, back = torch.cuda.Stream(), torch.cuda.Stream()
         outs = [self.out, torch.empty_like(self.out)]
+        series = [self.series, torch.empty_like(self.series)] if self.series is not None else [None, None]
         cur = torch.cuda.current_stream()
         front.wait_stream(cur)
         back.wait_stream(cur)
@@ -824,7 +857,7 @@ class BeamChain:
                     used, full = self.buf.add(self.stokes.view(self.nout, self.n_chan))
                     if used != self.nout or not full:
                         raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
-                    d_in = self.buf.tensor()
+                    d_in, r_in = self.buf.tensor(), self.buf.row_offsets()
                     filled = torch.cuda.Event()
                     filled.record(front)
                     if done is not None:
@@ -834,14 +867,15 @@ class BeamChain:
                 with torch.cuda.stream(back):
                     if pending is not None:
                         result = self.detect(threshold, bin_pow2, max_events, d_out=pending, stream=back)
-                    out = outs[k & 1]
                     back.wait_event(filled)
-                    self.plan.run(d_in, d_out=out)
+                    # the event is recorded after the u8 series too: it reads
+                    # the buffer's row offsets, which the carry then moves
+                    seen = self._dedisperse(d_in, r_in, outs[k & 1], series[k & 1])
                     done = torch.cuda.Event()
                     done.record(back)
                 if result is not None:
                     yield result
-                pending = out
+                pending = seen
                 self.parity ^= 1
                 k += 1
             if pending is not None:
@@ -856,6 +890,8 @@ class BeamChain:
             cur.wait_stream(back)
             if k and not k & 1:  # leave the last series in self.out, as step() does
                 self.out.copy_(outs[1])
+                if self.series is not None:
+                    self.series.copy_(series[1])
 
     def run_split(self, volts, small_cus=4, threshold=6.0, bin_pow2=6, max_events=1 << 22):
         """run_pipelined with the GPU split between the two streams by CU
@@ -878,6 +914,7 @@ class BeamChain:
         big, small = big_s.stream, small_s.stream
         stokes = [self.stokes, torch.empty_like(self.stokes)]
         outs = [self.out, torch.empty_like(self.out)]
+        series = [self.series, torch.empty_like(self.series)] if self.series is not None else [None, None]
         cur = torch.cuda.current_stream()
         big.wait_stream(cur)
         small.wait_stream(cur)
@@ -899,34 +936,32 @@ class BeamChain:
                     used, full = self.buf.add(sk.view(self.nout, self.n_chan))
                     if used != self.nout or not full:
                         raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
-                    d_in = self.buf.tensor()
+                    d_in, r_in = self.buf.tensor(), self.buf.row_offsets()
                     ev = torch.cuda.Event()
                     ev.record(small)
                     filled[k & 1] = ev
                 result, dd_done = None, None
                 if prev is not None:
                     with torch.cuda.stream(big):
-                        big.wait_event(prev[1])
-                        out = outs[(k - 1) & 1]
-                        self.plan.run(prev[0], d_out=out)
+                        big.wait_event(prev[2])
+                        seen = self._dedisperse(prev[0], prev[1], outs[(k - 1) & 1], series[(k - 1) & 1])
                         dd_done = torch.cuda.Event()
                         dd_done.record(big)
-                        result = self.detect(threshold, bin_pow2, max_events, d_out=out, stream=big)
+                        result = self.detect(threshold, bin_pow2, max_events, d_out=seen, stream=big)
                 with torch.cuda.stream(small):
                     if dd_done is not None:
                         small.wait_event(dd_done)
                     self.buf.rotate()
                 if result is not None:
                     yield result
-                prev = (d_in, ev)
+                prev = (d_in, r_in, ev)
                 self.parity ^= 1
                 k += 1
             if prev is not None:
                 with torch.cuda.stream(big):
-                    big.wait_event(prev[1])
-                    out = outs[(k - 1) & 1]
-                    self.plan.run(prev[0], d_out=out)
-                    result = self.detect(threshold, bin_pow2, max_events, d_out=out, stream=big)
+                    big.wait_event(prev[2])
+                    seen = self._dedisperse(prev[0], prev[1], outs[(k - 1) & 1], series[(k - 1) & 1])
+                    result = self.detect(threshold, bin_pow2, max_events, d_out=seen, stream=big)
                 yield result
         finally:
             cur.wait_stream(big)
@@ -936,6 +971,8 @@ class BeamChain:
             if k and not k & 1:
                 self.stokes.copy_(stokes[1])
                 self.out.copy_(outs[1])
+                if self.series is not None:
+                    self.series.copy_(series[1])
             big_s.close()
             small_s.close()
 
@@ -976,6 +1013,7 @@ class MultiBeam:
             raise PPLError("at most 48 beams")
         self.specs, self.dtype, self.inv_scale = list(specs), dtype, inv_scale
         self.plans, self.ppfs, self.clips, self.bufs, self.stokes, self.outs = [], [], [], [], [], []
+        self.series = []
         self.nout, self.overlap, self.chunk_spectra = [], [], []
         for sp in self.specs:
             n_chan = sp.n_sub * sp.n_ch
@@ -991,6 +1029,9 @@ class MultiBeam:
             self.bufs.append(DDBuffer(n_chan, sp.nsamp_buf, plan.maxshift + plan.rem, dtype, inv_scale))
             self.stokes.append(torch.empty((plan.nout, sp.n_sub, 1, sp.n_ch), dtype=torch.float32, device="cuda"))
             self.outs.append(torch.empty((sp.tdms, plan.nout), dtype=plan.out_dtype(), device="cuda"))
+            # u8: the float series the detector reads (see BeamChain)
+            self.series.append(torch.empty((sp.tdms, plan.nout), dtype=torch.float32, device="cuda")
+                               if dtype == "u8" else None)
             per = sp.integrate * sp.n_ch  # input samples per integrated spectrum
             m = max(1, min(plan.nout, max_chunk_samples // per))
             while plan.nout % m:
@@ -1085,6 +1126,9 @@ class MultiBeam:
                 raise PPLError("step: beam %d's buffer did not fill" % b)
         mark(2)
         dedisperse_batch(self.plans, [bf.tensor() for bf in self.bufs], self.outs, tile=self.tile)
+        if self.dtype == "u8":
+            for b, bf in enumerate(self.bufs):
+                self.plans[b].u8_series(self.outs[b], bf.row_offsets(), 1.0 / self.inv_scale, d_out=self.series[b])
         mark(3)
         for bf in self.bufs:
             bf.rotate()
@@ -1110,10 +1154,8 @@ class MultiBeam:
 
     def detect(self, b, threshold=6.0, bin_pow2=6, max_events=1 << 22):
         sp = self.specs[b]
-        if self.dtype == "u8":
-            return detect_u8(self.outs[b], sp.n_sub * sp.n_ch, 1.0 / self.inv_scale, sp.n_ch, sp.n_sub, threshold,
-                             bin_pow2, max_events)
-        return detect(self.outs[b], sp.n_ch, sp.n_sub, threshold, bin_pow2, max_events)
+        d = self.series[b] if self.dtype == "u8" else self.outs[b]
+        return detect(d, sp.n_ch, sp.n_sub, threshold, bin_pow2, max_events)
 
 
 # ------------------------------------------------------------ streaming
@@ -1141,6 +1183,8 @@ class StreamService:
             self.stokes = torch.empty((svc.chunk_spectra, sp.n_sub, 1, sp.n_ch), dtype=torch.float32,
                                       device="cuda")
             self.out = torch.empty((sp.tdms, svc.plan.nout), dtype=svc.plan.out_dtype(), device="cuda")
+            self.series = (torch.empty((sp.tdms, svc.plan.nout), dtype=torch.float32, device="cuda")
+                           if svc.dtype == "u8" else None)
             self.buffers = 0  # dedispersion buffers completed
 
     def __init__(self, spec, chunk_spectra, budget_bytes, dtype="f32", inv_scale=127.0 / 10.0, tile=None,
@@ -1160,13 +1204,14 @@ class StreamService:
 
     def footprint(self):
         """Device bytes one admitted beam holds: the two alternating
-        dedispersion buffers, the chunk's spectra, the dedispersed output, the
-        channeliser history and the clipper's state and work areas."""
+        dedispersion buffers, the chunk's spectra, the dedispersed output (for
+        u8 the int32 sums and the float series), the channeliser history and
+        the clipper's state and work areas."""
         sp = self.spec
         elem = {"f32": 4, "f16": 2, "u8": 1}[self.dtype]
         buf = 2 * self.n_chan * sp.nsamp_buf * elem
         stokes = self.chunk_spectra * self.n_chan * 4
-        out = sp.tdms * self.plan.nout * 4
+        out = sp.tdms * self.plan.nout * 4 * (2 if self.dtype == "u8" else 1)  # u8: sums and series
         ppf = 2 * sp.n_sub * 2 * (sp.n_taps - 1) * sp.n_ch * 8
         clip = 64 * self.n_chan * 4 + self.chunk_spectra * 256 + (16 << 20)
         return buf + stokes + out + ppf + clip
@@ -1213,8 +1258,8 @@ class StreamService:
             dedisperse_batch([self.plan] * len(full), [b.buf.tensor() for b in full], [b.out for b in full])
             for b in full:
                 if self.dtype == "u8":
-                    ints, vals, _ = detect_u8(b.out, self.n_chan, 1.0 / self.inv_scale, sp.n_ch, sp.n_sub,
-                                              threshold, bin_pow2)
+                    self.plan.u8_series(b.out, b.buf.row_offsets(), 1.0 / self.inv_scale, d_out=b.series)
+                    ints, vals, _ = detect(b.series, sp.n_ch, sp.n_sub, threshold, bin_pow2)
                 else:
                     ints, vals, _ = detect(b.out, sp.n_ch, sp.n_sub, threshold, bin_pow2)
                 results.append((b.key, b.buffers, ints, vals))

@@ -39,6 +39,11 @@ struct ppl_dedisp_plan {
     // offsets of each trial's reads, then per tile and channel the first
     // trial's delay
     int32_t* d_ring[2] = {nullptr, nullptr};
+    // u8 series, built on first use: per trial dm, the number of channels
+    // whose delay is k, for k = 0 .. maxdelay(dm): lenTab[lenOff[dm] + k];
+    // lenOff[tdms] is the table's length
+    int32_t* d_lenOff = nullptr;
+    uint16_t* d_lenTab = nullptr;
 };
 
 namespace {
@@ -1254,6 +1259,8 @@ extern "C" int ppl_dedisp_plan_destroy(ppl_dedisp_plan* plan)
     if (plan->d_delayT) (void)hipFree(plan->d_delayT);
     for (int32_t* t : plan->d_ring)
         if (t) (void)hipFree(t);
+    if (plan->d_lenOff) (void)hipFree(plan->d_lenOff);
+    if (plan->d_lenTab) (void)hipFree(plan->d_lenTab);
     delete plan;
     return PPL_OK;
 }
@@ -1504,6 +1511,151 @@ extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float*
     int rc = run_jobs(&job, 1, s, st);
     (void)hipFreeAsync(d_delayT, st);
     return rc;
+}
+
+// ------------------------------------------------------- u8 series
+namespace {
+
+// Rows (spectra) of a u8 buffer with a nonzero offset, in increasing order
+// (one workgroup: each thread a contiguous run of rows).
+__global__ __launch_bounds__(1024) void exceptional_rows_kernel(const float* __restrict__ rowOff, int nsamp,
+                                                                int* __restrict__ rows, int* __restrict__ nRows)
+{
+    __shared__ int waveTot[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int per = (nsamp + 1023) / 1024;
+    const int a = min(t * per, nsamp), b = min(a + per, nsamp);
+    int n = 0;
+    for (int i = a; i < b; ++i) n += rowOff[i] != 0.0f;
+    int incl = n;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) waveTot[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const int w = lane < 16 ? waveTot[lane] : 0;
+        int wi = w;
+        for (int d = 1; d < 16; d <<= 1) {
+            const int y = __shfl_up(wi, d, 64);
+            if (lane >= d) wi += y;
+        }
+        if (lane < 16) waveTot[lane] = wi - w;
+        if (lane == 15) *nRows = wi;
+    }
+    __syncthreads();
+    int o = waveTot[wave] + incl - n;
+    for (int i = a; i < b; ++i)
+        if (rowOff[i] != 0.0f) rows[o++] = i;
+}
+
+// out[dm][t] = float(S - offset) * scale, plus, for every exceptional row e
+// with 0 <= e - t <= maxdelay(dm) in increasing order, rowOff[e] times the
+// number of channels whose delay is e - t (each product rounded, then
+// added).  One workgroup per (256 samples, trial).
+__global__ __launch_bounds__(256) void u8_series_kernel(const int32_t* __restrict__ sums,
+                                                        const float* __restrict__ rowOff,
+                                                        const int* __restrict__ rows, const int* __restrict__ nRowsP,
+                                                        const uint16_t* __restrict__ lenTab,
+                                                        const int32_t* __restrict__ lenOff, int nOut, int offset,
+                                                        float scale, float* __restrict__ out)
+{
+    const int dm = blockIdx.y;
+    const int t0 = blockIdx.x * 256;
+    const int t = t0 + threadIdx.x;
+    const int base = lenOff[dm];
+    const int dmax = lenOff[dm + 1] - base - 1;
+    const int nRows = *nRowsP;
+    // first exceptional row at or after t0 (uniform)
+    int lo = 0, hi = nRows;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rows[mid] < t0) lo = mid + 1;
+        else hi = mid;
+    }
+    float corr = 0.0f;
+    for (int i = lo; i < nRows; ++i) {
+        const int e = rows[i];
+        if (e > t0 + 255 + dmax) break;
+        const int k = e - t;
+        if (k >= 0 && k <= dmax) {
+            const int n = lenTab[base + k];
+            if (n != 0) corr = corr + rowOff[e] * (float)n;
+        }
+    }
+    if (t < nOut) {
+        const size_t i = (size_t)dm * nOut + t;
+        out[i] = (float)(sums[i] - offset) * scale + corr;
+    }
+}
+
+// The channel-count table of a plan (see ppl_dedisp_plan::d_lenTab).
+int len_table(const ppl_dedisp_plan* cp, const int32_t** lenOff, const uint16_t** lenTab)
+{
+    ppl_dedisp_plan* p = const_cast<ppl_dedisp_plan*>(cp);
+    if (!p->d_lenTab) {
+        const int tdms = (int)p->cfg.tdms, nChan = (int)p->cfg.nChan;
+        std::vector<int32_t> off(tdms + 1);
+        long total = 0;
+        for (int dm = 0; dm < tdms; ++dm) {
+            off[dm] = (int32_t)total;
+            int dmax = 0;
+            for (int c = 0; c < nChan; ++c) dmax = std::max(dmax, p->delays[(size_t)dm * nChan + c]);
+            total += dmax + 1;
+            PPL_CHECK_ARG(total < (1L << 31), "u8 series: channel-count table too large");
+        }
+        off[tdms] = (int32_t)total;
+        std::vector<uint16_t> tab((size_t)total, 0);
+        PPL_CHECK_ARG(nChan < 65536, "u8 series: %d channels (at most 65535)", nChan);
+        for (int dm = 0; dm < tdms; ++dm)
+            for (int c = 0; c < nChan; ++c) ++tab[(size_t)off[dm] + p->delays[(size_t)dm * nChan + c]];
+        int32_t* dOff = nullptr;
+        uint16_t* dTab = nullptr;
+        PPL_HIP(hipMalloc(&dOff, off.size() * sizeof(int32_t)));
+        if (hipMalloc(&dTab, tab.size() * sizeof(uint16_t)) != hipSuccess) {
+            (void)hipFree(dOff);
+            set_error("u8 series: allocation of the channel-count table failed");
+            return PPL_E_NOMEM;
+        }
+        if (hipMemcpy(dOff, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dTab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(dOff);
+            (void)hipFree(dTab);
+            set_error("u8 series: upload of the channel-count table failed");
+            return PPL_E_HIP;
+        }
+        p->d_lenOff = dOff;
+        p->d_lenTab = dTab;
+    }
+    *lenOff = p->d_lenOff;
+    *lenTab = p->d_lenTab;
+    return PPL_OK;
+}
+
+}  // namespace
+
+extern "C" int ppl_dedisp_u8_series(const ppl_dedisp_plan* p, const int32_t* d_sums, const float* d_rowOffsets,
+                                    float scale, float* d_out, void* stream)
+{
+    PPL_CHECK_ARG(p && d_sums && d_rowOffsets && d_out, "null argument");
+    PPL_CHECK_ARG(p->cfg.dtype == PPL_U8, "u8 series of a plan of another dtype");
+    const int32_t* lenOff = nullptr;
+    const uint16_t* lenTab = nullptr;
+    int rc = len_table(p, &lenOff, &lenTab);
+    if (rc != PPL_OK) return rc;
+    hipStream_t st = stream_of(stream);
+    const int nsamp = (int)p->cfg.nsampBuf;
+    int* rows = nullptr;
+    PPL_HIP(hipMallocAsync((void**)&rows, sizeof(int) * ((size_t)nsamp + 1), st));
+    hipLaunchKernelGGL(exceptional_rows_kernel, dim3(1), dim3(1024), 0, st, d_rowOffsets, nsamp, rows + 1, rows);
+    PPL_LAUNCHED();
+    dim3 grid(cdiv((unsigned)p->nOut, 256), p->cfg.tdms);
+    hipLaunchKernelGGL(u8_series_kernel, grid, dim3(256), 0, st, d_sums, d_rowOffsets, rows + 1, rows, lenTab,
+                       lenOff, p->nOut, (int)(128 * p->cfg.nChan), scale, d_out);
+    PPL_LAUNCHED();
+    PPL_HIP(hipFreeAsync(rows, st));
+    return PPL_OK;
 }
 
 // ---------------------------------------------------------------- converters

@@ -162,8 +162,10 @@ void DedispersionModule::_launch()
     void* d_out = out->deviceDataForWrite();
     if (_dtype == PPL_U8) {
         ppl_check(ppl_dedisperse(_plan, ppl_ddbuf_data(_buffer), _sums, nullptr), "DedispersionModule");
-        ppl_check(ppl_dequantise_sums(static_cast<const int32_t*>(_sums), static_cast<float*>(d_out),
-                                      (size_t)_tdms * (size_t)_nOut, _sampleSize, 1.0f / _invScale, nullptr),
+        // the float series with the buffer's exceptional rows restored, before
+        // the rotate below moves its row offsets
+        ppl_check(ppl_dedisp_u8_series(_plan, static_cast<const int32_t*>(_sums), ppl_ddbuf_row_offsets(_buffer),
+                                       1.0f / _invScale, static_cast<float*>(d_out), nullptr),
                   "DedispersionModule");
     } else {
         ppl_check(ppl_dedisperse(_plan, ppl_ddbuf_data(_buffer), d_out, nullptr), "DedispersionModule");

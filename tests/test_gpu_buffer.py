@@ -75,3 +75,48 @@ def test_buffer_rejects_mismatched_blob(ppl):
     buf = ppl.DDBuffer(96, 512, 100)
     with pytest.raises(ppl.PPLError):
         buf.add_blob(torch.zeros((10, 5, 1, 16), device="cuda"))  # 80 channels, buffer has 96
+
+
+def test_u8_exceptional_rows_match_oracle(orc, ppl):
+    """u8 buffers: spectra most of whose samples fall outside the 8-bit range
+    (the clipper's rejected spectra, thousands of units below the data) are
+    stored relative to their middle channel's value, which is kept per row
+    and carried with the overlap; a spectrum with fewer saturated samples
+    than half is stored as it stands.  Codes and offsets equal the oracle's,
+    buffer after buffer."""
+    n_sub, n_ch, nsamp, overlap, inv = 6, 16, 512, 100, 127.0 / 10.0
+    n_chan = n_sub * n_ch
+    rng = np.random.default_rng(44)
+    blobs = []
+    for n in (300, 513, 77, 600):
+        b = rng.normal(size=(n, n_chan)).astype(np.float32)
+        rej = rng.choice(n, size=max(2, n // 40), replace=False)
+        b[rej] = (-3000.0 - 40.0 * rng.random(size=(len(rej), n_chan))).astype(np.float32)
+        b[rng.integers(n), : n_chan // 3] = 500.0  # a third saturated: not exceptional
+        blobs.append(b)
+    buf = ppl.DDBuffer(n_chan, nsamp, overlap, dtype="u8", inv_scale=inv)
+    got, offs = [], []
+    for b in blobs:
+        d = torch.from_numpy(b).cuda()
+        first = 0
+        while first < b.shape[0]:
+            used, full = buf.add(d, first)
+            first += used
+            if full:
+                got.append(buf.tensor().cpu().numpy().copy())
+                offs.append(buf.row_offsets().cpu().numpy().copy())
+                tail = offs[-1][nsamp - overlap:].copy()
+                buf.rotate()
+                assert np.array_equal(buf.row_offsets()[:overlap].cpu().numpy(), tail)
+    ob = orc.DDBuffer(n_chan, nsamp, overlap)
+    want = []
+    for b in blobs:
+        want.extend(list(ob.add(b)))
+    assert len(got) == len(want) >= 2
+    n_exc = 0
+    for g, o, w in zip(got, offs, want):
+        q, off = orc.quantise_u8_rows(w, inv)
+        assert np.array_equal(o, off)
+        assert np.array_equal(g, q)
+        n_exc += int(np.count_nonzero(off))
+    assert n_exc >= 8

@@ -136,3 +136,49 @@ def test_end_to_end_event_lists(orc, chain):
             if abs(dm - dm_idx) <= 2 and abs(start + t - PULSE["t0"]) <= 16:
                 found_pulse = True
     assert found_pulse
+
+
+def test_u8_chain_series_and_events_follow_f32(ppl):
+    """The u8 chain against the FP32 chain on one stream with broadband
+    impulses (which the clipper rejects and leaves thousands of units below
+    the data): the u8 series stays within the quantisation bound of the FP32
+    series everywhere -- the rejected spectra enter at their own level -- and
+    the event lists agree (recall and precision 0.99 or better)."""
+    from synth import beam_bandpass, device_voltages
+    n_sub, n_ch, nsamp, tdms, dm_hi = 8, 16, 4096, 128, 50.0
+    foff = -0.1953125 / n_ch
+    h = ppl.ppf_coefficients(8, n_ch, "hann")
+    bp = beam_bandpass(h, n_sub, n_ch, 4, 150.0, foff)
+    tsamp = 5.12e-6 * n_ch * 4
+    chains = {dt: ppl.BeamChain(n_sub, n_ch, 8, h, 4, bp, nsamp, tdms, 0.0, dm_hi / tdms, tsamp, 150.0, foff,
+                                dtype=dt, history=1000) for dt in ("f32", "u8")}
+    c32 = chains["f32"]
+    amp = tone_amplitude(h, n_ch, 64.0, 20.0, 4)
+    spc = 4 * n_ch
+    n_steps = 4
+    v = device_voltages(torch, n_sub, n_ch, 4, (c32.overlap + n_steps * c32.nout) * spc, 77, 150.0, foff,
+                        pulses=[(20.0 + 2 * j, 1500.0 + 500.0 * j, 4.0, amp) for j in range(24)],
+                        impulse_frac=0.003, wrap=False)
+    prime = v[:, :, :c32.overlap * spc].contiguous()
+    for ch in chains.values():
+        ch.prime(prime)
+    inv = chains["u8"].inv_scale
+    hits = miss = extra = 0
+    rejected_reach = 0.0
+    for k in range(n_steps):
+        a = c32.overlap * spc + k * c32.nout * spc
+        step_in = v[:, :, a:a + c32.nout * spc].contiguous()
+        for ch in chains.values():
+            ch.step(step_in)
+        ref, s8 = c32.out, chains["u8"].series
+        bound = n_sub * n_ch * 0.5 / inv * 1.001 + 1e-5 * ref.abs()
+        assert bool(torch.all((s8 - ref).abs() <= bound)), k
+        rejected_reach = min(rejected_reach, float(ref.min()))
+        e32 = {tuple(e) for e in c32.detect()[0][1:]}
+        e8 = {tuple(e) for e in chains["u8"].detect()[0][1:]}
+        hits += len(e32 & e8)
+        miss += len(e32 - e8)
+        extra += len(e8 - e32)
+    assert rejected_reach < -1000.0  # rejected spectra did land in the series
+    assert hits > 50
+    assert hits / (hits + miss) >= 0.99 and hits / (hits + extra) >= 0.99, (hits, miss, extra)

@@ -340,3 +340,34 @@ def test_tile_setters_of_the_c_abi(orc, ppl):
     assert L.ppl_dedisp_set_shape(plan.h, 32, 512, 17) != 0  # channel block above 16
     plan._refresh()
     assert plan.tile == (16, 512, 3, 4)                     # a refused call changes nothing
+
+
+def test_u8_series_matches_oracle_and_restores_rejected_rows(orc, ppl):
+    """ppl_dedisp_u8_series on a buffer with rejected spectra: bit for bit the
+    oracle's series, and within the quantisation bound of the FP32 sums of
+    the unquantised buffer everywhere -- also where a rejected spectrum lands,
+    which plain dequantisation misses by thousands.  (The bound holds while a
+    rejected spectrum's spread about its middle channel fits the 8-bit range,
+    here 5 units against 10: the clipper's rejected spectra are
+    -0.2 - bandpass - dataModel, flat for a flat bandpass model.)"""
+    cfg = dict(SMALL)
+    inv = np.float32(127.0 / 10.0)
+    rng = np.random.default_rng(71)
+    n, ns = cfg["n_chan"], cfg["nsamp_buf"]
+    x = rng.normal(size=(n, ns)).astype(np.float32)
+    rej = rng.choice(ns, size=20, replace=False)
+    x[:, rej] = (-2500.0 - 5.0 * rng.random(size=(n, rej.size))).astype(np.float32)
+    q, off = orc.quantise_u8_rows(x, inv)
+    assert np.count_nonzero(off) == rej.size
+    p, d = _oracle(orc, x, cfg)
+    plan = ppl.DedispPlan(**cfg, dtype="u8")
+    sums = plan.run(torch.from_numpy(q).cuda())
+    assert np.array_equal(sums.cpu().numpy(), orc.dedisperse(q, d, p["nout"]))
+    series = plan.u8_series(sums, torch.from_numpy(off).cuda(), float(1.0 / inv)).cpu().numpy()
+    want = orc.u8_series(sums.cpu().numpy(), off, d, p["nout"], np.float32(1.0 / inv))
+    assert np.array_equal(series, want)
+    ref = orc.dedisperse(x, d, p["nout"])
+    bound = n * 0.5 / float(inv) * 1.001 + 1e-5 * np.abs(ref)
+    assert np.all(np.abs(series - ref) <= bound)
+    plain = (sums.cpu().numpy() - 128 * n).astype(np.float32) * np.float32(1.0 / inv)
+    assert np.max(np.abs(plain - ref)) > 1000.0

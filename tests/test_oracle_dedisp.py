@@ -248,3 +248,41 @@ def test_reference_single_event(orc):
     one = np.nonzero(ev[1:, 2] == 1)[0] + 1
     assert vals[one].max() == 2048.0
     assert np.any((ev[one, 0] == 100) & (vals[one] == 2048.0))
+
+
+def test_u8_rows_quantiser_and_series_pinned(orc):
+    """The oracle's u8 buffer with exceptional rows and its series, against a
+    direct numpy statement: rows (columns of [nChan][nsamp]) with more than
+    half their samples outside [-128, 127] after rint(x * inv) are stored
+    relative to their middle channel; the series adds each such row's offset
+    once per channel that lands on it.  A row with a third of its samples
+    saturated stays as it is.  Rows with no exception quantise as before."""
+    rng = np.random.default_rng(9)
+    n, ns, inv = 24, 300, np.float32(12.7)
+    x = rng.normal(size=(n, ns)).astype(np.float32)
+    q0, off0 = orc.quantise_u8_rows(x, inv)
+    assert np.array_equal(q0, orc.quantise_u8(x, inv)) and not off0.any()
+    x[:, [5, 6, 200]] = -900.0 + rng.random(size=(n, 3)).astype(np.float32)
+    x[: n // 3, 100] = 400.0
+    q, off = orc.quantise_u8_rows(x, inv)
+    assert list(np.nonzero(off)[0]) == [5, 6, 200]
+    assert np.array_equal(off[[5, 6, 200]], x[n // 2, [5, 6, 200]])
+    y = np.rint((x - off[None, :]) * inv)
+    want = (np.clip(np.where(np.isnan(y), -128, y), -128, 127) + 128).astype(np.uint8)
+    assert np.array_equal(q, want)
+    # series: random delays non-decreasing in channel, against enumeration
+    tdms, n_out = 7, 200
+    d = np.sort(rng.integers(0, 90, size=(tdms, n)), axis=1).astype(np.int32)
+    d[:, 0] = 0
+    sums = orc.dedisperse(q, d, n_out)
+    got = orc.u8_series(sums, off, d, n_out, np.float32(1.0 / inv))
+    rows = [e for e in range(ns) if off[e] != 0]
+    for dm in range(tdms):
+        for t in range(n_out):
+            corr = np.float32(0.0)
+            for e in rows:
+                cnt = int(np.count_nonzero(t + d[dm] == e))
+                if cnt:
+                    corr = np.float32(corr + np.float32(off[e] * np.float32(cnt)))
+            v = np.float32(np.float32(sums[dm, t] - 128 * n) * np.float32(1.0 / inv)) + corr
+            assert got[dm, t] == np.float32(v), (dm, t)
