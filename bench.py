@@ -92,7 +92,11 @@ def parse():
 
 
 def gen_chunk(torch, ppl_coeffs, n_samp, sample0, seed, pulses, rfi, impulses=True):
-    """int16 voltages [sub][pol][n_samp][2] on the device."""
+    """int16 voltages [sub][pol][n_samp][2] on the device.  sample0 is the
+    chunk's first sample in the stream (tone phases); a pulse (dm, t0, width,
+    amp) starts t0 integrated spectra after the chunk's start in the top
+    channel and arrives later in lower channels by its dispersion delay; what
+    falls past the chunk's end is dropped."""
     from tests.synth import shift_table, tone_amplitude
     g = torch.Generator(device="cuda").manual_seed(seed)
     out = torch.empty((N_SUB, 2, n_samp, 2), dtype=torch.int16, device="cuda")
@@ -116,7 +120,7 @@ def gen_chunk(torch, ppl_coeffs, n_samp, sample0, seed, pulses, rfi, impulses=Tr
             x[:, :, 1] += (ramp * torch.sin(ph)).float()[None, :]
         for (dm, t0, width, amp) in pulses:
             d = shift[s * N_CH:(s + 1) * N_CH] * dm / TSAMP
-            a = torch.round((t0 + d) * spi).long() - sample0
+            a = torch.round((t0 + d) * spi).long()
             L = int(round(width * spi))
             idx = a[:, None] + torch.arange(L, device="cuda")[None, :]
             ok = (idx >= 0) & (idx < n_samp)
@@ -176,6 +180,62 @@ def event_keys(ints):
     return {tuple(r) for r in ints[1:]}  # without the dummy event
 
 
+def last_step_keeping_input(chain, volt):
+    """One step of a BeamChain by its parts, returning a copy of the
+    dedispersion input (the buffer, before its carry) of this step."""
+    chain.ppf.stokes_i16(volt, chain.nout * chain.integrate, d_out=chain.stokes)
+    chain.clip.run(chain.stokes)
+    chain.buf.add(chain.stokes.view(chain.nout, chain.n_chan))
+    keep = chain.buf.tensor().clone()
+    chain._dedisperse(chain.buf.tensor(), chain.buf.row_offsets(), chain.out, chain.series)
+    chain.buf.rotate()
+    chain.parity ^= 1
+    return keep
+
+
+def fp64_sample(torch, buf, plan, n_trials):
+    """FP64 dedispersion of a [nChan][nsamp] buffer (any dtype, taken as its
+    values; u8 codes are not meant here) over n_trials trials spread across
+    the range: (trial indices, [n_trials][nOut] float64)."""
+    delays = torch.from_numpy(plan.delays()).cuda().long()
+    tdms, n_chan = delays.shape
+    trials = torch.linspace(0, tdms - 1, n_trials, device="cuda").round().long()
+    x = buf.double()
+    nout = plan.nout
+    out = torch.zeros((n_trials, nout), dtype=torch.float64, device="cuda")
+    t = torch.arange(nout, device="cuda")
+    for k, dm in enumerate(trials.tolist()):
+        for c0 in range(0, n_chan, 512):
+            c1 = min(c0 + 512, n_chan)
+            idx = t[None, :] + delays[dm, c0:c1, None]
+            out[k] += x[c0:c1].gather(1, idx).sum(0)
+    return trials, out
+
+
+def count_recovered(step_events, pulses, step0, n_chunks, nout, overlap):
+    """Injected pulses found in the steps' event lists: a pulse of chunk c at
+    integrated spectrum t0 reaches the series of the step that reads c at
+    index overlap + t0 (channel 0 has no delay), or the next step's at
+    overlap + t0 - nOut; it counts as found if an event lies within 10
+    pc/cm^3 of its DM and within its width plus one 64-sample bin of that
+    index.  Returns "found/injected"."""
+    found = total = 0
+    for k, (ints, _) in enumerate(step_events):
+        ci = (step0 + k) % n_chunks
+        for (dm, t0, width, _amp) in pulses[ci]:
+            t = overlap + int(round(t0))
+            step_t = (k, t) if t < nout else (k + 1, t - nout)
+            if step_t[0] >= len(step_events):
+                continue
+            total += 1
+            ev = step_events[step_t[0]][0][1:]
+            if len(ev) == 0:
+                continue
+            ok = (np.abs(ev[:, 0] * DM_STEP - dm) <= 10.0) & (np.abs(ev[:, 1] - step_t[1]) <= width + 64)
+            found += bool(np.any(ok))
+    return "%d/%d" % (found, total)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -207,9 +267,9 @@ def main():
     chain = make_chain(dd_dtype)
     nout, overlap = chain.nout, chain.overlap
     n_samp = chain.samples_per_step()
-    def make_chunks(seed, impulses):
+    def make_chunks(seed, impulses, with_pulses=False):
         rng = np.random.default_rng(seed)
-        out = []
+        out, all_pulses = [], []
         for ci in range(args.chunks):
             pulses = []
             for _ in range(5):
@@ -221,7 +281,8 @@ def main():
             rfi = [(int(rng.integers(N_SUB)), int(rng.integers(N_CH)),
                     tone_amplitude(coeffs, N_CH, SIGMA, 50.0, INTEG)) for _ in range(10)]
             out.append(gen_chunk(torch, coeffs, n_samp, ci * n_samp, seed + ci, pulses, rfi, impulses))
-        return out
+            all_pulses.append(pulses)
+        return (out, all_pulses) if with_pulses else out
 
     log("generating inputs")
     t_gen = time.time()
@@ -310,14 +371,17 @@ def main():
     achieved = adds / t_dd / 1e12
     kname = {"f32": "dedisperse_kernel", "f16": "dedisperse_f16_kernel", "u8": "dedisperse_u8_kernel"}[dd_dtype]
     if dd_dtype == "f32" and chain.plan.tile[3] == 16:
-        kname = "dedisperse_f32x4_kernel"
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
+        kname = "dedisperse_ring_kernel"
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS_FP32, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_TFLOPS_FP32, 4), "traffic": traffic_from_profile(dd_dtype),
+                "traffic_source": "profiles/dedisperse_lofar_beam_counters.md (rocprofv3 --pmc FETCH_SIZE, "
+                                  "WRITE_SIZE per dispatch, scaled by a copy of known size; not measured in this run)",
                 "kernel": kname, "ops_per_launch": adds, "alg_bytes_per_launch": alg_bytes,
                 "hbm_gbs_algorithmic": round(alg_bytes / t_dd / 1e9, 1),
-                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32): issue peak 39.3 T/s; "
-                        "one 4-byte LDS read per add: 19.7 T/s at 128 B/clk/CU, 17.5 T/s at the 114 B/clk/CU "
-                        "ds_read_b32 rate measured on gfx950 (ds_read_b128: 228 B/clk/CU)"}
+                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32), priced against the FP32 peak; "
+                        "the kernel is VALU-bound: one v_add_f32 per add, a wave64 VALU instruction per clock "
+                        "and CU, 34.4 T adds/s at the 2.1 GHz it runs at under this load (39.3 T/s at 2.4 GHz); "
+                        "its 16-byte LDS reads (228 B/clk/CU) would allow about 57 adds/clk/CU"}
 
     # ---- mixed precision: the same inputs through chains with FP16 / u8
     # dedispersion input (FP32 / int32 sums), against a plain FP32 chain.
@@ -327,11 +391,13 @@ def main():
     # the same stream without the broadband impulses, where the pulses are
     # found.
     mixed = {}
+    pulses_recovered = None
     if dd_dtype == "f32" and not args.no_mixed:
-        pchunks = make_chunks(4048, False)
+        pchunks, ppulses = make_chunks(4048, False, with_pulses=True)
         pprime = pchunks[-1][:, :, :overlap * INTEG * N_CH].contiguous()
         nrm = float(np.sqrt(N_CHAN))  # series noise in the clipper's normalised units
         ref_out, ref_events = None, None
+        fp64 = None  # (trial indices, FP64 sums of the plain chain's last buffer)
         for mdt in ("f32", "f16", "u8"):
             log("mixed-precision chain " + mdt)
             mc = make_chain(mdt)
@@ -348,15 +414,23 @@ def main():
             mev = run_steps(mc, pchunks, args.steps, mt)
             torch.cuda.synchronize()
             mwall = time.perf_counter() - t0
-            if mdt == "u8":
-                series = (mc.out.double() - 128.0 * N_CHAN) / inv_scale
-            else:
-                series = mc.out.double()
             rec = {"value": TDMS * nout * args.steps / mwall, "ms_per_step": 1e3 * mwall / args.steps,
                    "dedisperse_ms": round(sum(ev[3].elapsed_time(ev[4]) for ev in mt) / args.steps, 3),
                    "tile_dm_t_chanblock_readbytes": list(mc.plan.tile),
                    "autotune_ms": None if m_tune is None else round(m_tune, 3),
                    "events": sum(len(e[0]) - 1 for e in mev)}
+            # one more step by hand, keeping the dedispersion input: its series
+            # against the FP64 sums of the plain chain's input on a DM sample
+            keep = last_step_keeping_input(mc, pchunks[args.steps % len(pchunks)])
+            series = (mc.series if mdt == "u8" else mc.out).double()
+            if mdt == "f32":
+                fp64 = fp64_sample(torch, keep, mc.plan, 16)
+                pulses_recovered = count_recovered(mev, ppulses, 0, len(pchunks), nout, overlap)
+            del keep
+            trials, ref64 = fp64
+            e64 = (series[trials] - ref64).abs()
+            rec.update({"series_vs_fp64_max_abs_err_sigma": float(e64.max()) / nrm,
+                        "series_vs_fp64_rms_err_sigma": float(e64.pow(2).mean().sqrt()) / nrm})
             if mdt == "f32":
                 ref_out, ref_events = series.clone(), mev
             else:
@@ -373,11 +447,17 @@ def main():
             mixed[mdt] = rec
             del mc, series
             torch.cuda.empty_cache()
+        del fp64
         mixed["stream"] = "the workload without broadband impulses (pulses and narrowband RFI only)"
         mixed["error_unit"] = "sigma of a dedispersed sample (sqrt(nChan) in normalised units)"
+        mixed["fp64_reference"] = ("FP64 sums, over 16 trials spread across the DM range and every output "
+                                   "sample, of the plain chain's last dedispersion input (the clipper's FP32 "
+                                   "output, which every chain here shares); the series_* errors without "
+                                   "vs_fp64 compare against the plain FP32 chain's series")
         mixed["note"] = ("f16 rounds each clipped sample to 11 bits; u8 quantises to 1/12.7 over +-10 sigma, "
-                         "and the spectra the clipper rejects (-0.2 minus the bandpass, thousands of raw "
-                         "units, as in the reference) saturate it, which is where its events differ")
+                         "and spectra most of whose samples leave that range (the clipper's rejected spectra, "
+                         "-0.2 minus the bandpass) are quantised relative to their middle channel and restored "
+                         "at their own level in the series (ppl_dedisp_u8_series)")
 
     mbeam = None
     if not args.no_multibeam:
@@ -429,6 +509,12 @@ def main():
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "graph_ms_per_step": round(graph_ms, 3),
         "events_per_step": [len(e[0]) for e in plain_events],
+        "events_note": ("on this stream the clipper rejects the 0.5 % impulse spectra and leaves them at -0.2 "
+                        "minus the bandpass, as the reference does (RFI_Clipper.cpp:127-150,488); nearly every "
+                        "DM trial's series then lands on one and stays far below the threshold, so a step's "
+                        "event list is the dummy event alone; pulses_recovered counts the injected pulses "
+                        "found on the same stream without the impulses (the mixed run)"),
+        "pulses_recovered": pulses_recovered,
         "clipper_walk": chain.clip.counters(),
         "mixed": mixed,
         "precision_256ch": prec,
@@ -531,7 +617,7 @@ def precision(args, torch, ppl):
         names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "carry", "detect"]
         stages = {nm: round(sum(e[j].elapsed_time(e[j + 1]) for e in timing) / args.prec_steps, 3)
                   for j, nm in enumerate(names)}
-        series = (ch.out.double() - 128.0 * n_chan) / inv_scale if dd == "u8" else ch.out.double()
+        series = (ch.series if dd == "u8" else ch.out).double()
         rec = {"dft": dft, "dedisp_input": dd, "ms_per_step": round(1e3 * wall, 3),
                "value": TDMS * nout / wall, "real_time_factor": nout * tsamp / wall, "stages_ms": stages,
                "events": sum(len(e) - 1 for e in events)}
@@ -842,6 +928,11 @@ def cpu_baseline(n_spec, coeffs, bp, nout):
     I = orc.stokes_i_f32(spec)
     I4 = orc.integrate(I, INTEG)
     t["stokes_integrate"] = time.perf_counter() - t0
+    # the clipper's first `history` spectra are its learning phase (band minima
+    # on every spectrum); prime it past that off the clock so the timed spectra
+    # take its steady-state path, as a long-running stream does
+    reps = -(-10000 // n_spec)
+    clip.run(np.ascontiguousarray(np.concatenate([I4] * reps)[:10000]))
     t0 = time.perf_counter()
     clip.run(I4)
     t["clipper"] = time.perf_counter() - t0
@@ -869,9 +960,9 @@ def cpu_baseline(n_spec, coeffs, bp, nout):
         pass
     return {"value": TDMS / per_spectrum, "unit": "DM-trials*samples/s", "cores": threads,
             "kind": "port",
-            "sample": "oracle chain (FP32 FIR+FFT PPF, Stokes-I, x4, clipper, buffer, OpenMP brute-force "
-                      "dedispersion of 2048 DMs, detection) over %d integrated spectra of the same "
-                      "workload, %.1f s; scaled per output sample" % (n_spec, total),
+            "sample": "oracle chain (FP32 FIR+FFT PPF, Stokes-I, x4, clipper past its learning phase, "
+                      "buffer, OpenMP brute-force dedispersion of 2048 DMs, detection) over %d integrated "
+                      "spectra of the same workload, %.1f s; scaled per output sample" % (n_spec, total),
             "real_time_factor": TSAMP / per_spectrum,
             "stages_s": {k: round(v, 3) for k, v in t.items()},
             "cpu_model": model, "nproc": os.cpu_count()}
