@@ -561,7 +561,7 @@ def precision(args, torch, ppl):
     bp = beam_bandpass(coeffs, N_SUB, n_ch, INTEG, FCH1, foff, SIGMA)
     inv_scale = 127.0 / 10.0
     chains = [("fp32", "f32"), ("bf16x3", "f16"), ("fp16x2", "f16"), ("bf16x3", "u8")]
-    tiles = {"f32": (16, 512, 4), "f16": (32, 512, 4), "u8": (32, 512, 4)}
+    tiles = {"f32": None, "f16": (32, 512, 4), "u8": (32, 512, 4)}  # f32: autotuned
     # channeliser error on a sample: 2 subbands x 96 blocks against FP64
     rng = np.random.default_rng(77)
     sample = np.clip(np.rint(rng.normal(0, SIGMA, size=(2, 2, 96 * n_ch, 2))), -32768, 32767).astype(np.int16)
@@ -592,11 +592,14 @@ def precision(args, torch, ppl):
                        "on 65536-sample buffers, pulses + narrowband RFI" % (tsamp * 1e3),
            "nOut_per_step": nout, "data_s_per_step": round(nout * tsamp, 1), "dft_error_vs_fp64": dft_err}
     ref_series, ref_events = None, None
+    fp64 = None
     nrm = float(np.sqrt(n_chan))
     for dft, dd in chains:
         log("precision: chain dft=%s dedisp=%s" % (dft, dd))
         ch = ppl.BeamChain(N_SUB, n_ch, N_TAPS, coeffs, INTEG, bp, NSAMP_BUF, TDMS, 0.0, DM_STEP, tsamp, FCH1, foff,
                            dtype=dd, inv_scale=inv_scale, tile=tiles[dd], dft=dft)
+        if tiles[dd] is None:
+            ch.autotune()
         ch.prime(prime)
         for _ in range(args.warmup):
             ch.step(volt)
@@ -617,10 +620,20 @@ def precision(args, torch, ppl):
         names = ["ppf_stokes", "clipper", "buffer", "dedisperse", "carry", "detect"]
         stages = {nm: round(sum(e[j].elapsed_time(e[j + 1]) for e in timing) / args.prec_steps, 3)
                   for j, nm in enumerate(names)}
+        # one more step by hand, keeping its dedispersion input: the plain
+        # chain's gives the FP64 reference sums on a DM sample
+        keep = last_step_keeping_input(ch, volt)
+        if fp64 is None:
+            fp64 = fp64_sample(torch, keep, ch.plan, 16)
+        del keep
         series = (ch.series if dd == "u8" else ch.out).double()
-        rec = {"dft": dft, "dedisp_input": dd, "ms_per_step": round(1e3 * wall, 3),
+        e64 = (series[fp64[0]] - fp64[1]).abs()
+        rec = {"dft": dft, "dedisp_input": dd, "tile_dm_t_chanblock_readbytes": list(ch.plan.tile),
+               "ms_per_step": round(1e3 * wall, 3),
                "value": TDMS * nout / wall, "real_time_factor": nout * tsamp / wall, "stages_ms": stages,
-               "events": sum(len(e) - 1 for e in events)}
+               "events": sum(len(e) - 1 for e in events),
+               "series_vs_fp64_max_abs_err_sigma": float(e64.max()) / nrm,
+               "series_vs_fp64_rms_err_sigma": float(e64.pow(2).mean().sqrt()) / nrm}
         if ref_series is None:
             ref_series, ref_events = series.clone(), events
         else:
@@ -639,6 +652,10 @@ def precision(args, torch, ppl):
     del volt, prime
     torch.cuda.empty_cache()
     out["error_unit"] = "channeliser: max / RMS error over the spectrum's RMS; series: sigma of a dedispersed sample"
+    out["fp64_reference"] = ("series_vs_fp64: against FP64 sums, over 16 trials spread across the DM range and "
+                             "every output sample, of the fp32+f32 chain's last dedispersion input (its FP32 "
+                             "channeliser and clipper output); the other chains' errors include their "
+                             "channeliser's.  series_* without vs_fp64 compare with the fp32+f32 chain's series")
     return out
 
 
