@@ -299,3 +299,23 @@ def test_run_into_rejects_a_mismatched_buffer(ppl):
     cb.run(db)
     assert torch.equal(da, db)
     assert ca.stats() == cb.stats()
+
+
+@pytest.mark.parametrize("case", ["mean +35", "mean +36", "25% bright in 3 bands", "25% bright in 4 bands",
+                                  "one band all bright", "26 bright channels", "25 bright channels"])
+def test_clipper_steady_state_known_answers(orc, ppl, case):
+    """The hand-worked steady-state cases of the oracle tests (tolerance
+    edge, history rollback, bad-band and total-kept rules) through the
+    device clipper: outputs, flags and running statistics as the oracle's."""
+    from test_oracle_clipper import KA_HIST, _ka_bp, ka_stream
+    x = ka_stream(case)
+    ref = x.copy()
+    oc = orc.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    oc.run(ref)
+    gc = ppl.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    d = torch.from_numpy(x).cuda()
+    gc.run(d)
+    assert np.array_equal(gc.flags(x.shape[0]), oc.last_flags(x.shape[0])[0])
+    assert np.array_equal(d.cpu().numpy(), ref)
+    st_o, _ = oc.state()
+    assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])

@@ -223,3 +223,116 @@ def test_clipper_nbins_not_multiple_of_8(orc):
     c = orc.Clipper(orc.read_bp(os.path.join(GOLDEN, "band31.bp")))
     with pytest.raises(ValueError):
         c.run(np.ones((2, 3, 1, 5), np.float32))
+
+
+# ---- steady-state known answers (RFI_Clipper.cpp:288-440), worked by hand.
+#
+# A flat bandpass and spectra whose channels alternate 1400, 1600 (128 bins,
+# 8 bands of 16): every band has mean 1500 and sigma exactly 100.  After the
+# 1000-spectrum learning phase the state is exactly meanRunAve = 1500,
+# rmsRunAve = 100, meanOverRMS = 15 and the bandpass sits at 1500, so in the
+# steady state (dataModel = 0, :305):
+#  * margin = crFactor * rmsRunAve = 1000: a channel 2000 above its level is
+#    clipped, one at +/-100 kept (:311-342);
+#  * tol = srFactor * bpRms / sqrt(nBins) = 400 / sqrt(128) = 35.355 (:421-425):
+#    a spectrum whose kept mean is 35 above the running mean is kept, 36 above
+#    rejected (:415-420);
+#  * the running mean adds the newest ring entry before this spectrum's sum is
+#    pushed (:394-399), so a kept spectrum moves meanRunAve one spectrum late;
+#    a rejected one is popped and replaced by a copy of the previous entry
+#    (:432-435) and never moves it;
+#  * a band with fewer than 80 % kept channels scores 1, an empty band 4 more,
+#    fewer than 80 % kept over the spectrum 4 more, and a score of 4 rejects
+#    (:355-372);
+#  * a kept spectrum's clipped channels take the last good spectrum's values
+#    (:319-335), and kept spectra come out zero-DMed with unit RMS: -1, +1;
+#    a rejected one is -0.2 minus the bandpass, -1500.2 (:127-150, 488).
+KA_HIST = 1000
+
+
+def _ka_normal(n):
+    x = np.empty((n, 8, 1, 16), np.float32)
+    x[..., 0::2] = 1400.0
+    x[..., 1::2] = 1600.0
+    return x
+
+
+def _ka_bright(per_band):
+    """One spectrum with the first per_band[b] channels of band b 2000 higher."""
+    s = _ka_normal(1).reshape(128)
+    for b, n in enumerate(per_band):
+        s[b * 16:b * 16 + n] += 2000.0
+    return s.reshape(1, 8, 1, 16)
+
+
+def _ka_offset(d):
+    return _ka_normal(1) + np.float32(d)
+
+
+# name -> (case spectrum, kept?)
+KA_CASES = {
+    "mean +35": (_ka_offset(35.0), True),
+    "mean +36": (_ka_offset(36.0), False),
+    "25% bright in 3 bands": (_ka_bright([4, 4, 4, 0, 0, 0, 0, 0]), True),
+    "25% bright in 4 bands": (_ka_bright([4, 4, 4, 4, 0, 0, 0, 0]), False),
+    "one band all bright": (_ka_bright([16, 0, 0, 0, 0, 0, 0, 0]), False),
+    "26 bright channels": (_ka_bright([4, 4, 3, 3, 3, 3, 3, 3]), False),
+    "25 bright channels": (_ka_bright([4, 3, 3, 3, 3, 3, 3, 3]), True),
+}
+
+
+def ka_stream(case):
+    """Learning phase, 20 steady spectra, the case spectrum, 3 more normal ones."""
+    return np.concatenate([_ka_normal(KA_HIST + 20), KA_CASES[case][0], _ka_normal(3)])
+
+
+def _ka_bp():
+    from synth import bandpass_values
+    return bandpass_values(128, 141.5625, -0.01220703125, 1000.0, 40.0, poly=(0.0,))
+
+
+def test_steady_state_exact_state(orc):
+    c = orc.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    x = _ka_normal(KA_HIST + 20)
+    c.run(x)
+    st, sizes = c.state()
+    assert (st[0], st[1], st[4]) == (1500.0, 100.0, 15.0) and list(sizes) == [KA_HIST, KA_HIST]
+    assert np.all(c.bandpass(128) == 1500.0)
+    good, _ = c.last_flags(x.shape[0])
+    # rejected while the mean buffer holds fewer than 1000 entries: the
+    # 1000th spectrum is the first kept one
+    assert not good[:KA_HIST - 1].any() and good[KA_HIST - 1:].all()
+    assert np.all(x[KA_HIST:, :, 0, 0::2] == -1.0) and np.all(x[KA_HIST:, :, 0, 1::2] == 1.0)
+
+
+@pytest.mark.parametrize("case", sorted(KA_CASES))
+def test_steady_state_known_answers(orc, case):
+    spec, kept = KA_CASES[case]
+    t = KA_HIST + 20
+    x = ka_stream(case)
+    c = orc.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    c.run(x[:t + 1])
+    good, nclip = c.last_flags(t + 1)
+    assert bool(good[t]) == kept
+    n_bright = int(np.count_nonzero(spec.reshape(128) > 2000.0))
+    assert nclip[t] == n_bright
+    # the running mean is unchanged right after the case spectrum either way:
+    # its update reads the previous ring entry
+    assert c.state()[0][0] == 1500.0
+    rest = x[t + 1:].copy()
+    c.run(rest)
+    m = c.state()[0][0]
+    control = orc.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    cx = np.concatenate([_ka_normal(KA_HIST + 20 + 1), _ka_normal(3)])
+    control.run(cx)
+    if kept:
+        kept_mean = spec.reshape(128)[spec.reshape(128) < 2000.0].astype(np.float64).mean()
+        assert m == pytest.approx(1500.0 + (kept_mean - 1500.0) / KA_HIST, abs=2e-4)
+        # the clipped channels took the last good values: the spectrum is a
+        # normal one again
+        assert np.all(x[t, :, 0, 0::2] == -1.0) and np.all(x[t, :, 0, 1::2] == 1.0)
+    else:
+        # rolled back: the state and every later output equal the control's
+        assert m == 1500.0 and np.array_equal(c.state()[0], control.state()[0])
+        assert np.all(x[t, :, 0, :] == np.float32(-0.2) - np.float32(1500.0))
+        assert np.array_equal(rest, cx[t + 1:])

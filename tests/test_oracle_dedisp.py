@@ -286,3 +286,55 @@ def test_u8_rows_quantiser_and_series_pinned(orc):
                     corr = np.float32(corr + np.float32(off[e] * np.float32(cnt)))
             v = np.float32(np.float32(sums[dm, t] - 128 * n) * np.float32(1.0 / inv)) + corr
             assert got[dm, t] == np.float32(v), (dm, t)
+
+
+def _dyadic_sum(row, t, bf):
+    """A window's sum as the analyser's binning tree defines it
+    (DedispersionAnalyser.cpp:105-128): the sum of a window of 2^k samples is
+    the float sum of its two halves' sums."""
+    if bf == 1:
+        return np.float32(row[t])
+    h = bf // 2
+    return np.float32(_dyadic_sum(row, t, h) + _dyadic_sum(row, t + h, h))
+
+
+@pytest.mark.parametrize("integer", [True, False])
+def test_detection_brute_force_every_level(orc, integer):
+    """Every dyadic window of every level, enumerated: window [t, t + 2^k)
+    with t a multiple of 2^k inside each whole 64-sample segment, k = 0..6,
+    is an event exactly when its sum reaches threshold * rms * sqrt(2^k) in
+    float (rms = sqrt(nChannels * nSubbands), DedispersionAnalyser.cpp:
+    87-128).  The event set and values must be these windows and sums, at
+    every level.  On integer data every summation order gives the same sums,
+    so the enumeration alone decides; on float data the sums follow the
+    halving tree."""
+    rng = np.random.default_rng(31 if integer else 32)
+    tdms, nsamp, nch, nsub, bin_pow2, thr = 3, 64 * 4 + 23, 4, 4, 6, 6.0
+    if integer:
+        data = rng.integers(-6, 10, size=(tdms, nsamp)).astype(np.float32)
+        data[0, 5] = 30.0
+        data[1, 100:104] = 25.0
+    else:
+        data = rng.normal(2.0, 4.0, size=(tdms, nsamp)).astype(np.float32)
+        data[0, 10] = 40.5
+        data[1, 66:68] = 25.25
+        data[2, 130:146] += 9.0
+    rms = np.float32(np.sqrt(np.float32(nch) * np.float32(nsub)))
+    ev, vals = orc.detect(data, nch, nsub, thr, bin_pow2)
+    got = {(int(d), int(t), int(b)): np.float32(v) for (d, t, b), v in zip(ev[1:], vals[1:])}
+    want = {}
+    for dm in range(tdms):
+        for seg in range(nsamp // 64):
+            for k in range(bin_pow2 + 1):
+                bf = 1 << k
+                level = np.float32(np.float32(thr) * rms) * np.float32(np.sqrt(np.float32(bf)))
+                for t in range(seg * 64, seg * 64 + 64, bf):
+                    s = (np.float32(np.sum(data[dm, t:t + bf], dtype=np.float64)) if integer
+                         else _dyadic_sum(data[dm], t, bf))
+                    if s >= level:
+                        want[(dm, t, bf)] = s
+    assert len(ev) - 1 == len(got)  # no window reported twice
+    assert set(got) == set(want)
+    assert all(got[k] == want[k] for k in want)
+    # every level has events, so a detector dropping one fails here
+    assert {b for (_, _, b) in want} == {1 << k for k in range(bin_pow2 + 1)}
