@@ -390,10 +390,16 @@ typedef struct {
     int zeroDM;               /* zeroDMing active */
     int bandMatching;         /* Band matching (must be 1) */
     int useMeanOverRMS;       /* computeRMSfromMean active (default 1) */
+    const unsigned* kill_index; /* indices of the .bp file's K lines (BandPassAdapter.cpp:58-71), or NULL */
+    int n_kill;               /* entries in kill_index; 0: no killed bands */
 } ppl_clipper_config;
 
 /* RFI_Clipper (src/lib/src/RFI_Clipper.cpp:22-104): bandpass model from the
- * .bp values (BandPassAdapter.cpp:30-107), running state on the device. */
+ * .bp values (BandPassAdapter.cpp:30-107), running state on the device.
+ * Killed channels (K lines) are checked as the adapter checks them
+ * (binIndex(index) == index on the file's map, else PPL_E_ARG with the
+ * adapter's message) and hold 0.0 in the bandpass on every map
+ * (BandPass.cpp:101-157). */
 int ppl_clipper_create(ppl_clipper** clip, const ppl_clipper_config* cfg);
 int ppl_clipper_destroy(ppl_clipper* clip);
 /* RFI_Clipper::run (:154-573) in place on I [nT][nSub][nPol][nCh]; W (may be
@@ -427,6 +433,13 @@ int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nS
 /* Scalars after the last run: meanRunAve, rmsRunAve (WeightedSpectrumDataSet
  * setMean/setRMS, :569-570); synchronises. */
 int ppl_clipper_stats(ppl_clipper* clip, float* mean, float* rms);
+/* The live bandpass model, RFI_Clipper::bandPass() (RFI_Clipper.h:38, kept
+ * current by setMean/setRMS after every spectrum, RFI_Clipper.cpp:562-563):
+ * its constant coefficient, mean and RMS on the current map and, when
+ * values is not NULL, its nValues (= the run's bins) values on the run map
+ * (killed bins 0.0).  Before the first run: the file's model, no values. */
+int ppl_clipper_model(ppl_clipper* clip, float* params0, float* mean, float* rms, float* values,
+                      unsigned nValues);
 /* Per-spectrum flags of the last run (1 = spectrum kept), host buffer. */
 int ppl_clipper_flags(ppl_clipper* clip, uint8_t* h_good, unsigned nT);
 /* Walk counters since creation (indices below); writes min(n, PPL_CLIP_NCOUNTERS)

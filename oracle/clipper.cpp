@@ -28,6 +28,9 @@ struct BinMapR {
     void setEnd(double end) { setBinWidth((end - lower) / nBins); }  // BinMap.cpp:67-70
     double binAssignmentNumber(int i) const { return lower + (width * i); }
     double endValue() const { return lower + width * nBins; }
+    int binIndex(double v) const { return (int)(0.5 + ((v - lower) / width)); }     // BinMap.cpp:49-52
+    double binStart(unsigned i) const { return binAssignmentNumber(i) - halfwidth; }  // :72-75
+    double binEnd(unsigned i) const { return binAssignmentNumber(i) + halfwidth; }    // :77-80
 };
 
 // boost::circular_buffer<float> semantics used by the clipper.
@@ -58,7 +61,12 @@ struct BandPassR {
     int primaryId = 0, currentId = 0;
     std::vector<float> params{0.0f};
     std::map<int, std::vector<float>> dataSets;
+    std::map<int, BinMapR> maps;  // the map of each data set
     std::map<int, float> rms_, median_, mean_;
+    // Range<float>::subranges of the killed bands (src/lib/Range.h:40-92): each
+    // (min, max) with min <= max
+    std::vector<std::pair<float, float>> killed;
+    bool killOutside = false;  // a kill reached outside a map (the reference writes out of bounds)
 
     // BandPass::_evaluate :159-166 -- tot is float, each term in double.
     float evaluate(float v) const {
@@ -67,13 +75,34 @@ struct BandPassR {
             tot += (double)params[i] * std::pow((double)v, (double)(int)i);
         return tot;
     }
-    // BandPass::_buildData :101-108 (no killed bands)
+    // BandPass::_buildData :101-109, then _zeroChannelsMap
     void buildData(const BinMapR& map, float scale) {
         int id = idOf(map);
         std::vector<float> d(map.nBins);
         for (unsigned i = 0; i < map.nBins; ++i) d[i] = scale * evaluate((float)map.binAssignmentNumber(i));
         dataSets[id] = d;
+        maps[id] = map;
+        zeroChannelsMap(id);
     }
+    // BandPass::_zeroChannelsMap :111-125: every killed range's bins, from
+    // binIndex(min) to binIndex(max) inclusive, are set to 0.0
+    void zeroChannelsMap(int id) {
+        const BinMapR& map = maps[id];
+        std::vector<float>& d = dataSets[id];
+        for (const auto& r : killed) {
+            int lo = map.binIndex(r.first), hi = map.binIndex(r.second);
+            if (hi < lo) std::swap(lo, hi);
+            if ((int)d.size() < hi) d.resize(hi + 1);
+            if (lo < 0 || hi >= (int)d.size()) { killOutside = true; continue; }
+            do { d[lo] = 0.0f; } while (++lo <= hi);
+        }
+    }
+    // BandPass::killBand :151-157 and killChannel :141-144 (on the current map)
+    void killBand(float start, float end) {
+        killed.emplace_back(std::min(start, end), std::max(start, end));
+        for (auto& kv : dataSets) zeroChannelsMap(kv.first);
+    }
+    void killChannel(unsigned index) { killBand((float)current.binStart(index), (float)current.binEnd(index)); }
     // BandPass::reBin :81-99
     void reBin(const BinMapR& map) {
         current = map;
@@ -157,6 +186,20 @@ extern "C" orc_clipper* orc_clipper_create(const double* v, int n, float crFacto
                                            int maxHistory, int zeroDM, int bandMatching,
                                            int useMeanOverRMS)
 {
+    return orc_clipper_create_kill(v, n, nullptr, 0, crFactor, srFactor, maxHistory, zeroDM, bandMatching,
+                                   useMeanOverRMS);
+}
+
+// The same with the indices of the file's K lines.  BandPassAdapter takes
+// the kill line's first number for both the index and the frequency
+// (BandPassAdapter.cpp:58-71), keeps them in a QMap (unique, ascending), and
+// after setRMS checks binIndex(frequency) == index on the file's map before
+// killChannel(index) (:101-106).  Returns nullptr when a check fails or a
+// kill reaches outside the map (where the reference writes out of bounds).
+extern "C" orc_clipper* orc_clipper_create_kill(const double* v, int n, const unsigned* kills, int nKill,
+                                                float crFactor, float srFactor, int maxHistory, int zeroDM,
+                                                int bandMatching, int useMeanOverRMS)
+{
     if (n < 6) return nullptr;
     orc_clipper* c = new orc_clipper;
     // BandPassAdapter::deserialise :76-100.  Values pass through QVector<float>.
@@ -175,6 +218,14 @@ extern "C" orc_clipper* orc_clipper_create(const double* v, int n, float crFacto
     c->bp.setData(map, params);
     c->bp.setMedian(median);
     c->bp.setRMS(rms);
+    std::map<unsigned, float> killChannels;
+    for (int k = 0; k < nKill; ++k) killChannels[kills[k]] = (float)(double)kills[k];
+    for (const auto& kv : killChannels) {
+        const unsigned index = (unsigned)kv.second;  // foreach over the QMap's values
+        if (map.binIndex(killChannels[index]) != (int)index) { delete c; return nullptr; }
+        c->bp.killChannel(index);
+    }
+    if (c->bp.killOutside) { delete c; return nullptr; }
     // RFI_Clipper::RFI_Clipper :54-79
     c->crFactor = crFactor;
     c->srFactor = srFactor;
@@ -215,6 +266,7 @@ extern "C" int orc_clipper_run(orc_clipper* C, float* I, float* W, unsigned nSam
     C->map.setStart(C->startFrequency);
     C->map.setEnd(C->endFrequency);
     C->bp.reBin(C->map);
+    if (C->bp.killOutside) return -2;  // a killed range outside the run map
 
     for (unsigned t = 0; t < nSamples; ++t) {
         const std::vector<float> bandPass = C->bp.currentSet();
@@ -444,7 +496,7 @@ extern "C" int orc_bandpass_probe(unsigned nBins, double start, double width, co
 extern "C" int orc_clipper_bandpass(orc_clipper* c, float* out, unsigned nBins)
 {
     const std::vector<float>& d = c->bp.currentSet();
-    if (d.size() != nBins) return -1;
+    if (d.size() < nBins) return -1;  // a kill past the map's end lengthens it (BandPass.cpp:118)
     for (unsigned i = 0; i < nBins; ++i) out[i] = d[i];
     return 0;
 }

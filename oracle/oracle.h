@@ -107,6 +107,13 @@ orc_clipper* orc_clipper_create(const double* bp_values, int n_values,
                                 float crFactor, float srFactor, int maxHistory,
                                 int zeroDM, int bandMatching,
                                 int useMeanOverRMS);
+/* The same with the indices of the .bp file's K (kill) lines
+ * (BandPassAdapter.cpp:58-71,101-106, BandPass.cpp:111-157); nullptr when an
+ * index fails the adapter's frequency check or reaches outside the map. */
+orc_clipper* orc_clipper_create_kill(const double* bp_values, int n_values,
+                                     const unsigned* kill_index, int n_kill,
+                                     float crFactor, float srFactor, int maxHistory,
+                                     int zeroDM, int bandMatching, int useMeanOverRMS);
 void orc_clipper_destroy(orc_clipper* c);
 /* In-place run over one blob [nT][nSub][nPol][nCh]; weights likewise
  * (may be NULL).  Returns 0, or -1 if nSub*nCh is not divisible by 8. */

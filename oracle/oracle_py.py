@@ -55,6 +55,7 @@ def _declare(L):
         "orc_integrate_f32": (None, [vp, u, u, u, u, u, u, vp]),
         "orc_integrate_f64": (None, [vp, u, u, u, u, u, u, vp]),
         "orc_clipper_create": (vp, [vp, i, f, f, i, i, i, i]),
+        "orc_clipper_create_kill": (vp, [vp, i, vp, i, f, f, i, i, i, i]),
         "orc_clipper_destroy": (None, [vp]),
         "orc_clipper_run": (i, [vp, vp, vp, u, u, u, u]),
         "orc_clipper_last_flags": (i, [vp, vp, vp, u]),
@@ -179,12 +180,13 @@ def integrate(x, window, bin_ch=1):
 # -------------------------------------------------------------- clipper
 class Clipper:
     def __init__(self, bp_values, cr=10.0, sr=4.0, history=10000, zero_dm=True,
-                 band_matching=True, mean_over_rms=True):
+                 band_matching=True, mean_over_rms=True, kill=()):
         v = np.ascontiguousarray(bp_values, np.float64)
-        self.h = lib().orc_clipper_create(_p(v), v.size, cr, sr, history, int(zero_dm),
-                                          int(band_matching), int(mean_over_rms))
+        k = np.ascontiguousarray(list(kill), np.uint32)
+        self.h = lib().orc_clipper_create_kill(_p(v), v.size, _p(k) if k.size else None, k.size, cr, sr,
+                                               history, int(zero_dm), int(band_matching), int(mean_over_rms))
         if not self.h:
-            raise ValueError("bad bandpass values")
+            raise ValueError("bad bandpass values or kill indices")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -197,6 +199,8 @@ class Clipper:
         n_t, n_sub, n_pol, n_ch = I.shape
         wp = _p(W) if W is not None else None
         rc = lib().orc_clipper_run(self.h, _p(I), wp, n_t, n_sub, n_pol, n_ch)
+        if rc == -2:
+            raise ValueError("a killed band reaches outside the run map")
         if rc != 0:
             raise ValueError("nBins must be divisible by 8")
         return I
@@ -232,22 +236,36 @@ def bandpass_probe(n_bins, start, width, params, rms, median=float("nan"), rebin
     return tuple(float(v) for v in o)
 
 
-def read_bp(path):
+def read_bp(path, with_kills=False):
     """Numbers of a .bp bandpass file in order (BandPassAdapter::deserialise,
     src/lib/src/BandPassAdapter.cpp:30-100): '#' lines are comments, one
-    number per line; 'K' kill lines are not supported here."""
-    vals = []
+    number per line.  A line starting with 'K' holds an index and a
+    frequency (:58-71): at least two fields, the first an unsigned integer;
+    with_kills returns (numbers, sorted unique kill indices), otherwise a
+    file with kill lines is refused."""
+    vals, kills = [], set()
     with open(path) as fh:
-        for line in fh:
-            line = line.strip()
-            if not line or line.startswith("#"):
+        for n, raw in enumerate(fh, 1):
+            if raw.startswith("#"):
                 continue
-            if line.startswith("K"):
-                raise ValueError("kill lines are not supported")
+            if raw.startswith("K"):
+                nums = raw[1:].split()
+                if len(nums) < 2:
+                    raise ValueError("line %d: expecting two numbers on Kill line: index frequency" % n)
+                if not nums[0].isdigit() or int(nums[0]) > 0xFFFFFFFF:
+                    raise ValueError("line %d: cannot convert index to unsigned integer" % n)
+                kills.add(int(nums[0]))
+                continue
+            line = raw.strip()
+            if not line:
+                continue
             vals.append(float(line))
     if len(vals) < 6:
         raise ValueError("expecting at least 6 parameters")
-    return np.array(vals, np.float64)
+    if kills and not with_kills:
+        raise ValueError("the file has kill lines: read it with with_kills=True")
+    v = np.array(vals, np.float64)
+    return (v, sorted(kills)) if with_kills else v
 
 
 # --------------------------------------------------------- dedispersion

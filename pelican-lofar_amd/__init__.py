@@ -65,7 +65,8 @@ class PacketSeq(C.Structure):
 class ClipperConfig(C.Structure):
     _fields_ = [("bp_values", C.c_void_p), ("n_values", C.c_int), ("crFactor", C.c_float),
                 ("srFactor", C.c_float), ("maxHistory", C.c_int), ("zeroDM", C.c_int),
-                ("bandMatching", C.c_int), ("useMeanOverRMS", C.c_int)]
+                ("bandMatching", C.c_int), ("useMeanOverRMS", C.c_int), ("kill_index", C.c_void_p),
+                ("n_kill", C.c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/ppl_amd.h
@@ -136,6 +137,7 @@ SIGNATURES = {
     "ppl_clipper_run_into": (i, [vp, vp, u, u, u, vp, u, vp, vp, vp]),
     "ppl_clipper_stats": (i, [vp, vp, vp]),
     "ppl_clipper_flags": (i, [vp, vp, u]),
+    "ppl_clipper_model": (i, [vp, vp, vp, vp, vp, u]),
     "ppl_clipper_counters": (i, [vp, vp, i]),
     "ppl_clipper_run_batch": (i, [vp, vp, i, vp]),
 }
@@ -660,10 +662,14 @@ class Clipper:
     """RFI_Clipper with its running state on the device."""
 
     def __init__(self, bp_values, cr=10.0, sr=4.0, history=10000, zero_dm=True,
-                 band_matching=True, mean_over_rms=True):
+                 band_matching=True, mean_over_rms=True, kill=()):
+        """kill: the indices of the .bp file's K lines (killed channels)."""
         self._bp = np.ascontiguousarray(bp_values, np.float64)
+        self._kill = np.ascontiguousarray(list(kill), np.uint32)
         cfg = ClipperConfig(self._bp.ctypes.data_as(C.c_void_p), self._bp.size, cr, sr, history,
-                            int(zero_dm), int(band_matching), int(mean_over_rms))
+                            int(zero_dm), int(band_matching), int(mean_over_rms),
+                            self._kill.ctypes.data_as(C.c_void_p) if self._kill.size else None,
+                            int(self._kill.size))
         h = C.c_void_p()
         check(lib().ppl_clipper_create(C.byref(h), C.byref(cfg)))
         self.h = h
@@ -709,6 +715,16 @@ class Clipper:
         out = np.zeros(n_t, np.uint8)
         check(lib().ppl_clipper_flags(self.h, out.ctypes.data_as(C.c_void_p), n_t))
         return out
+
+    def model(self, n_bins=None):
+        """The live bandpass model (RFI_Clipper::bandPass()): (params[0],
+        mean, rms) and, with n_bins (the run map's bins), its values."""
+        p0, m, r = C.c_float(), C.c_float(), C.c_float()
+        vals = None if n_bins is None else np.zeros(n_bins, np.float32)
+        check(lib().ppl_clipper_model(self.h, C.byref(p0), C.byref(m), C.byref(r),
+                                      None if vals is None else vals.ctypes.data_as(C.c_void_p),
+                                      0 if vals is None else n_bins))
+        return (p0.value, m.value, r.value) if vals is None else (p0.value, m.value, r.value, vals)
 
 
 # ------------------------------------------------------------- beam chain

@@ -53,6 +53,9 @@ struct HostMap {
     double lower = 0.0, width = 1.0;
     double binAssignmentNumber(int i) const { return lower + (width * i); }
     double endValue() const { return lower + width * nBins; }
+    int binIndex(double v) const { return (int)(0.5 + ((v - lower) / width)); }       // BinMap.cpp:49-52
+    double binStart(unsigned i) const { return binAssignmentNumber(i) - width / 2.0; }  // :72-75
+    double binEnd(unsigned i) const { return binAssignmentNumber(i) + width / 2.0; }    // :77-80
 };
 
 struct HostBandPass {
@@ -169,9 +172,17 @@ struct SpecRec {
     int ver;           // last-good version in force for this spectrum
 };
 
-__device__ __forceinline__ float bp_value(const double* __restrict__ terms, int nTerms, int nBins, int i,
-                                          float p0, float scale)
+// Killed bins (BandPass::_zeroChannelsMap, BandPass.cpp:111-125) hold 0.0 in
+// every model the clipper builds; kill is a bit mask over the run map's bins,
+// or null.
+__device__ __forceinline__ bool bin_killed(const unsigned* __restrict__ kill, int i)
 {
+    return kill != nullptr && ((kill[i >> 5] >> (i & 31)) & 1u);
+}
+__device__ __forceinline__ float bp_value(const double* __restrict__ terms, int nTerms, int nBins, int i,
+                                          float p0, float scale, const unsigned* __restrict__ kill)
+{
+    if (bin_killed(kill, i)) return 0.0f;
     // evaluate(): tot = 0; tot += params[0]*1; tot += params[j]*pow(v,j)
     float tot = 0.0f;
     tot = (float)((double)tot + (double)p0 * 1.0);
@@ -303,13 +314,17 @@ __device__ __forceinline__ float q_at(const float4& q, int j) { return (&q.x)[j]
 template <int NT>
 struct BinTerms {
     double t[NT > 0 ? NT : 1];
-    __device__ __forceinline__ void load(const double* __restrict__ terms, int, int nBins, int i)
+    bool killed;
+    __device__ __forceinline__ void load(const double* __restrict__ terms, int, int nBins, int i,
+                                         const unsigned* __restrict__ kill)
     {
 #pragma unroll
         for (int j = 0; j < NT; ++j) t[j] = terms[(size_t)j * nBins + i];
+        killed = bin_killed(kill, i);
     }
     __device__ __forceinline__ float bp(float p0, float scale) const
     {
+        if (killed) return 0.0f;
         float tot = 0.0f;
         tot = (float)((double)tot + (double)p0 * 1.0);
 #pragma unroll
@@ -320,17 +335,20 @@ struct BinTerms {
 template <>
 struct BinTerms<-1> {
     const double* terms;
+    const unsigned* kill;
     int nTerms, nBins, i;
-    __device__ __forceinline__ void load(const double* __restrict__ tm, int nt, int nb, int bin)
+    __device__ __forceinline__ void load(const double* __restrict__ tm, int nt, int nb, int bin,
+                                         const unsigned* __restrict__ km)
     {
         terms = tm;
         nTerms = nt;
         nBins = nb;
         i = bin;
+        kill = km;
     }
     __device__ __forceinline__ float bp(float p0, float scale) const
     {
-        return bp_value(terms, nTerms, nBins, i, p0, scale);
+        return bp_value(terms, nTerms, nBins, i, p0, scale, kill);
     }
 };
 
@@ -472,6 +490,7 @@ struct BeamState {
     float *meanBuf, *rmsBuf, *lastGood, *lastGoodVer, *bpLo, *bpHi;
     double *meanPrefix, *rmsPrefix;
     const double* terms;
+    const unsigned* kill;  // killed bins of the run map (bit mask), or null
     Box* box;
     PreRec* pre;
     LearnRec* learn;
@@ -541,8 +560,8 @@ __global__ void clip_bounds_kernel(BatchIO io)
         *box = b;
     }
     if (i < K.nBins && valid) {
-        const float a = bp_value(terms, K.nTerms, K.nBins, i, p0lo, K.scale);
-        const float b = bp_value(terms, K.nTerms, K.nBins, i, p0hi, K.scale);
+        const float a = bp_value(terms, K.nTerms, K.nBins, i, p0lo, K.scale, bs.kill);
+        const float b = bp_value(terms, K.nTerms, K.nBins, i, p0hi, K.scale, bs.kill);
         bpLo[i] = fminf(a, b);
         bpHi[i] = fmaxf(a, b);
     }
@@ -797,6 +816,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     float* __restrict__ lastGood = bst.lastGood;
     float* __restrict__ lastGoodVer = bst.lastGoodVer;
     const double* __restrict__ terms = bst.terms;
+    const unsigned* __restrict__ kill = bst.kill;
     const Box* __restrict__ box = bst.box;
     const PreRec* __restrict__ pre = bst.pre;
     SpecRec* __restrict__ rec = bst.rec;
@@ -1523,7 +1543,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             const int b = tid >> 6;
             float mm = 1e6f;
             for (int i = b * cpb + (tid & 63); i < (b + 1) * cpb; i += 64) {
-                const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+                const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale, kill);
                 mm = (bpv < mm) ? bpv : mm;
             }
 #pragma unroll
@@ -1546,7 +1566,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 for (int i = b * cpb + tid; i < (b + 1) * cpb; i += kWalkThreads) {
                     const int sb = i / nCh, c = i % nCh;
                     const float v = I[idx(sb, 0, t) + c];
-                    const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+                    const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale, kill);
                     md = (v < md) ? v : md;
                     mm = (bpv < mm) ? bpv : mm;
                 }
@@ -1670,7 +1690,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 int band = 0;
                 if (i < nBins) {
                     const float v = vv[j];
-                    const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale);
+                    const float bpv = bp_value(terms, K.nTerms, nBins, i, s.params0, K.scale, kill);
                     band = band_of(i, invCpbW);
                     if (v - dataModel - bpv > margin) {
                         const int sb = i / nCh, c = i % nCh;
@@ -1902,7 +1922,7 @@ __global__ __launch_bounds__(kTileThreads) void clip_stats_kernel(BatchIO io)
         if (i >= nBins) return;
         BinTerms<NT> bt[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bt[j].load(terms, K.nTerms, nBins, i + j);
+        for (int j = 0; j < 4; ++j) bt[j].load(terms, K.nTerms, nBins, i + j, bs.kill);
 #pragma unroll
         for (int q = 0; q < kRowsPerWave; ++q) {
             const int r = wave + 4 * q;
@@ -1975,7 +1995,7 @@ __global__ __launch_bounds__(256) void clip_write_kernel(BatchIO io)
     const long rowStride = (long)nBins * nPol;
     BinTerms<NT> bt[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bt[j].load(terms, K.nTerms, nBins, i + j);
+    for (int j = 0; j < 4; ++j) bt[j].load(terms, K.nTerms, nBins, i + j, bs.kill);
     for (int r = 0; r < nRows; ++r) {
         const long rowOff = (long)(t0 + r) * rowStride;
         const SpecRec sr = recs[r];
@@ -2021,6 +2041,14 @@ struct ppl_clipper {
     ClipScalars* d_S = nullptr;
     float *d_meanBuf = nullptr, *d_rmsBuf = nullptr, *d_lastGood = nullptr;
     double *d_meanPrefix = nullptr, *d_rmsPrefix = nullptr, *d_terms = nullptr;
+    // killed bands of the .bp file, as Range<float> (min, max) pairs on the
+    // file's map, and their bins on the run map
+    std::vector<std::pair<float, float>> killed;
+    unsigned* d_kill = nullptr;
+    // host copies of the run map's polynomial terms and kill mask, for the
+    // live model's values (ppl_clipper_model)
+    std::vector<double> h_terms;
+    std::vector<unsigned> h_kill;
     float2* d_stats = nullptr;
     SpecRec* d_rec = nullptr;
     PreRec* d_pre = nullptr;
@@ -2044,6 +2072,8 @@ void free_state(ppl_clipper* c)
     (void)hipFree(c->d_meanPrefix);
     (void)hipFree(c->d_rmsPrefix);
     (void)hipFree(c->d_terms);
+    (void)hipFree(c->d_kill);
+    c->d_kill = nullptr;
     (void)hipFree(c->d_stats);
     (void)hipFree(c->d_rec);
     (void)hipFree(c->d_pre);
@@ -2076,6 +2106,21 @@ int setup_map(ppl_clipper* c, unsigned nSub, unsigned nPol, unsigned nCh)
     m.lower = (double)c->startFrequency;
     m.width = ((double)c->endFrequency - m.lower) / nBins;  // BinMap::setEnd
     c->bp.reBin(m);
+    // the killed ranges' bins on the run map (_zeroChannelsMap :111-125):
+    // binIndex(min) to binIndex(max) inclusive; bins past the map's end are
+    // dropped (the reference grows its vector to hold them), and a range
+    // that starts before bin 0 or ends exactly at nBins is refused (the
+    // reference writes outside its vector there)
+    std::vector<unsigned> killMask;
+    for (const auto& r : c->killed) {
+        int lo = m.binIndex(r.first), hi = m.binIndex(r.second);
+        if (hi < lo) std::swap(lo, hi);
+        PPL_CHECK_ARG(lo >= 0 && hi != (int)nBins,
+                      "RFI_Clipper: a killed band (%g to %g) reaches outside the %u-bin map", r.first, r.second,
+                      nBins);
+        if (killMask.empty()) killMask.assign((nBins + 31) / 32, 0u);
+        for (int i = lo; i <= hi && i < (int)nBins; ++i) killMask[(size_t)i >> 5] |= 1u << (i & 31);
+    }
     // _buildData and setMean both use the float width ratio
     const float scale = (float)(c->bp.current.width / c->bp.primary.width);
     const int nTerms = (int)c->bp.params.size() - 1;
@@ -2130,6 +2175,12 @@ int setup_map(ppl_clipper* c, unsigned nSub, unsigned nPol, unsigned nCh)
     PPL_HIP(hipMemset(c->d_lastGood, 0, nBins * sizeof(float)));
     if (!terms.empty())
         PPL_HIP(hipMemcpy(c->d_terms, terms.data(), terms.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!killMask.empty()) {
+        PPL_HIP(hipMalloc(&c->d_kill, killMask.size() * sizeof(unsigned)));
+        PPL_HIP(hipMemcpy(c->d_kill, killMask.data(), killMask.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
+    c->h_terms = terms;
+    c->h_kill = killMask;
     c->nBins = (int)nBins;
     return PPL_OK;
 }
@@ -2161,6 +2212,41 @@ extern "C" int ppl_clipper_create(ppl_clipper** out, const ppl_clipper_config* c
     c->bp.setData(m, params);
     c->bp.setMedian(median);
     c->bp.setRMS(rms);
+    // kill lines (BandPassAdapter.cpp:58-71,101-106): the line's first number
+    // is both the index and the frequency; unique indices in ascending order
+    // (a QMap), each checked against the file's map, then killChannel(index)
+    // on the current (file's) map: the range [binStart, binEnd] as floats
+    // (BandPass.cpp:141-157, Range<float>)
+    if (cfg->n_kill < 0 || (cfg->n_kill > 0 && !cfg->kill_index)) {
+        delete c;
+        ::ppl::set_error("null kill list");
+        return PPL_E_ARG;
+    }
+    std::vector<unsigned> kills(cfg->kill_index, cfg->kill_index + cfg->n_kill);
+    std::sort(kills.begin(), kills.end());
+    kills.erase(std::unique(kills.begin(), kills.end()), kills.end());
+    for (unsigned index : kills) {
+        const float freq = (float)(double)index;
+        if (m.binIndex(freq) != (int)index) {
+            delete c;
+            ::ppl::set_error("BandPassAdapter:  Frequency kill mismatch: index %u does not match frequency %g", index,
+                             (double)freq);
+            return PPL_E_ARG;
+        }
+        const float a = (float)c->bp.current.binStart(index), b = (float)c->bp.current.binEnd(index);
+        c->killed.emplace_back(std::min(a, b), std::max(a, b));
+        // zeroing the file's map: a range ending exactly at nChan is written
+        // out of bounds by the reference (BandPass.cpp:118-123)
+        int lo = m.binIndex(std::min(a, b)), hi = m.binIndex(std::max(a, b));
+        if (hi < lo) std::swap(lo, hi);
+        if (lo < 0 || hi == (int)nChan) {
+            delete c;
+            ::ppl::set_error("BandPassAdapter: kill index %u reaches outside the %u-channel map", index, nChan);
+            return PPL_E_ARG;
+        }
+    }
+    c->cfg.kill_index = nullptr;
+    c->cfg.n_kill = 0;
     c->startFrequency = (float)c->bp.current.lower;
     c->endFrequency = (float)c->bp.current.endValue();
     *out = c;
@@ -2241,6 +2327,7 @@ static int prepare(ppl_clipper* c, const ppl_clip_job& j)
         h.meanPrefix = c->d_meanPrefix;
         h.rmsPrefix = c->d_rmsPrefix;
         h.terms = c->d_terms;
+        h.kill = c->d_kill;
         h.box = c->d_box;
         h.pre = c->d_pre;
         h.learn = c->d_learn;
@@ -2347,6 +2434,44 @@ extern "C" int ppl_clipper_stats(ppl_clipper* c, float* mean, float* rms)
     PPL_HIP(hipMemcpy(&S, c->d_S, sizeof S, hipMemcpyDeviceToHost));
     *mean = S.meanRunAve;
     *rms = S.rmsRunAve;
+    return PPL_OK;
+}
+
+// The live bandpass model: RFI_Clipper::bandPass() after the spectra run so
+// far (setMean / setRMS at the end of every spectrum, RFI_Clipper.cpp:562-563;
+// BandPass.cpp:44-74): the constant coefficient, the current map's mean and
+// RMS, and optionally its values on the run map (scale * evaluate(v), killed
+// bins 0.0), as the device evaluates them.
+extern "C" int ppl_clipper_model(ppl_clipper* c, float* params0, float* mean, float* rms, float* values,
+                                 unsigned nValues)
+{
+    PPL_CHECK_ARG(c && params0 && mean && rms, "null argument");
+    if (!c->d_S) {
+        PPL_CHECK_ARG(!values, "the run map is fixed by the first run; no values before it");
+        *params0 = c->bp.params[0];
+        *mean = c->bp.curMean;
+        *rms = c->bp.curRms;
+        return PPL_OK;
+    }
+    ClipScalars S;
+    PPL_HIP(hipMemcpy(&S, c->d_S, sizeof S, hipMemcpyDeviceToHost));
+    *params0 = S.params0;
+    *mean = S.bpMean;
+    *rms = S.bpRms;
+    if (values) {
+        PPL_CHECK_ARG(nValues == (unsigned)c->nBins, "values: %u entries for a %d-bin map", nValues, c->nBins);
+        const int nb = c->nBins, nt = c->K.nTerms;
+        for (int i = 0; i < nb; ++i) {
+            if (!c->h_kill.empty() && ((c->h_kill[(size_t)i >> 5] >> (i & 31)) & 1u)) {
+                values[i] = 0.0f;
+                continue;
+            }
+            float tot = 0.0f;
+            tot = (float)((double)tot + (double)S.params0 * 1.0);
+            for (int j = 0; j < nt; ++j) tot = (float)((double)tot + c->h_terms[(size_t)j * nb + i]);
+            values[i] = c->K.scale * tot;
+        }
+    }
     return PPL_OK;
 }
 

@@ -144,7 +144,10 @@ class BandPass : public DataBlob {
     // coefficients.  The adapter drops the first five numbers only, so the
     // median becomes the constant coefficient and the file's coefficients
     // move up one power; then setMedian(median) and setRMS(rms).
-    static BandPass fromFileValues(const std::vector<double>& v)
+    // Kill lines (:58-71,101-106): the unique indices in ascending order
+    // (a QMap), each checked against the file's map (the line's first number
+    // is also its frequency) and killed on it.
+    static BandPass fromFileValues(const std::vector<double>& v, const std::vector<unsigned>& kills = {})
     {
         if (v.size() < 6) throw std::runtime_error("BandPassAdapter: data format wrong");
         BandPass bp;
@@ -156,7 +159,35 @@ class BandPass : public DataBlob {
         bp.setData(map, params);
         bp.setMedian((float)v[5]);
         bp.setRMS((float)v[4]);
+        std::vector<unsigned> k(kills);
+        std::sort(k.begin(), k.end());
+        k.erase(std::unique(k.begin(), k.end()), k.end());
+        for (unsigned index : k) {
+            const float freq = (float)(double)index;
+            if (map.binIndex(freq) != (int)index)
+                throw std::runtime_error("BandPassAdapter:  Frequency kill mismatch: index " + std::to_string(index) +
+                                         " does not match frequency " + std::to_string(freq));
+            bp.killChannel(index);
+        }
         return bp;
+    }
+
+    // The clipper's model after a run (RFI_Clipper.cpp:184-187,562-563):
+    // current on the run map, with the constant coefficient, mean and RMS
+    // that setMean / setRMS left; the values are rebuilt as setMean
+    // rebuilds them (:63-74).
+    void setRunState(const BinMap& runMap, float params0, float mean, float rms)
+    {
+        reBin(runMap);
+        const float scale = (float)(_currentMap.width() / _primaryMap.width());
+        if (params0 != _params[0] || mean != _mean[_currentId]) {
+            _params[0] = params0;
+            _mean[_currentId] = mean;
+            _mean[_primaryId] = mean / scale;
+            _data.clear();
+            buildData(_currentMap, scale);
+        }
+        setRMS(rms);
     }
 
   private:

@@ -9,22 +9,35 @@
 namespace pelican {
 namespace ampp {
 
-std::vector<double> readBandPassFile(const std::string& fileName)
+std::vector<double> readBandPassFile(const std::string& fileName, std::vector<unsigned>* kills)
 {
     std::ifstream in(fileName);
     if (!in) throw std::runtime_error("RFI_Clipper: Cannot open File \"" + fileName + "\"");
     std::vector<double> params;
     std::string line;
     int lineNo = 0;
+    if (kills) kills->clear();
     while (std::getline(in, line)) {
         ++lineNo;
         if (!line.empty() && line[0] == '#') continue;  // comment
-        if (!line.empty() && line[0] == 'K')
-            // kill lines name a channel index and frequency; none of the
-            // reference's .bp files carries one, and this build does not
-            // model killed bands
-            throw std::runtime_error("BandPassAdapter: kill lines are not supported (line " +
-                                     std::to_string(lineNo) + ")");
+        if (!line.empty() && line[0] == 'K') {
+            // kill line (BandPassAdapter.cpp:58-71): "index frequency", at
+            // least two whitespace-separated fields, the first an unsigned
+            // integer (the reference takes it for the frequency too)
+            std::istringstream fields(line.substr(1));
+            std::vector<std::string> nums;
+            for (std::string w; fields >> w;) nums.push_back(w);
+            if (nums.size() < 2)
+                throw std::runtime_error("BandPassAdapter: syntax error on line " + std::to_string(lineNo) +
+                                         ": expecting two numbers on Kill line: index frequency");
+            const std::string& n0 = nums[0];
+            if (n0.empty() || n0.find_first_not_of("0123456789") != std::string::npos || n0.size() > 10 ||
+                std::stoull(n0) > 0xFFFFFFFFull)
+                throw std::runtime_error("BandPassAdapter: syntax error on line " + std::to_string(lineNo) +
+                                         " - cannot convert index to unsigned integer");
+            if (kills) kills->push_back((unsigned)std::stoul(n0));
+            continue;
+        }
         // QByteArray::trimmed().toDouble(): the whole trimmed line is a number
         size_t a = line.find_first_not_of(" \t\r\n\v\f");
         size_t b = line.find_last_not_of(" \t\r\n\v\f");
@@ -56,11 +69,15 @@ RFI_Clipper::RFI_Clipper(const ConfigNode& config) : AbstractModule(config)
     } catch (const std::exception& e) {
         throw std::runtime_error(std::string("RFI_Clipper: ") + e.what());
     }
-    _bp = readBandPassFile(path);
-    _bandPass = BandPass::fromFileValues(_bp);
+    _bp = readBandPassFile(path, &_kill);
+    _bandPass = BandPass::fromFileValues(_bp, _kill);
+    _startFrequency = _bandPass.startFrequency();
+    _endFrequency = _bandPass.endFrequency();
     ppl_clipper_config cfg{};
     cfg.bp_values = _bp.data();
     cfg.n_values = (int)_bp.size();
+    cfg.kill_index = _kill.empty() ? nullptr : _kill.data();
+    cfg.n_kill = (int)_kill.size();
     cfg.crFactor = config.hasAttribute("channelRejectionRMS") ? config.getAttribute("channelRejectionRMS").toFloat()
                                                               : 10.0f;
     cfg.srFactor = config.hasAttribute("spectrumRejectionRMS")
@@ -90,6 +107,14 @@ void RFI_Clipper::run(WeightedSpectrumDataSet* weightedStokes)
     ppl_check(ppl_clipper_stats(_clip, &mean, &rms), "RFI_Clipper");
     weightedStokes->setMean(mean);
     weightedStokes->setRMS(rms);
+    // the live model: the run map (:184-187) with the device's constant
+    // coefficient, mean and RMS after the last spectrum (:562-563)
+    float p0 = 0.0f, bpMean = 0.0f, bpRms = 0.0f;
+    ppl_check(ppl_clipper_model(_clip, &p0, &bpMean, &bpRms, nullptr, 0), "RFI_Clipper");
+    BinMap map(nSub * nCh);
+    map.setStart(_startFrequency);
+    map.setEnd(_endFrequency);
+    _bandPass.setRunState(map, p0, bpMean, bpRms);
 }
 
 std::vector<unsigned char> RFI_Clipper::lastFlags() const

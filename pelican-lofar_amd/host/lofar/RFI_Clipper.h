@@ -17,8 +17,9 @@ namespace pelican {
 namespace ampp {
 
 // The numbers of a .bp file in order: nChan, start, end, width, rms,
-// median, polynomial coefficients...  Throws with the reference's messages.
-std::vector<double> readBandPassFile(const std::string& fileName);
+// median, polynomial coefficients...; the indices of its K (kill) lines go
+// to *kills when given.  Throws with the reference's messages.
+std::vector<double> readBandPassFile(const std::string& fileName, std::vector<unsigned>* kills = nullptr);
 
 class RFI_Clipper : public AbstractModule {
   public:
@@ -35,15 +36,19 @@ class RFI_Clipper : public AbstractModule {
     // Per-spectrum flags of the last run (1 = kept).
     std::vector<unsigned char> lastFlags() const;
     const std::vector<double>& bandPassValues() const { return _bp; }
-    // The bandpass read from the file (RFI_Clipper.h:38): BandPassAdapter's
-    // model on the file's map, its median and RMS.  The running model the
-    // clipper updates per spectrum stays on the device.
+    const std::vector<unsigned>& killedChannels() const { return _kill; }
+    // The bandpass model (RFI_Clipper.h:38): BandPassAdapter's model from
+    // the file until the first run, then the live model on the run map as
+    // setMean / setRMS leave it after the last spectrum (its constant
+    // coefficient, mean and RMS read back from the device).
     const BandPass& bandPass() const { return _bandPass; }
 
   private:
     bool _active = true;
     std::vector<double> _bp;
+    std::vector<unsigned> _kill;
     BandPass _bandPass;
+    float _startFrequency = 0.0f, _endFrequency = 0.0f;
     ppl_clipper* _clip = nullptr;
     unsigned _lastNT = 0;
 };

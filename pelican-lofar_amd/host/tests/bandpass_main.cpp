@@ -4,6 +4,7 @@
 //   bandpass_main <config.xml> stream <stokes.f32> <nT> <nSub> <nPol> <nCh> <chunk>
 //   bandpass_main <config.xml> spectrum <spectrum.f32> <nBins>
 //   bandpass_main - bp <file.bp> [<nBins2> <start2> <width2>]
+//   bandpass_main - clip <file.bp> <history> <stokes.f32> <nT> <nSub> <nCh> <chunk>
 //
 // config.xml: a root element holding <BandPassRecorder> and, optionally,
 // <BandPassOutput> sections.  "stream" feeds the float32 [t][sub][pol][ch]
@@ -15,7 +16,11 @@
 // spectrum's file (<spectrum.f32>.clean).  "bp" reads a .bp file as the
 // clipper does (BandPass::fromFileValues), optionally rebins it, and prints
 // the current map's median, RMS, mean, first and last bin, the constant
-// coefficient and the map's start and end frequencies.
+// coefficient and the map's start and end frequencies (kill lines applied).
+// "clip" runs the RFI_Clipper module (zero-DMing, 10 / 4 rms) on the float32
+// [t][sub][1][ch] file in blobs of `chunk` spectra, in place, and prints its
+// bandPass() afterwards: constant coefficient, mean, RMS, bins, and the
+// values to <stokes.f32>.bp; the clipped spectra go to <stokes.f32>.out.
 #include <cstdio>
 #include <algorithm>
 #include <cstdlib>
@@ -61,7 +66,9 @@ int main(int argc, char** argv)
     try {
         if (std::string(argv[2]) == "bp") {
             if (argc != 4 && argc != 7) throw std::runtime_error("bp needs a file and optionally a map");
-            BandPass b = BandPass::fromFileValues(readBandPassFile(argv[3]));
+            std::vector<unsigned> kills;
+            const std::vector<double> v = readBandPassFile(argv[3], &kills);
+            BandPass b = BandPass::fromFileValues(v, kills);
             if (argc == 7) {
                 BinMap m((unsigned)std::atoi(argv[4]));
                 m.setStart(std::atof(argv[5]));
@@ -74,6 +81,38 @@ int main(int argc, char** argv)
                       << ", \"last\": " << g9(b.intensityOfBin((unsigned)n - 1)) << ", \"p0\": "
                       << g9(b.params()[0]) << ", \"start\": " << g9(b.startFrequency())
                       << ", \"end\": " << g9(b.endFrequency()) << ", \"bins\": " << n << "}" << std::endl;
+            return 0;
+        }
+        if (std::string(argv[2]) == "clip") {
+            if (argc != 10) throw std::runtime_error("clip needs a file, history, stokes file, nT, nSub, nCh, chunk");
+            const std::string xml = std::string("<RFI_Clipper active=\"true\" channelRejectionRMS=\"10\" "
+                                                "spectrumRejectionRMS=\"4\"><BandPassData file=\"") +
+                                    argv[3] + "\"/><History maximum=\"" + argv[4] +
+                                    "\"/><zeroDMing active=\"true\"/><Band matching=\"true\"/></RFI_Clipper>";
+            RFI_Clipper clipper{ConfigNode(xml)};
+            const unsigned nT = (unsigned)std::atoi(argv[6]), nSub = (unsigned)std::atoi(argv[7]);
+            const unsigned nCh = (unsigned)std::atoi(argv[8]), chunk = (unsigned)std::atoi(argv[9]);
+            if (!chunk) throw std::runtime_error("chunk must be positive");
+            const size_t row = (size_t)nSub * nCh;
+            std::vector<float> all = readFloats(argv[5], row * nT);
+            SpectrumDataSetStokes stokes;
+            for (unsigned t0 = 0; t0 < nT; t0 += chunk) {
+                const unsigned n = std::min(chunk, nT - t0);
+                stokes.resize(n, nSub, 1, nCh);
+                std::copy(all.begin() + (long)(row * t0), all.begin() + (long)(row * (t0 + n)), stokes.data());
+                WeightedSpectrumDataSet ws(&stokes);
+                clipper.run(&ws);
+                const float* out = stokes.data();
+                std::copy(out, out + row * n, all.begin() + (long)(row * t0));
+            }
+            const BandPass& b = clipper.bandPass();
+            const std::vector<float>& vals = b.currentSet();
+            std::ofstream(std::string(argv[5]) + ".bp", std::ios::binary)
+                .write(reinterpret_cast<const char*>(vals.data()), (std::streamsize)(vals.size() * sizeof(float)));
+            std::ofstream(std::string(argv[5]) + ".out", std::ios::binary)
+                .write(reinterpret_cast<const char*>(all.data()), (std::streamsize)(all.size() * sizeof(float)));
+            std::cout << "{\"p0\": " << g9(b.params()[0]) << ", \"mean\": " << g9(b.mean())
+                      << ", \"rms\": " << g9(b.rms()) << ", \"bins\": " << vals.size() << "}" << std::endl;
             return 0;
         }
         std::ifstream xf(argv[1]);

@@ -319,3 +319,91 @@ def test_clipper_steady_state_known_answers(orc, ppl, case):
     assert np.array_equal(d.cpu().numpy(), ref)
     st_o, _ = oc.state()
     assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
+
+
+def test_clipper_kill_lines_bit_exact(orc, ppl, tmp_path):
+    """A .bp file with kill lines (BandPassAdapter.cpp:58-71,101-106): the
+    killed bins hold 0.0 in every model the clipper builds (BandPass.cpp:
+    101-157); outputs, flags, statistics and the live model equal the
+    oracle's through learning into the steady state."""
+    from test_oracle_clipper import write_bp
+    path = write_bp(str(tmp_path / "k.bp"), 128, 0.0, 1.0, 40.0, 1000.0, [0.0, 0.5], ["K 5 5", "K 77 77"])
+    v, kills = orc.read_bp(path, with_kills=True)
+    x = _stream(2600, 8, 16, seed=61)
+    oc = orc.Clipper(v, 10.0, 4.0, 1000, True, kill=kills)
+    gc = ppl.Clipper(v, 10.0, 4.0, 1000, True, kill=kills)
+    for a in range(0, x.shape[0], 650):
+        chunk = np.ascontiguousarray(x[a:a + 650])
+        ref = chunk.copy()
+        oc.run(ref)
+        d = torch.from_numpy(chunk).cuda()
+        gc.run(d)
+        assert np.array_equal(gc.flags(chunk.shape[0]), oc.last_flags(chunk.shape[0])[0]), a
+        assert np.array_equal(d.cpu().numpy(), ref, equal_nan=True), a
+    p0, mean, rms, vals = gc.model(128)
+    st, _ = oc.state()
+    assert (np.float32(p0), np.float32(mean), np.float32(rms)) == (st[2], st[3], st[1])
+    assert np.array_equal(vals, oc.bandpass(128))
+    assert np.all(vals[[5, 6, 77, 78]] == 0.0) and np.count_nonzero(vals == 0.0) == 4
+
+
+def test_clipper_kill_mismatch_refused(ppl, orc, tmp_path):
+    from test_oracle_clipper import write_bp
+    path = write_bp(str(tmp_path / "m.bp"), 128, 141.5, -0.0122, 40.0, 1000.0, [0.0], ["K 5 141.44"])
+    v, kills = orc.read_bp(path, with_kills=True)
+    with pytest.raises(ppl.PPLError, match="kill mismatch"):
+        ppl.Clipper(v, kill=kills)
+
+
+def test_clipper_live_model(orc, ppl):
+    """RFI_Clipper::bandPass() after every run is the model setMean / setRMS
+    left (RFI_Clipper.cpp:562-563): constant coefficient, mean, RMS and
+    values as the oracle's, before and after the steady state."""
+    n_sub, n_ch = 8, 16
+    x = _stream(2400, n_sub, n_ch, seed=8)
+    bp = _bp(n_sub, n_ch)
+    oc = orc.Clipper(bp, 10.0, 4.0, 1000, True)
+    gc = ppl.Clipper(bp, 10.0, 4.0, 1000, True)
+    p0, mean, rms = gc.model()
+    for a in range(0, x.shape[0], 800):
+        chunk = np.ascontiguousarray(x[a:a + 800])
+        ref = chunk.copy()
+        oc.run(ref)
+        gc.run(torch.from_numpy(chunk).cuda())
+        p0, mean, rms, vals = gc.model(n_sub * n_ch)
+        st, _ = oc.state()
+        assert (np.float32(p0), np.float32(mean), np.float32(rms)) == (st[2], st[3], st[1]), a
+        assert np.array_equal(vals, oc.bandpass(n_sub * n_ch)), a
+
+
+def test_clipper_module_bandpass_is_live(orc, tmp_path):
+    """The C++ RFI_Clipper module (ppl_bandpass clip) on the reference's
+    band31.bp: clipped spectra and its bandPass() after the run (the live
+    model on the run map) as the oracle's."""
+    import json
+    import os
+    import subprocess
+    from test_oracle_clipper import _band31_stream
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "pelican-lofar_amd", "ppl_bandpass")
+    bp_path = os.path.join(root, "tests", "golden", "band31.bp")
+    v, x = _band31_stream(orc, 2500, seed=14, spikes=(9, 700), impulses=list(range(1200, 2500, 150)))
+    x = np.ascontiguousarray(x, np.float32)
+    f = tmp_path / "s.f32"
+    x.tofile(f)
+    r = subprocess.run([exe, "-", "clip", bp_path, "1000", str(f), str(x.shape[0]), "31", "64", "700"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    oc = orc.Clipper(v, 10.0, 4.0, 1000, True)
+    ref = x.copy()
+    for a in range(0, x.shape[0], 700):
+        chunk = np.ascontiguousarray(ref[a:a + 700])
+        oc.run(chunk)
+        ref[a:a + 700] = chunk
+    got = np.fromfile(str(f) + ".out", np.float32).reshape(x.shape)
+    assert np.array_equal(got, ref, equal_nan=True)
+    st, _ = oc.state()
+    assert (np.float32(out["p0"]), np.float32(out["mean"]), np.float32(out["rms"])) == (st[2], st[3], st[1])
+    vals = np.fromfile(str(f) + ".bp", np.float32)
+    assert out["bins"] == 1984 and np.array_equal(vals, oc.bandpass(1984))

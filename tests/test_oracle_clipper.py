@@ -336,3 +336,58 @@ def test_steady_state_known_answers(orc, case):
         assert m == 1500.0 and np.array_equal(c.state()[0], control.state()[0])
         assert np.all(x[t, :, 0, :] == np.float32(-0.2) - np.float32(1500.0))
         assert np.array_equal(rest, cx[t + 1:])
+
+
+# ---- kill lines (BandPassAdapter.cpp:58-71,101-106; BandPass.cpp:111-157)
+def write_bp(path, n_chan, start, width, rms, median, poly, kill_lines=()):
+    with open(path, "w") as f:
+        f.write("# test bandpass\n%d\n%r\n%r\n%r\n%r\n%r\n" % (n_chan, start, start + width * (n_chan - 1), width,
+                                                            rms, median))
+        for c in poly:
+            f.write("%r\n" % c)
+        for k in kill_lines:
+            f.write(k + "\n")
+    return path
+
+
+def test_kill_lines_known_answer(orc, tmp_path):
+    """A map with frequencies equal to bin numbers (start 0, width 1), so the
+    adapter's check binIndex(index) == index holds.  killChannel(k) kills
+    [binStart(k), binEnd(k)] = [k - 1/2, k + 1/2], and _zeroChannelsMap zeroes
+    binIndex(k - 1/2) = k through binIndex(k + 1/2) = k + 1: two bins per
+    kill line.  The second field of a kill line is not read (the reference
+    takes the first number for the frequency too) and a repeated index
+    counts once."""
+    path = write_bp(str(tmp_path / "k.bp"), 64, 0.0, 1.0, 40.0, 1000.0, [0.0, 2.0],
+                    ["K 5 5.0", "K 20 123.4", "K 5 7"])
+    v, kills = orc.read_bp(path, with_kills=True)
+    assert kills == [5, 20]
+    with pytest.raises(ValueError):
+        orc.read_bp(path)  # refused unless the caller takes the kills
+    x = (1000.0 + np.random.default_rng(3).normal(0, 10, size=(4, 8, 1, 8))).astype(np.float32)
+    plain = orc.Clipper(v, history=1000)
+    plain.run(x.copy())
+    killed = orc.Clipper(v, history=1000, kill=kills)
+    killed.run(x.copy())
+    a, b = plain.bandpass(64), killed.bandpass(64)
+    zero = [5, 6, 20, 21]
+    assert np.all(b[zero] == 0.0) and np.all(a[zero] != 0.0)
+    rest = np.setdiff1d(np.arange(64), zero)
+    assert np.array_equal(a[rest], b[rest])
+
+
+def test_kill_line_errors(orc, tmp_path):
+    # the reference's frequency check fails on a real map (frequency 5 is far
+    # outside 141.5-142.3 MHz), and the last bin's range ends at nChan
+    p = write_bp(str(tmp_path / "a.bp"), 64, 141.5, -0.0122, 40.0, 1000.0, [0.0], ["K 5 141.44"])
+    v, kills = orc.read_bp(p, with_kills=True)
+    with pytest.raises(ValueError):
+        orc.Clipper(v, kill=kills)
+    p = write_bp(str(tmp_path / "b.bp"), 64, 0.0, 1.0, 40.0, 1000.0, [0.0], ["K 63 63"])
+    v, kills = orc.read_bp(p, with_kills=True)
+    with pytest.raises(ValueError):
+        orc.Clipper(v, kill=kills)
+    for bad in ("K 5", "K x 5", "K -3 3"):
+        p = write_bp(str(tmp_path / "c.bp"), 64, 0.0, 1.0, 40.0, 1000.0, [0.0], [bad])
+        with pytest.raises(ValueError):
+            orc.read_bp(p, with_kills=True)
