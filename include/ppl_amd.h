@@ -446,12 +446,14 @@ int ppl_clipper_flags(ppl_clipper* clip, uint8_t* h_good, unsigned nT);
  * values and synchronises.  FAST: spectra decided from the precomputed
  * channel mask; SLOW: spectra walked by the full workgroup, split by reason:
  * LEARN (history not yet full), BOX (state outside the speculation box, or
- * beyond the precomputed range), UNDECIDED (a channel the box does not
- * decide), FILL (last-good slots to fill), VER (last-good versions used up);
- * STOP: sub-chunks ended early to re-centre the box. */
+ * beyond the precomputed range), UNDECIDED (channels the box does not
+ * decide: more than one, or one where the fast path could not run), FILL
+ * (last-good slots to fill), VER (last-good versions used up); STOP:
+ * sub-chunks ended early to re-centre the box; RESOLVED: FAST spectra with
+ * exactly one undecided channel, decided at their own state. */
 enum { PPL_CLIP_FAST = 0, PPL_CLIP_SLOW = 1, PPL_CLIP_LEARN = 2, PPL_CLIP_BOX = 3,
        PPL_CLIP_UNDECIDED = 4, PPL_CLIP_FILL = 5, PPL_CLIP_VER = 6, PPL_CLIP_STOP = 7,
-       PPL_CLIP_NCOUNTERS = 8 };
+       PPL_CLIP_RESOLVED = 8, PPL_CLIP_NCOUNTERS = 9 };
 int ppl_clipper_counters(ppl_clipper* clip, long long* out, int n);
 
 #ifdef __cplusplus

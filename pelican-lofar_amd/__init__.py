@@ -701,7 +701,7 @@ class Clipper:
         check(lib().ppl_clipper_stats(self.h, C.byref(m), C.byref(r)))
         return m.value, r.value
 
-    COUNTER_NAMES = ("fast", "slow", "learn", "box", "undecided", "fill", "ver", "stop")
+    COUNTER_NAMES = ("fast", "slow", "learn", "box", "undecided", "fill", "ver", "stop", "resolved")
 
     def counters(self):
         """Walk counters since creation: spectra decided from the precomputed

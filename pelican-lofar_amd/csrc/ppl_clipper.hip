@@ -37,7 +37,8 @@ using namespace ppl;
 constexpr int kMaxParams = 8;
 enum {
     CNT_FAST = PPL_CLIP_FAST, CNT_SLOW = PPL_CLIP_SLOW, CNT_LEARN = PPL_CLIP_LEARN, CNT_BOX = PPL_CLIP_BOX,
-    CNT_UNDECIDED = PPL_CLIP_UNDECIDED, CNT_FILL = PPL_CLIP_FILL, CNT_VER = PPL_CLIP_VER, CNT_STOP = PPL_CLIP_STOP
+    CNT_UNDECIDED = PPL_CLIP_UNDECIDED, CNT_FILL = PPL_CLIP_FILL, CNT_VER = PPL_CLIP_VER, CNT_STOP = PPL_CLIP_STOP,
+    CNT_RESOLVED = PPL_CLIP_RESOLVED
 };
 // Spectra per sub-chunk of the walk, and sub-chunk launches beyond
 // nT / kSubLen that a run makes room for (each early end costs one).
@@ -253,7 +254,19 @@ struct PreRec {
     float mean;
     float fracInc;
     int badBands;
-    int flags;  // bit0: some channel undecided; bit1: a kept non-zero channel whose last-good slot was empty
+    int flags;  // PRE_* bits
+    // exactly one undecided channel (PRE_ONE): the same three for the mask
+    // with that channel kept, the channel and its value
+    float meanK, fracIncK;
+    int badBandsK, undBin;
+    float undVal;
+};
+enum {
+    PRE_UND = 1,      // some channel undecided over the box
+    PRE_FILL = 2,     // a kept (or undecided) non-zero channel whose last-good slot was empty
+    PRE_REJ = 4,      // (walk) the mask in force rejects the spectrum for certain: 4 bad bands or more
+    PRE_ONE = 8,      // exactly one channel undecided: its decision picks one of two records
+    PRE_PREDKEPT = 16 // (PRE_ONE) the undecided channel is kept at the sub-chunk-start state
 };
 
 constexpr int kMaxVer = 64;  // last-good versions kept per run for the apply pass
@@ -589,8 +602,12 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
     PreRec* __restrict__ pre = bs.pre;
     const int nT = io.nT[beam], subLen = io.subLen[beam], ldMode = io.ldMode[beam];
     __shared__ float vals[2][kPreRows][kTileBins + 1];
+    // the same rows with the undecided channels kept, for rows with one
+    __shared__ float valsK[2][kPreRows][kTileBins + 1];
     __shared__ int cnt[kPreRows][8];
     __shared__ int rowFlags[kPreRows];
+    __shared__ int undCnt[kPreRows], undBinS[kPreRows];
+    __shared__ float undValS[kPreRows];
     const int tStart = S->tNext;
     if (tStart >= nT) return;
     const Box bx = *box;
@@ -606,7 +623,10 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
     const long rowStride = (long)nBins * nPol;
     const int nTiles = (nBins + kTileBins - 1) / kTileBins;
     for (int i = tid; i < kPreRows * 8; i += kTileThreads) (&cnt[0][0])[i] = 0;
-    if (tid < kPreRows) rowFlags[tid] = 0;
+    if (tid < kPreRows) {
+        rowFlags[tid] = 0;
+        undCnt[tid] = 0;
+    }
     __syncthreads();
 
     float4 raw[kPreRowsPerWave];
@@ -647,9 +667,19 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
                 const float eHi = v - q_at(lo4, j);  // largest fl(I - bp) over the box
                 const float eLo = v - q_at(hi4, j);  // smallest
                 keep[j] = inb && eHi <= bx.mlo;
-                und |= inb && !keep[j] && !(eLo > bx.mhi);
-                fill |= keep[j] && q_at(lg4, j) == 0.0f && v != 0.0f;
-                if (inb) vals[buf][r][lane * 4 + j] = keep[j] ? v : 0.0f;
+                const bool uj = inb && !keep[j] && !(eLo > bx.mhi);
+                und |= uj;
+                fill |= (keep[j] || uj) && q_at(lg4, j) == 0.0f && v != 0.0f;
+                if (inb) {
+                    vals[buf][r][lane * 4 + j] = keep[j] ? v : 0.0f;
+                    valsK[buf][r][lane * 4 + j] = (keep[j] || uj) ? v : 0.0f;
+                }
+                if (uj) {
+                    // rare: with two or more the bin and value are not used
+                    atomicAdd(&undCnt[r], 1);
+                    undBinS[r] = i + j;
+                    undValS[r] = v;
+                }
             }
             for (int b = bandLo; b <= bandHi; ++b) {
                 int c = 0;
@@ -662,21 +692,25 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
         }
     };
 
+    // lanes 0..7 of wave 0 fold the rows as judged, lanes 8..15 the same
+    // rows with the undecided channels kept (one fold instruction stream)
     float sum = 0.0f;
+    const int foldRow = lane & (kPreRows - 1);
+    const bool folder = wave == 0 && lane < 2 * kPreRows && foldRow < nRows;
     issue(0);
     judge(0, 0);
     __syncthreads();
     for (int tile = 0; tile < nTiles; ++tile) {
         const int buf = tile & 1;
         if (tile + 1 < nTiles) issue(tile + 1);
-        if (wave == 0 && lane < nRows) {
+        if (folder) {
             const int nb = min(kTileBins, nBins - tile * kTileBins);
-            sum = fold_sum(&vals[buf][lane][0], nb, sum);
+            sum = fold_sum(lane < kPreRows ? &vals[buf][foldRow][0] : &valsK[buf][foldRow][0], nb, sum);
         }
         if (tile + 1 < nTiles) judge(tile + 1, buf ^ 1);
         __syncthreads();
     }
-    if (wave == 0 && lane < nRows) {
+    if (folder && lane < kPreRows) {
         int c[8];
 #pragma unroll
         for (int b = 0; b < 8; ++b) c[b] = cnt[lane][b];
@@ -688,7 +722,31 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
         r.fracInc = fracInc;
         r.badBands = badBands;
         r.flags = rowFlags[lane];
-        pre[k0 + lane] = r;
+        pre[k0 + lane].mean = r.mean;
+        pre[k0 + lane].fracInc = r.fracInc;
+        pre[k0 + lane].badBands = r.badBands;
+        if (!(r.flags & PRE_UND) || undCnt[lane] != 1) pre[k0 + lane].flags = r.flags;
+    } else if (folder && undCnt[foldRow] == 1) {
+        // the record with the one undecided channel kept, and the decision
+        // it gets at the sub-chunk-start state (steady state: dataModel 0,
+        // margin crFactor * rmsRunAve, RFI_Clipper.cpp:305-311)
+        const int row = foldRow, u = undBinS[row];
+        int c[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) c[b] = cnt[row][b] + (band_of(u, invCpb) == b ? 1 : 0);
+        int total, badBands;
+        float fracInc;
+        band_stats(c, nBins, cpb, total, badBands, fracInc);
+        const float v = undValS[row];
+        const float bpu = bp_value(bs.terms, K.nTerms, nBins, u, S->params0, K.scale, bs.kill);
+        const bool predKept = !((v - 0.0f) - bpu > K.crFactor * S->rmsRunAve);
+        PreRec& r = pre[k0 + row];
+        r.meanK = sum / (float)(unsigned)total;
+        r.fracIncK = fracInc;
+        r.badBandsK = badBands;
+        r.undBin = u;
+        r.undVal = v;
+        r.flags = rowFlags[row] | PRE_ONE | (predKept ? PRE_PREDKEPT : 0);
     }
 }
 
@@ -793,6 +851,28 @@ struct Staged {
     int badBands, flags;
     float meanC, meanFC, rmsFC, pad;
 };
+// Staged for a spectrum with one undecided channel (PRE_ONE): the record
+// with that channel kept, the channel, its value and its bandpass terms.
+struct VarRec {
+    float meanK, fracIncK;
+    int badBandsK, undBin;
+    float undVal;
+    int killed;
+    double terms[kMaxParams];
+};
+// The undecided channel's decision at a state (RFI_Clipper.cpp:319-322 with
+// dataModel 0): kept unless (I - 0) - bandpass > margin.
+__device__ __forceinline__ bool var_kept(const VarRec& v, int nTerms, float p0, float scale, float margin)
+{
+    float bp = 0.0f;
+    if (!v.killed) {
+        float tot = 0.0f;
+        tot = (float)((double)tot + (double)p0 * 1.0);
+        for (int j = 0; j < nTerms; ++j) tot = (float)((double)tot + v.terms[j]);
+        bp = scale * tot;
+    }
+    return !((v.undVal - 0.0f) - bp > margin);
+}
 
 
 // The sequential walk over one sub-chunk of spectra.  Thread 0 runs ahead
@@ -840,6 +920,12 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     // the fold inputs of the staged spectra again, one array per field, so
     // that the single-lane folds read four spectra per LDS access
     __shared__ __align__(16) float soaMeanFC[kStage], soaMeanC[kStage], soaRmsFC[kStage], soaFracInc[kStage];
+    // the speculative block's records: the precomputed mean (predicted
+    // variant for one undecided channel) and whether its mask rejects it
+    __shared__ __align__(16) float soaMean[kStage];
+    __shared__ __align__(16) int soaRej[kStage];
+    __shared__ int lastKept[kStage];  // last spectrum before k the mask keeps, or -1
+    __shared__ VarRec var[kStage];
     // first spectrum at or after k the fast run must not take: undecided
     // channels (blkU), or undecided channels or a last-good fill (blkUF)
     __shared__ int blkU[kStage], blkUF[kStage];
@@ -1006,6 +1092,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         int badSpectra = s.badSpectra, num = s.num, meanStart = s.meanStart, rmsStart = s.rmsStart;
         int status = FAST_LIMIT;
         int k = k0;
+        int nResolved = 0;
         Staged cur = stg[k0];
         while (k < kLim) {
             const Staged nxt = stg[min(k + 1, kStage - 1)];  // consumed next iteration
@@ -1031,9 +1118,23 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 status = FAST_BOX;
                 break;
             }
+            // one undecided channel: its decision at this state picks the
+            // precomputed record (:319-342)
+            float cMean = cur.mean, cMeanC = cur.meanC, cFrac = cur.fracInc;
+            bool cRej = (cur.flags & PRE_REJ) != 0;
+            if (cur.flags & PRE_ONE) {
+                const VarRec& v = var[k];
+                if (var_kept(v, K.nTerms, p0, K.scale, margin)) {
+                    cMean = v.meanK;
+                    cMeanC = v.meanK / capf;
+                    cFrac = v.fracIncK;
+                    cRej = v.badBandsK >= 4;
+                }
+                ++nResolved;
+            }
             // phaseC (:345-440)
             const float mRAn = (mRA - cur.meanFC) + mBackC;
-            const float x = cur.mean - mRAn;
+            const float x = cMean - mRAn;
             const float a32 = K.srFactor * bpRms;
             const float tolA = (float)((double)a32 * rcpSqrtN);
             bool gt;
@@ -1045,11 +1146,11 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             } else {
                 gt = x > (float)((double)a32 / K.sqrtNBins);
             }
-            const bool good = !(gt || (cur.flags & 4));
+            const bool good = !(gt || cRej);
             // push, or push then pop_back + push_back(back()): the slot ends up
             // with the new value or with the previous back
             rmsBuf[rmsStart] = good ? sRMSn : rBack;
-            meanBuf[meanStart] = good ? cur.mean : mBack;
+            meanBuf[meanStart] = good ? cMean : mBack;
             rmsStart = rmsStart + 1 == cap ? 0 : rmsStart + 1;
             meanStart = meanStart + 1 == cap ? 0 : meanStart + 1;
             rBack2 = rBack;
@@ -1057,11 +1158,11 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             if (good) {
                 rBack = sRMSn;
                 rBackC = rBCn;
-                mBack = cur.mean;
-                mBackC = cur.meanC;
+                mBack = cMean;
+                mBackC = cMeanC;
             }
             badSpectra += good ? 0 : 1;
-            fbc += cur.fracInc;
+            fbc += cFrac;
             SpecRec rr;
             rr.params0 = p0;
             rr.dataModel = 0.0f;
@@ -1101,6 +1202,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         }
         if (status == FAST_LIMIT && kLim < kBlk) status = FAST_MAXN;
         s.count[CNT_FAST] += (t - stageBase) - k0;
+        s.count[CNT_RESOLVED] += nResolved;
         s.meanRunAve = mRA;
         s.rmsRunAve = rA;
         s.params0 = p0;
@@ -1152,11 +1254,40 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     float* const spQ = rowChunk + 4 * kStage;                      // offset step of k
     int* const spD = reinterpret_cast<int*>(rowChunk + 5 * kStage); // k moves the offset
     float* const spP = rowChunk + 6 * kStage;                      // offset before k
-    float* const spPA = rowChunk + 7 * kStage;                     // offset after k
-    float* const spBM = rowChunk + 8 * kStage;                     // bandpass mean after k
+    float* const spMBC = rowChunk + 7 * kStage;                    // newest mean entry before k / history
+    float* const spRBC = rowChunk + 8 * kStage;                    // newest rms entry before k / history
     float* const spF = rowChunk + 9 * kStage;                      // bad-channel fraction after k
-    static_assert(10 * kStage <= kRowChunk, "speculation arrays alias the row chunk");
-    __shared__ int sh_fail;
+    float* const spMB = rowChunk + 10 * kStage;                    // newest mean-ring entry before k
+    float* const spRB = rowChunk + 11 * kStage;                    // newest rms-ring entry before k
+    static_assert(12 * kStage <= kRowChunk, "speculation arrays alias the row chunk");
+    __shared__ int sh_fail, sh_nrej, sh_nres;
+    __shared__ float sh_pEnd;
+    // spectrumRMS of a steady spectrum whose running mean before it is mPrev
+    // (:288-311), and that value over the history length, as the scalar run
+    // takes them
+    auto spec_rms = [&](float mPrev, bool useMOR, float mOR, double rcpMOR, float& sR, float& rb) {
+        sR = s.spectrumRMS;
+        if (useMOR) sR = fabsf((float)((double)mPrev * rcpMOR));
+        rb = (float)((double)sR * rcpCap);
+        if (!(fabsf(rb) >= 0x1p-125f) || (useMOR && !(sR >= 0x1p-125f))) {
+            if (useMOR) sR = fabsf(mPrev / mOR);
+            rb = sR / capf;
+        }
+    };
+    // ---- speculative steady-state block (whole workgroup).  Assumes that
+    // every spectrum from t on stays inside the box, that each one with a
+    // single undecided channel gets the decision it had at the stage-start
+    // state, and that a spectrum is rejected exactly when its (so predicted)
+    // mask has 4 bad bands or more.  The four recurrences (running mean,
+    // running rms, bandpass offset, bad-channel fraction) are then folds over
+    // values known in advance -- a rejection keeps the newest ring entry
+    // (:432-435), so the entry each spectrum's running averages add is that
+    // of the last kept spectrum before it; each fold is run by one lane with
+    // exactly the operations of the scalar run, in the same order, so every
+    // value is bit-identical to it.  The assumptions are then checked in
+    // parallel and the block is committed up to the first spectrum that
+    // breaks one; that spectrum goes to the scalar run.  Returns the spectra
+    // committed.
     auto spec_block = [&](int t, int span) -> int {
         const int k0 = t - stageBase, kEnd = k0 + span;
         const bool useMOR = K.useMeanOverRMS != 0;
@@ -1167,39 +1298,51 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         // the first multiple of four on, four spectra per LDS access.
         constexpr int B = 16;
         auto ld4 = [](const float* a, int k) { return *reinterpret_cast<const float4*>(a + k); };
+        auto ld4i = [](const int* a, int k) { return *reinterpret_cast<const int4*>(a + k); };
         auto st4 = [](float* a, int k, float x, float y, float z, float w) {
             *reinterpret_cast<float4*>(a + k) = make_float4(x, y, z, w);
         };
-        if (tid == 0) {
-            float v = s.meanRunAve, bc = s.meanBack / capf;
+        // the mean-ring entry each spectrum's update adds: the last kept
+        // spectrum's mean before it, or the state's newest entry
+        {
+            const int k = tid;
+            if (k >= k0 && k < kEnd) {
+                const int j = lastKept[k];
+                spMB[k] = j >= k0 ? soaMean[j] : s.meanBack;
+                spMBC[k] = j >= k0 ? soaMeanC[j] : s.meanBack / capf;
+            }
+        }
+        __syncthreads();
+        // a two-lane fold: (v - front/cap) + back/cap per spectrum
+        auto fold2 = [&](float v, const float* fc, const float* bc, float* out) {
             int k = k0;
             for (; k < kEnd && (k & 3); ++k) {
-                v = (v - soaMeanFC[k]) + bc;
-                spM[k] = v;
-                bc = soaMeanC[k];
+                v = (v - fc[k]) + bc[k];
+                out[k] = v;
             }
             for (; k + B <= kEnd; k += B) {
-                float fc[B], mc[B], o[B];
+                float a[B], c[B];
 #pragma unroll
                 for (int j = 0; j < B; j += 4) {
-                    const float4 a = ld4(soaMeanFC, k + j), c = ld4(soaMeanC, k + j);
-                    fc[j] = a.x; fc[j + 1] = a.y; fc[j + 2] = a.z; fc[j + 3] = a.w;
-                    mc[j] = c.x; mc[j + 1] = c.y; mc[j + 2] = c.z; mc[j + 3] = c.w;
+                    const float4 x = ld4(fc, k + j), y = ld4(bc, k + j);
+                    a[j] = x.x; a[j + 1] = x.y; a[j + 2] = x.z; a[j + 3] = x.w;
+                    c[j] = y.x; c[j + 1] = y.y; c[j + 2] = y.z; c[j + 3] = y.w;
                 }
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
-                    v = (v - fc[j]) + bc;
-                    o[j] = v;
-                    bc = mc[j];
+                    v = (v - a[j]) + c[j];
+                    a[j] = v;
                 }
 #pragma unroll
-                for (int j = 0; j < B; j += 4) st4(spM, k + j, o[j], o[j + 1], o[j + 2], o[j + 3]);
+                for (int j = 0; j < B; j += 4) st4(out, k + j, a[j], a[j + 1], a[j + 2], a[j + 3]);
             }
             for (; k < kEnd; ++k) {
-                v = (v - soaMeanFC[k]) + bc;
-                spM[k] = v;
-                bc = soaMeanC[k];
+                v = (v - fc[k]) + bc[k];
+                out[k] = v;
             }
+        };
+        if (tid == 0) {
+            fold2(s.meanRunAve, soaMeanFC, spMBC, spM);
         } else if (tid == 64) {
             float f = s.fractionBadChannels;
             int k = k0;
@@ -1208,7 +1351,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 spF[k] = f;
             }
             for (; k + B <= kEnd; k += B) {
-                float fi[B], o[B];
+                float fi[B];
 #pragma unroll
                 for (int j = 0; j < B; j += 4) {
                     const float4 a = ld4(soaFracInc, k + j);
@@ -1217,10 +1360,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     f += fi[j];
-                    o[j] = f;
+                    fi[j] = f;
                 }
 #pragma unroll
-                for (int j = 0; j < B; j += 4) st4(spF, k + j, o[j], o[j + 1], o[j + 2], o[j + 3]);
+                for (int j = 0; j < B; j += 4) st4(spF, k + j, fi[j], fi[j + 1], fi[j + 2], fi[j + 3]);
             }
             for (; k < kEnd; ++k) {
                 f += soaFracInc[k];
@@ -1231,16 +1374,23 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         {
             const int k = tid;
             if (k >= k0 && k < kEnd) {
-                const float mPrev = k == k0 ? s.meanRunAve : spM[k - 1];
-                float sR = s.spectrumRMS;
-                if (useMOR) sR = fabsf((float)((double)mPrev * rcpMOR));
-                float rb = (float)((double)sR * rcpCap);
-                if (!(fabsf(rb) >= 0x1p-125f) || (useMOR && !(sR >= 0x1p-125f))) {
-                    if (useMOR) sR = fabsf(mPrev / mOR);
-                    rb = sR / capf;
-                }
+                float sR, rb;
+                spec_rms(k == k0 ? s.meanRunAve : spM[k - 1], useMOR, mOR, rcpMOR, sR, rb);
                 spS[k] = sR;
                 spB[k] = rb;
+                // the rms-ring entry this spectrum's update adds: the last
+                // kept spectrum's value before it (recomputed here, the
+                // same operations as its own thread's), or the state's
+                const int j = lastKept[k];
+                if (j >= k0) {
+                    float sRj, rbj;
+                    spec_rms(j == k0 ? s.meanRunAve : spM[j - 1], useMOR, mOR, rcpMOR, sRj, rbj);
+                    spRB[k] = sRj;
+                    spRBC[k] = rbj;
+                } else {
+                    spRB[k] = s.rmsBack;
+                    spRBC[k] = s.rmsBack / capf;
+                }
                 // the bandpass mean before k is the running mean after k-1 (an
                 // unchanged one differs from it at most in the sign of a zero,
                 // which neither the test nor the step below can see)
@@ -1259,88 +1409,45 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         }
         __syncthreads();
         if (tid == 0) {
-            float r = s.rmsRunAve, bc = s.rmsBack / capf;
-            int k = k0;
-            for (; k < kEnd && (k & 3); ++k) {
-                r = (r - soaRmsFC[k]) + bc;
-                spR[k] = r;
-                bc = spB[k];
-            }
-            for (; k + B <= kEnd; k += B) {
-                float fc[B], bb[B], o[B];
-#pragma unroll
-                for (int j = 0; j < B; j += 4) {
-                    const float4 a = ld4(soaRmsFC, k + j), c = ld4(spB, k + j);
-                    fc[j] = a.x; fc[j + 1] = a.y; fc[j + 2] = a.z; fc[j + 3] = a.w;
-                    bb[j] = c.x; bb[j + 1] = c.y; bb[j + 2] = c.z; bb[j + 3] = c.w;
-                }
-#pragma unroll
-                for (int j = 0; j < B; ++j) {
-                    r = (r - fc[j]) + bc;
-                    o[j] = r;
-                    bc = bb[j];
-                }
-#pragma unroll
-                for (int j = 0; j < B; j += 4) st4(spR, k + j, o[j], o[j + 1], o[j + 2], o[j + 3]);
-            }
-            for (; k < kEnd; ++k) {
-                r = (r - soaRmsFC[k]) + bc;
-                spR[k] = r;
-                bc = spB[k];
-            }
+            fold2(s.rmsRunAve, soaRmsFC, spRBC, spR);
         } else if (tid == 64) {
-            float p = s.params0, bm = s.bpMean;
+            // the bandpass offset: p + q where the running mean moved
+            float p = s.params0;
             const int* const spDi = spD;
             int k = k0;
             for (; k < kEnd && (k & 3); ++k) {
                 spP[k] = p;
-                if (spDi[k]) {
-                    p += spQ[k];
-                    bm = spM[k];
-                }
-                spPA[k] = p;
-                spBM[k] = bm;
+                if (spDi[k]) p += spQ[k];
             }
             for (; k + B <= kEnd; k += B) {
-                float q[B], m[B], op[B], opa[B], obm[B];
+                float q[B], op[B];
                 int d[B];
 #pragma unroll
                 for (int j = 0; j < B; j += 4) {
-                    const float4 a = ld4(spQ, k + j), c = ld4(spM, k + j);
-                    const int4 e = *reinterpret_cast<const int4*>(spDi + k + j);
+                    const float4 a = ld4(spQ, k + j);
+                    const int4 e = ld4i(spDi, k + j);
                     q[j] = a.x; q[j + 1] = a.y; q[j + 2] = a.z; q[j + 3] = a.w;
-                    m[j] = c.x; m[j + 1] = c.y; m[j + 2] = c.z; m[j + 3] = c.w;
                     d[j] = e.x; d[j + 1] = e.y; d[j + 2] = e.z; d[j + 3] = e.w;
                 }
-                // selects, not branches: the chain is one add and one
-                // select per spectrum (p + q is taken only where it moves)
+                // a select, not a branch: one add and one select per spectrum
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     op[j] = p;
                     const float pn = p + q[j];
                     p = d[j] ? pn : p;
-                    bm = d[j] ? m[j] : bm;
-                    opa[j] = p;
-                    obm[j] = bm;
                 }
 #pragma unroll
-                for (int j = 0; j < B; j += 4) {
-                    st4(spP, k + j, op[j], op[j + 1], op[j + 2], op[j + 3]);
-                    st4(spPA, k + j, opa[j], opa[j + 1], opa[j + 2], opa[j + 3]);
-                    st4(spBM, k + j, obm[j], obm[j + 1], obm[j + 2], obm[j + 3]);
-                }
+                for (int j = 0; j < B; j += 4) st4(spP, k + j, op[j], op[j + 1], op[j + 2], op[j + 3]);
             }
             for (; k < kEnd; ++k) {
                 spP[k] = p;
-                if (spDi[k]) {
-                    p += spQ[k];
-                    bm = spM[k];
-                }
-                spPA[k] = p;
-                spBM[k] = bm;
+                if (spDi[k]) p += spQ[k];
             }
+            sh_pEnd = p;
         } else if (tid == 128) {
             sh_fail = kEnd;
+            sh_nrej = 0;
+            sh_nres = 0;
         }
         __syncthreads();
         {
@@ -1349,19 +1456,25 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 const float margin = K.crFactor * spR[k];
                 const float p0 = spP[k];
                 bool ok = p0 >= bx.p0lo && p0 <= bx.p0hi && margin >= bx.mlo && margin <= bx.mhi;
-                const float x = stg[k].mean - spM[k];
-                const float a32 = K.srFactor * (k == k0 ? s.bpRms : spR[k - 1]);
-                const float tolA = (float)((double)a32 * rcpSqrtN);
-                bool gt;
-                if (tolA >= 0x1p-126f && tolA < 0x1p127f) {
-                    const bool gtHi = x > __int_as_float(__float_as_int(tolA) + 1);
-                    const bool gtLo = x > __int_as_float(__float_as_int(tolA) - 1);
-                    gt = gtHi;
-                    if (gtLo != gtHi) gt = x > (float)((double)a32 / K.sqrtNBins);
-                } else {
-                    gt = x > (float)((double)a32 / K.sqrtNBins);
+                const int fl = stg[k].flags;
+                if (fl & PRE_ONE)
+                    ok = ok && var_kept(var[k], K.nTerms, p0, K.scale, margin) == ((fl & PRE_PREDKEPT) != 0);
+                if (!soaRej[k]) {
+                    // a spectrum the mask keeps must pass the tolerance test
+                    const float x = soaMean[k] - spM[k];
+                    const float a32 = K.srFactor * (k == k0 ? s.bpRms : spR[k - 1]);
+                    const float tolA = (float)((double)a32 * rcpSqrtN);
+                    bool gt;
+                    if (tolA >= 0x1p-126f && tolA < 0x1p127f) {
+                        const bool gtHi = x > __int_as_float(__float_as_int(tolA) + 1);
+                        const bool gtLo = x > __int_as_float(__float_as_int(tolA) - 1);
+                        gt = gtHi;
+                        if (gtLo != gtHi) gt = x > (float)((double)a32 / K.sqrtNBins);
+                    } else {
+                        gt = x > (float)((double)a32 / K.sqrtNBins);
+                    }
+                    ok = ok && !gt;
                 }
-                ok = ok && !(gt || (stg[k].flags & 4));
                 if (!ok) atomicMin(&sh_fail, k);
             }
         }
@@ -1370,37 +1483,47 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         {
             const int k = tid;
             if (k >= k0 && k < kf) {
+                const bool rej = soaRej[k] != 0;
                 SpecRec rr;
                 rr.params0 = spP[k];
                 rr.dataModel = 0.0f;
                 rr.margin = K.crFactor * spR[k];
                 rr.meanRunAve = spM[k];
-                rr.flags = 1;
+                rr.flags = rej ? 0 : 1;
                 rr.ver = s.ver;
                 rec[stageBase + k] = rr;
-                rmsBuf[(s.rmsStart + (k - k0)) % K.cap] = spS[k];
-                meanBuf[(s.meanStart + (k - k0)) % K.cap] = stg[k].mean;
+                rmsBuf[(s.rmsStart + (k - k0)) % K.cap] = rej ? spRB[k] : spS[k];
+                meanBuf[(s.meanStart + (k - k0)) % K.cap] = rej ? spMB[k] : soaMean[k];
+                if (rej) atomicAdd(&sh_nrej, 1);
+                if (stg[k].flags & PRE_ONE) atomicAdd(&sh_nres, 1);
             }
         }
         __syncthreads();
         const int n = kf - k0;
         if (tid == 0 && n > 0) {
             const int kl = kf - 1;
+            const bool rejL = soaRej[kl] != 0;
             s.meanRunAve = spM[kl];
             s.rmsRunAve = spR[kl];
-            s.params0 = spPA[kl];
-            s.bpMean = spBM[kl];
+            s.params0 = kf < kEnd ? spP[kf] : sh_pEnd;
+            // setMean moved the bandpass mean to the running mean at the
+            // last spectrum whose running mean changed
+            int jd = kl;
+            while (jd >= k0 && !spD[jd]) --jd;
+            if (jd >= k0) s.bpMean = spM[jd];
             s.bpRms = spR[kl];
             s.spectrumRMS = spS[kl];
             s.fractionBadChannels = spF[kl];
             s.num = (s.num + n) % period;
             s.meanStart = (s.meanStart + n) % K.cap;
             s.rmsStart = (s.rmsStart + n) % K.cap;
-            s.meanBack2 = n >= 2 ? stg[kl - 1].mean : s.meanBack;
-            s.meanBack = stg[kl].mean;
-            s.rmsBack2 = n >= 2 ? spS[kl - 1] : s.rmsBack;
-            s.rmsBack = spS[kl];
+            s.meanBack2 = spMB[kl];
+            s.meanBack = rejL ? spMB[kl] : soaMean[kl];
+            s.rmsBack2 = spRB[kl];
+            s.rmsBack = rejL ? spRB[kl] : spS[kl];
+            s.badSpectra += sh_nrej;
             s.count[CNT_FAST] += n;
+            s.count[CNT_RESOLVED] += sh_nres;
         }
         return n;
     };
@@ -1429,8 +1552,11 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                             mode = MODE_SPEC;
                             break;
                         }
+                        // too short to speculate: the scalar run takes it and
+                        // the spectrum that ends it, then speculation resumes
+                        scalarNext = span + 1;
                     }
-                    const int r = fast_run(t, scalarNext > 0 ? scalarNext : kStage);
+                    const int r = fast_run(t, scalarNext);
                     scalarNext = 0;
                     if (r == FAST_MAXN) continue;
                     if (r == FAST_BOX) {
@@ -1487,37 +1613,67 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
             const int nPre = bx.valid ? min(stageN, preN - (t - tStart)) : 0;
             for (int k = tid; k < nPre; k += kWalkThreads) {
                 const PreRec r = pre[t - tStart + k];
+                const int fl = r.flags | (r.badBands >= 4 ? PRE_REJ : 0);
                 stg[k].mean = r.mean;
                 stg[k].fracInc = r.fracInc;
                 stg[k].badBands = r.badBands;
-                stg[k].flags = r.flags;
+                stg[k].flags = fl;
                 stg[k].meanC = r.mean / capf;
-                soaMeanC[k] = stg[k].meanC;
-                soaFracInc[k] = r.fracInc;
+                // the speculative block takes the record the one undecided
+                // channel's decision at the stage-start state predicts
+                bool useK = false;
+                if (fl & PRE_ONE) {
+                    VarRec& v = var[k];
+                    v.meanK = r.meanK;
+                    v.fracIncK = r.fracIncK;
+                    v.badBandsK = r.badBandsK;
+                    v.undBin = r.undBin;
+                    v.undVal = r.undVal;
+                    v.killed = bin_killed(kill, r.undBin) ? 1 : 0;
+                    for (int j = 0; j < K.nTerms; ++j) v.terms[j] = terms[(size_t)j * nBins + r.undBin];
+                    useK = var_kept(v, K.nTerms, s.params0, K.scale, K.crFactor * s.rmsRunAve);
+                    stg[k].flags = useK ? (fl | PRE_PREDKEPT) : (fl & ~PRE_PREDKEPT);
+                }
+                soaMean[k] = useK ? r.meanK : r.mean;
+                soaMeanC[k] = useK ? r.meanK / capf : stg[k].meanC;
+                soaFracInc[k] = useK ? r.fracIncK : r.fracInc;
+                soaRej[k] = (useK ? r.badBandsK : r.badBands) >= 4 ? 1 : 0;
             }
-            for (int k = tid; k < nPre; k += kWalkThreads)
-                if (stg[k].badBands >= 4) stg[k].flags |= 4;
             {
-                // suffix minima over the stage (one thread per entry)
+                // suffix minima over the stage (one thread per entry) of the
+                // spectra neither the fast run nor a speculative block takes:
+                // undecided channels other than one (blkU), or those or a
+                // last-good fill (blkUF)
                 const int k = tid;
-                int bu = k, buf = k;
+                int bu = k, buf = k, lk = -1;
                 if (k < nPre) {
-                    bu = (stg[k].flags & 1) ? k : kStage;
-                    buf = (stg[k].flags & 3) ? k : kStage;
+                    const int fl = stg[k].flags;
+                    const bool unres = (fl & PRE_UND) && !(fl & PRE_ONE);
+                    bu = unres ? k : kStage;
+                    buf = (unres || (fl & PRE_FILL)) ? k : kStage;
+                    lk = soaRej[k] ? -1 : k;
                 }
                 blkU[k] = bu;
                 blkUF[k] = buf;
+                lastKept[k] = lk;
                 __syncthreads();
+                // and the prefix maxima of the kept indices (inclusive here,
+                // shifted by one below)
                 for (int off = 1; off < kStage; off <<= 1) {
                     if (k + off < kStage) {
                         bu = min(bu, blkU[k + off]);
                         buf = min(buf, blkUF[k + off]);
                     }
+                    if (k >= off) lk = max(lk, lastKept[k - off]);
                     __syncthreads();
                     blkU[k] = bu;
                     blkUF[k] = buf;
+                    lastKept[k] = lk;
                     __syncthreads();
                 }
+                lk = k > 0 ? lastKept[k - 1] : -1;
+                __syncthreads();
+                lastKept[k] = lk;
             }
             const int fv = s.rmsSize == K.cap && s.meanSize == K.cap && K.cap > kStage;
             if (fv) {
