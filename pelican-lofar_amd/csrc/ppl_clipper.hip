@@ -265,8 +265,9 @@ enum {
     PRE_UND = 1,      // some channel undecided over the box
     PRE_FILL = 2,     // a kept (or undecided) non-zero channel whose last-good slot was empty
     PRE_REJ = 4,      // (walk) the mask in force rejects the spectrum for certain: 4 bad bands or more
-    PRE_ONE = 8,      // exactly one channel undecided: its decision picks one of two records
-    PRE_PREDKEPT = 16 // (PRE_ONE) the undecided channel is kept at the sub-chunk-start state
+    PRE_ONE = 8,       // exactly one channel undecided: its decision picks one of two records
+    PRE_PREDKEPT = 16, // (PRE_ONE) the undecided channel is kept at the predicting state
+    PRE_TOLREJ = 32    // (walk) the mask keeps it, but its mean exceeds the tolerance at the stage-start state
 };
 
 constexpr int kMaxVer = 64;  // last-good versions kept per run for the apply pass
@@ -1459,8 +1460,9 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 const int fl = stg[k].flags;
                 if (fl & PRE_ONE)
                     ok = ok && var_kept(var[k], K.nTerms, p0, K.scale, margin) == ((fl & PRE_PREDKEPT) != 0);
-                if (!soaRej[k]) {
-                    // a spectrum the mask keeps must pass the tolerance test
+                if (!soaRej[k] || (fl & PRE_TOLREJ)) {
+                    // the tolerance test (:421-425) must keep the spectra
+                    // predicted kept and reject those predicted rejected by it
                     const float x = soaMean[k] - spM[k];
                     const float a32 = K.srFactor * (k == k0 ? s.bpRms : spR[k - 1]);
                     const float tolA = (float)((double)a32 * rcpSqrtN);
@@ -1473,7 +1475,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                     } else {
                         gt = x > (float)((double)a32 / K.sqrtNBins);
                     }
-                    ok = ok && !gt;
+                    ok = ok && gt == ((fl & PRE_TOLREJ) != 0);
                 }
                 if (!ok) atomicMin(&sh_fail, k);
             }
@@ -1637,7 +1639,14 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 soaMean[k] = useK ? r.meanK : r.mean;
                 soaMeanC[k] = useK ? r.meanK / capf : stg[k].meanC;
                 soaFracInc[k] = useK ? r.fracIncK : r.fracInc;
-                soaRej[k] = (useK ? r.badBandsK : r.badBands) >= 4 ? 1 : 0;
+                // a spectrum the mask keeps is predicted rejected when its
+                // mean exceeds the tolerance at the stage-start state
+                // (:394-425): the speculative block then checks that it is
+                const bool maskRej = (useK ? r.badBandsK : r.badBands) >= 4;
+                const float tolS = (float)((double)(K.srFactor * s.bpRms) / K.sqrtNBins);
+                const bool tolRej = !maskRej && soaMean[k] - s.meanRunAve > tolS;
+                if (tolRej) stg[k].flags |= PRE_TOLREJ;
+                soaRej[k] = (maskRej || tolRej) ? 1 : 0;
             }
             {
                 // suffix minima over the stage (one thread per entry) of the

@@ -415,3 +415,40 @@ def test_clipper_module_bandpass_is_live(orc, tmp_path):
     assert (np.float32(out["p0"]), np.float32(out["mean"]), np.float32(out["rms"])) == (st[2], st[3], st[1])
     vals = np.fromfile(str(f) + ".bp", np.float32)
     assert out["bins"] == 1984 and np.array_equal(vals, oc.bandpass(1984))
+
+
+def test_clipper_steady_cases_in_speculative_blocks(orc, ppl):
+    """The hand-worked steady-state cases scattered through a long steady
+    stream (past the first 4096-spectrum sub-chunk, where the walk runs
+    speculative blocks): spectra rejected by the tolerance test, by bad
+    bands or by the total-kept rule, runs of consecutive rejections, and
+    kept offsets, at varying distances.  Outputs, flags and statistics as
+    the oracle's, and most spectra on the fast paths."""
+    from test_oracle_clipper import KA_CASES, KA_HIST, _ka_bp, _ka_normal
+    rng = np.random.default_rng(77)
+    n_t = KA_HIST + 9000
+    x = _ka_normal(n_t)
+    names = sorted(KA_CASES)
+    t = KA_HIST + 30
+    while t < n_t - 10:
+        run = 1 + int(rng.integers(0, 3)) if rng.random() < 0.2 else 1
+        for j in range(run):
+            x[t + j] = KA_CASES[names[int(rng.integers(len(names)))]][0][0]
+        t += run + int(rng.integers(5, 140))
+    # small noise so that spectra differ (multiples of 1/8: exact sums)
+    x += (rng.integers(-8, 9, size=x.shape) / 8.0).astype(np.float32)
+    ref = x.copy()
+    oc = orc.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    gc = ppl.Clipper(_ka_bp(), 10.0, 4.0, KA_HIST, True)
+    for a in range(0, n_t, 5000):
+        r = np.ascontiguousarray(ref[a:a + 5000])
+        oc.run(r)
+        ref[a:a + 5000] = r
+        d = torch.from_numpy(np.ascontiguousarray(x[a:a + 5000])).cuda()
+        gc.run(d)
+        assert np.array_equal(gc.flags(r.shape[0]), oc.last_flags(r.shape[0])[0]), a
+        assert np.array_equal(d.cpu().numpy(), r), a
+    st_o, _ = oc.state()
+    assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
+    cnt = gc.counters()
+    assert cnt["fast"] > 5000, cnt
