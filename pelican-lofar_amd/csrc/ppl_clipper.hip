@@ -283,7 +283,7 @@ constexpr int kRowsPerWave = kTileRows / 4;
 // The precompute and learning records: spectra per workgroup.  Each row's
 // ordered sum is one lane's dependent chain, so fewer rows per workgroup
 // spread a sub-chunk's rows over more CUs.
-constexpr int kPreRows = 8;
+constexpr int kPreRows = 4;
 constexpr int kPreRowsPerWave = kPreRows / 4;
 
 // Loads of 4 consecutive bins (i % 4 == 0) of polarisation 0 of one spectrum.
@@ -1261,7 +1261,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
     float* const spMB = rowChunk + 10 * kStage;                    // newest mean-ring entry before k
     float* const spRB = rowChunk + 11 * kStage;                    // newest rms-ring entry before k
     static_assert(12 * kStage <= kRowChunk, "speculation arrays alias the row chunk");
-    __shared__ int sh_fail, sh_nrej, sh_nres;
+    __shared__ int sh_fail, sh_nrej, sh_nres, sh_still;
     __shared__ float sh_pEnd;
     // spectrumRMS of a steady spectrum whose running mean before it is mPrev
     // (:288-311), and that value over the history length, as the scalar run
@@ -1305,6 +1305,7 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         };
         // the mean-ring entry each spectrum's update adds: the last kept
         // spectrum's mean before it, or the state's newest entry
+        if (tid == 0) sh_still = 0;
         {
             const int k = tid;
             if (k >= k0 && k < kEnd) {
@@ -1406,11 +1407,41 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
                 }
                 spQ[k] = qd;
                 spD[k] = d;
+                if (!d) sh_still = 1;
             }
         }
         __syncthreads();
         if (tid == 0) {
             fold2(s.rmsRunAve, soaRmsFC, spRBC, spR);
+        } else if (tid == 64 && !sh_still) {
+            // the bandpass offset when every spectrum moves it: p + q
+            float p = s.params0;
+            int k = k0;
+            for (; k < kEnd && (k & 3); ++k) {
+                spP[k] = p;
+                p += spQ[k];
+            }
+            for (; k + B <= kEnd; k += B) {
+                float q[B];
+#pragma unroll
+                for (int j = 0; j < B; j += 4) {
+                    const float4 a = ld4(spQ, k + j);
+                    q[j] = a.x; q[j + 1] = a.y; q[j + 2] = a.z; q[j + 3] = a.w;
+                }
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    const float pn = p + q[j];
+                    q[j] = p;
+                    p = pn;
+                }
+#pragma unroll
+                for (int j = 0; j < B; j += 4) st4(spP, k + j, q[j], q[j + 1], q[j + 2], q[j + 3]);
+            }
+            for (; k < kEnd; ++k) {
+                spP[k] = p;
+                p += spQ[k];
+            }
+            sh_pEnd = p;
         } else if (tid == 64) {
             // the bandpass offset: p + q where the running mean moved
             float p = s.params0;
