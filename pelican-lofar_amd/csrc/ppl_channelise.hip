@@ -6,8 +6,13 @@
 // by side).  The FIR keeps the last nTaps blocks of its channels in
 // registers, so every input sample is read from HBM once (plus nTaps-1
 // history blocks per run), and the polyphase sum is evaluated in FP32 in the
-// reference's order (oldest block first, separate multiply and add,
-// src/lib/src/PPFChanneliser.cpp:276-289).  The N-point forward DFT is a
+// reference's order (oldest block first, src/lib/src/PPFChanneliser.cpp:
+// 276-289), each tap one fused multiply-add.  The FIR and the FFT are the
+// one part of the chain held to a tolerance rather than to bits (the
+// reference's FFTW butterflies cannot be reproduced), so they contract
+// multiply-adds: one rounding instead of two, and a third fewer VALU
+// instructions in this VALU-bound kernel.  Stokes-I and the integration keep
+// the reference's separate roundings.  The N-point forward DFT is a
 // Cooley-Tukey split N = R x G: an R-point DFT in registers, twiddles, then
 // a G-point radix-2 DIF FFT across lanes with xor shuffles.  Stokes-I
 // (src/lib/src/StokesGenerator.cpp:93-99) and the integration over
@@ -84,6 +89,7 @@ struct LaneTw {
 template <int G, int S = 0>
 __device__ __forceinline__ void lane_fft(float& re, float& im, const LaneTw& L)
 {
+#pragma clang fp contract(fast)
     constexpr int m = G >> (S + 1);
     if constexpr (m >= 1) {
         const float pr = xor_lane<m>(re);
@@ -247,6 +253,8 @@ __global__ __launch_bounds__(256) void ppf_kernel(
             }
             // FIR, oldest block first, coefficient index t*N + n
             C2 f0[R], f1[R];
+            {
+#pragma clang fp contract(fast)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 f0[r] = f1[r] = {0.f, 0.f};
@@ -270,6 +278,7 @@ __global__ __launch_bounds__(256) void ppf_kernel(
                 const C2 a = f0[r], c = f1[r], t = rtw[r];
                 f0[r] = {a.re * t.re - a.im * t.im, a.re * t.im + a.im * t.re};
                 f1[r] = {c.re * t.re - c.im * t.im, c.re * t.im + c.im * t.re};
+            }
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
