@@ -88,6 +88,8 @@ def parse():
     ap.add_argument("--svc-accel", type=float, default=10.0,
                     help="data seconds per wall second in the streaming run (arrival schedule speed-up)")
     ap.add_argument("--svc-dtype", default="f16", choices=["f32", "f16", "u8"])
+    ap.add_argument("--svc-recall-sessions", type=int, default=16,
+                    help="sessions replayed through plain FP32 chains for the event recall")
     return ap.parse_args()
 
 
@@ -891,7 +893,10 @@ def service(args, torch, ppl, coeffs, bp):
     lat_ms = np.array(lat) * 1e3
     # recall against plain FP32 per-beam runs of the same chunk sequences
     log("service: recall check")
-    keys = sorted([k for k in events if events[k]], key=lambda k: -len(events[k]))[:3]
+    # a sample of the sessions that produced events, spread over the run
+    cand = sorted([k for k in events if events[k]], key=lambda k: k[1])
+    step = max(1, len(cand) // max(args.svc_recall_sessions, 1))
+    keys = cand[::step][:args.svc_recall_sessions]
     n_ref = n_got = inter = 0
     for key in keys:
         plain = ppl.StreamService(spec, chunk_spec, 1 << 40, dtype="f32", max_beams=1)
@@ -919,7 +924,7 @@ def service(args, torch, ppl, coeffs, bp):
             "throughput_data_s_per_s": round(data_s / wall, 1),
             "throughput_dm_trials_samples_per_s": data_s / TSAMP * TDMS / wall,
             "event_recall_vs_plain_f32": inter / max(n_ref, 1), "event_precision_vs_plain_f32": inter / max(n_got, 1),
-            "recall_sessions": len(keys), "recall_events_ref": n_ref}
+            "recall_sessions": len(keys), "recall_sessions_with_events": len(cand), "recall_events_ref": n_ref}
 
 
 def cpu_baseline(n_spec, coeffs, bp, nout):

@@ -1,8 +1,9 @@
 // ppl_channelise_mfma.hip -- polyphase channeliser whose DFT runs on the
 // matrix cores as split-precision GEMMs with FP32 accumulation.
 //
-// The FIR is the plain method's: FP32, oldest block first, separate multiply
-// and add (src/lib/src/PPFChanneliser.cpp:276-289).  The N-point DFT of each
+// The FIR is the plain method's: FP32, oldest block first (src/lib/src/
+// PPFChanneliser.cpp:276-289), each tap one fused multiply-add (the spectra
+// are held to a tolerance against FP64, not to the reference's bits).  The N-point DFT of each
 // filtered block, N = 16 R, is split Cooley-Tukey style with n = n1 + R n2 and
 // k = k2 + 16 k1:
 //
@@ -255,6 +256,7 @@ __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __re
                 C2 f = {0.f, 0.f};
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) {
+#pragma clang fp contract(fast)
                     const float cre = w[q][t].re * hc[q][t], cim = w[q][t].im * hc[q][t];
                     f.re = f.re + cre;
                     f.im = f.im + cim;
