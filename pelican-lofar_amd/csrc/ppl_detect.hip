@@ -94,34 +94,64 @@ __device__ __forceinline__ void segment_masks(const T* data, long rowBase, int s
 constexpr int kSegPerWave = 16;
 constexpr int kSegPerBlock = 4 * kSegPerWave;
 
-// Hits of one segment at every level (the row's value on this lane is v).
-__device__ __forceinline__ int segment_hits(float v, int lane, int W, int P, const Thresholds& th)
+// Lanes that hold a level-n bin of a 2^P-sample segment: the multiples of
+// 2^n below 2^P.
+template <int P>
+constexpr uint64_t level_lanes(int n)
 {
-    uint64_t masks[kMaxLevels];
-    float vals[kMaxLevels];
-    masks[0] = __ballot(lane < W && v >= th.v[0]);
+    uint64_t m = 0;
+    for (int l = 0; l < (1 << P); l += 1 << n) m |= 1ull << l;
+    return m;
+}
+
+// Hits of one segment at every level (the row's value on this lane is v):
+// per level one add of the lane 2^(n-1) above (a DPP move folded into the
+// add for n <= 4) and one compare whose lane mask is the ballot; the lanes
+// that hold no bin are masked off by a constant.
+template <int P>
+__device__ __forceinline__ int segment_hits(float v, const Thresholds& th)
+{
+    int c = __popcll(__builtin_amdgcn_ballot_w64(v >= th.v[0]) & level_lanes<P>(0));
     float cur = v;
-    if (P >= 1) level_up<1>(cur, lane, W, th, masks, vals);
-    if (P >= 2) level_up<2>(cur, lane, W, th, masks, vals);
-    if (P >= 3) level_up<3>(cur, lane, W, th, masks, vals);
-    if (P >= 4) level_up<4>(cur, lane, W, th, masks, vals);
-    if (P >= 5) level_up<5>(cur, lane, W, th, masks, vals);
-    if (P >= 6) level_up<6>(cur, lane, W, th, masks, vals);
-    int c = 0;
-    for (int n = 0; n <= P; ++n) c += __popcll(masks[n]);
+    if constexpr (P >= 1) {
+        cur = cur + from_lane_above<1>(cur);
+        c += __popcll(__builtin_amdgcn_ballot_w64(cur >= th.v[1]) & level_lanes<P>(1));
+    }
+    if constexpr (P >= 2) {
+        cur = cur + from_lane_above<2>(cur);
+        c += __popcll(__builtin_amdgcn_ballot_w64(cur >= th.v[2]) & level_lanes<P>(2));
+    }
+    if constexpr (P >= 3) {
+        cur = cur + from_lane_above<4>(cur);
+        c += __popcll(__builtin_amdgcn_ballot_w64(cur >= th.v[3]) & level_lanes<P>(3));
+    }
+    if constexpr (P >= 4) {
+        cur = cur + from_lane_above<8>(cur);
+        c += __popcll(__builtin_amdgcn_ballot_w64(cur >= th.v[4]) & level_lanes<P>(4));
+    }
+    if constexpr (P >= 5) {
+        cur = cur + from_lane_above<16>(cur);
+        c += __popcll(__builtin_amdgcn_ballot_w64(cur >= th.v[5]) & level_lanes<P>(5));
+    }
+    if constexpr (P >= 6) {
+        cur = cur + from_lane_above<32>(cur);
+        c += __popcll(__builtin_amdgcn_ballot_w64(cur >= th.v[6]) & level_lanes<P>(6));
+    }
     return c;
 }
 
 // Pass 1: hits per segment (at most 127, one byte each) and per workgroup.
-template <typename T>
+// P (the levels above 0) is a template parameter, so the level loop and
+// the lane masks are fixed at compile time.
+template <typename T, int P>
 __global__ __launch_bounds__(256) void detect_count_kernel(const T* __restrict__ data, int nsamp, int nSeg,
-                                                           int total, int P, Thresholds th, int offset,
+                                                           int total, Thresholds th, int offset,
                                                            float scale, uint8_t* __restrict__ counts,
                                                            int* __restrict__ blockSums)
 {
     __shared__ int part[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int W = 1 << P;
+    constexpr int W = 1 << P;
     const int first = blockIdx.x * kSegPerBlock + wave * kSegPerWave;
     int dm = first / nSeg, seg = first - dm * nSeg;
     float v[kSegPerWave];
@@ -139,7 +169,7 @@ __global__ __launch_bounds__(256) void detect_count_kernel(const T* __restrict__
 #pragma unroll
     for (int k = 0; k < kSegPerWave; ++k) {
         if (first + k < total) {
-            const int c = segment_hits(v[k], lane, W, P, th);
+            const int c = segment_hits<P>(v[k], th);
             if (lane == 0) counts[first + k] = (uint8_t)c;
             sum += c;
         }
@@ -290,8 +320,24 @@ int launch_detect(const T* d_data, int tdms, int nsamp, int offset, float scale,
     uint8_t* counts = static_cast<uint8_t*>(scratch);
     int* off = reinterpret_cast<int*>(counts + (size_t)(total + 255) / 256 * 256);
     if (total > 0) {
-        hipLaunchKernelGGL(detect_count_kernel<T>, dim3(nBlocks), dim3(256), 0, st, d_data, nsamp, nSeg, total, P,
-                           th, offset, scale, counts, off);
+#define PPL_COUNT(PP)                                                                                         \
+    case PP:                                                                                                  \
+        hipLaunchKernelGGL((detect_count_kernel<T, PP>), dim3(nBlocks), dim3(256), 0, st, d_data, nsamp, nSeg, \
+                           total, th, offset, scale, counts, off);                                            \
+        break;
+        switch (P) {
+            PPL_COUNT(0)
+            PPL_COUNT(1)
+            PPL_COUNT(2)
+            PPL_COUNT(3)
+            PPL_COUNT(4)
+            PPL_COUNT(5)
+            PPL_COUNT(6)
+            default:
+                set_error("detect: binPow2 %d outside 0..6", P);
+                return PPL_E_ARG;
+        }
+#undef PPL_COUNT
         PPL_LAUNCHED();
     }
     hipLaunchKernelGGL(detect_scan_kernel, dim3(1), dim3(1024), 0, st, off, nBlocks, evTotal, maxEvents, evInts,
