@@ -368,6 +368,26 @@ int ppl_ppf_run_c32(ppl_ppf_plan* plan, const float* d_in, unsigned nBlk, float*
 int ppl_ppf_stokes_i16(ppl_ppf_plan* plan, const int16_t* d_in, unsigned nBlk, float* d_stokes,
                        void* stream);
 
+/* One plan's chunk in ppl_ppf_stokes_i16_batch. */
+enum { PPL_PPF_MAX_JOBS = 48 };
+typedef struct {
+    ppl_ppf_plan* plan;
+    const int16_t* d_in;   /* int16 voltages [sub][pol][strideBlk*nCh]; the first nBlk blocks of
+                              every stream are read */
+    unsigned nBlk;         /* blocks to channelise, a multiple of the plan's integrate; 0 skips the job */
+    unsigned strideBlk;    /* blocks per (sub, pol) stream of d_in, >= nBlk (0: nBlk) */
+    float* d_stokes;       /* [nBlk/integrate][sub][1][nCh] */
+} ppl_ppf_job;
+/* ppl_ppf_stokes_i16 of up to PPL_PPF_MAX_JOBS plans (beams of any channel
+ * count, tap count and DFT method) in one launch per (channel count, taps,
+ * method) group, and one launch that saves every plan's FIR history.  Each
+ * job's output equals ppl_ppf_stokes_i16 on the same plan and chunk, bit for
+ * bit.  A plan appears at most once per call (its chunks run in order).  No
+ * counterpart in the reference, which channelises one beam per module
+ * (src/lib/src/PPFChanneliser.cpp:81-187); for callers that step many beams
+ * together. */
+int ppl_ppf_stokes_i16_batch(const ppl_ppf_job* jobs, int n, void* stream);
+
 /* StokesGenerator::run (src/lib/src/StokesGenerator.cpp:40-108):
  * spectra [nBlk][nSub][2][nCh] complex -> [nBlk][nSub][nStokes][nCh]
  * (nStokes 1: I; 4: I, Q, U, V). */

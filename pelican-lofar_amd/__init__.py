@@ -129,6 +129,7 @@ SIGNATURES = {
     "ppl_ppf_run_i16": (i, [vp, vp, u, vp, vp]),
     "ppl_ppf_run_c32": (i, [vp, vp, u, vp, vp]),
     "ppl_ppf_stokes_i16": (i, [vp, vp, u, vp, vp]),
+    "ppl_ppf_stokes_i16_batch": (i, [vp, i, vp]),
     "ppl_stokes": (i, [vp, u, u, u, u, vp, vp]),
     "ppl_integrate": (i, [vp, u, u, u, u, u, u, vp, vp]),
     "ppl_clipper_create": (i, [vp, vp]),
@@ -616,6 +617,39 @@ class PPF:
         return d_out
 
 
+class PPFJob(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("d_in", C.c_void_p), ("nBlk", C.c_uint), ("strideBlk", C.c_uint),
+                ("d_stokes", C.c_void_p)]
+
+
+MAX_PPF_JOBS = 48
+
+
+def ppf_stokes_i16_batch(jobs, stream=None):
+    """PPF.stokes_i16 of many plans in one launch per (channel count, taps,
+    DFT method) and one history launch: jobs = [(PPF, int16 voltages
+    [sub][pol][samples][2], n_blk, d_out)].  The first n_blk blocks of each
+    (sub, pol) stream are read, so a chunk can be a prefix of a longer
+    voltage buffer; d_out is [n_blk / integrate][sub][1][nCh] (contiguous)."""
+    n = len(jobs)
+    if not 1 <= n <= MAX_PPF_JOBS:
+        raise PPLError("%d channeliser jobs (1..%d)" % (n, MAX_PPF_JOBS))
+    arr = (PPFJob * n)()
+    for k, (ppf, d_in, n_blk, d_out) in enumerate(jobs):
+        if not (d_in.is_contiguous() and d_out.is_contiguous()):
+            raise PPLError("job %d: voltages and output must be contiguous" % k)
+        if d_in.shape[0] != ppf.n_sub or d_in.shape[1] != 2 or d_in.shape[2] % ppf.n_ch:
+            raise PPLError("job %d: voltages of shape %s for %d subbands x %d channels"
+                           % (k, tuple(d_in.shape), ppf.n_sub, ppf.n_ch))
+        stride = d_in.shape[2] // ppf.n_ch
+        if n_blk > stride:
+            raise PPLError("job %d: %d blocks from a buffer of %d" % (k, n_blk, stride))
+        if d_out.numel() < (n_blk // ppf.integrate) * ppf.n_sub * ppf.n_ch:
+            raise PPLError("job %d: output too small" % k)
+        arr[k] = PPFJob(ppf.h, _ptr(d_in), n_blk, stride, _ptr(d_out))
+    check(lib().ppl_ppf_stokes_i16_batch(arr, n, _stream(stream)))
+
+
 def stokes(d_spectra, n_stokes=1, d_out=None, stream=None):
     import torch
     n_blk, n_sub, _, n_ch, _ = d_spectra.shape
@@ -1016,13 +1050,18 @@ class MultiBeam:
     clipper run for every beam, the transposition into each beam's buffer,
     one batched dedispersion launch for every beam and the overlap carries.
 
-    Each beam reads a voltage chunk of chunk_spectra[b] integrated spectra
-    (a divisor of its nOut), fed repeatedly as a continuous stream; a step
-    takes nOut[b] / chunk_spectra[b] channeliser calls per beam.  Every
-    launch goes to the current stream, so steps can be captured in two HIP
-    graphs (even and odd steps: the FIR histories and buffers alternate)."""
+    Each beam reads a voltage chunk of chunk_spectra[b] integrated spectra,
+    fed repeatedly as a continuous stream: n spectra are channelised in
+    chunks of chunk_spectra[b] read from the chunk's start, the last one
+    shorter (chunk_lengths).  The chunk sizes give every beam the same number
+    of chunks per step (at most max_chunk_samples input samples per
+    subband and polarisation in the largest), and each round of chunks, one
+    per beam, is one batched channeliser launch per (channel count, DFT
+    method) plus one history launch.  Every launch goes to the current
+    stream, so steps can be captured in two HIP graphs (even and odd steps:
+    the FIR histories and buffers alternate)."""
 
-    def __init__(self, specs, dtype="f32", inv_scale=127.0 / 10.0, max_chunk_samples=1 << 20, tile=None,
+    def __init__(self, specs, dtype="f32", inv_scale=127.0 / 10.0, max_chunk_samples=1 << 22, tile=None,
                  dft="fp32"):
         import torch
         if len(specs) > min(MAX_DEDISP_JOBS, 48):
@@ -1048,11 +1087,11 @@ class MultiBeam:
             # u8: the float series the detector reads (see BeamChain)
             self.series.append(torch.empty((sp.tdms, plan.nout), dtype=torch.float32, device="cuda")
                                if dtype == "u8" else None)
-            per = sp.integrate * sp.n_ch  # input samples per integrated spectrum
-            m = max(1, min(plan.nout, max_chunk_samples // per))
-            while plan.nout % m:
-                m -= 1
-            self.chunk_spectra.append(m)
+        # chunks per step: the largest beam's input in chunks of at most
+        # max_chunk_samples samples; every beam's step in that many chunks
+        per = [sp.integrate * sp.n_ch for sp in self.specs]  # input samples per integrated spectrum
+        rounds = max(-(-n * q // max_chunk_samples) for n, q in zip(self.nout, per))
+        self.chunk_spectra = [max(1, -(-n // rounds)) for n in self.nout]
         self.tile = _tile4(tile) if tile else (0, 0, 0, 4)
         self.graphs = None
         self.parity = 0
@@ -1103,23 +1142,34 @@ class MultiBeam:
             p.set_tile(*t)
         return ms
 
-    def _channelise(self, b, d_volt, n_spec, d_out):
-        """n_spec integrated spectra of beam b into d_out, chunk by chunk."""
-        sp, m = self.specs[b], self.chunk_spectra[b]
-        done = 0
-        while done < n_spec:
-            k = min(m, n_spec - done)
-            src = d_volt if k == m else d_volt[:, :, :k * sp.integrate * sp.n_ch].contiguous()
-            self.ppfs[b].stokes_i16(src, k * sp.integrate, d_out=d_out[done:done + k])
-            done += k
+    def chunk_lengths(self, b, n_spec):
+        """The chunks (integrated spectra each) in which n_spec spectra of
+        beam b are channelised, each read from the start of its voltage chunk."""
+        m = self.chunk_spectra[b]
+        return [min(m, n_spec - d) for d in range(0, n_spec, m)]
+
+    def _channelise(self, volts, n_specs, d_outs):
+        """n_specs[b] integrated spectra of every beam b into d_outs[b]: round
+        r channelises chunk r of every beam that has one, in one batched call."""
+        sched = [self.chunk_lengths(b, n) for b, n in enumerate(n_specs)]
+        done = [0] * len(sched)
+        for r in range(max(len(x) for x in sched)):
+            jobs = []
+            for b, lens in enumerate(sched):
+                if r < len(lens):
+                    k, sp = lens[r], self.specs[b]
+                    jobs.append((self.ppfs[b], volts[b], k * sp.integrate, d_outs[b][done[b]:done[b] + k]))
+                    done[b] += k
+            ppf_stokes_i16_batch(jobs)
 
     def prime(self, volts):
-        """Fill every beam's overlap (untimed set-up, beam by beam)."""
+        """Fill every beam's overlap (untimed set-up)."""
         import torch
+        sts = [torch.empty((ov, sp.n_sub, 1, sp.n_ch), dtype=torch.float32, device="cuda")
+               for ov, sp in zip(self.overlap, self.specs)]
+        self._channelise(volts, self.overlap, sts)
         for b, sp in enumerate(self.specs):
-            ov = self.overlap[b]
-            st = torch.empty((ov, sp.n_sub, 1, sp.n_ch), dtype=torch.float32, device="cuda")
-            self._channelise(b, volts[b], ov, st)
+            ov, st = self.overlap[b], sts[b]
             self.clips[b].run(st)
             used, full = self.bufs[b].add(st.view(ov, sp.n_sub * sp.n_ch))
             if used != ov or full:
@@ -1131,8 +1181,7 @@ class MultiBeam:
         def mark(i):
             if events is not None:
                 events[i].record()
-        for b in range(len(self.specs)):
-            self._channelise(b, volts[b], self.nout[b], self.stokes[b])
+        self._channelise(volts, self.nout, self.stokes)
         mark(0)
         clipper_run_batch(self.clips, self.stokes)
         mark(1)

@@ -20,8 +20,10 @@
 // are fused behind the FFT, so only int16 voltages are read and only the
 // integrated Stokes-I spectra are written.
 #include "ppl_internal.h"
+#include "ppl_ppf_batch.h"
 
 #include <cmath>
+#include <numeric>
 #include <vector>
 
 struct ppl_ppf_plan {
@@ -142,11 +144,14 @@ __device__ __forceinline__ void reg_dft(C2* x)
 // OUT: 0 = integrated Stokes-I [nBlk/integ][sub][1][N]; 1 = complex spectra
 // [nBlk][sub][2][N].  TAPS is the FIR length (compile-time, so the sliding
 // windows stay in registers).
+// The runs of one wave: waveId counts the waves of the launch (or of one
+// job of a batched launch); `stride` is the samples per (sub, pol) stream of
+// `in`, of which the first nBlk*N are read.
 template <typename Tin, int N, int TAPS, int OUT>
-__global__ __launch_bounds__(256) void ppf_kernel(
-    const Tin* __restrict__ in, const typename HistT<Tin>::type* __restrict__ hist, float* __restrict__ out,
-    const float* __restrict__ coeffs, const float* __restrict__ tw, int nSub, long nBlk, int runLen,
-    int nRuns, int integ)
+__device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typename HistT<Tin>::type* __restrict__ hist,
+                                         float* __restrict__ out, const float* __restrict__ coeffs,
+                                         const float* __restrict__ tw, int nSub, long nBlk, long stride, int runLen,
+                                         int nRuns, int integ, long waveId)
 {
     constexpr int G = N < 64 ? N : 64;
     constexpr int R = N / G;
@@ -155,13 +160,13 @@ __global__ __launch_bounds__(256) void ppf_kernel(
     const int lane = threadIdx.x & 63;
     const int grp = lane / G;
     const int l = lane % G;
-    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long wavesPerSub = (nRuns + RUNS_PER_WAVE - 1) / RUNS_PER_WAVE;
     const int sub = (int)(waveId / wavesPerSub);
     if (sub >= nSub) return;
-    const int run = (int)(waveId % wavesPerSub) * RUNS_PER_WAVE + grp;
+    const int run0 = (int)(waveId % wavesPerSub) * RUNS_PER_WAVE;  // the wave's first run
+    const int run = run0 + grp;
     const bool active = run < nRuns;
-    const long T = nBlk * N;  // samples per (sub, pol)
+    const long T = stride;
     const Tin* x0 = in + (long)(2 * sub + 0) * T;
     const Tin* x1 = in + (long)(2 * sub + 1) * T;
     const auto* h0 = hist + (long)(2 * sub + 0) * H * N;
@@ -231,15 +236,17 @@ __global__ __launch_bounds__(256) void ppf_kernel(
     for (int r = 0; r < R; ++r) acc[r] = 0.0f;
     const int lastRunLen = (int)(nBlk - (long)(nRuns - 1) * runLen);
     const int len = active ? (run == nRuns - 1 ? lastRunLen : runLen) : 0;
+    // steps of the wave (uniform): a full run unless its only run is the last
+    const int waveLen = run0 < nRuns - 1 ? runLen : lastRunLen;
     const int kLane = bitrev<G>(l);
     int inWindow = 0;  // blocks summed into the current integration window
     // output row of the current window: b0 is a multiple of integ (runLen is),
     // so row (b0 + s) / integ is counted instead of divided
     long uo = OUT == 0 ? b0 / integ : 0;
     // every lane runs every step: the lane FFT shuffles need the whole wave.
-    // Steps past runLen (up to the next multiple of TAPS) are not live: they
-    // store nothing.
-    for (int s0 = 0; s0 < runLen; s0 += TAPS) {
+    // Steps past a run's length (a shorter last run, and up to the next
+    // multiple of TAPS) are not live: they store nothing.
+    for (int s0 = 0; s0 < waveLen; s0 += TAPS) {
 #pragma unroll
         for (int u = 0; u < TAPS; ++u) {
             const int s = s0 + u;
@@ -321,6 +328,28 @@ __global__ __launch_bounds__(256) void ppf_kernel(
     }
 }
 
+template <typename Tin, int N, int TAPS, int OUT>
+__global__ __launch_bounds__(256) void ppf_kernel(
+    const Tin* __restrict__ in, const typename HistT<Tin>::type* __restrict__ hist, float* __restrict__ out,
+    const float* __restrict__ coeffs, const float* __restrict__ tw, int nSub, long nBlk, int runLen,
+    int nRuns, int integ)
+{
+    const long waveId = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    ppf_wave<Tin, N, TAPS, OUT>(in, hist, out, coeffs, tw, nSub, nBlk, nBlk * N, runLen, nRuns, integ, waveId);
+}
+
+// Many plans' fused PPF -> Stokes-I -> integration of one channel count and
+// tap count in one launch: each workgroup finds its job in the table.
+template <int N, int TAPS>
+__global__ __launch_bounds__(256) void ppf_batch_kernel(const PPFBatch B)
+{
+    int local;
+    const PPFJob& J = find_batch_job(B, local);
+    const long waveId = (long)local * 4 + (threadIdx.x >> 6);
+    ppf_wave<uint32_t, N, TAPS, 0>(J.in, J.hist, J.out, J.coeffs, B.tw, J.nSub, J.nBlk, J.stride, J.runLen,
+                                   J.nRuns, J.integ, waveId);
+}
+
 // Save the last nTaps-1 blocks of every (sub, pol) stream (new history),
 // taking them from the chunk or, for short chunks, from the old history.
 template <typename Tin>
@@ -337,6 +366,21 @@ __global__ void save_history_kernel(const Tin* __restrict__ in, const Tin* __res
     newh[i] = b >= 0 ? in[stream * nBlk * N + b * N + n] : oldh[stream * per + (H + b) * N + n];
 }
 
+// History update of every job of a batched launch.
+__global__ void save_history_batch_kernel(const HistBatch B)
+{
+    int local;
+    const HistJob& J = find_batch_job(B, local);
+    const long i = (long)local * blockDim.x + threadIdx.x;
+    const long per = (long)J.H * J.N;
+    if (i >= per * J.nStreams) return;
+    const long stream = i / per;
+    const long j = i % per;
+    const long slot = j / J.N, n = j % J.N;
+    const long b = J.nBlk - J.H + slot;
+    J.newh[i] = b >= 0 ? J.in[stream * J.stride + b * J.N + n] : J.oldh[stream * per + (J.H + b) * J.N + n];
+}
+
 template <typename Tin>
 int run_ppf(ppl_ppf_plan* p, const Tin* in, typename HistT<Tin>::type* histCur,
             typename HistT<Tin>::type* histNext, unsigned nBlk, float* out, int mode, hipStream_t st)
@@ -345,12 +389,12 @@ int run_ppf(ppl_ppf_plan* p, const Tin* in, typename HistT<Tin>::type* histCur,
     const int integ = mode == 0 ? (int)p->cfg.integrate : 1;
     PPL_CHECK_ARG(nBlk > 0, "Empty time data blob");
     PPL_CHECK_ARG(nBlk % integ == 0, "nBlk (%u) must be a multiple of the integration (%d)", nBlk, integ);
-    // runs of L blocks, L a multiple of the integration window
-    int L = 256;
-    if (mode == 0) L = ((L + integ - 1) / integ) * integ;
-    if ((unsigned)L > nBlk) L = (int)nBlk;
-    const int nRuns = (int)cdiv(nBlk, L);
+    // runs of L blocks, L a multiple of the integration window and of the tap count
     const int G = N < 64 ? N : 64;
+    const long nb = nBlk;
+    const int mult = std::lcm(integ, T);
+    const int L = ppf_run_length(&nb, &S, 1, 64 / G, kFillWaves, mult, 4 * T);
+    const int nRuns = (int)cdiv(nBlk, L);
     const long wavesPerSub = (nRuns + (64 / G) - 1) / (64 / G);
     const long waves = wavesPerSub * S;
     const unsigned blocks = (unsigned)cdiv((unsigned)waves, 4);
@@ -389,6 +433,81 @@ int run_ppf(ppl_ppf_plan* p, const Tin* in, typename HistT<Tin>::type* histCur,
     }
     return PPL_OK;
 }
+
+// FP32 batched launch of a group of jobs sharing N and TAPS: sets each
+// job's runs and first workgroup.
+template <int N, int TAPS>
+int launch_fp32_batch(PPFBatch& B, hipStream_t st)
+{
+    constexpr int G = N < 64 ? N : 64;
+    constexpr int RPW = 64 / G;
+    long nb[kMaxPPFJobs];
+    int ns[kMaxPPFJobs];
+    int mult = TAPS;
+    for (int k = 0; k < B.n; ++k) {
+        nb[k] = B.job[k].nBlk;
+        ns[k] = B.job[k].nSub;
+        mult = std::lcm(mult, B.job[k].integ);
+    }
+    const int L = ppf_run_length(nb, ns, B.n, RPW, kFillWaves, mult, 4 * TAPS);
+    long blocks = 0;
+    for (int k = 0; k < B.n; ++k) {
+        PPFJob& J = B.job[k];
+        J.runLen = L;
+        J.nRuns = (J.nBlk + L - 1) / L;
+        const long waves = (long)J.nSub * ((J.nRuns + RPW - 1) / RPW);
+        J.blockStart = (int)blocks;
+        blocks += (waves + 3) / 4;
+        if (blocks >= (1L << 31)) {
+            set_error("channeliser batch: %ld workgroups exceed the grid", blocks);
+            return PPL_E_ARG;
+        }
+    }
+    hipLaunchKernelGGL((ppf_batch_kernel<N, TAPS>), dim3((unsigned)blocks), dim3(256), 0, st, B);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
+int launch_fp32_batch_any(PPFBatch& B, int N, int taps, hipStream_t st)
+{
+#define PPL_PPFB_CASE(N_)                                           \
+    if (N == N_) {                                                  \
+        if (taps == 4) return launch_fp32_batch<N_, 4>(B, st);      \
+        if (taps == 8) return launch_fp32_batch<N_, 8>(B, st);      \
+        if (taps == 16) return launch_fp32_batch<N_, 16>(B, st);    \
+    }
+    PPL_PPFB_CASE(8)
+    PPL_PPFB_CASE(16)
+    PPL_PPFB_CASE(32)
+    PPL_PPFB_CASE(64)
+    PPL_PPFB_CASE(128)
+    PPL_PPFB_CASE(256)
+#undef PPL_PPFB_CASE
+    set_error("channeliser: no kernel for %d channels x %d taps", N, taps);
+    return PPL_E_ARG;
+}
+
+}  // namespace
+
+int ppl::ppf_run_length(const long* nBlk, const int* nSub, int n, int unitsPerRun, long fill, int mult, int minLen)
+{
+    auto units = [&](long L) {
+        long u = 0;
+        for (int j = 0; j < n; ++j) u += (long)nSub[j] * (((nBlk[j] + L - 1) / L + unitsPerRun - 1) / unitsPerRun);
+        return u;
+    };
+    auto round_up = [&](long v) { return (v + mult - 1) / mult * mult; };
+    long L = round_up(256);
+    const long floorLen = round_up(minLen);
+    while (units(L) < fill) {
+        const long h = round_up(L / 2);
+        if (h >= L || h < floorLen) break;
+        L = h;
+    }
+    return (int)L;
+}
+
+namespace {
 
 bool pow2_supported(unsigned n) { return n >= 8 && n <= 256 && (n & (n - 1)) == 0; }
 
@@ -448,6 +567,7 @@ int ppl_mfma_create(ppl_ppf_plan* p);
 void ppl_mfma_destroy(ppl_ppf_plan* p);
 int ppl_mfma_run(ppl_ppf_plan* p, const int16_t* d_in, const uint32_t* histCur, unsigned nBlk,
                  float* d_out, int mode, hipStream_t st);
+int ppl_mfma_run_batch(ppl::PPFBatch& B, int nCh, int taps, int prec, hipStream_t st);
 
 extern "C" int ppl_ppf_plan_create(ppl_ppf_plan** plan, const ppl_ppf_config* cfg)
 {
@@ -569,6 +689,83 @@ extern "C" int ppl_ppf_stokes_i16(ppl_ppf_plan* p, const int16_t* d_in, unsigned
                                p->d_hist[p->cur ^ 1], nBlk, d_stokes, 0, stream_of(stream));
     if (rc == PPL_OK) p->cur ^= 1;
     return rc;
+}
+
+extern "C" int ppl_ppf_stokes_i16_batch(const ppl_ppf_job* jobs, int n, void* stream)
+{
+    PPL_CHECK_ARG(jobs && n >= 1 && n <= kMaxPPFJobs, "%d channeliser jobs (1..%d)", n, kMaxPPFJobs);
+    const hipStream_t st = stream_of(stream);
+    for (int k = 0; k < n; ++k) {
+        const ppl_ppf_job& j = jobs[k];
+        PPL_CHECK_ARG(j.plan, "job %d: null plan", k);
+        if (!j.nBlk) continue;
+        PPL_CHECK_ARG(j.d_in && j.d_stokes, "job %d: null argument", k);
+        PPL_CHECK_ARG(j.nBlk < (1u << 31), "job %d: %u blocks", k, j.nBlk);
+        PPL_CHECK_ARG(j.nBlk % j.plan->cfg.integrate == 0,
+                      "job %d: nBlk (%u) must be a multiple of the integration (%u)", k, j.nBlk,
+                      j.plan->cfg.integrate);
+        PPL_CHECK_ARG(j.strideBlk == 0 || j.strideBlk >= j.nBlk, "job %d: stride of %u blocks below nBlk %u", k,
+                      j.strideBlk, j.nBlk);
+        for (int m = 0; m < k; ++m)
+            PPL_CHECK_ARG(jobs[m].plan != j.plan || !jobs[m].nBlk,
+                          "job %d: the plan of job %d again (a plan's chunks run in order)", k, m);
+    }
+    auto fill = [&](const ppl_ppf_job& j) {
+        const ppl_ppf_plan* p = j.plan;
+        PPFJob J{};
+        J.in = reinterpret_cast<const uint32_t*>(j.d_in);
+        J.hist = p->d_hist[p->cur];
+        J.out = j.d_stokes;
+        J.coeffs = p->d_coeffs;
+        J.stride = (long)(j.strideBlk ? j.strideBlk : j.nBlk) * p->cfg.nCh;
+        J.nBlk = (int)j.nBlk;
+        J.nSub = (int)p->cfg.nSub;
+        J.integ = (int)p->cfg.integrate;
+        return J;
+    };
+    // one launch per (channel count, taps, DFT method)
+    bool done[kMaxPPFJobs] = {};
+    for (int k = 0; k < n; ++k) {
+        if (done[k] || !jobs[k].nBlk) continue;
+        const ppl_ppf_config& c = jobs[k].plan->cfg;
+        PPFBatch B{};
+        B.tw = jobs[k].plan->d_twiddle;
+        for (int m = k; m < n; ++m) {
+            const ppl_ppf_config& d = jobs[m].plan->cfg;
+            if (done[m] || !jobs[m].nBlk || d.nCh != c.nCh || d.nTaps != c.nTaps || d.precision != c.precision)
+                continue;
+            B.job[B.n++] = fill(jobs[m]);
+            done[m] = true;
+        }
+        const int rc = c.precision == PPL_DFT_FP32 ? launch_fp32_batch_any(B, (int)c.nCh, (int)c.nTaps, st)
+                                                  : ppl_mfma_run_batch(B, (int)c.nCh, (int)c.nTaps, c.precision, st);
+        if (rc != PPL_OK) return rc;
+    }
+    // every job's new history in one launch
+    HistBatch H{};
+    long blocks = 0;
+    for (int k = 0; k < n; ++k) {
+        const ppl_ppf_job& j = jobs[k];
+        if (!j.nBlk) continue;
+        const ppl_ppf_plan* p = j.plan;
+        HistJob& h = H.job[H.n++];
+        h.in = reinterpret_cast<const uint32_t*>(j.d_in);
+        h.oldh = p->d_hist[p->cur];
+        h.newh = p->d_hist[p->cur ^ 1];
+        h.N = (int)p->cfg.nCh;
+        h.H = (int)p->cfg.nTaps - 1;
+        h.nStreams = 2 * (int)p->cfg.nSub;
+        h.stride = (long)(j.strideBlk ? j.strideBlk : j.nBlk) * h.N;
+        h.nBlk = j.nBlk;
+        h.blockStart = (int)blocks;
+        blocks += ((long)h.nStreams * h.H * h.N + 255) / 256;
+    }
+    if (!H.n) return PPL_OK;
+    hipLaunchKernelGGL(save_history_batch_kernel, dim3((unsigned)blocks), dim3(256), 0, st, H);
+    PPL_LAUNCHED();
+    for (int k = 0; k < n; ++k)
+        if (jobs[k].nBlk) jobs[k].plan->cur ^= 1;
+    return PPL_OK;
 }
 
 // history buffers for the split-precision path

@@ -29,6 +29,7 @@
 // Stokes-I (StokesGenerator.cpp:93-99) and the time integration
 // (StokesIntegrator.cpp:97-114) follow in FP32 in the reference's order.
 #include "ppl_internal.h"
+#include "ppl_ppf_batch.h"
 
 #include <cmath>
 
@@ -133,15 +134,15 @@ __device__ __forceinline__ C2 load_c(const uint32_t* __restrict__ p, long idx)
 }
 
 // OUT: 0 = integrated Stokes-I [nBlk/integ][sub][1][N]; 1 = complex spectra
-// [nBlk][sub][2][N].  One workgroup runs the blocks [run*runLen, +runLen) of
-// one subband, both polarisations, in tiles of tile_blocks<N>() blocks.
+// [nBlk][sub][2][N].  Workgroup `group` (of the launch, or of one job of a
+// batched launch) runs the blocks [run*runLen, +runLen) of one subband, both
+// polarisations, in tiles of tile_blocks<N>() blocks; `stride` is the samples
+// per (sub, pol) stream of `in`, of which the first nBlk*N are read.
 template <int N, int TAPS, int PREC, int OUT>
-__global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __restrict__ in,
-                                                            const uint32_t* __restrict__ hist,
-                                                            float* __restrict__ out,
-                                                            const float* __restrict__ coeffs,
-                                                            const float* __restrict__ tw, int nSub, long nBlk,
-                                                            int runLen, int nRuns, int integ)
+__device__ __forceinline__ void ppf_mfma_group(const uint32_t* __restrict__ in, const uint32_t* __restrict__ hist,
+                                               float* __restrict__ out, const float* __restrict__ coeffs,
+                                               const float* __restrict__ tw, int nSub, long nBlk, long stride,
+                                               int runLen, int nRuns, int integ, int group)
 {
     constexpr int R = N / 16;
     static_assert(R == 4 || R == 16, "split-precision DFT: 64 or 256 channels");
@@ -159,11 +160,11 @@ __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __re
     __shared__ float2 F[2][kTB][N];   // filtered blocks
     __shared__ float P[kTB][N];       // Stokes-I per block (OUT 0)
 
-    const int sub = blockIdx.x / nRuns;
-    const int run = blockIdx.x % nRuns;
+    const int sub = group / nRuns;
+    const int run = group % nRuns;
     if (sub >= nSub) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long T = nBlk * N;
+    const long T = stride;
     const long b0 = (long)run * runLen;
     const int len = (int)min((long)runLen, nBlk - b0);
 
@@ -364,11 +365,35 @@ __global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __re
     }
 }
 
+template <int N, int TAPS, int PREC, int OUT>
+__global__ __launch_bounds__(kThreads) void ppf_mfma_kernel(const uint32_t* __restrict__ in,
+                                                            const uint32_t* __restrict__ hist,
+                                                            float* __restrict__ out,
+                                                            const float* __restrict__ coeffs,
+                                                            const float* __restrict__ tw, int nSub, long nBlk,
+                                                            int runLen, int nRuns, int integ)
+{
+    ppf_mfma_group<N, TAPS, PREC, OUT>(in, hist, out, coeffs, tw, nSub, nBlk, nBlk * N, runLen, nRuns, integ,
+                                       (int)blockIdx.x);
+}
+
+// Many plans' fused channeliser of one channel count, tap count and
+// precision in one launch: each workgroup finds its job in the table.
+template <int N, int TAPS, int PREC>
+__global__ __launch_bounds__(kThreads) void ppf_mfma_batch_kernel(const PPFBatch B)
+{
+    int local;
+    const PPFJob& J = find_batch_job(B, local);
+    ppf_mfma_group<N, TAPS, PREC, 0>(J.in, J.hist, J.out, J.coeffs, B.tw, J.nSub, J.nBlk, J.stride, J.runLen,
+                                     J.nRuns, J.integ, local);
+}
+
 template <int N, int TAPS, int PREC>
 int launch(const ppl_ppf_config& c, const uint32_t* in, const uint32_t* hist, float* out, const float* coeffs,
            const float* tw, long nBlk, int mode, hipStream_t st)
 {
-    const int runLen = 256;
+    const int nSub = (int)c.nSub;
+    const int runLen = ppf_run_length(&nBlk, &nSub, 1, 1, kFillGroups, tile_blocks<N>(), 4 * tile_blocks<N>());
     const int nRuns = (int)((nBlk + runLen - 1) / runLen);
     const long grid = (long)c.nSub * nRuns;
     if (grid >= (1L << 31)) {
@@ -398,7 +423,62 @@ int launch_taps(const ppl_ppf_config& c, const uint32_t* in, const uint32_t* his
     return PPL_E_ARG;
 }
 
+// One batched launch of a group of jobs sharing N, TAPS and PREC: sets each
+// job's runs and first workgroup.
+template <int N, int TAPS, int PREC>
+int launch_batch(PPFBatch& B, hipStream_t st)
+{
+    long nb[kMaxPPFJobs];
+    int ns[kMaxPPFJobs];
+    for (int k = 0; k < B.n; ++k) {
+        nb[k] = B.job[k].nBlk;
+        ns[k] = B.job[k].nSub;
+    }
+    const int L = ppf_run_length(nb, ns, B.n, 1, kFillGroups, tile_blocks<N>(), 4 * tile_blocks<N>());
+    long blocks = 0;
+    for (int k = 0; k < B.n; ++k) {
+        PPFJob& J = B.job[k];
+        J.runLen = L;
+        J.nRuns = (J.nBlk + L - 1) / L;
+        J.blockStart = (int)blocks;
+        blocks += (long)J.nSub * J.nRuns;
+        if (blocks >= (1L << 31)) {
+            set_error("channeliser batch: %ld workgroups exceed the grid", blocks);
+            return PPL_E_ARG;
+        }
+    }
+    hipLaunchKernelGGL((ppf_mfma_batch_kernel<N, TAPS, PREC>), dim3((unsigned)blocks), dim3(kThreads), 0, st, B);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
+template <int N, int PREC>
+int launch_batch_taps(PPFBatch& B, int taps, hipStream_t st)
+{
+    switch (taps) {
+        case 4: return launch_batch<N, 4, PREC>(B, st);
+        case 8: return launch_batch<N, 8, PREC>(B, st);
+        case 16: return launch_batch<N, 16, PREC>(B, st);
+    }
+    set_error("channeliser: %d taps", taps);
+    return PPL_E_ARG;
+}
+
 }  // namespace
+
+// The split-precision part of ppl_ppf_stokes_i16_batch: jobs of one channel
+// count (64 or 256), tap count and precision.
+int ppl_mfma_run_batch(ppl::PPFBatch& B, int nCh, int taps, int prec, hipStream_t st)
+{
+    if (nCh == 64)
+        return prec == PPL_DFT_BF16X3 ? launch_batch_taps<64, PPL_DFT_BF16X3>(B, taps, st)
+                                      : launch_batch_taps<64, PPL_DFT_FP16X2>(B, taps, st);
+    if (nCh == 256)
+        return prec == PPL_DFT_BF16X3 ? launch_batch_taps<256, PPL_DFT_BF16X3>(B, taps, st)
+                                      : launch_batch_taps<256, PPL_DFT_FP16X2>(B, taps, st);
+    set_error("split-precision DFT channeliser: %d channels", nCh);
+    return PPL_E_CONFIG;
+}
 
 int ppl_mfma_create(ppl_ppf_plan* p)
 {

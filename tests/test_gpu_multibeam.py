@@ -38,19 +38,17 @@ def test_multibeam_equals_single_beam_chains(ppl, dtype, dft):
         ch = ppl.BeamChain(sp.n_sub, sp.n_ch, 8, sp.coeffs, 4, sp.bp_values, sp.nsamp_buf, sp.tdms, 0.0,
                            sp.dm_step, sp.tsamp, sp.fch1, sp.foff, dtype=dtype, history=1000, dft=prec)
         singles.append(ch)
-    # the single chains read the same periodic stream: the chunk repeated
-    reps = [mb.nout[b] // mb.chunk_spectra[b] for b in range(len(specs))]
-    step_in = [volts[b].repeat(1, 1, reps[b], 1).contiguous() for b in range(len(specs))]
-    prime_in = []
-    for b, sp in enumerate(specs):
-        n = mb.overlap[b] * 4 * sp.n_ch
-        whole = volts[b].repeat(1, 1, n // mb.chunk_samples(b) + 1, 1)
-        prime_in.append(whole[:, :, :n].contiguous())
+    # the single chains read the same stream: MultiBeam's chunks, each a
+    # prefix of the beam's voltage chunk, one after the other
+    def stream(b, n_spec):
+        per = 4 * specs[b].n_ch
+        return torch.cat([volts[b][:, :, :k * per] for k in mb.chunk_lengths(b, n_spec)], dim=2).contiguous()
+
+    step_in = [stream(b, mb.nout[b]) for b in range(len(specs))]
+    prime_in = [stream(b, mb.overlap[b]) for b in range(len(specs))]
     mb.prime(volts)
     for b, ch in enumerate(singles):
         ch.prime(prime_in[b])
-    # MultiBeam's steps read each chunk from its start, after the prime's
-    # partial chunk: the single chains' step input is the chunk repeated
     mb_in = volts
     for k in range(2):
         mb.step(mb_in)
