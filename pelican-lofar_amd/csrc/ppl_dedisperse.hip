@@ -576,8 +576,14 @@ __global__ __launch_bounds__(256) void dedisperse_f16_kernel(const DDBatch B)
             } else {
                 for (int k0 = 0; k0 < rowDw; k0 += 64) {
                     const size_t g = byte0 + 4 * (size_t)(k0 + lane);
-                    if (k0 + lane < rowDw && g + 4 <= totalBytes)
+                    if (k0 + lane < rowDw && g + 4 <= totalBytes) {
                         __builtin_amdgcn_global_load_lds(inBytes + g, dst + 4 * k0, 4, 0, 0);
+                    } else if (k0 + lane < rowDw && g < totalBytes) {
+                        // the input's last, partial dword (one half)
+                        uint32_t v = 0u;
+                        for (size_t b = 0; g + b < totalBytes; ++b) v |= (uint32_t)inBytes[g + b] << (8 * b);
+                        reinterpret_cast<uint32_t*>(dst)[k0 + lane] = v;
+                    }
                 }
             }
         }
@@ -748,8 +754,14 @@ __global__ __launch_bounds__(256) void dedisperse_u8_kernel(const DDBatch B)
             } else {
                 for (int k0 = 0; k0 < rowDw; k0 += 64) {
                     const size_t g = byte0 + 4 * (size_t)(k0 + lane);
-                    if (k0 + lane < rowDw && g + 4 <= totalBytes)
+                    if (k0 + lane < rowDw && g + 4 <= totalBytes) {
                         __builtin_amdgcn_global_load_lds(in + g, dst + 4 * k0, 4, 0, 0);
+                    } else if (k0 + lane < rowDw && g < totalBytes) {
+                        // the input's last, partial dword
+                        uint32_t v = 0u;
+                        for (size_t b = 0; g + b < totalBytes; ++b) v |= (uint32_t)in[g + b] << (8 * b);
+                        reinterpret_cast<uint32_t*>(dst)[k0 + lane] = v;
+                    }
                 }
             }
         }

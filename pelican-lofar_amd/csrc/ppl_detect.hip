@@ -179,8 +179,101 @@ __global__ __launch_bounds__(256) void detect_count_kernel(const T* __restrict__
     if (threadIdx.x == 0) blockSums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
+// Four consecutive samples of a row as floats (16-byte load).
+template <typename T>
+__device__ __forceinline__ float4 sample_quad(const T* data, long idx, int offset, float scale);
+template <>
+__device__ __forceinline__ float4 sample_quad<float>(const float* data, long idx, int, float)
+{
+    return *reinterpret_cast<const float4*>(data + idx);
+}
+template <>
+__device__ __forceinline__ float4 sample_quad<int32_t>(const int32_t* data, long idx, int offset, float scale)
+{
+    const int4 v = *reinterpret_cast<const int4*>(data + idx);
+    return make_float4((float)(v.x - offset) * scale, (float)(v.y - offset) * scale, (float)(v.z - offset) * scale,
+                       (float)(v.w - offset) * scale);
+}
+
+// Value of lane + D (D <= 8, inside a row of 16 lanes; 0 past the row).
+__device__ __forceinline__ int int_from_lane_above(int x, int D)
+{
+    switch (D) {
+        case 1: return __builtin_amdgcn_update_dpp(0, x, 0x101, 0xf, 0xf, true);
+        case 2: return __builtin_amdgcn_update_dpp(0, x, 0x102, 0xf, 0xf, true);
+        case 4: return __builtin_amdgcn_update_dpp(0, x, 0x104, 0xf, 0xf, true);
+        default: return __builtin_amdgcn_update_dpp(0, x, 0x108, 0xf, 0xf, true);
+    }
+}
+
+// Pass 1 with 16-byte loads, for segments of 16..64 samples (P = 4..6):
+// lane m of a segment's LS = 2^(P-2) lanes holds its samples 4m..4m+3.
+// Levels 0..2 are summed in the lane (x+y, z+w, then their sum, the
+// reference's pair order); level n > 2 adds the level n-1 bin of the lane
+// 2^(n-3) above (DPP row shift), kept on lanes m that are multiples of
+// 2^(n-2).  Each lane counts its own bins' hits and the segment's lanes
+// reduce them onto its first lane.  Same counts as detect_count_kernel.
+template <typename T, int P>
+__global__ __launch_bounds__(256) void detect_count4_kernel(const T* __restrict__ data, int nsamp, int nSeg,
+                                                            int total, Thresholds th, int offset, float scale,
+                                                            uint8_t* __restrict__ counts, int* __restrict__ blockSums)
+{
+    static_assert(P >= 4 && P <= 6, "16..64-sample segments");
+    constexpr int LS = 1 << (P - 2);        // lanes per segment
+    constexpr int SPL = 64 / LS;            // segments per load
+    constexpr int NL = kSegPerWave / SPL;   // loads per lane
+    __shared__ int part[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & (LS - 1);
+    const int first = blockIdx.x * kSegPerBlock + wave * kSegPerWave;
+    float4 v[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+        const int id = first + L * SPL + lane / LS;
+        v[L] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (id < total) {
+            const int dm = id / nSeg, seg = id - dm * nSeg;
+            v[L] = sample_quad<T>(data, (long)dm * nsamp + ((long)seg << P) + 4 * m, offset, scale);
+        }
+    }
+    int sum = 0;
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+        const float4 x = v[L];
+        int c = (x.x >= th.v[0]) + (x.y >= th.v[0]) + (x.z >= th.v[0]) + (x.w >= th.v[0]);
+        const float a = x.x + x.y, b = x.z + x.w;
+        c += (a >= th.v[1]) + (b >= th.v[1]);
+        float r = a + b;
+        c += r >= th.v[2];
+        r = r + from_lane_above<1>(r);
+        c += (m & 1) == 0 && r >= th.v[3];
+        r = r + from_lane_above<2>(r);
+        c += (m & 3) == 0 && r >= th.v[4];
+        if constexpr (P >= 5) {
+            r = r + from_lane_above<4>(r);
+            c += (m & 7) == 0 && r >= th.v[5];
+        }
+        if constexpr (P >= 6) {
+            r = r + from_lane_above<8>(r);
+            c += (m & 15) == 0 && r >= th.v[6];
+        }
+#pragma unroll
+        for (int d = 1; d < LS; d <<= 1) c += int_from_lane_above(c, d);
+        const int id = first + L * SPL + lane / LS;
+        if (m == 0 && id < total) {
+            counts[id] = (uint8_t)c;
+            sum += c;
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) part[wave] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
 // Pass 2 (one workgroup): exclusive prefix of the workgroup sums, in place,
-// 4096 at a time (four per thread), with off[nBlocks] = their total;
+// 16384 at a time (sixteen consecutive per thread), with off[nBlocks] = their total;
 // *evTotal = 1 + total, and the dummy event (0, 0, 1, 0) in slot 0
 // (DedispersionAnalyser.cpp:69).
 __global__ __launch_bounds__(1024) void detect_scan_kernel(int* __restrict__ off, int nBlocks,
@@ -190,13 +283,27 @@ __global__ __launch_bounds__(1024) void detect_scan_kernel(int* __restrict__ off
     __shared__ int waveTot[16];
     __shared__ int chunkTot;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int K = 16;
     int carry = 0;
-    for (int base = 0; base < nBlocks; base += 4096) {
-        const int i0 = base + 4 * t;
-        int x[4];
+    for (int base = 0; base < nBlocks; base += 1024 * K) {
+        const int i0 = base + K * t;
+        int x[K];
+        if (i0 + K <= nBlocks) {  // off is 256-byte aligned and i0 a multiple of 16
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = i0 + k < nBlocks ? off[i0 + k] : 0;
-        const int s = x[0] + x[1] + x[2] + x[3];
+            for (int k = 0; k < K; k += 4) {
+                const int4 q = *reinterpret_cast<const int4*>(off + i0 + k);
+                x[k] = q.x;
+                x[k + 1] = q.y;
+                x[k + 2] = q.z;
+                x[k + 3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) x[k] = i0 + k < nBlocks ? off[i0 + k] : 0;
+        }
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) s += x[k];
         int incl = s;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -219,7 +326,7 @@ __global__ __launch_bounds__(1024) void detect_scan_kernel(int* __restrict__ off
         __syncthreads();
         int e = carry + waveTot[wave] + incl - s;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < K; ++k) {
             if (i0 + k < nBlocks) off[i0 + k] = e;
             e += x[k];
         }
@@ -319,7 +426,22 @@ int launch_detect(const T* d_data, int tdms, int nsamp, int offset, float scale,
     const int nBlocks = (total + kSegPerBlock - 1) / kSegPerBlock;
     uint8_t* counts = static_cast<uint8_t*>(scratch);
     int* off = reinterpret_cast<int*>(counts + (size_t)(total + 255) / 256 * 256);
-    if (total > 0) {
+    // 16-byte loads where the rows are 16-byte aligned and segments hold 16..64 samples
+    const bool quads = P >= 4 && nsamp % 4 == 0 && (reinterpret_cast<uintptr_t>(d_data) & 15) == 0;
+    if (total > 0 && quads) {
+#define PPL_COUNT4(PP)                                                                                       \
+    case PP:                                                                                                 \
+        hipLaunchKernelGGL((detect_count4_kernel<T, PP>), dim3(nBlocks), dim3(256), 0, st, d_data, nsamp, nSeg, \
+                           total, th, offset, scale, counts, off);                                           \
+        break;
+        switch (P) {
+            PPL_COUNT4(4)
+            PPL_COUNT4(5)
+            PPL_COUNT4(6)
+        }
+#undef PPL_COUNT4
+        PPL_LAUNCHED();
+    } else if (total > 0) {
 #define PPL_COUNT(PP)                                                                                         \
     case PP:                                                                                                  \
         hipLaunchKernelGGL((detect_count_kernel<T, PP>), dim3(nBlocks), dim3(256), 0, st, d_data, nsamp, nSeg, \

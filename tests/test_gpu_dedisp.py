@@ -194,8 +194,8 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
     """Every instantiated tile, channel block and read width, odd channel
     counts and odd delay parities (the FP16 form reads pairs from a shifted
     copy, the 16-byte FP32 form from four copies shifted by 0..3, staged
-    through its four-stage ring: 53 channels end part way through a round
-    of four stages)."""
+    through its three-stage ring: 53 channels end part way through a round
+    of stages)."""
     # 18 samples of delay per trial in the lowest channel; a third of that for
     # the 16-byte form, whose ring stage holds a spread of up to 256 samples
     # at 512-sample tiles
@@ -222,6 +222,40 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
         with pytest.raises(ppl.PPLError):  # the u8 form needs 4 samples per lane
             plan.run(dev)
         return
+    out = plan.run(dev).cpu().numpy()
+    assert np.array_equal(out, orc.dedisperse(host, d, p["nout"]))
+
+
+@pytest.mark.parametrize("dtype,tile", [("u8", (32, 512, 0, 4)), ("u8", (16, 256, 3, 4)), ("f16", (32, 512, 0, 4)),
+                                        ("f16", (16, 512, 2, 4)), ("f32", (32, 512, 0, 16))])
+def test_last_partial_dword_of_the_input(orc, ppl, dtype, tile):
+    """A buffer whose byte count is not a multiple of four, with the delays
+    of the top trial reaching the last sample of the last channel: the
+    kernels that stage whole dwords read the input's final, partial dword
+    byte by byte."""
+    # dm_step 0.31: maxshift 391, and the top trial's delay in the last channel is 391 too
+    base = dict(n_chan=53, tdms=64, dm_low=0.0, dm_step=0.31, tsamp=0.00032768, fch1=150.0, foff=-0.05)
+    cfg = None
+    for nsamp in range(2001, 3000, 2):  # odd lengths: 53 * nsamp is odd
+        c = dict(base, nsamp_buf=nsamp)
+        p, d = _oracle(orc, None, c)
+        if p["nout"] - 1 + int(d.max()) == nsamp - 1:
+            cfg = c
+            break
+    assert cfg is not None
+    rng = np.random.default_rng(nsamp)
+    x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
+    x[-1, -1] = 9.0  # the last sample, read by the top trial's last output
+    if dtype == "f32":
+        dev, host = torch.from_numpy(x).cuda(), x
+    elif dtype == "f16":
+        host = orc.f32_to_f16_bits(x)
+        dev = torch.from_numpy(host.view(np.float16)).cuda()
+    else:
+        host = orc.quantise_u8(x, np.float32(12.7))
+        dev = torch.from_numpy(host).cuda()
+    plan = ppl.DedispPlan(**cfg, dtype=dtype)
+    plan.set_tile(*tile)
     out = plan.run(dev).cpu().numpy()
     assert np.array_equal(out, orc.dedisperse(host, d, p["nout"]))
 

@@ -16,9 +16,12 @@ def _data(tdms, nsamp, nch, nsub, seed, pulses=()):
     return x
 
 
-@pytest.mark.parametrize("bin_pow2", [6, 4, 0])
-def test_events_identical(orc, ppl, bin_pow2):
-    x = _data(64, 64 * 40 + 19, 16, 8, 1, [(10, 300, 9, 20), (11, 301, 9, 15), (40, 2000, 64, 3)])
+@pytest.mark.parametrize("bin_pow2", [6, 5, 4, 2, 0])
+@pytest.mark.parametrize("tail", [19, 12])
+def test_events_identical(orc, ppl, bin_pow2, tail):
+    """Rows of a length that is a multiple of four take the 16-byte count
+    pass (segments of 16..64 samples), others the one-sample pass."""
+    x = _data(64, 64 * 40 + tail, 16, 8, 1, [(10, 300, 9, 20), (11, 301, 9, 15), (40, 2000, 64, 3)])
     ev_o, val_o = orc.detect(x, 16, 8, 6.0, bin_pow2)
     ev_g, val_g, n = ppl.detect(torch.from_numpy(x).cuda(), 16, 8, 6.0, bin_pow2)
     assert n == len(ev_o)
@@ -70,11 +73,12 @@ def test_reference_single_event(orc, ppl):
 
 
 @pytest.mark.parametrize("threshold", [2.5, 4.0])
-def test_events_identical_many_workgroups(orc, ppl, threshold):
+@pytest.mark.parametrize("tail", [37, 36])
+def test_events_identical_many_workgroups(orc, ppl, threshold, tail):
     """Low thresholds on noise: hits in most segments of 512 DM rows x 600
-    segments (4800 workgroups of 64 segments, more than one 4096-entry round
-    of the workgroup scan), and a truncated list of the same events."""
-    x = _data(512, 64 * 600 + 37, 16, 8, 5)
+    segments (4800 workgroups of 64 segments), and a truncated list of the
+    same events."""
+    x = _data(512, 64 * 600 + tail, 16, 8, 5)
     ev_o, val_o = orc.detect(x, 16, 8, threshold, 6, max_events=1 << 22)
     d = torch.from_numpy(x).cuda()
     ev_g, val_g, n = ppl.detect(d, 16, 8, threshold, 6, max_events=1 << 22)
@@ -83,6 +87,16 @@ def test_events_identical_many_workgroups(orc, ppl, threshold):
     assert np.array_equal(val_g, val_o)
     ev_t, val_t, n_t = ppl.detect(d, 16, 8, threshold, 6, max_events=777)
     assert n_t == n and np.array_equal(ev_t, ev_o[:777]) and np.array_equal(val_t, val_o[:777])
+
+
+def test_events_identical_two_scan_rounds(orc, ppl):
+    """1024 DM rows x 1100 segments: 17600 workgroups, more than one
+    16384-entry round of the workgroup scan."""
+    x = _data(1024, 64 * 1100 + 36, 16, 8, 6, [(700, 40000, 20, 9)])
+    ev_o, val_o = orc.detect(x, 16, 8, 4.0, 6, max_events=1 << 22)
+    ev_g, val_g, n = ppl.detect(torch.from_numpy(x).cuda(), 16, 8, 4.0, 6, max_events=1 << 22)
+    assert n == len(ev_o) > 1000
+    assert np.array_equal(ev_g, ev_o) and np.array_equal(val_g, val_o)
 
 
 def test_count_only_and_empty_shapes(orc, ppl):
