@@ -541,19 +541,17 @@ __global__ void clip_begin_kernel(BatchIO io)
     if (i < nBins) lastGoodVer[i] = lastGood[i];
 }
 
-// The box around the sub-chunk-start state and the bandpass at its two offsets.
-__global__ void clip_bounds_kernel(BatchIO io)
+// The box around the sub-chunk-start state s and the bandpass at its two
+// offsets: thread i0 (of a set of `stride` threads) writes the box when it
+// is thread 0, and bins i0, i0 + stride, ...  Nothing is written once the
+// run is done (s.tNext >= nT).
+__device__ __forceinline__ void bounds_of_state(const BeamState& bs, const ClipScalars& s, int nT, int i0, int stride)
 {
-    const int beam = blockIdx.z;
-    const BeamState& bs = *io.st[beam];
-    const ClipScalars* __restrict__ S = bs.S;
-    const ClipConst K = bs.K;
+    const ClipConst& K = bs.K;
     const double* __restrict__ terms = bs.terms;
     Box* __restrict__ box = bs.box;
     float* __restrict__ bpLo = bs.bpLo;
     float* __restrict__ bpHi = bs.bpHi;
-    const int nT = io.nT[beam];
-    const ClipScalars s = *S;
     if (s.tNext >= nT) return;
     const bool steady = s.rmsSize == K.cap && s.meanSize == K.cap;
     const float rms = s.rmsRunAve;
@@ -563,8 +561,7 @@ __global__ void clip_bounds_kernel(BatchIO io)
     const float margin = K.crFactor * rms;
     const bool valid = steady && isfinite(rms) && rms > 0.0f && isfinite(s.params0) && isfinite(dp) &&
                        p0lo < p0hi && margin > 0.0f && isfinite(margin);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
+    if (i0 == 0) {
         Box b;
         b.p0lo = p0lo;
         b.p0hi = p0hi;
@@ -573,12 +570,23 @@ __global__ void clip_bounds_kernel(BatchIO io)
         b.valid = valid ? 1 : 0;
         *box = b;
     }
-    if (i < K.nBins && valid) {
+    if (!valid) return;
+    for (int i = i0; i < K.nBins; i += stride) {
         const float a = bp_value(terms, K.nTerms, K.nBins, i, p0lo, K.scale, bs.kill);
         const float b = bp_value(terms, K.nTerms, K.nBins, i, p0hi, K.scale, bs.kill);
         bpLo[i] = fminf(a, b);
         bpHi[i] = fmaxf(a, b);
     }
+}
+
+// The box of a run's first sub-chunk (the walk sets it up for the next ones
+// as it ends).
+__global__ void clip_bounds_kernel(BatchIO io)
+{
+    const int beam = blockIdx.z;
+    const BeamState& bs = *io.st[beam];
+    const ClipScalars s = *bs.S;
+    bounds_of_state(bs, s, io.nT[beam], blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Precompute over the spectra of one sub-chunk: the kept-channel sum in bin
@@ -589,7 +597,7 @@ __global__ void clip_bounds_kernel(BatchIO io)
 // (fl(I - bp) falls as bp rises; bp is monotone in the offset).  Clipped
 // channels enter the fold as +0.0, which leaves a sum started at +0.0
 // unchanged.
-__global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
+__device__ __forceinline__ void pre_rows(const BatchIO& io)
 {
     const int beam = blockIdx.z;
     const BeamState& bs = *io.st[beam];
@@ -754,7 +762,7 @@ __global__ __launch_bounds__(kTileThreads) void clip_pre_kernel(BatchIO io)
 // Learning-phase records (LearnRec) of the sub-chunk's spectra, one row per
 // lane of wave 0 in bin order, tiles staged in LDS as in the precompute.
 // Runs only while the clipper is learning at the sub-chunk's start.
-__global__ __launch_bounds__(kTileThreads) void clip_learn_kernel(BatchIO io)
+__device__ __forceinline__ void learn_rows(const BatchIO& io)
 {
     const int beam = blockIdx.z;
     const BeamState& bs = *io.st[beam];
@@ -875,6 +883,14 @@ __device__ __forceinline__ bool var_kept(const VarRec& v, int nTerms, float p0, 
     return !((v.undVal - 0.0f) - bp > margin);
 }
 
+// A sub-chunk's records ahead of its walk: the precompute in the steady
+// state (a valid box), the learning records while the rms ring fills; the
+// two conditions exclude each other, so one of the bodies works.
+__global__ __launch_bounds__(kTileThreads) void clip_prep_kernel(BatchIO io)
+{
+    pre_rows(io);
+    learn_rows(io);
+}
 
 // The sequential walk over one sub-chunk of spectra.  Thread 0 runs ahead
 // through spectra the precompute decided; the workgroup joins in for the
@@ -2048,6 +2064,10 @@ __global__ __launch_bounds__(kWalkThreads) void clip_walk_kernel(BatchIO io, int
         s.tNext = t;
         *S = s;
     }
+    // the box for the next sub-chunk's precompute, around the state reached
+    // (every thread is past its last read of the box and bandpass bounds)
+    __syncthreads();
+    bounds_of_state(bst, s, nT, tid, kWalkThreads);
 }
 
 // Replacement, clipSample and flattening of one bin of one spectrum
@@ -2576,12 +2596,11 @@ static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n,
     hipLaunchKernelGGL(clip_begin_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
     PPL_LAUNCHED();
     const int maxSub = std::min(maxNT, kSubLen);
+    // the first sub-chunk's box; each walk sets up the next one's as it ends
+    hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
+    PPL_LAUNCHED();
     for (int it = 0; it < nIter; ++it) {
-        hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
-        PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_pre_kernel, dim3(cdiv(maxSub, kPreRows), 1, nb), dim3(kTileThreads), 0, st, io);
-        PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_learn_kernel, dim3(cdiv(maxSub, kPreRows), 1, nb), dim3(kTileThreads), 0, st, io);
+        hipLaunchKernelGGL(clip_prep_kernel, dim3(cdiv(maxSub, kPreRows), 1, nb), dim3(kTileThreads), 0, st, io);
         PPL_LAUNCHED();
         hipLaunchKernelGGL(clip_walk_kernel, dim3(1, 1, nb), dim3(kWalkThreads), 0, st, io, it == nIter - 1 ? 1 : 0);
         PPL_LAUNCHED();
