@@ -20,12 +20,13 @@ the device before timing (a ring of distinct chunks).
 
 Prints one JSON line.  value = DM trials x output samples per second of the
 whole chain (tdms * nOut * steps / wall), with the real-time factor, per-stage
-times, the dedispersion roofline and the CPU baseline alongside.  With
---two-stream the steps are also timed on two streams (step k's channeliser,
-clipper and buffer fill beside step k-1's dedispersion), and with --cu-split N
-on two streams over disjoint CU slots (the clipper, buffer fill and carry on N
-slots of every XCD, beside step k-1's dedispersion); value is then the fastest
-schedule's, and every timed schedule is reported.
+times, the dedispersion roofline and the CPU baseline alongside.  The steps
+are timed one after the other on one stream, and in the overlapped schedule a
+streaming deployment runs (step k's clipper walk, one workgroup, on CU slot 0
+of every XCD beside step k-1's dedispersion on the other slots), timed in its
+steady state over the same number of steps; --two-stream and --cu-split N
+time two more pipelined schedules.  value is the fastest schedule's, and
+every timed schedule is reported.
 """
 import argparse
 import json
@@ -83,6 +84,8 @@ def parse():
                     help="also time the CU-split schedule with this many CU slots per XCD for the clipper, "
                          "buffer fill and carry (the rest run the channeliser, dedispersion and detection); "
                          "0: off")
+    ap.add_argument("--no-overlapped", action="store_true",
+                    help="skip the overlapped schedule (clipper walk beside the previous step's dedispersion)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 256-channel split-precision run")
     ap.add_argument("--prec-steps", type=int, default=2, help="timed steps per chain of the 256-channel run")
@@ -326,43 +329,55 @@ def main():
     torch.cuda.synchronize()
     graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
 
-    # ---- the two-stream schedule: step k's PPF, clipper, buffer fill and
-    # carry on one stream overlap step k-1's dedispersion and detection on
-    # another (the order a streaming deployment runs, as the reference's
-    # asynchronous DedispersionModule does).  The same work per step; every
-    # step's work completes inside the timed region.
+    # ---- pipelined schedules (a streaming deployment's order: step k's
+    # front work beside step k-1's dedispersion and detection), timed in
+    # their steady state: one run over warm + K + 1 chunks, the window from
+    # the detection of step warm-1 to that of step warm+K-1, with a
+    # synchronisation at both ends.  The window holds exactly K steps'
+    # work: K PPF/clipper/buffer stages (steps warm+1 .. warm+K; step warm's
+    # finished at the first synchronisation) and K dedispersions and
+    # detections (steps warm .. warm+K-1).
+    def pipelined(run, base):
+        warm = max(2, args.warmup)
+        n = warm + args.steps + 1
+        res, t0, t1 = [], None, None
+        for r in run(chunks[(base + k) % len(chunks)] for k in range(n)):
+            res.append(r)
+            if len(res) == warm:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            elif len(res) == warm + args.steps:
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+        w = t1 - t0
+        return {"ms_per_step": 1e3 * w / args.steps, "value": TDMS * nout * args.steps / w,
+                "real_time_factor": nout * TSAMP * args.steps / w,
+                "events_per_step": [len(e[0]) for e in res[warm:warm + args.steps]]}
+
+    # the two-stream schedule: PPF, clipper, buffer fill and carry on one
+    # stream, dedispersion and detection on another
     pipe = None
     if args.two_stream:
         log("timed steps, two-stream schedule")
-        base = args.warmup + args.steps
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pev = list(chain.run_pipelined(chunks[(base + k) % len(chunks)] for k in range(args.steps)))
-        torch.cuda.synchronize()
-        pwall = time.perf_counter() - t0
-        pipe = {"ms_per_step": 1e3 * pwall / args.steps, "value": TDMS * nout * args.steps / pwall,
-                "real_time_factor": nout * TSAMP * args.steps / pwall,
-                "events_per_step": [len(e[0]) for e in pev]}
-    # ---- the CU-split schedule: as above, with the clipper, buffer fill and
-    # carry on their own CU slots of every XCD, so that the clipper's
-    # one-workgroup walk runs beside the dedispersion grid.
+        pipe = pipelined(chain.run_pipelined, args.warmup + args.steps)
+    # the CU-split schedule: as above, with the clipper, buffer fill and
+    # carry on their own CU slots of every XCD
     split = None
     if args.cu_split:
         log("timed steps, CU-split schedule (%d CU slots per XCD)" % args.cu_split)
-        base = args.warmup + 2 * args.steps
-        list(chain.run_split([chunks[base % len(chunks)]], small_cus=args.cu_split))  # untimed
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sev = list(chain.run_split((chunks[(base + 1 + k) % len(chunks)] for k in range(args.steps)),
-                                   small_cus=args.cu_split))
-        torch.cuda.synchronize()
-        swall = time.perf_counter() - t0
-        split = {"ms_per_step": 1e3 * swall / args.steps, "value": TDMS * nout * args.steps / swall,
-                 "real_time_factor": nout * TSAMP * args.steps / swall, "cu_slots_per_xcd": args.cu_split,
-                 "events_per_step": [len(e[0]) for e in sev]}
+        split = pipelined(lambda it: chain.run_split(it, small_cus=args.cu_split), args.warmup + 2 * args.steps)
+        split["cu_slots_per_xcd"] = args.cu_split
+    # the overlapped schedule: step k's clipper walk (one workgroup) on CU
+    # slot 0 of every XCD beside step k-1's dedispersion on the other slots;
+    # PPF, the clipper's precompute and write passes and the buffer work on
+    # the whole GPU
+    over = None
+    if not args.no_overlapped:
+        log("timed steps, overlapped schedule")
+        over = pipelined(chain.run_overlapped, args.warmup + 3 * args.steps)
     sequential = {"ms_per_step": ms_per_step, "value": value, "real_time_factor": rtf}
     schedule = "one-stream"
-    for name, alt in (("two-stream", pipe), ("cu-split", split)):
+    for name, alt in (("two-stream", pipe), ("cu-split", split), ("overlapped", over)):
         if alt is not None and alt["value"] > value:
             ms_per_step, value, rtf = alt["ms_per_step"], alt["value"], alt["real_time_factor"]
             schedule = name
@@ -510,6 +525,7 @@ def main():
         "one_stream": sequential,
         "two_stream": pipe,
         "cu_split": split,
+        "overlapped": over,
         "stages_ms": {k: round(v, 3) for k, v in stages.items()},
         "graph_ms_per_step": round(graph_ms, 3),
         "events_per_step": [len(e[0]) for e in plain_events],

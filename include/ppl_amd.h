@@ -450,6 +450,23 @@ int ppl_clipper_run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, i
  * remaining clipped spectra (from first + *consumed) with ppl_ddbuf_add. */
 int ppl_clipper_run_into(ppl_clipper* clip, float* d_I, unsigned nT, unsigned nSub, unsigned nCh,
                          ppl_ddbuf* buf, unsigned first, unsigned* consumed, int* full, void* stream);
+/* RFI_Clipper::run (as ppl_clipper_run) in three calls, so that the
+ * sequential walk can run on a stream of its own beside other work:
+ *   _front: the run's start and the precompute of all its spectra (the
+ *           whole chunk is one sub-chunk; a wider box than ppl_clipper_run's);
+ *   _walk:  the walk over the chunk, one workgroup (then the precompute and
+ *           walk of whatever follows a point where the state left the box);
+ *   _back:  the statistics and write passes.
+ * The caller orders them: the walk after the front, the back after the walk
+ * (events between streams).  The results equal ppl_clipper_run's bit for
+ * bit.  Calls out of this order fail with PPL_E_ARG.  No counterpart in the
+ * reference, whose clipper runs inside its module (RFI_Clipper.cpp:154-573);
+ * for schedules that overlap the walk with the previous buffer's
+ * dedispersion. */
+int ppl_clipper_run_front(ppl_clipper* clip, float* d_I, float* d_W, unsigned nT, unsigned nSub,
+                          unsigned nPol, unsigned nCh, void* stream);
+int ppl_clipper_run_walk(ppl_clipper* clip, void* stream);
+int ppl_clipper_run_back(ppl_clipper* clip, void* stream);
 /* Scalars after the last run: meanRunAve, rmsRunAve (WeightedSpectrumDataSet
  * setMean/setRMS, :569-570); synchronises. */
 int ppl_clipper_stats(ppl_clipper* clip, float* mean, float* rms);

@@ -141,6 +141,9 @@ SIGNATURES = {
     "ppl_clipper_model": (i, [vp, vp, vp, vp, vp, u]),
     "ppl_clipper_counters": (i, [vp, vp, i]),
     "ppl_clipper_run_batch": (i, [vp, vp, i, vp]),
+    "ppl_clipper_run_front": (i, [vp, vp, vp, u, u, u, u, vp]),
+    "ppl_clipper_run_walk": (i, [vp, vp]),
+    "ppl_clipper_run_back": (i, [vp, vp]),
 }
 
 
@@ -719,6 +722,22 @@ class Clipper:
                                     _stream(stream)))
         return d_I
 
+    def run_front(self, d_I, d_W=None, stream=None):
+        """First of the three calls of a run split over streams: the run's
+        start and the precompute of all its spectra."""
+        n_t, n_sub, n_pol, n_ch = d_I.shape
+        check(lib().ppl_clipper_run_front(self.h, _ptr(d_I), _ptr(d_W), n_t, n_sub, n_pol, n_ch,
+                                          _stream(stream)))
+
+    def run_walk(self, stream=None):
+        """The sequential walk of the run started by run_front."""
+        check(lib().ppl_clipper_run_walk(self.h, _stream(stream)))
+
+    def run_back(self, stream=None):
+        """The statistics and write passes of the run; the spectra then equal
+        run()'s."""
+        check(lib().ppl_clipper_run_back(self.h, _stream(stream)))
+
     def run_into(self, d_I, ddbuf, first=0, stream=None):
         """Clip d_I [nT][nSub][1][nCh] in place, then add spectra [first, nT)
         to ddbuf; returns (consumed, full) as DDBuffer.add does."""
@@ -942,6 +961,93 @@ class BeamChain:
                 self.out.copy_(outs[1])
                 if self.series is not None:
                     self.series.copy_(series[1])
+
+    def run_overlapped(self, volts, threshold=6.0, bin_pow2=6, max_events=1 << 22):
+        """Steps over the voltage chunks in `volts` with step k's clipper walk
+        beside step k-1's dedispersion, yielding each step's (ints, vals,
+        count) detection in order.
+
+        The clipper runs split in three (Clipper.run_front / run_walk /
+        run_back): the precompute of the whole chunk, the sequential walk
+        (one workgroup per beam), then the statistics and write passes.
+        Three streams: the current one runs PPF+Stokes, the clipper's front
+        and back, the buffer fill and the carry; a walk stream on CU slot 0
+        of every XCD runs the walk; a dedispersion stream on the other slots
+        runs dedispersion and detection.  The two CU ranges do not overlap:
+        a kernel whose range overlaps the running dedispersion grid's is not
+        dispatched before that grid's last workgroup is (measured: the walk
+        then ends after the dedispersion instead of beside it).  Ordering
+        per step k: the walk and dedispersion k-1 wait
+        for the clipper's front (so they do not share the GPU with PPF k and
+        the precompute); the back waits for the walk; the carry that starts
+        buffer k+1 overwrites the copy dedispersion k-1 read, so it waits for
+        that dedispersion.  The results equal step()'s."""
+        import torch
+        per_xcd = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count // 8
+        dd_s, walk_s = CuRangeStream(1, per_xcd - 1), CuRangeStream(0, 1)
+        dd, walk = dd_s.stream, walk_s.stream
+        main = torch.cuda.current_stream()
+        outs = [self.out, torch.empty_like(self.out)]
+        series = [self.series, torch.empty_like(self.series)] if self.series is not None else [None, None]
+        dd.wait_stream(main)
+        walk.wait_stream(main)
+        prev = None  # (buffer, row offsets, filled event) of step k-1
+        k = 0
+        try:
+            for d_volt in volts:
+                self.ppf.stokes_i16(d_volt, self.nout * self.integrate, d_out=self.stokes)
+                self.clip.run_front(self.stokes)
+                fronted = torch.cuda.Event()
+                fronted.record(main)
+                with torch.cuda.stream(walk):
+                    walk.wait_event(fronted)
+                    self.clip.run_walk(stream=walk)
+                    walked = torch.cuda.Event()
+                    walked.record(walk)
+                seen, done = None, None
+                if prev is not None:
+                    with torch.cuda.stream(dd):
+                        dd.wait_event(fronted)
+                        dd.wait_event(prev[2])
+                        seen = self._dedisperse(prev[0], prev[1], outs[(k - 1) & 1], series[(k - 1) & 1],
+                                                stream=dd)
+                        done = torch.cuda.Event()
+                        done.record(dd)
+                main.wait_event(walked)
+                self.clip.run_back()
+                used, full = self.buf.add(self.stokes.view(self.nout, self.n_chan))
+                if used != self.nout or not full:
+                    raise PPLError("step: the buffer did not fill with %d spectra" % self.nout)
+                d_in, r_in = self.buf.tensor(), self.buf.row_offsets()
+                filled = torch.cuda.Event()
+                filled.record(main)
+                if done is not None:
+                    main.wait_event(done)
+                self.buf.rotate()
+                if seen is not None:
+                    with torch.cuda.stream(dd):
+                        result = self.detect(threshold, bin_pow2, max_events, d_out=seen, stream=dd)
+                    yield result
+                prev = (d_in, r_in, filled)
+                self.parity ^= 1
+                k += 1
+            if prev is not None:
+                with torch.cuda.stream(dd):
+                    dd.wait_event(prev[2])
+                    seen = self._dedisperse(prev[0], prev[1], outs[(k - 1) & 1], series[(k - 1) & 1], stream=dd)
+                    result = self.detect(threshold, bin_pow2, max_events, d_out=seen, stream=dd)
+                yield result
+        finally:
+            main.wait_stream(walk)
+            main.wait_stream(dd)
+            torch.cuda.synchronize()
+            # leave the last step's series in out, as step() does
+            if k and not k & 1:
+                self.out.copy_(outs[1])
+                if self.series is not None:
+                    self.series.copy_(series[1])
+            dd_s.close()
+            walk_s.close()
 
     def run_split(self, volts, small_cus=4, threshold=6.0, bin_pow2=6, max_events=1 << 22):
         """run_pipelined with the GPU split between the two streams by CU

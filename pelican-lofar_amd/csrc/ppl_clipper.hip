@@ -2248,6 +2248,13 @@ __global__ __launch_bounds__(256) void clip_write_kernel(BatchIO io)
 
 }  // namespace
 
+// A run's launch set-up: the beams' descriptors and the grid sizes.
+struct RunSetup {
+    BatchIO io{};
+    int nb = 0, maxBins = 0, maxNT = 0, maxSub = 0, nIter = 0, nTerms = -2;
+    ppl_clipper* used[kMaxBeams];
+};
+
 struct ppl_clipper {
     ppl_clipper_config cfg;
     HostBandPass bp;
@@ -2275,6 +2282,11 @@ struct ppl_clipper {
     int recCap = 0;
     std::vector<uint8_t> lastFlags;
     int lastNT = 0;
+    // a run split over streams (ppl_clipper_run_front / _walk / _back): its
+    // set-up and the phase reached (0: none, 1: front, 2: walks, 3 and 4:
+    // front and walk of an empty run)
+    RunSetup split;
+    int phase = 0;
 };
 
 namespace {
@@ -2524,8 +2536,8 @@ static int prepare(ppl_clipper* c, const ppl_clip_job& j)
         c->d_learn = nullptr;
         c->d_stats = nullptr;
         PPL_HIP(hipMalloc(&c->d_rec, sizeof(SpecRec) * j.nT));
-        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * std::min((int)j.nT, kSubLen)));
-        PPL_HIP(hipMalloc(&c->d_learn, sizeof(LearnRec) * std::min((int)j.nT, kSubLen)));
+        PPL_HIP(hipMalloc(&c->d_pre, sizeof(PreRec) * j.nT));     // a split run precomputes the whole run
+        PPL_HIP(hipMalloc(&c->d_learn, sizeof(LearnRec) * j.nT));
         PPL_HIP(hipMalloc(&c->d_stats, sizeof(float2) * j.nT));
         c->recCap = (int)j.nT;
         dirty = true;
@@ -2555,16 +2567,15 @@ static int prepare(ppl_clipper* c, const ppl_clip_job& j)
     return PPL_OK;
 }
 
-// One run over a batch of clippers (beams): each stage is one launch for
-// all beams (blockIdx.z), the walk one workgroup per beam.  Beams keep
-// their own shapes and spectrum counts; shorter beams' blocks return early.
-static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n, hipStream_t st)
+// Validates and prepares a batch of clippers (beams) for one run, with
+// sub-chunks of at most subCap spectra.
+static int setup_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n, int subCap, RunSetup& R)
 {
     PPL_CHECK_ARG(clips && jobs && n >= 0, "null argument");
     PPL_CHECK_ARG(n <= kMaxBeams, "at most %d beams per clipper batch (got %d)", kMaxBeams, n);
-    BatchIO io{};
-    int maxBins = 0, maxNT = 0, maxIter = 0, nb = 0, nTerms = -2;
-    ppl_clipper* used[kMaxBeams];
+    R = RunSetup{};
+    BatchIO& io = R.io;
+    int maxIter = 0;
     auto aligned = [](const void* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
     for (int k = 0; k < n; ++k) {
         for (int q = 0; q < k; ++q)
@@ -2574,49 +2585,89 @@ static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n,
         const ppl_clip_job& j = jobs[k];
         if (j.nT == 0) continue;
         ppl_clipper* c = clips[k];
+        const int nb = R.nb;
         io.st[nb] = c->d_bs;
         io.I[nb] = j.d_I;
         io.W[nb] = j.d_W;
         io.nT[nb] = (int)j.nT;
-        io.subLen[nb] = std::min((int)j.nT, kSubLen);
+        io.subLen[nb] = std::min((int)j.nT, subCap);
         // access mode of the row-tiled passes (see load_quad)
         io.ldMode[nb] = !(aligned(j.d_I) && aligned(j.d_W)) ? 2 : j.nPol == 1 ? 0 : j.nCh % 4 == 0 ? 1 : 2;
-        maxBins = std::max(maxBins, c->nBins);
-        maxNT = std::max(maxNT, (int)j.nT);
+        R.maxBins = std::max(R.maxBins, c->nBins);
+        R.maxNT = std::max(R.maxNT, (int)j.nT);
         maxIter = std::max(maxIter, (int)cdiv(j.nT, (unsigned)io.subLen[nb]));
-        nTerms = nTerms == -2 ? c->K.nTerms : (nTerms == c->K.nTerms ? nTerms : -1);
-        used[nb++] = c;
+        R.nTerms = R.nTerms == -2 ? c->K.nTerms : (R.nTerms == c->K.nTerms ? R.nTerms : -1);
+        R.used[R.nb++] = c;
     }
-    if (nb == 0) return PPL_OK;
     // Sub-chunks: box around the state reached, precompute, walk.  The walk
-    // advances S->tNext by up to kSubLen spectra, or less when the state
+    // advances S->tNext by up to subLen spectra, or less when the state
     // leaves the box; the last launch walks whatever remains.  Launches past
     // a beam's end return at once.
-    const int nIter = maxIter + kExtraIter;
-    hipLaunchKernelGGL(clip_begin_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
+    R.nIter = maxIter + kExtraIter;
+    R.maxSub = std::min(R.maxNT, subCap);
+    return PPL_OK;
+}
+
+// The run's start: the walk counters' position and first box, and the first
+// sub-chunk's precompute.
+static int launch_front(const RunSetup& R, hipStream_t st)
+{
+    const BatchIO& io = R.io;
+    hipLaunchKernelGGL(clip_begin_kernel, dim3(cdiv(R.maxBins, 256), 1, R.nb), dim3(256), 0, st, io);
     PPL_LAUNCHED();
-    const int maxSub = std::min(maxNT, kSubLen);
     // the first sub-chunk's box; each walk sets up the next one's as it ends
-    hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(maxBins, 256), 1, nb), dim3(256), 0, st, io);
+    hipLaunchKernelGGL(clip_bounds_kernel, dim3(cdiv(R.maxBins, 256), 1, R.nb), dim3(256), 0, st, io);
     PPL_LAUNCHED();
-    for (int it = 0; it < nIter; ++it) {
-        hipLaunchKernelGGL(clip_prep_kernel, dim3(cdiv(maxSub, kPreRows), 1, nb), dim3(kTileThreads), 0, st, io);
-        PPL_LAUNCHED();
-        hipLaunchKernelGGL(clip_walk_kernel, dim3(1, 1, nb), dim3(kWalkThreads), 0, st, io, it == nIter - 1 ? 1 : 0);
+    hipLaunchKernelGGL(clip_prep_kernel, dim3(cdiv(R.maxSub, kPreRows), 1, R.nb), dim3(kTileThreads), 0, st, io);
+    PPL_LAUNCHED();
+    return PPL_OK;
+}
+
+// The walks: the first sub-chunk's, then precompute and walk for the others.
+static int launch_walks(const RunSetup& R, hipStream_t st)
+{
+    const BatchIO& io = R.io;
+    for (int it = 0; it < R.nIter; ++it) {
+        if (it > 0) {
+            hipLaunchKernelGGL(clip_prep_kernel, dim3(cdiv(R.maxSub, kPreRows), 1, R.nb), dim3(kTileThreads), 0,
+                               st, io);
+            PPL_LAUNCHED();
+        }
+        hipLaunchKernelGGL(clip_walk_kernel, dim3(1, 1, R.nb), dim3(kWalkThreads), 0, st, io,
+                           it == R.nIter - 1 ? 1 : 0);
         PPL_LAUNCHED();
     }
+    return PPL_OK;
+}
+
+// The statistics and write passes over the walked spectra.
+static int launch_back(const RunSetup& R, hipStream_t st)
+{
     int rc = PPL_OK;
-    switch (nTerms) {
-        case 0: rc = launch_apply<0>(io, nb, maxBins, maxNT, st); break;
-        case 1: rc = launch_apply<1>(io, nb, maxBins, maxNT, st); break;
-        case 2: rc = launch_apply<2>(io, nb, maxBins, maxNT, st); break;
-        case 3: rc = launch_apply<3>(io, nb, maxBins, maxNT, st); break;
-        case 4: rc = launch_apply<4>(io, nb, maxBins, maxNT, st); break;
-        default: rc = launch_apply<-1>(io, nb, maxBins, maxNT, st);
+    switch (R.nTerms) {
+        case 0: rc = launch_apply<0>(R.io, R.nb, R.maxBins, R.maxNT, st); break;
+        case 1: rc = launch_apply<1>(R.io, R.nb, R.maxBins, R.maxNT, st); break;
+        case 2: rc = launch_apply<2>(R.io, R.nb, R.maxBins, R.maxNT, st); break;
+        case 3: rc = launch_apply<3>(R.io, R.nb, R.maxBins, R.maxNT, st); break;
+        case 4: rc = launch_apply<4>(R.io, R.nb, R.maxBins, R.maxNT, st); break;
+        default: rc = launch_apply<-1>(R.io, R.nb, R.maxBins, R.maxNT, st);
     }
     if (rc != PPL_OK) return rc;
-    for (int k = 0; k < nb; ++k) used[k]->lastNT = io.nT[k];
+    for (int k = 0; k < R.nb; ++k) R.used[k]->lastNT = R.io.nT[k];
     return PPL_OK;
+}
+
+// One run over a batch of clippers (beams): each stage is one launch for
+// all beams (blockIdx.z), the walk one workgroup per beam.  Beams keep
+// their own shapes and spectrum counts; shorter beams' blocks return early.
+static int run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int n, hipStream_t st)
+{
+    RunSetup R;
+    int rc = setup_batch(clips, jobs, n, kSubLen, R);
+    if (rc != PPL_OK || R.nb == 0) return rc;
+    if ((rc = launch_front(R, st)) != PPL_OK) return rc;
+    if ((rc = launch_walks(R, st)) != PPL_OK) return rc;
+    return launch_back(R, st);
 }
 
 static int clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub, unsigned nPol,
@@ -2635,6 +2686,50 @@ extern "C" int ppl_clipper_run(ppl_clipper* c, float* d_I, float* d_W, unsigned 
 extern "C" int ppl_clipper_run_batch(ppl_clipper* const* clips, const ppl_clip_job* jobs, int nBeams, void* stream)
 {
     return run_batch(clips, jobs, nBeams, stream_of(stream));
+}
+
+extern "C" int ppl_clipper_run_front(ppl_clipper* c, float* d_I, float* d_W, unsigned nT, unsigned nSub,
+                                     unsigned nPol, unsigned nCh, void* stream)
+{
+    PPL_CHECK_ARG(c, "null argument");
+    PPL_CHECK_ARG(c->phase == 0, "RFI_Clipper: a split run is in progress (front called twice)");
+    ppl_clip_job j{d_I, d_W, nT, nSub, nPol, nCh};
+    ppl_clipper* one = c;
+    const int rc = setup_batch(&one, &j, 1, std::max((int)nT, 1), c->split);
+    if (rc != PPL_OK) return rc;
+    if (c->split.nb == 0) {
+        c->phase = 3;  // nothing to walk: walk and back are no-ops
+        return PPL_OK;
+    }
+    const int r2 = launch_front(c->split, stream_of(stream));
+    if (r2 == PPL_OK) c->phase = 1;
+    return r2;
+}
+
+extern "C" int ppl_clipper_run_walk(ppl_clipper* c, void* stream)
+{
+    PPL_CHECK_ARG(c, "null argument");
+    PPL_CHECK_ARG(c->phase == 1 || c->phase == 3, "RFI_Clipper: walk without a front");
+    if (c->phase == 3) {
+        c->phase = 4;
+        return PPL_OK;
+    }
+    const int rc = launch_walks(c->split, stream_of(stream));
+    if (rc == PPL_OK) c->phase = 2;
+    return rc;
+}
+
+extern "C" int ppl_clipper_run_back(ppl_clipper* c, void* stream)
+{
+    PPL_CHECK_ARG(c, "null argument");
+    PPL_CHECK_ARG(c->phase == 2 || c->phase == 4, "RFI_Clipper: back without a walk");
+    if (c->phase == 4) {
+        c->phase = 0;
+        return PPL_OK;
+    }
+    const int rc = launch_back(c->split, stream_of(stream));
+    c->phase = 0;
+    return rc;
 }
 
 extern "C" int ppl_clipper_stats(ppl_clipper* c, float* mean, float* rms)

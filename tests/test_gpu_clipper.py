@@ -135,6 +135,50 @@ def test_clipper_sub_chunks_and_drift(orc, ppl):
     assert (np.float32(gc.stats()[0]), np.float32(gc.stats()[1])) == (st_o[0], st_o[1])
 
 
+def test_split_run_equals_run(orc, ppl):
+    """A run split over two streams (front and back on one, the walk on
+    another, ordered by events) gives what run() gives, bit for bit: through
+    the learning phase, the steady state and a drift that carries the state
+    out of the box (the split run's whole-chunk box ends early and the walk
+    continues from a fresh one); calls out of order are refused."""
+    n_sub, n_ch, n_t = 8, 16, 14000
+    x = _stream(n_t, n_sub, n_ch, seed=79)
+    ramp = np.clip((np.arange(n_t) - 6000) / 3000.0, 0.0, 1.0) * 0.03 + 1.0
+    x = (x * ramp[:, None, None, None]).astype(np.float32)
+    bp = _bp(n_sub, n_ch)
+    a, b = ppl.Clipper(bp, 10.0, 4.0, 1000, True), ppl.Clipper(bp, 10.0, 4.0, 1000, True)
+    walk = torch.cuda.Stream()
+    main = torch.cuda.current_stream()
+    for t0 in range(0, n_t, 3500):
+        chunk = np.ascontiguousarray(x[t0:t0 + 3500])
+        da, db = torch.from_numpy(chunk).cuda(), torch.from_numpy(chunk).cuda()
+        a.run(da)
+        b.run_front(db)
+        ev = torch.cuda.Event()
+        ev.record(main)
+        walk.wait_event(ev)
+        b.run_walk(stream=walk)
+        ev2 = torch.cuda.Event()
+        ev2.record(walk)
+        main.wait_event(ev2)
+        b.run_back()
+        assert torch.equal(torch.nan_to_num(da, nan=-1e30), torch.nan_to_num(db, nan=-1e30)), t0
+        assert np.array_equal(a.flags(chunk.shape[0]), b.flags(chunk.shape[0])), t0
+        assert a.stats() == b.stats(), t0
+    ca, cb = a.counters(), b.counters()
+    assert ca["fast"] + ca["slow"] == cb["fast"] + cb["slow"] == n_t
+    with pytest.raises(ppl.PPLError, match="without a front"):
+        b.run_walk()
+    with pytest.raises(ppl.PPLError, match="without a walk"):
+        b.run_back()
+    d = torch.from_numpy(np.ascontiguousarray(x[:100])).cuda()
+    b.run_front(d)
+    with pytest.raises(ppl.PPLError, match="front called twice"):
+        b.run_front(d)
+    b.run_walk()
+    b.run_back()
+
+
 def test_clipper_batch_equals_separate_runs(orc, ppl):
     """Beams of different shapes and spectrum counts in one batched run give
     the same outputs, flags and running statistics as separate runs, through

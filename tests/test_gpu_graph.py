@@ -123,6 +123,44 @@ def test_cu_split_steps_match_eager(orc, ppl, dtype):
     assert np.array_equal(split.out.cpu().numpy(), eager.out.cpu().numpy())
 
 
+@pytest.mark.parametrize("dtype", ["f32", "u8"])
+def test_overlapped_steps_match_eager(orc, ppl, dtype):
+    """The overlapped schedule (step k's clipper walk on CU slot 0 of every XCD
+    beside step k-1's dedispersion on CU slots 1.. of every XCD) gives every
+    step's events bit for bit as step() does, leaves the last series in out,
+    and the chain carries on with step() afterwards."""
+    coeffs, _, bp = _inputs(orc)
+    mk = lambda: ppl.BeamChain(N_SUB, N_CH, N_TAPS, coeffs, INTEG, bp, NSAMP_BUF, TDMS, 0.0, DM_STEP, TSAMP,
+                               FCH1, FOFF, history=1000, dtype=dtype)
+    eager, ov = mk(), mk()
+    per = eager.samples_per_step()
+    n_prime = eager.overlap * INTEG * N_CH
+    n_steps = 7
+    x = noise_voltages(N_SUB, n_prime + n_steps * per, 64.0, seed=43)
+    xd = torch.from_numpy(x).cuda()
+    prime = xd[:, :, :n_prime].contiguous()
+    eager.prime(prime)
+    ov.prime(prime)
+    chunks = [xd[:, :, n_prime + k * per:n_prime + (k + 1) * per].contiguous() for k in range(n_steps)]
+    want = []
+    for c in chunks:
+        eager.step(c)
+        want.append(eager.detect()[:2])
+    last = eager.out.cpu().numpy().copy()
+    got = [r[:2] for r in ov.run_overlapped(chunks)]
+    torch.cuda.synchronize()
+    assert len(got) == n_steps
+    for k, ((gi, gv), (wi, wv)) in enumerate(zip(got, want)):
+        assert np.array_equal(gi, wi) and np.array_equal(gv, wv), k
+    assert np.array_equal(ov.out.cpu().numpy(), last)
+    assert ov.clip.stats() == eager.clip.stats()
+    tail = noise_voltages(N_SUB, per, 64.0, seed=44)
+    tc = torch.from_numpy(tail).cuda()
+    eager.step(tc)
+    ov.step(tc)
+    assert np.array_equal(ov.out.cpu().numpy(), eager.out.cpu().numpy())
+
+
 def test_cu_range_stream_rejects_bad_ranges(ppl):
     """CU ranges outside the slots of an XCD, or empty, are refused."""
     per_xcd = torch.cuda.get_device_properties(0).multi_processor_count // 8
