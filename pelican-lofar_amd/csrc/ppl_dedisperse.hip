@@ -258,9 +258,11 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 // other vector memory operation is issued in the channel loop.  The
 // compiler's own fence before __syncthreads (a wait for every DMA in
 // flight) is kept out with an s_barrier in inline assembly, which also
-// keeps the compiler from moving LDS accesses across it.  The instruction
-// mix per channel and wave (counters on the LOFAR beam): 64 adds, 8
-// address adds, about 6 other VALU instructions; VALU is the bound.
+// keeps the compiler from moving LDS accesses across it.  Row addresses
+// are scalar (the DMA's scalar base plus a fixed lane offset) and the loop
+// is unrolled by DEPTH, so stage addresses are constants: the vector
+// instructions per channel and wave are the 64 adds and 8 read-address adds
+// alone.  VALU is the bound.
 constexpr int kRingQuads = 192;           // float4 per copy of a staged row
 constexpr int kRingDw = 4 * kRingQuads;   // dwords per copy
 constexpr int kRingDma = kRingDw / 64;    // DMA instructions per copy
@@ -286,33 +288,52 @@ __device__ __forceinline__ void bare_barrier() { asm volatile("s_barrier" ::: "m
 
 // kRingDma LDS-DMA dword loads of consecutive 64-dword rows: row k from
 // global address g + 256k to LDS address lds + 256k (+ 4 * lane each).  The
-// instruction offset applies to both addresses, so one address register and
-// one M0 serve all twelve.  M0 is restored before the block ends.  The
-// compiler's hazard checks do not look into inline assembly, so the wait
-// state between the M0 write and the first DMA is written out.
-__device__ __forceinline__ void dma_rows12(const float* g, uint32_t lds)
+// address is the uniform row start g in scalar registers plus the lane's
+// byte offset (4 * lane, a loop-invariant register), so placing a row costs
+// no vector instruction.  The instruction offset applies to both addresses,
+// so one address and one M0 serve all twelve.  M0 is restored before the
+// block ends.  The compiler's hazard checks do not look into inline
+// assembly, so the wait state between the M0 write and the first DMA is
+// written out.
+__device__ __forceinline__ void dma_rows12(const float* g, uint32_t laneBytes, uint32_t lds)
 {
     static_assert(kRingDma == 12, "twelve rows");
     uint32_t saved;
     asm volatile("s_mov_b32 %0, m0\n\t"
-                 "s_mov_b32 m0, %2\n\t"
+                 "s_mov_b32 m0, %3\n\t"
                  "s_nop 0\n\t"  // a scalar write of M0 needs one wait state before an LDS DMA reads it
-                 "global_load_lds_dword %1, off\n\t"
-                 "global_load_lds_dword %1, off offset:256\n\t"
-                 "global_load_lds_dword %1, off offset:512\n\t"
-                 "global_load_lds_dword %1, off offset:768\n\t"
-                 "global_load_lds_dword %1, off offset:1024\n\t"
-                 "global_load_lds_dword %1, off offset:1280\n\t"
-                 "global_load_lds_dword %1, off offset:1536\n\t"
-                 "global_load_lds_dword %1, off offset:1792\n\t"
-                 "global_load_lds_dword %1, off offset:2048\n\t"
-                 "global_load_lds_dword %1, off offset:2304\n\t"
-                 "global_load_lds_dword %1, off offset:2560\n\t"
-                 "global_load_lds_dword %1, off offset:2816\n\t"
+                 "global_load_lds_dword %1, %2\n\t"
+                 "global_load_lds_dword %1, %2 offset:256\n\t"
+                 "global_load_lds_dword %1, %2 offset:512\n\t"
+                 "global_load_lds_dword %1, %2 offset:768\n\t"
+                 "global_load_lds_dword %1, %2 offset:1024\n\t"
+                 "global_load_lds_dword %1, %2 offset:1280\n\t"
+                 "global_load_lds_dword %1, %2 offset:1536\n\t"
+                 "global_load_lds_dword %1, %2 offset:1792\n\t"
+                 "global_load_lds_dword %1, %2 offset:2048\n\t"
+                 "global_load_lds_dword %1, %2 offset:2304\n\t"
+                 "global_load_lds_dword %1, %2 offset:2560\n\t"
+                 "global_load_lds_dword %1, %2 offset:2816\n\t"
                  "s_nop 0\n\t"
                  "s_mov_b32 m0, %0"
                  : "=&s"(saved)
-                 : "v"(g), "s"(lds)
+                 : "v"(laneBytes), "s"(g), "s"(lds)
+                 : "memory");
+}
+
+// One LDS-DMA dword load, the same addressing: dword `lane` from g (scalar
+// registers) to lds + 4 * lane, for the lanes the caller's branch leaves on.
+__device__ __forceinline__ void dma_dword(const int32_t* g, uint32_t laneBytes, uint32_t lds)
+{
+    uint32_t saved;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %3\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dword %1, %2\n\t"
+                 "s_nop 0\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved)
+                 : "v"(laneBytes), "s"(g), "s"(lds)
                  : "memory");
 }
 
@@ -410,19 +431,22 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
     // count is fixed), then the gather of this wave's read offsets of the
     // channel (lanes 0..ND-1).  Past the last channel: a dummy stage (the
     // last channel again), so that every channel of the loop issues one.
+    const uint32_t laneBytes = 4 * lane;
     auto stage = [&](unsigned char* buf, int c, int dmin) {
         c = min(c, nChan - 1);
         const int row0 = c * nsamp + t0 + dmin + wave;  // uniform
         float* dst = reinterpret_cast<float*>(buf + wave * kRingQuads * 16);
         if (row0 + kRingDw - 1 <= lastElem) {
-            dma_rows12(in + row0 + lane, lds_addr(dst));
+            dma_rows12(in + row0, laneBytes, lds_addr(dst));
         } else {
 #pragma unroll
-            for (int k = 0; k < kRingDma; ++k)
-                __builtin_amdgcn_global_load_lds(in + min(row0 + 64 * k + lane, lastElem), dst + 64 * k, 4, 0, 0);
+            for (int k = 0; k < kRingDma; ++k) {
+                const int rowk = __builtin_amdgcn_readfirstlane(row0 + 64 * k);  // uniform
+                __builtin_amdgcn_global_load_lds(in + min(rowk + lane, lastElem), dst + 64 * k, 4, 0, 0);
+            }
         }
-        int32_t* small = reinterpret_cast<int32_t*>(buf + 4 * kRingQuads * 16 + wave * 64);
-        if (lane < ND) __builtin_amdgcn_global_load_lds(offsTab + c * TD + lane, small, 4, 0, 0);
+        const uint32_t small = lds_addr(buf + 4 * kRingQuads * 16 + wave * 64);
+        if (lane < ND) dma_dword(offsTab + c * TD, laneBytes, small);
     };
     auto compute = [&](const unsigned char* buf) {
         static_assert((NQ == 1 || NQ == 2) && ND % 4 == 0, "reads in blocks of four trials x NQ float4");
@@ -457,18 +481,24 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
     for (int k = 0; k < DEPTH - 1; ++k) stage(ring[k], k, dmin_of(k));
     // dmin of the channel staged at the next top, loaded a channel ahead
     int nextDmin = dmin_of(DEPTH - 1);
-    int slot = 0;  // stage of channel c
-    for (int c = 0; c < nChan; ++c) {
-        // this wave's DMA into channel c's stage has landed (the DMA of the
-        // next DEPTH-2 stages may be in flight); then the barrier: every
-        // wave's part has landed and every wave is done with channel c-1
-        wait_vmcnt<(DEPTH - 2) * (kRingDma + 1)>();
-        bare_barrier();
-        const int free = slot == 0 ? DEPTH - 1 : slot - 1;  // channel c-1's stage
-        stage(ring[free], c + DEPTH - 1, nextDmin);
-        nextDmin = dmin_of(c + DEPTH);
-        compute(ring[slot]);
-        slot = slot + 1 == DEPTH ? 0 : slot + 1;
+    // channel c uses stage c % DEPTH: the loop is unrolled by DEPTH so that
+    // every stage address is a constant
+    for (int c0 = 0; c0 < nChan; c0 += DEPTH) {
+#pragma unroll
+        for (int slot = 0; slot < DEPTH; ++slot) {
+            const int c = c0 + slot;
+            if (c < nChan) {  // uniform
+                // this wave's DMA into channel c's stage has landed (the DMA of
+                // the next DEPTH-2 stages may be in flight); then the barrier:
+                // every wave's part has landed and every wave is done with
+                // channel c-1
+                wait_vmcnt<(DEPTH - 2) * (kRingDma + 1)>();
+                bare_barrier();
+                stage(ring[(slot + DEPTH - 1) % DEPTH], c + DEPTH - 1, nextDmin);  // channel c-1's stage
+                nextDmin = dmin_of(c + DEPTH);
+                compute(ring[slot]);
+            }
+        }
     }
     // no LDS DMA may land after the workgroup has ended
     wait_vmcnt<0>();
