@@ -449,14 +449,16 @@ def detect(d_data, n_channels, n_subbands, threshold=6.0, bin_pow2=6, max_events
            stream=None):
     """DedispersionAnalyser::analyse on a device [tdms][nsamp] float tensor."""
     tdms, nsamp = d_data.shape
-    ints = np.zeros((max(max_events, 1), 3), np.int32)
-    vals = np.zeros(max(max_events, 1), np.float32)
+    # uninitialised: only the pages of the events written are ever touched,
+    # and the results are views of these arrays (no second copy)
+    ints = np.empty((max(max_events, 1), 3), np.int32)
+    vals = np.empty(max(max_events, 1), np.float32)
     count = C.c_int(0)
     check(lib().ppl_detect(_ptr(d_data), tdms, nsamp, n_channels, n_subbands, threshold, bin_pow2,
                            ints.ctypes.data_as(C.c_void_p), vals.ctypes.data_as(C.c_void_p),
                            max_events, C.byref(count), _stream(stream)))
     n = min(count.value, max_events)
-    return ints[:n].copy(), vals[:n].copy(), count.value
+    return ints[:n], vals[:n], count.value
 
 
 DETECT_NOALLOC_MAX_SEGMENTS = 1 << 23
@@ -475,15 +477,15 @@ def detect_noalloc(d_data, n_channels, n_subbands, d_ints, d_vals, d_count, thre
 def detect_u8(d_data, n_chan, scale, n_channels, n_subbands, threshold=6.0, bin_pow2=6,
               max_events=1 << 20, stream=None):
     tdms, nsamp = d_data.shape
-    ints = np.zeros((max_events, 3), np.int32)
-    vals = np.zeros(max_events, np.float32)
+    ints = np.empty((max(max_events, 1), 3), np.int32)  # as in detect()
+    vals = np.empty(max(max_events, 1), np.float32)
     count = C.c_int(0)
     check(lib().ppl_detect_u8(_ptr(d_data), tdms, nsamp, n_chan, scale, n_channels, n_subbands,
                               threshold, bin_pow2, ints.ctypes.data_as(C.c_void_p),
                               vals.ctypes.data_as(C.c_void_p), max_events, C.byref(count),
                               _stream(stream)))
     n = min(count.value, max_events)
-    return ints[:n].copy(), vals[:n].copy(), count.value
+    return ints[:n], vals[:n], count.value
 
 
 # -------------------------------------------------------------- channeliser

@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -528,6 +530,35 @@ int staging_reserve(size_t events)
     return PPL_OK;
 }
 
+// Copy n events out of the pinned staging.  Long lists are split over a few
+// threads: the destination is usually fresh pageable memory, and faulting
+// its pages in is most of the time.
+void copy_events(int32_t* h_ints, float* h_vals, const int32_t* sInts, const float* sVals, size_t n)
+{
+    constexpr size_t kPerThread = size_t(1) << 17;  // events
+    const size_t T = std::min<size_t>(8, (n + kPerThread - 1) / kPerThread);
+    auto part = [&](size_t k) {
+        const size_t a = n * k / T, b = n * (k + 1) / T;
+        std::memcpy(h_ints + 3 * a, sInts + 3 * a, sizeof(int32_t) * 3 * (b - a));
+        std::memcpy(h_vals + a, sVals + a, sizeof(float) * (b - a));
+    };
+    if (T <= 1) {
+        std::memcpy(h_ints, sInts, sizeof(int32_t) * 3 * n);
+        std::memcpy(h_vals, sVals, sizeof(float) * n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    size_t k = 1;
+    try {
+        for (; k < T; ++k) pool.emplace_back(part, k);
+    } catch (const std::system_error&) {
+        // no thread to be had: this thread copies the parts not started
+    }
+    for (size_t r = k; r < T; ++r) part(r);
+    part(0);
+    for (auto& t : pool) t.join();
+}
+
 template <typename T>
 int run_detect(const T* d_data, int tdms, int nsamp, int offset, float scale, int nChannels, int nSubbands,
                float threshold, unsigned binPow2, int32_t* h_ints, float* h_vals, int max_events, int* count,
@@ -562,10 +593,7 @@ int run_detect(const T* d_data, int tdms, int nsamp, int offset, float scale, in
     PPL_HIP(hipStreamSynchronize(st));
     const int nEvents = *hTotal;
     const int n = std::min(nEvents, std::max(max_events, 0));
-    if (n > 0) {
-        std::memcpy(h_ints, sInts, sizeof(int32_t) * 3 * (size_t)n);
-        std::memcpy(h_vals, sVals, sizeof(float) * (size_t)n);
-    }
+    if (n > 0) copy_events(h_ints, h_vals, sInts, sVals, (size_t)n);
     *count = nEvents;
     return PPL_OK;
 }
