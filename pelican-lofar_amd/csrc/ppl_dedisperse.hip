@@ -246,16 +246,16 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 //
 // One channel per ring stage, DEPTH stages.  Wave s loads copy s of the row
 // (from element s on: the same cache lines for all four) by LDS DMA, always
-// kRingDw dwords (kRingDma instructions), plus one gather instruction that
-// brings the wave's ND read offsets of the channel from a per-plan ring
-// table (offsets already in LDS bytes).  So every wave has exactly
-// kRingDma + 1 DMA instructions per channel in flight, and at the top of
+// kRingDw dwords (kRingDma instructions).  So every wave has exactly
+// kRingDma DMA instructions per channel in flight, and at the top of
 // channel c it waits (explicit vmcnt) only for its DMA into c's stage; the
 // workgroup then meets at a bare barrier (every wave's part of the stage
 // has landed, every wave is done with channel c-1), and the DMA of channel
 // c + DEPTH - 1 goes into the stage channel c-1 used.  The first-trial delay
-// that places a row comes by scalar load (lgkmcnt), a channel ahead.  No
-// other vector memory operation is issued in the channel loop.  The
+// that places a row and the wave's ND read offsets of the channel (from a
+// per-plan ring table, already in LDS bytes) come by scalar loads
+// (lgkmcnt), a channel ahead, so LDS serves only the rows.  No other vector
+// memory operation is issued in the channel loop.  The
 // compiler's own fence before __syncthreads (a wait for every DMA in
 // flight) is kept out with an s_barrier in inline assembly, which also
 // keeps the compiler from moving LDS accesses across it.  Row addresses
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 constexpr int kRingQuads = 192;           // float4 per copy of a staged row
 constexpr int kRingDw = 4 * kRingQuads;   // dwords per copy
 constexpr int kRingDma = kRingDw / 64;    // DMA instructions per copy
-constexpr int kRingStage = 4 * kRingQuads * 16 + 4 * 64;  // four copies, then 64 bytes of offsets per wave
+constexpr int kRingStage = 4 * kRingQuads * 16;  // four copies
 // Three stages (two channels of DMA lead): 37 KB of LDS a workgroup, so
 // four workgroups (16 waves) share a CU; four stages (49 KB, three
 // workgroups) measured 10 % slower on the LOFAR beam.
@@ -321,22 +321,6 @@ __device__ __forceinline__ void dma_rows12(const float* g, uint32_t laneBytes, u
                  : "memory");
 }
 
-// One LDS-DMA dword load, the same addressing: dword `lane` from g (scalar
-// registers) to lds + 4 * lane, for the lanes the caller's branch leaves on.
-__device__ __forceinline__ void dma_dword(const int32_t* g, uint32_t laneBytes, uint32_t lds)
-{
-    uint32_t saved;
-    asm volatile("s_mov_b32 %0, m0\n\t"
-                 "s_mov_b32 m0, %3\n\t"
-                 "s_nop 0\n\t"
-                 "global_load_lds_dword %1, %2\n\t"
-                 "s_nop 0\n\t"
-                 "s_mov_b32 m0, %0"
-                 : "=&s"(saved)
-                 : "v"(laneBytes), "s"(g), "s"(lds)
-                 : "memory");
-}
-
 // LDS address (byte offset in the workgroup's LDS) of a __shared__ pointer.
 __device__ __forceinline__ uint32_t lds_addr(const void* p)
 {
@@ -348,13 +332,6 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p)
 // compiler takes its registers.  The compiler's wait tracking cannot
 // follow the four stages' DMA through the loop and would wait for every
 // DMA in flight before the first read of a loop iteration.
-__device__ __forceinline__ void lds_read_x2(uint32_t a, int4& x, int4& y)
-{
-    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:16\n\ts_waitcnt lgkmcnt(0)"
-                 : "=&v"(x), "=&v"(y)
-                 : "v"(a)
-                 : "memory");
-}
 __device__ __forceinline__ void lds_read_4x1(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, float4 (&v)[8])
 {
     asm volatile("ds_read_b128 %0, %4\n\t"
@@ -387,7 +364,7 @@ template <int ND, int NQ, int DEPTH>
 __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
 {
     static_assert(ND <= 16, "a wave's offsets fill 64 bytes");
-    static_assert(DEPTH >= 3 && (DEPTH - 2) * (kRingDma + 1) < 64, "ring depth");
+    static_assert(DEPTH >= 3 && (DEPTH - 2) * kRingDma < 64, "ring depth");
     // The stages are read only by inline assembly and written only by DMA,
     // so one array holds them all.
     __shared__ __align__(16) unsigned char ring[DEPTH][kRingStage];
@@ -410,12 +387,13 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lastElem = nChan * nsamp - 1;  // make_job checks that it fits an int
     // ring table: offs[tile][c][TD] (LDS byte offset of each trial's reads in
-    // a stage), then dmin[tile][c] (the tile's first-trial delay)
-    const int32_t* __restrict__ offsTab = tab + ((size_t)dmTile * nChan) * TD + wave * ND;
-    // first-trial delays through the constant address space: scalar loads,
-    // counted by lgkmcnt, not by the DMA's vmcnt (consecutive channels share
-    // a scalar cache line)
+    // a stage), then dmin[tile][c] (the tile's first-trial delay).  Both are
+    // read through the constant address space: scalar loads, counted by
+    // lgkmcnt, not by the DMA's vmcnt, issued a channel ahead.  A wave's ND
+    // offsets of a channel are one load; consecutive channels' first-trial
+    // delays share a scalar cache line.
     using ConstI32 = const __attribute__((address_space(4))) int32_t;
+    ConstI32* offsTab = (ConstI32*)(tab + ((size_t)dmTile * nChan) * TD + wave * ND);
     ConstI32* dminTab = (ConstI32*)(tab + (size_t)nDMTiles * nChan * TD + (size_t)dmTile * nChan);
 
     float acc[ND][NQ][4];
@@ -445,16 +423,10 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
                 __builtin_amdgcn_global_load_lds(in + min(rowk + lane, lastElem), dst + 64 * k, 4, 0, 0);
             }
         }
-        const uint32_t small = lds_addr(buf + 4 * kRingQuads * 16 + wave * 64);
-        if (lane < ND) dma_dword(offsTab + c * TD, laneBytes, small);
     };
-    auto compute = [&](const unsigned char* buf) {
+    auto compute = [&](const unsigned char* buf, const int (&o)[ND]) {
         static_assert((NQ == 1 || NQ == 2) && ND % 4 == 0, "reads in blocks of four trials x NQ float4");
-        const uint32_t base = lds_addr(buf);
-        int4 oa, ob;
-        lds_read_x2(base + 4 * kRingQuads * 16 + wave * 64, oa, ob);
-        const int o[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
-        const uint32_t lb = base + 16 * lane;
+        const uint32_t lb = lds_addr(buf) + 16 * lane;
 #pragma unroll
         for (int j0 = 0; j0 < ND; j0 += 4) {
             float4 v[8];
@@ -475,12 +447,19 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
         }
     };
     auto dmin_of = [&](int c) { return dminTab[min(c, nChan - 1)]; };
+    auto offs_of = [&](int c, int (&o)[ND]) {
+        const int cc = min(c, nChan - 1);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) o[j] = offsTab[cc * TD + j];
+    };
 
     // prologue: channels 0 .. DEPTH-2 into their stages
 #pragma unroll
     for (int k = 0; k < DEPTH - 1; ++k) stage(ring[k], k, dmin_of(k));
     // dmin of the channel staged at the next top, loaded a channel ahead
     int nextDmin = dmin_of(DEPTH - 1);
+    int oCur[ND];
+    offs_of(0, oCur);
     // channel c uses stage c % DEPTH: the loop is unrolled by DEPTH so that
     // every stage address is a constant
     for (int c0 = 0; c0 < nChan; c0 += DEPTH) {
@@ -492,11 +471,14 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
                 // the next DEPTH-2 stages may be in flight); then the barrier:
                 // every wave's part has landed and every wave is done with
                 // channel c-1
-                wait_vmcnt<(DEPTH - 2) * (kRingDma + 1)>();
+                wait_vmcnt<(DEPTH - 2) * kRingDma>();
                 bare_barrier();
                 stage(ring[(slot + DEPTH - 1) % DEPTH], c + DEPTH - 1, nextDmin);  // channel c-1's stage
+                compute(ring[slot], oCur);
+                // the next channel's scalars, landing while the wave waits at
+                // the next barrier
                 nextDmin = dmin_of(c + DEPTH);
-                compute(ring[slot]);
+                offs_of(c + 1, oCur);
             }
         }
     }
