@@ -88,10 +88,11 @@ struct LaneTw {
 // form pr + sgn*x in one rounding (the sum, or the partner's difference),
 // then multiply by their twiddle ((1, 0) on the sum lane, an exact no-op up
 // to the sign of a zero).  The last stage's twiddle is 1 on every lane.
+// Every fused multiply-add is written out (the file is built without
+// contraction), so each kernel built from this code rounds alike.
 template <int G, int S = 0>
 __device__ __forceinline__ void lane_fft(float& re, float& im, const LaneTw& L)
 {
-#pragma clang fp contract(fast)
     constexpr int m = G >> (S + 1);
     if constexpr (m >= 1) {
         const float pr = xor_lane<m>(re);
@@ -102,8 +103,8 @@ __device__ __forceinline__ void lane_fft(float& re, float& im, const LaneTw& L)
             re = dr;
             im = di;
         } else {
-            re = dr * L.twr[S] - di * L.twi[S];
-            im = dr * L.twi[S] + di * L.twr[S];
+            re = __builtin_fmaf(dr, L.twr[S], -(di * L.twi[S]));
+            im = __builtin_fmaf(dr, L.twi[S], di * L.twr[S]);
         }
         lane_fft<G, S + 1>(re, im, L);
     }
@@ -141,13 +142,24 @@ __device__ __forceinline__ void reg_dft(C2* x)
     }
 }
 
+// Wide channelisers with long filters hold TAPS x N/64 complex samples per
+// lane and polarisation in their FIR windows; where both polarisations'
+// windows would take more than about 128 registers (two waves per SIMD), each
+// polarisation of a run gets a wave of its own instead (pol split).
+constexpr bool pol_split(int N, int TAPS) { return (N < 64 ? 1 : N / 64) * TAPS >= 32; }
+
 // OUT: 0 = integrated Stokes-I [nBlk/integ][sub][1][N]; 1 = complex spectra
 // [nBlk][sub][2][N].  TAPS is the FIR length (compile-time, so the sliding
 // windows stay in registers).
 // The runs of one wave: waveId counts the waves of the launch (or of one
 // job of a batched launch); `stride` is the samples per (sub, pol) stream of
 // `in`, of which the first nBlk*N are read.
-template <typename Tin, int N, int TAPS, int OUT>
+// PS (pol split, Stokes-I only, N >= 64): waveId counts pairs of waves, two
+// pairs per workgroup; wave (threadIdx.x >> 6) & 1 of a pair channelises that
+// polarisation of the pair's run, and the second hands its power to the
+// first through LDS at every block, where I = powerX + powerY is summed as
+// in the one-wave form.  The workgroup meets at a barrier every block.
+template <typename Tin, int N, int TAPS, int OUT, bool PS = false>
 __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typename HistT<Tin>::type* __restrict__ hist,
                                          float* __restrict__ out, const float* __restrict__ coeffs,
                                          const float* __restrict__ tw, int nSub, long nBlk, long stride, int runLen,
@@ -157,20 +169,28 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
     constexpr int R = N / G;
     constexpr int RUNS_PER_WAVE = 64 / G;
     constexpr int H = TAPS - 1;
+    static_assert(!PS || (OUT == 0 && G == 64), "pol split: Stokes-I, one run per pair of waves");
     const int lane = threadIdx.x & 63;
     const int grp = lane / G;
     const int l = lane % G;
+    const int pol = PS ? (int)((threadIdx.x >> 6) & 1) : 0;  // the wave's polarisation (pol split)
     const long wavesPerSub = (nRuns + RUNS_PER_WAVE - 1) / RUNS_PER_WAVE;
     const int sub = (int)(waveId / wavesPerSub);
-    if (sub >= nSub) return;
+    if constexpr (!PS) {
+        if (sub >= nSub) return;
+    }
+    // a pair past the last subband still meets its workgroup's barriers
+    const bool subOk = sub < nSub;
+    const int sb = subOk ? sub : 0;
     const int run0 = (int)(waveId % wavesPerSub) * RUNS_PER_WAVE;  // the wave's first run
     const int run = run0 + grp;
-    const bool active = run < nRuns;
+    const bool active = subOk && run < nRuns;
     const long T = stride;
-    const Tin* x0 = in + (long)(2 * sub + 0) * T;
-    const Tin* x1 = in + (long)(2 * sub + 1) * T;
-    const auto* h0 = hist + (long)(2 * sub + 0) * H * N;
-    const auto* h1 = hist + (long)(2 * sub + 1) * H * N;
+    // pol split: x0 and h0 are the wave's polarisation, x1 and h1 unused
+    const Tin* x0 = in + (long)(2 * sb + pol) * T;
+    const Tin* x1 = in + (long)(2 * sb + 1) * T;
+    const auto* h0 = hist + (long)(2 * sb + pol) * H * N;
+    const auto* h1 = hist + (long)(2 * sb + 1) * H * N;
 
     // coefficients of this lane's channels, twiddles of this lane
     float hc[R][TAPS];
@@ -221,10 +241,10 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
             if (active) {
                 if (b >= 0) {
                     a = load_sample(x0, b * N + n);
-                    c = load_sample(x1, b * N + n);
+                    if constexpr (!PS) c = load_sample(x1, b * N + n);
                 } else {
                     a = load_sample(h0, (long)(H + b) * N + n);
-                    c = load_sample(h1, (long)(H + b) * N + n);
+                    if constexpr (!PS) c = load_sample(h1, (long)(H + b) * N + n);
                 }
             }
             w0[t + 1][r] = a;
@@ -237,7 +257,15 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
     const int lastRunLen = (int)(nBlk - (long)(nRuns - 1) * runLen);
     const int len = active ? (run == nRuns - 1 ? lastRunLen : runLen) : 0;
     // steps of the wave (uniform): a full run unless its only run is the last
-    const int waveLen = run0 < nRuns - 1 ? runLen : lastRunLen;
+    int waveLen = run0 < nRuns - 1 ? runLen : lastRunLen;
+    if constexpr (PS) {
+        // both pairs of the workgroup take the same steps
+        const long first = waveId - ((threadIdx.x >> 7) & 1);
+        auto steps = [&](long id) {
+            return id / nRuns >= nSub ? 0 : (int)(id % nRuns) == nRuns - 1 ? lastRunLen : runLen;
+        };
+        waveLen = max(steps(first), steps(first + 1));
+    }
     const int kLane = bitrev<G>(l);
     int inWindow = 0;  // blocks summed into the current integration window
     // output row of the current window: b0 is a multiple of integ (runLen is),
@@ -252,16 +280,21 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
             const int s = s0 + u;
             const bool live = s < len;
             const long b = b0 + s;
+            // Every lane loads, from a block clamped into the chunk: a load
+            // under a branch would be waited for on its own.  The steps past a
+            // run's length follow all of its live steps and store nothing
+            // (integration windows never straddle the end of a run), so what
+            // they load reaches no output.
+            const long bl = b < nBlk ? b : nBlk - 1;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int n = l + G * r;
-                w0[u][r] = live ? load_sample(x0, b * N + n) : C2{0.f, 0.f};
-                w1[u][r] = live ? load_sample(x1, b * N + n) : C2{0.f, 0.f};
+                w0[u][r] = load_sample(x0, bl * N + n);
+                if constexpr (!PS) w1[u][r] = load_sample(x1, bl * N + n);
             }
-            // FIR, oldest block first, coefficient index t*N + n
+            // FIR, oldest block first, coefficient index t*N + n, one
+            // multiply-add per tap (written out, as in lane_fft)
             C2 f0[R], f1[R];
-            {
-#pragma clang fp contract(fast)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 f0[r] = f1[r] = {0.f, 0.f};
@@ -269,12 +302,10 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
                 for (int t = 0; t < TAPS; ++t) {
                     const int slot = (u + 1 + t) % TAPS;
                     const float coeff = hc[r][t];
-                    const float a_re = w0[slot][r].re * coeff, a_im = w0[slot][r].im * coeff;
-                    const float c_re = w1[slot][r].re * coeff, c_im = w1[slot][r].im * coeff;
-                    f0[r].re = f0[r].re + a_re;
-                    f0[r].im = f0[r].im + a_im;
-                    f1[r].re = f1[r].re + c_re;
-                    f1[r].im = f1[r].im + c_im;
+                    f0[r].re = __builtin_fmaf(w0[slot][r].re, coeff, f0[r].re);
+                    f0[r].im = __builtin_fmaf(w0[slot][r].im, coeff, f0[r].im);
+                    f1[r].re = __builtin_fmaf(w1[slot][r].re, coeff, f1[r].re);
+                    f1[r].im = __builtin_fmaf(w1[slot][r].im, coeff, f1[r].im);
                 }
             }
             // DFT: registers, twiddle, lanes
@@ -283,9 +314,8 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 const C2 a = f0[r], c = f1[r], t = rtw[r];
-                f0[r] = {a.re * t.re - a.im * t.im, a.re * t.im + a.im * t.re};
-                f1[r] = {c.re * t.re - c.im * t.im, c.re * t.im + c.im * t.re};
-            }
+                f0[r] = {__builtin_fmaf(a.re, t.re, -(a.im * t.im)), __builtin_fmaf(a.re, t.im, a.im * t.re)};
+                f1[r] = {__builtin_fmaf(c.re, t.re, -(c.im * t.im)), __builtin_fmaf(c.re, t.im, c.im * t.re)};
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -301,6 +331,36 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
                         o[0] = make_float2(f0[r].re, f0[r].im);
                         o[N] = make_float2(f1[r].re, f1[r].im);
                     }
+                }
+            } else if constexpr (PS) {
+                // the second wave's power to the first; two buffers by step
+                // parity, so one barrier per step orders every write and read
+                __shared__ float xch[2][2][R][64];
+                const int pr = (threadIdx.x >> 7) & 1, par = s & 1;
+                float pw[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    pw[r] = f0[r].re * f0[r].re + f0[r].im * f0[r].im;
+                    if (pol == 1) xch[pr][par][r][lane] = pw[r];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float I = pw[r] + xch[pr][par][r][lane];  // powerX + powerY on the first wave
+                    acc[r] = acc[r] + I;
+                }
+                if (++inWindow == integ) {  // (s + 1) % integ == 0
+                    inWindow = 0;
+                    if (live && pol == 0) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int k = r + R * kLane;
+                            out[(uo * nSub + sub) * N + k] = acc[r];
+                        }
+                    }
+                    ++uo;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
                 }
             } else {
 #pragma unroll
@@ -338,6 +398,17 @@ __global__ __launch_bounds__(256) void ppf_kernel(
     ppf_wave<Tin, N, TAPS, OUT>(in, hist, out, coeffs, tw, nSub, nBlk, nBlk * N, runLen, nRuns, integ, waveId);
 }
 
+// The pol-split form of ppf_kernel (Stokes-I): two runs per workgroup.
+template <typename Tin, int N, int TAPS>
+__global__ __launch_bounds__(256) void ppf_pair_kernel(
+    const Tin* __restrict__ in, const typename HistT<Tin>::type* __restrict__ hist, float* __restrict__ out,
+    const float* __restrict__ coeffs, const float* __restrict__ tw, int nSub, long nBlk, int runLen,
+    int nRuns, int integ)
+{
+    const long pairId = (long)blockIdx.x * 2 + (threadIdx.x >> 7);
+    ppf_wave<Tin, N, TAPS, 0, true>(in, hist, out, coeffs, tw, nSub, nBlk, nBlk * N, runLen, nRuns, integ, pairId);
+}
+
 // Many plans' fused PPF -> Stokes-I -> integration of one channel count and
 // tap count in one launch: each workgroup finds its job in the table.
 template <int N, int TAPS>
@@ -345,9 +416,15 @@ __global__ __launch_bounds__(256) void ppf_batch_kernel(const PPFBatch B)
 {
     int local;
     const PPFJob& J = find_batch_job(B, local);
-    const long waveId = (long)local * 4 + (threadIdx.x >> 6);
-    ppf_wave<uint32_t, N, TAPS, 0>(J.in, J.hist, J.out, J.coeffs, B.tw, J.nSub, J.nBlk, J.stride, J.runLen,
-                                   J.nRuns, J.integ, waveId);
+    if constexpr (pol_split(N, TAPS)) {
+        const long pairId = (long)local * 2 + (threadIdx.x >> 7);
+        ppf_wave<uint32_t, N, TAPS, 0, true>(J.in, J.hist, J.out, J.coeffs, B.tw, J.nSub, J.nBlk, J.stride,
+                                             J.runLen, J.nRuns, J.integ, pairId);
+    } else {
+        const long waveId = (long)local * 4 + (threadIdx.x >> 6);
+        ppf_wave<uint32_t, N, TAPS, 0>(J.in, J.hist, J.out, J.coeffs, B.tw, J.nSub, J.nBlk, J.stride, J.runLen,
+                                       J.nRuns, J.integ, waveId);
+    }
 }
 
 // Save the last nTaps-1 blocks of every (sub, pol) stream (new history),
@@ -381,6 +458,21 @@ __global__ void save_history_batch_kernel(const HistBatch B)
     J.newh[i] = b >= 0 ? J.in[stream * J.stride + b * J.N + n] : J.oldh[stream * per + (J.H + b) * J.N + n];
 }
 
+// The Stokes-I launch of one channel count and tap count: the pol-split
+// form where it applies (blocks: one wave per run; pairBlocks: a pair).
+template <typename Tin, int N, int TAPS>
+void launch_stokes_i(const Tin* in, const typename HistT<Tin>::type* hist, float* out, const float* coeffs,
+                     const float* tw, int S, long nBlk, int L, int nRuns, int integ, unsigned blocks,
+                     unsigned pairBlocks, hipStream_t st)
+{
+    if constexpr (pol_split(N, TAPS))
+        hipLaunchKernelGGL((ppf_pair_kernel<Tin, N, TAPS>), dim3(pairBlocks), dim3(256), 0, st, in, hist, out,
+                           coeffs, tw, S, nBlk, L, nRuns, integ);
+    else
+        hipLaunchKernelGGL((ppf_kernel<Tin, N, TAPS, 0>), dim3(blocks), dim3(256), 0, st, in, hist, out, coeffs,
+                           tw, S, nBlk, L, nRuns, integ);
+}
+
 template <typename Tin>
 int run_ppf(ppl_ppf_plan* p, const Tin* in, typename HistT<Tin>::type* histCur,
             typename HistT<Tin>::type* histNext, unsigned nBlk, float* out, int mode, hipStream_t st)
@@ -393,16 +485,19 @@ int run_ppf(ppl_ppf_plan* p, const Tin* in, typename HistT<Tin>::type* histCur,
     const int G = N < 64 ? N : 64;
     const long nb = nBlk;
     const int mult = std::lcm(integ, T);
-    const int L = ppf_run_length(&nb, &S, 1, 64 / G, kFillWaves, mult, 4 * T);
+    // pol split: two waves per run, so half the runs fill the GPU
+    const bool ps = mode == 0 && pol_split(N, T);
+    const int L = ppf_run_length(&nb, &S, 1, 64 / G, ps ? kFillWaves / 2 : kFillWaves, mult, 4 * T);
     const int nRuns = (int)cdiv(nBlk, L);
     const long wavesPerSub = (nRuns + (64 / G) - 1) / (64 / G);
     const long waves = wavesPerSub * S;
     const unsigned blocks = (unsigned)cdiv((unsigned)waves, 4);
+    const unsigned pairBlocks = (unsigned)cdiv((unsigned)waves, 2);  // pol split: a run per pair of waves
 #define PPL_PPF_LAUNCH(N_, T_)                                                                          \
     if (N == N_ && T == T_) {                                                                           \
         if (mode == 0)                                                                                  \
-            hipLaunchKernelGGL((ppf_kernel<Tin, N_, T_, 0>), dim3(blocks), dim3(256), 0, st, in, histCur,\
-                               out, p->d_coeffs, p->d_twiddle, S, (long)nBlk, L, nRuns, integ);        \
+            launch_stokes_i<Tin, N_, T_>(in, histCur, out, p->d_coeffs, p->d_twiddle, S, (long)nBlk, L,   \
+                                         nRuns, integ, blocks, pairBlocks, st);                         \
         else                                                                                            \
             hipLaunchKernelGGL((ppf_kernel<Tin, N_, T_, 1>), dim3(blocks), dim3(256), 0, st, in, histCur,\
                                out, p->d_coeffs, p->d_twiddle, S, (long)nBlk, L, nRuns, 1);            \
@@ -449,15 +544,17 @@ int launch_fp32_batch(PPFBatch& B, hipStream_t st)
         ns[k] = B.job[k].nSub;
         mult = std::lcm(mult, B.job[k].integ);
     }
-    const int L = ppf_run_length(nb, ns, B.n, RPW, kFillWaves, mult, 4 * TAPS);
+    // pol split: two waves per run, so half the runs fill the GPU
+    constexpr bool PS = pol_split(N, TAPS);
+    const int L = ppf_run_length(nb, ns, B.n, RPW, PS ? kFillWaves / 2 : kFillWaves, mult, 4 * TAPS);
     long blocks = 0;
     for (int k = 0; k < B.n; ++k) {
         PPFJob& J = B.job[k];
         J.runLen = L;
         J.nRuns = (J.nBlk + L - 1) / L;
-        const long waves = (long)J.nSub * ((J.nRuns + RPW - 1) / RPW);
+        const long waves = (long)J.nSub * ((J.nRuns + RPW - 1) / RPW);  // pairs of waves under pol split
         J.blockStart = (int)blocks;
-        blocks += (waves + 3) / 4;
+        blocks += PS ? (waves + 1) / 2 : (waves + 3) / 4;
         if (blocks >= (1L << 31)) {
             set_error("channeliser batch: %ld workgroups exceed the grid", blocks);
             return PPL_E_ARG;

@@ -82,19 +82,23 @@ def test_stokes_and_integrate_bit_exact(orc, ppl):
     assert np.array_equal(integ2, orc.integrate(I, 2, 4))
 
 
-@pytest.mark.parametrize("n_ch", [16, 64])
-def test_fused_stokes_matches_stages(orc, ppl, n_ch):
+@pytest.mark.parametrize("n_ch,n_taps,n_sub,n_blk", [(16, 8, 4, 128), (64, 8, 4, 128), (256, 8, 3, 72),
+                                                     (128, 16, 3, 100), (256, 16, 3, 72)])
+def test_fused_stokes_matches_stages(orc, ppl, n_ch, n_taps, n_sub, n_blk):
     """Fused PPF -> Stokes-I -> x4 equals the unfused kernels bit for bit, and
-    the FP64 chain within tolerance."""
-    n_sub, n_taps, n_blk = 4, 8, 128
+    the FP64 chain within tolerance.  The 256- and 128-channel cases with
+    8 and 16 taps take the pol-split form (a wave per polarisation); with
+    72 blocks of 256 channels the runs are 32, 32 and 8 blocks, so a
+    workgroup's two runs straddle subbands, the last run is short and the
+    last workgroup's second pair has no run."""
     h = _coeffs(orc, n_taps, n_ch)
     x = noise_voltages(n_sub, n_blk * n_ch, 64.0, seed=9)
     xd = torch.from_numpy(x).cuda()
     fused = ppl.PPF(n_sub, n_ch, n_taps, h.astype(np.float32), integrate=4).stokes_i16(xd, n_blk)
     spec = ppl.PPF(n_sub, n_ch, n_taps, h.astype(np.float32)).spectra_i16(xd, n_blk)
     staged = ppl.integrate(ppl.stokes(spec, 1), 4)
-    assert torch.equal(fused, staged)
     ref = orc.PPF(n_sub, 2, n_ch, n_taps, h).run_f64(x, n_blk)
     I64 = orc.integrate(orc.stokes_i_f64(ref), 4)
     err = np.max(np.abs(fused.cpu().numpy() - I64)) / np.mean(I64)
+    assert torch.equal(fused, staged), (err, float((fused - staged).abs().max()))
     assert err < 1e-4, err
