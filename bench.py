@@ -397,10 +397,11 @@ def main():
                                   "WRITE_SIZE per dispatch, scaled by a copy of known size; not measured in this run)",
                 "kernel": kname, "ops_per_launch": adds, "alg_bytes_per_launch": alg_bytes,
                 "hbm_gbs_algorithmic": round(alg_bytes / t_dd / 1e9, 1),
-                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32), priced against the FP32 peak; "
-                        "the kernel is VALU-bound: one v_add_f32 per add, a wave64 VALU instruction per clock "
-                        "and CU, 34.4 T adds/s at the 2.1 GHz it runs at under this load (39.3 T/s at 2.4 GHz); "
-                        "its 16-byte LDS reads (228 B/clk/CU) would allow about 57 adds/clk/CU"}
+                "note": "ops are channel-order FP32 adds (no FMA, no packed FP32), priced against the FP32 peak: "
+                        "one v_add_f32 per add plus one address add per eight (72 VALU instructions per 64 "
+                        "adds); counters show 0.64 VALU wave-instructions per CU-clock against the 1.19 the "
+                        "channeliser sustains, so LDS binds: the 16-byte reads plus the LDS-DMA staging of "
+                        "four shifted copies of every row (a variant staging one copy ran 17 % faster)"}
 
     # ---- mixed precision: the same inputs through chains with FP16 / u8
     # dedispersion input (FP32 / int32 sums), against a plain FP32 chain.
