@@ -38,7 +38,7 @@ struct ppl_dedisp_plan {
     // of 16 and 32 trials (index 0, 1): per tile and channel the LDS byte
     // offsets of each trial's reads, then per tile and channel the first
     // trial's delay
-    int32_t* d_ring[2] = {nullptr, nullptr};
+    int32_t* d_ring[3] = {nullptr, nullptr, nullptr};  // ring tables for DM tiles of 16, 32, 64
     // u8 series, built on first use: per trial dm, the number of channels
     // whose delay is k, for k = 0 .. maxdelay(dm): lenTab[lenOff[dm] + k];
     // lenOff[tdms] is the table's length
@@ -244,10 +244,11 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 // in channel order, one FP32 add per sample, so the result is bit-identical
 // to the one-sample kernel.
 //
-// One channel per ring stage, DEPTH stages.  Wave s loads copy s of the row
-// (from element s on: the same cache lines for all four) by LDS DMA, always
-// kRingDw dwords (kRingDma instructions).  So every wave has exactly
-// kRingDma DMA instructions per channel in flight, and at the top of
+// One channel per ring stage, DEPTH stages.  Wave s loads copy s & 3 of the
+// row (from element s & 3 on: the same cache lines for all four; with eight
+// waves, wave s takes half s >> 2 of it) by LDS DMA, always the same number
+// of dwords (twelve or eight DMA instructions).  So every wave has a fixed
+// count of DMA instructions per channel in flight, and at the top of
 // channel c it waits (explicit vmcnt) only for its DMA into c's stage; the
 // workgroup then meets at a bare barrier (every wave's part of the stage
 // has landed, every wave is done with channel c-1), and the DMA of channel
@@ -262,14 +263,18 @@ __global__ __launch_bounds__(256) void dedisperse_kernel(const DDBatch B)
 // are scalar (the DMA's scalar base plus a fixed lane offset) and the loop
 // is unrolled by DEPTH, so stage addresses are constants: the vector
 // instructions per channel and wave are the 64 adds and 8 read-address adds
-// alone.  VALU is the bound.
-constexpr int kRingQuads = 192;           // float4 per copy of a staged row
-constexpr int kRingDw = 4 * kRingQuads;   // dwords per copy
-constexpr int kRingDma = kRingDw / 64;    // DMA instructions per copy
-constexpr int kRingStage = 4 * kRingQuads * 16;  // four copies
-// Three stages (two channels of DMA lead): 37 KB of LDS a workgroup, so
-// four workgroups (16 waves) share a CU; four stages (49 KB, three
-// workgroups) measured 10 % slower on the LOFAR beam.
+// alone.  LDS is the bound: the 16-byte reads and the DMA writes of the four
+// copies (a timing variant staging one copy ran 17 % faster), which is why
+// the eight-wave form shares each staged row between twice the trials.
+// Four-wave workgroups (tiles of 16 and 32 trials) hold 192 float4 per copy
+// of a staged row; eight-wave workgroups (64 trials, whose delay spread is
+// twice as wide) hold 256, two waves to a copy.
+constexpr int ring_quads(int waves) { return waves == 8 ? 256 : 192; }
+constexpr int ring_quads_of_tile(int TD) { return ring_quads(TD == 64 ? 8 : 4); }
+// Three stages (two channels of DMA lead): 37 KB of LDS a four-wave
+// workgroup, so four of them (16 waves) share a CU; four stages (49 KB,
+// three workgroups) measured 10 % slower on the LOFAR beam.  An eight-wave
+// workgroup takes 49 KB, two to a CU.
 #ifndef PPL_RING_DEPTH
 #define PPL_RING_DEPTH 3
 #endif
@@ -286,18 +291,17 @@ __device__ __forceinline__ void wait_vmcnt()
 
 __device__ __forceinline__ void bare_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-// kRingDma LDS-DMA dword loads of consecutive 64-dword rows: row k from
+// Twelve LDS-DMA dword loads of consecutive 64-dword rows: row k from
 // global address g + 256k to LDS address lds + 256k (+ 4 * lane each).  The
 // address is the uniform row start g in scalar registers plus the lane's
 // byte offset (4 * lane, a loop-invariant register), so placing a row costs
 // no vector instruction.  The instruction offset applies to both addresses,
-// so one address and one M0 serve all twelve.  M0 is restored before the
+// so one address and one M0 serve all of them.  M0 is restored before the
 // block ends.  The compiler's hazard checks do not look into inline
 // assembly, so the wait state between the M0 write and the first DMA is
 // written out.
 __device__ __forceinline__ void dma_rows12(const float* g, uint32_t laneBytes, uint32_t lds)
 {
-    static_assert(kRingDma == 12, "twelve rows");
     uint32_t saved;
     asm volatile("s_mov_b32 %0, m0\n\t"
                  "s_mov_b32 m0, %3\n\t"
@@ -314,6 +318,28 @@ __device__ __forceinline__ void dma_rows12(const float* g, uint32_t laneBytes, u
                  "global_load_lds_dword %1, %2 offset:2304\n\t"
                  "global_load_lds_dword %1, %2 offset:2560\n\t"
                  "global_load_lds_dword %1, %2 offset:2816\n\t"
+                 "s_nop 0\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved)
+                 : "v"(laneBytes), "s"(g), "s"(lds)
+                 : "memory");
+}
+
+// The same for eight rows (a half copy of an eight-wave stage).
+__device__ __forceinline__ void dma_rows8(const float* g, uint32_t laneBytes, uint32_t lds)
+{
+    uint32_t saved;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %3\n\t"
+                 "s_nop 0\n\t"  // a scalar write of M0 needs one wait state before an LDS DMA reads it
+                 "global_load_lds_dword %1, %2\n\t"
+                 "global_load_lds_dword %1, %2 offset:256\n\t"
+                 "global_load_lds_dword %1, %2 offset:512\n\t"
+                 "global_load_lds_dword %1, %2 offset:768\n\t"
+                 "global_load_lds_dword %1, %2 offset:1024\n\t"
+                 "global_load_lds_dword %1, %2 offset:1280\n\t"
+                 "global_load_lds_dword %1, %2 offset:1536\n\t"
+                 "global_load_lds_dword %1, %2 offset:1792\n\t"
                  "s_nop 0\n\t"
                  "s_mov_b32 m0, %0"
                  : "=&s"(saved)
@@ -360,14 +386,23 @@ __device__ __forceinline__ void lds_read_4x2(uint32_t a0, uint32_t a1, uint32_t 
                  : "memory");
 }
 
-template <int ND, int NQ, int DEPTH>
-__global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
+// WAVES = 4 or 8 waves a workgroup, each summing ND trials: a tile of
+// TD = WAVES * ND trials shares every staged row, so eight waves stage a
+// row once for twice the trials.
+template <int ND, int NQ, int DEPTH, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void dedisperse_ring_kernel(const DDBatch B)
 {
+    static_assert(WAVES == 4 || WAVES == 8, "four or eight waves");
+    constexpr int QUADS = ring_quads(WAVES);
+    constexpr int PARTS = WAVES / 4;                // waves per copy
+    constexpr int PART_DW = 4 * QUADS / PARTS;      // dwords of a copy each of them stages
+    constexpr int DMA = PART_DW / 64;               // DMA instructions per wave and channel
+    static_assert(DMA == 12 || DMA == 8, "twelve or eight rows");
     static_assert(ND <= 16, "a wave's offsets fill 64 bytes");
-    static_assert(DEPTH >= 3 && (DEPTH - 2) * kRingDma < 64, "ring depth");
+    static_assert(DEPTH >= 3 && (DEPTH - 2) * DMA < 64, "ring depth");
     // The stages are read only by inline assembly and written only by DMA,
     // so one array holds them all.
-    __shared__ __align__(16) unsigned char ring[DEPTH][kRingStage];
+    __shared__ __align__(16) unsigned char ring[DEPTH][4 * QUADS * 16];
     int local;
     const DDJob& J = find_job(B, local);
     const float* __restrict__ in = static_cast<const float*>(J.in);
@@ -376,7 +411,7 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
     const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, nOut = J.nOut;
     const int nDMTiles = J.nDMTiles;
     constexpr int TT = 256 * NQ;
-    constexpr int TD = 4 * ND;
+    constexpr int TD = WAVES * ND;
 
     int tTile, dmTile;
     tile_of(local, nDMTiles, tTile, dmTile);
@@ -385,6 +420,7 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
     const int dm0 = dmTile * TD;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int copy = wave & 3, part = wave >> 2;  // the copy and the part of it this wave stages
     const int lastElem = nChan * nsamp - 1;  // make_job checks that it fits an int
     // ring table: offs[tile][c][TD] (LDS byte offset of each trial's reads in
     // a stage), then dmin[tile][c] (the tile's first-trial delay).  Both are
@@ -404,21 +440,23 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
 #pragma unroll
             for (int k = 0; k < 4; ++k) acc[j][q][k] = 0.0f;
 
-    // channel c into a stage: copy `wave` of the row (kRingDma instructions,
-    // each lane's address clamped to the input rather than masked, so the
-    // count is fixed), then the gather of this wave's read offsets of the
-    // channel (lanes 0..ND-1).  Past the last channel: a dummy stage (the
-    // last channel again), so that every channel of the loop issues one.
+    // channel c into a stage: part `part` of copy `copy` of the row (DMA
+    // instructions, each lane's address clamped to the input rather than
+    // masked, so the count is fixed).  Past the last channel: a dummy stage
+    // (the last channel again), so that every channel of the loop issues one.
     const uint32_t laneBytes = 4 * lane;
     auto stage = [&](unsigned char* buf, int c, int dmin) {
         c = min(c, nChan - 1);
-        const int row0 = c * nsamp + t0 + dmin + wave;  // uniform
-        float* dst = reinterpret_cast<float*>(buf + wave * kRingQuads * 16);
-        if (row0 + kRingDw - 1 <= lastElem) {
-            dma_rows12(in + row0, laneBytes, lds_addr(dst));
+        const int row0 = c * nsamp + t0 + dmin + copy + part * PART_DW;  // uniform
+        float* dst = reinterpret_cast<float*>(buf + copy * QUADS * 16) + part * PART_DW;
+        if (row0 + PART_DW - 1 <= lastElem) {
+            if constexpr (DMA == 12)
+                dma_rows12(in + row0, laneBytes, lds_addr(dst));
+            else
+                dma_rows8(in + row0, laneBytes, lds_addr(dst));
         } else {
 #pragma unroll
-            for (int k = 0; k < kRingDma; ++k) {
+            for (int k = 0; k < DMA; ++k) {
                 const int rowk = __builtin_amdgcn_readfirstlane(row0 + 64 * k);  // uniform
                 __builtin_amdgcn_global_load_lds(in + min(rowk + lane, lastElem), dst + 64 * k, 4, 0, 0);
             }
@@ -471,7 +509,7 @@ __global__ __launch_bounds__(256) void dedisperse_ring_kernel(const DDBatch B)
                 // the next DEPTH-2 stages may be in flight); then the barrier:
                 // every wave's part has landed and every wave is done with
                 // channel c-1
-                wait_vmcnt<(DEPTH - 2) * kRingDma>();
+                wait_vmcnt<(DEPTH - 2) * DMA>();
                 bare_barrier();
                 stage(ring[(slot + DEPTH - 1) % DEPTH], c + DEPTH - 1, nextDmin);  // channel c-1's stage
                 compute(ring[slot], oCur);
@@ -933,7 +971,8 @@ int elem_bytes(int dtype) { return dtype == PPL_F32 ? 4 : dtype == PPL_F16 ? 2 :
 int ring_table(const ppl_dedisp_plan* cp, int TD, const int32_t** out)
 {
     ppl_dedisp_plan* p = const_cast<ppl_dedisp_plan*>(cp);
-    const int slot = TD == 16 ? 0 : 1;
+    const int slot = TD == 16 ? 0 : TD == 32 ? 1 : 2;
+    const int quads = ring_quads_of_tile(TD);
     if (!p->d_ring[slot]) {
         if (p->delays.empty()) {
             set_error("dedisperse: no host delay table for the 16-byte kernel");
@@ -950,7 +989,7 @@ int ring_table(const ppl_dedisp_plan* cp, int TD, const int32_t** out)
                 for (int j = 0; j < TD; ++j) {
                     const int dm = std::min(k * TD + j, tdms - 1);
                     const int r = p->delays[(size_t)dm * nChan + c] - d0;
-                    t[((size_t)k * nChan + c) * TD + j] = (r & 3) * kRingQuads * 16 + (r >> 2) * 16;
+                    t[((size_t)k * nChan + c) * TD + j] = (r & 3) * quads * 16 + (r >> 2) * 16;
                 }
             }
         int32_t* d = nullptr;
@@ -982,16 +1021,16 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
             return PPL_E_ARG;
         }
         // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1
-        // of a copy, which holds kRingQuads
-        rowUnits = kRingQuads;
-        perChannel = span / 4 + TT / 4 <= kRingQuads ? 0 : budget + 1;
+        // of a copy, which holds ring_quads_of_tile(TD)
+        rowUnits = ring_quads_of_tile(TD);
+        perChannel = span / 4 + TT / 4 <= rowUnits ? 0 : budget + 1;
         if ((long)p->cfg.nChan * p->cfg.nsampBuf >= (1L << 31) ||
             (long)p->cfg.nChan * p->tdmsPad >= (1L << 31)) {
             set_error("dedisperse: 16-byte reads need nChan*nsampBuf below 2^31");
             return PPL_E_CONFIG;
         }
-        if (TD != 16 && TD != 32) {
-            set_error("dedisperse: 16-byte reads take DM tiles of 16 or 32 trials");
+        if (TD != 16 && TD != 32 && TD != 64) {
+            set_error("dedisperse: 16-byte reads take DM tiles of 16, 32 or 64 trials");
             return PPL_E_ARG;
         }
         if (perChannel == 0) {
@@ -1043,6 +1082,7 @@ template <typename Tin>
 int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, hipStream_t stream)
 {
     dim3 grid(blocks), block(256);
+    const dim3 block8(512);  // the eight-wave ring kernel
     if constexpr (std::is_same<Tin, _Float16>::value) {
 #define PPL_DD16_CASE(ND_, NT_)                                                                  \
     if (s.nd == ND_ && s.nt == NT_) {                                                            \
@@ -1076,7 +1116,7 @@ int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, 
         if (s.rw == 16) {
 #define PPL_DDX4_CASE(ND_, NT_)                                                                 \
     if (s.nd == ND_ && s.nt == NT_) {                                                           \
-        hipLaunchKernelGGL((dedisperse_ring_kernel<ND_, NT_ / 4, PPL_RING_DEPTH>), grid, block, 0, stream, B); \
+        hipLaunchKernelGGL((dedisperse_ring_kernel<ND_, NT_ / 4, PPL_RING_DEPTH, 4>), grid, block, 0, stream, B); \
         PPL_LAUNCHED();                                                                         \
         return PPL_OK;                                                                          \
     }
@@ -1085,6 +1125,17 @@ int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, 
             PPL_DDX4_CASE(4, 4)
             PPL_DDX4_CASE(8, 4)
 #undef PPL_DDX4_CASE
+            // 64 trials: eight waves of 8 trials (the shape's nd counts trials / 4)
+            if (s.nd == 16 && s.nt == 8) {
+                hipLaunchKernelGGL((dedisperse_ring_kernel<8, 2, PPL_RING_DEPTH, 8>), grid, block8, 0, stream, B);
+                PPL_LAUNCHED();
+                return PPL_OK;
+            }
+            if (s.nd == 16 && s.nt == 4) {
+                hipLaunchKernelGGL((dedisperse_ring_kernel<8, 1, PPL_RING_DEPTH, 8>), grid, block8, 0, stream, B);
+                PPL_LAUNCHED();
+                return PPL_OK;
+            }
             set_error("dedisperse: unsupported 16-byte tile %dx%d", 4 * s.nd, 64 * s.nt);
             return PPL_E_ARG;
         }
@@ -1148,7 +1199,8 @@ int run_jobs(const ppl_dedisp_job* jobs, int n, const Shape& s, hipStream_t st)
 }
 
 const Shape kShapes[] = {{8, 8, 0}, {8, 4, 0}, {4, 8, 0}, {4, 4, 0}, {2, 8, 0}, {16, 4, 0}, {16, 2, 0},
-                         {4, 8, 0, 16}, {8, 8, 0, 16}, {4, 4, 0, 16}, {8, 4, 0, 16}};
+                         {4, 8, 0, 16}, {8, 8, 0, 16}, {4, 4, 0, 16}, {8, 4, 0, 16}, {16, 8, 0, 16},
+                         {16, 4, 0, 16}};
 
 bool shape_known(int nd, int nt, int rw)
 {

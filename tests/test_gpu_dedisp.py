@@ -27,7 +27,7 @@ def test_delay_table_matches_oracle(orc, ppl):
 
 @pytest.mark.parametrize("tile", [None, (32, 256), (16, 512), (8, 512), (16, 256), (64, 256),
                                   (16, 512, 0, 16), (32, 512, 0, 16), (32, 512, 1, 16), (16, 256, 0, 16),
-                                  (32, 256, 0, 16)])
+                                  (32, 256, 0, 16), (64, 512, 0, 16), (64, 256, 0, 16)])
 def test_f32_bit_exact_small(orc, ppl, tile):
     rng = np.random.default_rng(11)
     x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
@@ -189,7 +189,7 @@ def test_linearity_full_size(ppl):
 @pytest.mark.parametrize("dtype", ["f32", "f16", "u8"])
 @pytest.mark.parametrize("tile", [(32, 512, 2), (64, 256, 4), (16, 512, 8), (32, 256, 16), (8, 512, 0),
                                   (64, 128, 3), (16, 512, 3, 16), (32, 512, 1, 16), (32, 512, 0, 16),
-                                  (16, 256, 0, 16), (32, 256, 0, 16)])
+                                  (16, 256, 0, 16), (32, 256, 0, 16), (64, 512, 0, 16), (64, 256, 0, 16)])
 def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
     """Every instantiated tile, channel block and read width, odd channel
     counts and odd delay parities (the FP16 form reads pairs from a shifted
@@ -262,16 +262,17 @@ def test_last_partial_dword_of_the_input(orc, ppl, dtype, tile):
 
 def test_wide_reads_reject_a_spread_too_wide(ppl):
     """A DM tile whose delay spread (about 4600 samples over 32 trials) needs
-    a staged row longer than a ring stage holds (768 samples a copy) is
-    refused, not truncated; 16-byte tiles that are not instantiated are
-    refused when set."""
+    a staged row longer than a ring stage holds (768 samples a copy; 1024
+    for 64 trials) is refused, not truncated; 16-byte tiles that are not
+    instantiated are refused when set."""
     cfg = dict(n_chan=64, nsamp_buf=16384, tdms=64, dm_low=0.0, dm_step=6.0, tsamp=0.00032768,
                fch1=150.0, foff=-0.05)
     plan = ppl.DedispPlan(**cfg)
-    plan.set_tile(32, 512, 0, 16)
-    with pytest.raises(ppl.PPLError):
-        plan.run(torch.zeros((64, 16384), device="cuda"))
-    for tile in [(64, 512), (64, 256), (16, 1024), (8, 512)]:
+    for tile in [(32, 512, 0, 16), (64, 512, 0, 16)]:
+        plan.set_tile(*tile)
+        with pytest.raises(ppl.PPLError):
+            plan.run(torch.zeros((64, 16384), device="cuda"))
+    for tile in [(128, 512), (16, 1024), (8, 512)]:
         with pytest.raises(ppl.PPLError, match="not instantiated"):
             plan.set_tile(*tile, 0, 16)
 
