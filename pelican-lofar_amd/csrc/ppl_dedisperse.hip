@@ -906,6 +906,282 @@ __global__ __launch_bounds__(256) void dedisperse_u8_kernel(const DDBatch B)
     }
 }
 
+// u8 input through a ring of staged rows, the u8 counterpart of the FP32
+// ring kernel.  Lane l sums the dwords l + 64q (q < NQ) of the tile's row
+// window, four samples t0 + 4(l + 64q) + {0,1,2,3} each, with the SWAR scheme
+// of dedisperse_u8_kernel (bytes 0/2 and 1/3 into the 16-bit halves of two
+// accumulators, two channels per v_add3_u32, widened to int32 every 254
+// channels).  A stage holds a channel pair; each channel's row is one copy
+// staged by LDS DMA from the dword at or below its first byte (so the DMA
+// stays dword-aligned), then three copies shifted by 1..3 bytes built in LDS
+// with alignbyte, so every read is one aligned dword.  The read offsets
+// (copy and dword, with the row's misalignment folded in) and the first-trial
+// delays come by scalar loads from a per-plan table, a pair ahead.  Per pair:
+// wait for the DMA of pair p+1, a barrier (the shifted copies of pair p are
+// complete, every wave is done with pair p-1), the DMA of pair p+2 into pair
+// p-1's stage, the shifted copies of pair p+1, then the sums of pair p.
+// Integer sums are exact in any order, so the output equals the other u8
+// kernel's.
+constexpr int kU8CopyDw = 256;                       // dwords per copy of a staged row
+constexpr int kU8StageBytes = 2 * 4 * kU8CopyDw * 4;  // a channel pair, four copies each
+
+// One LDS-DMA dword load from a per-lane address (a vector register pair)
+// to lds + 4 * lane, M0 saved and restored as in dma_rows12.
+__device__ __forceinline__ void dma_dword_lane(const void* g, uint32_t lds)
+{
+    uint32_t saved;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %2\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dword %1, off\n\t"
+                 "s_nop 0\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved)
+                 : "v"(g), "s"(lds)
+                 : "memory");
+}
+
+// One LDS-DMA dword load, scalar base plus the lane's byte offset.
+__device__ __forceinline__ void dma_dword_s(const void* g, uint32_t laneBytes, uint32_t lds)
+{
+    uint32_t saved;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %3\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dword %1, %2\n\t"
+                 "s_nop 0\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved)
+                 : "v"(laneBytes), "s"(g), "s"(lds)
+                 : "memory");
+}
+
+// A barrier after this wave's LDS writes have landed.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Eight dword reads, two from each of four addresses (the second 256 bytes
+// on: the q = 1 dword), waiting for them before the block ends.
+__device__ __forceinline__ void lds_read4x2(const uint32_t (&a)[4], uint32_t (&v)[8])
+{
+    asm volatile("ds_read_b32 %0, %8\n\t"
+                 "ds_read_b32 %1, %9\n\t"
+                 "ds_read_b32 %2, %8 offset:256\n\t"
+                 "ds_read_b32 %3, %9 offset:256\n\t"
+                 "ds_read_b32 %4, %10\n\t"
+                 "ds_read_b32 %5, %11\n\t"
+                 "ds_read_b32 %6, %10 offset:256\n\t"
+                 "ds_read_b32 %7, %11 offset:256\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
+                   "=&v"(v[7])
+                 : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3])
+                 : "memory");
+}
+
+// Eight dword reads from eight addresses, waiting for them before the block
+// ends.
+__device__ __forceinline__ void lds_read8(const uint32_t (&a)[8], uint32_t (&v)[8])
+{
+    asm volatile("ds_read_b32 %0, %8\n\t"
+                 "ds_read_b32 %1, %9\n\t"
+                 "ds_read_b32 %2, %10\n\t"
+                 "ds_read_b32 %3, %11\n\t"
+                 "ds_read_b32 %4, %12\n\t"
+                 "ds_read_b32 %5, %13\n\t"
+                 "ds_read_b32 %6, %14\n\t"
+                 "ds_read_b32 %7, %15\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
+                   "=&v"(v[7])
+                 : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7])
+                 : "memory");
+}
+
+template <int ND, int NQ>
+__global__ __launch_bounds__(256) void dedisperse_u8_ring_kernel(const DDBatch B)
+{
+    static_assert((NQ == 1 || NQ == 2) && ND % 4 == 0 && ND <= 16, "u8 ring tile");
+    constexpr int DEPTH = 3;
+    __shared__ __align__(16) unsigned char ring[DEPTH][kU8StageBytes];
+    int local;
+    const DDJob& J = find_job(B, local);
+    const uint8_t* __restrict__ in = static_cast<const uint8_t*>(J.in);
+    int32_t* __restrict__ out = static_cast<int32_t*>(J.out);
+    const int32_t* __restrict__ tab = J.delayT;
+    const int nChan = J.nChan, nsamp = J.nsamp, tdms = J.tdms, nOut = J.nOut;
+    const int nDMTiles = J.nDMTiles;
+    constexpr int TT = 256 * NQ;
+    constexpr int TD = 4 * ND;
+
+    int tTile, dmTile;
+    tile_of(local, nDMTiles, tTile, dmTile);
+    const int t0 = tTile * TT;
+    if (t0 >= nOut) return;  // padding of the XCD-major grid
+    const int dm0 = dmTile * TD;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lastDw = (nChan * nsamp - 1) >> 2;  // the input's last dword (make_job bounds the size)
+    using ConstI32 = const __attribute__((address_space(4))) int32_t;
+    ConstI32* offsTab = (ConstI32*)(tab + ((size_t)dmTile * nChan) * TD + wave * ND);
+    ConstI32* dminTab = (ConstI32*)(tab + (size_t)nDMTiles * nChan * TD + (size_t)dmTile * nChan);
+    const uint32_t* in32 = reinterpret_cast<const uint32_t*>(in);
+
+    int32_t acc[ND][NQ][4];
+    uint32_t h02[ND][NQ], h13[ND][NQ];
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            h02[j][q] = h13[j][q] = 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[j][q][k] = 0;
+        }
+    auto flush = [&]() {
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                acc[j][q][0] += (int32_t)(h02[j][q] & 0xffffu);
+                acc[j][q][2] += (int32_t)(h02[j][q] >> 16);
+                acc[j][q][1] += (int32_t)(h13[j][q] & 0xffffu);
+                acc[j][q][3] += (int32_t)(h13[j][q] >> 16);
+                h02[j][q] = h13[j][q] = 0u;
+            }
+    };
+
+    // copy 0 of channel c (past the last channel: the last channel again, so
+    // that every pair issues the same DMA) into half h of a stage: wave w
+    // loads dwords 64w .. 64w+63 of the row, from the dword holding its first
+    // byte (the misalignment is in the read offsets)
+    const uint32_t laneBytes = 4 * lane;
+    auto dmin_of = [&](int c) { return dminTab[min(c, nChan - 1)]; };
+    auto stage_row = [&](unsigned char* buf, int h, int c, int dmin) {
+        c = min(c, nChan - 1);
+        const int first = c * nsamp + t0 + dmin;  // uniform byte index
+        const int dw0 = (first >> 2) + 64 * wave;
+        unsigned char* dst = buf + h * (4 * kU8CopyDw * 4) + 256 * wave;
+        if (dw0 + 63 <= lastDw)
+            dma_dword_s(in32 + dw0, laneBytes, lds_addr(dst));
+        else
+            dma_dword_lane(in32 + min(dw0 + lane, lastDw), lds_addr(dst));
+    };
+    auto stage_pair = [&](unsigned char* buf, int p, int dmin0, int dmin1) {
+        stage_row(buf, 0, 2 * p, dmin0);
+        stage_row(buf, 1, 2 * p + 1, dmin1);
+    };
+    // copies 1..3 of both rows of a stage from copy 0: dword m of copy s is
+    // bytes 4m + s .. 4m + s + 3 of the row (thread t builds dword t of each
+    // row's three copies; the last one reads one dword past copy 0, into
+    // copy 1, which no read ever reaches).  A pair whose second channel is
+    // past the last one gets zeros there (after its DMA has landed), so the
+    // sums run the same for every pair.
+    auto shift_pair = [&](unsigned char* buf, int p) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint32_t* row = reinterpret_cast<uint32_t*>(buf + h * (4 * kU8CopyDw * 4));
+            if (h == 1 && 2 * p + 1 >= nChan) {  // uniform
+#pragma unroll
+                for (int k = 0; k < 4; ++k) row[k * kU8CopyDw + tid] = 0u;
+            } else {
+                const uint32_t a = row[tid], b = row[tid + 1];
+                row[kU8CopyDw + tid] = __builtin_amdgcn_alignbyte(b, a, 1);
+                row[2 * kU8CopyDw + tid] = __builtin_amdgcn_alignbyte(b, a, 2);
+                row[3 * kU8CopyDw + tid] = __builtin_amdgcn_alignbyte(b, a, 3);
+            }
+        }
+    };
+    auto offs_of = [&](int c, int (&o)[ND]) {
+        const int cc = min(c, nChan - 1);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) o[j] = offsTab[cc * TD + j];
+    };
+    // the sums of a pair.  v[2 (jj NQ + q) + h] is trial j0 + jj, dword q,
+    // row h.
+    auto compute = [&](const unsigned char* buf, const int (&o0)[ND], const int (&o1)[ND]) {
+        const uint32_t lb0 = lds_addr(buf) + 4 * lane;
+        const uint32_t lb1 = lb0 + 4 * kU8CopyDw * 4;
+        constexpr int JB = 4 / NQ;  // trials per block of eight reads
+#pragma unroll
+        for (int j0 = 0; j0 < ND; j0 += JB) {
+            uint32_t v[8];
+            if constexpr (NQ == 2) {
+                const uint32_t a[4] = {lb0 + o0[j0], lb1 + o1[j0], lb0 + o0[j0 + 1], lb1 + o1[j0 + 1]};
+                lds_read4x2(a, v);
+            } else {
+                const uint32_t a[8] = {lb0 + o0[j0],     lb1 + o1[j0],     lb0 + o0[j0 + 1], lb1 + o1[j0 + 1],
+                                       lb0 + o0[j0 + 2], lb1 + o1[j0 + 2], lb0 + o0[j0 + 3], lb1 + o1[j0 + 3]};
+                lds_read8(a, v);
+            }
+#pragma unroll
+            for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const uint32_t x0 = v[2 * (jj * NQ + q)];
+                    const uint32_t x1 = v[2 * (jj * NQ + q) + 1];
+                    h02[j0 + jj][q] = h02[j0 + jj][q] + (x0 & 0x00ff00ffu) + (x1 & 0x00ff00ffu);
+                    h13[j0 + jj][q] = h13[j0 + jj][q] + __builtin_amdgcn_perm(0u, x0, 0x0c030c01u) +
+                                      __builtin_amdgcn_perm(0u, x1, 0x0c030c01u);
+                }
+        }
+    };
+
+    const int nPairs = (nChan + 1) / 2;
+    // prologue: pairs 0 and 1 staged, pair 0's copies built
+    stage_pair(ring[0], 0, dmin_of(0), dmin_of(1));
+    stage_pair(ring[1], 1, dmin_of(2), dmin_of(3));
+    wait_vmcnt<2>();  // pair 0 landed
+    lds_barrier();
+    shift_pair(ring[0], 0);
+    int o0[ND], o1[ND];
+    offs_of(0, o0);
+    offs_of(1, o1);
+    // first-trial delays of the pair staged next, loaded a pair ahead
+    int nd0 = dmin_of(4), nd1 = dmin_of(5);
+    int sinceFlush = 0;
+    for (int p0 = 0; p0 < nPairs; p0 += DEPTH) {
+#pragma unroll
+        for (int slot = 0; slot < DEPTH; ++slot) {
+            const int p = p0 + slot;
+            if (p < nPairs) {  // uniform
+                wait_vmcnt<0>();  // pair p+1 landed
+                lds_barrier();    // pair p's copies built, every wave done with pair p-1
+                stage_pair(ring[(slot + 2) % DEPTH], p + 2, nd0, nd1);
+                // pair p+1's shifted copies (its copy 0 landed before the barrier)
+                shift_pair(ring[(slot + 1) % DEPTH], p + 1);
+                if (sinceFlush >= 254) {
+                    flush();
+                    sinceFlush = 0;
+                }
+                compute(ring[slot], o0, o1);
+                sinceFlush += 2;
+                // the next pair's scalars
+                offs_of(2 * p + 2, o0);
+                offs_of(2 * p + 3, o1);
+                nd0 = dmin_of(2 * p + 6);
+                nd1 = dmin_of(2 * p + 7);
+            }
+        }
+    }
+    wait_vmcnt<0>();  // no LDS DMA may land after the workgroup has ended
+    flush();
+    const int dmw = dm0 + wave * ND;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        const int dm = dmw + j;
+        if (dm < tdms) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int t = t0 + 4 * lane + 256 * q;
+                int32_t* o = out + (size_t)dm * nOut + t;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (t + k < nOut) o[k] = acc[j][q][k];
+            }
+        }
+    }
+}
+
 // delayT[c][dm] = trunc(shift[c] * (mstartdm + dm*mdmstep)) for dm < tdms,
 // rows padded with the last trial (compiled without contraction, so the
 // product and sum round separately, as in the host table).
@@ -973,6 +1249,7 @@ int ring_table(const ppl_dedisp_plan* cp, int TD, const int32_t** out)
     ppl_dedisp_plan* p = const_cast<ppl_dedisp_plan*>(cp);
     const int slot = TD == 16 ? 0 : TD == 32 ? 1 : 2;
     const int quads = ring_quads_of_tile(TD);
+    const bool u8 = p->cfg.dtype == PPL_U8;  // a plan has one input type, so its tables have one format
     if (!p->d_ring[slot]) {
         if (p->delays.empty()) {
             set_error("dedisperse: no host delay table for the 16-byte kernel");
@@ -986,10 +1263,14 @@ int ring_table(const ppl_dedisp_plan* cp, int TD, const int32_t** out)
             for (int c = 0; c < nChan; ++c) {
                 const int32_t d0 = p->delays[(size_t)(k * TD) * nChan + c];
                 dminPart[(size_t)k * nChan + c] = d0;
+                // u8: the row's first byte sits mis bytes into its first staged
+                // dword (tiles start at multiples of 256 samples)
+                const int mis = (int)(((long)c * p->cfg.nsampBuf + d0) & 3);
                 for (int j = 0; j < TD; ++j) {
                     const int dm = std::min(k * TD + j, tdms - 1);
                     const int r = p->delays[(size_t)dm * nChan + c] - d0;
-                    t[((size_t)k * nChan + c) * TD + j] = (r & 3) * quads * 16 + (r >> 2) * 16;
+                    t[((size_t)k * nChan + c) * TD + j] =
+                        u8 ? ((mis + r) & 3) * kU8CopyDw * 4 + ((mis + r) >> 2) * 4 : (r & 3) * quads * 16 + (r >> 2) * 16;
                 }
             }
         int32_t* d = nullptr;
@@ -1015,9 +1296,27 @@ int make_job(const ppl_dedisp_plan* p, const Shape& s, const void* in, void* out
     int rowUnits = 0, perChannel = 0;
     int budget = kMaxLDS;
     const int32_t* ringTab = nullptr;
-    if (s.rw == 16) {
+    if (s.rw == 16 && p->cfg.dtype == PPL_U8) {
+        // the u8 ring kernel: tiles of 16 or 32 trials and 256 or 512 samples
+        if ((TD != 16 && TD != 32) || (TT != 256 && TT != 512)) {
+            set_error("dedisperse: the u8 ring takes DM tiles of 16 or 32 trials and 256 or 512 samples");
+            return PPL_E_ARG;
+        }
+        if ((long)p->cfg.nChan * p->cfg.nsampBuf >= (1L << 31) || ((long)p->cfg.nChan * p->cfg.nsampBuf) % 4 != 0) {
+            set_error("dedisperse: the u8 ring needs nChan*nsampBuf below 2^31 and a multiple of 4");
+            return PPL_E_CONFIG;
+        }
+        // the widest read reaches dword (3 + span) / 4 + TT / 4 - 1 of a copy;
+        // the last dword of a shifted copy is not valid
+        rowUnits = kU8CopyDw;
+        perChannel = (span + 3) / 4 + TT / 4 <= kU8CopyDw - 1 ? 0 : budget + 1;
+        if (perChannel == 0) {
+            const int rc = ring_table(p, TD, &ringTab);
+            if (rc != PPL_OK) return rc;
+        }
+    } else if (s.rw == 16) {
         if (p->cfg.dtype != PPL_F32 || s.nt % 4 != 0) {
-            set_error("dedisperse: 16-byte reads need FP32 input and a tile of 256k samples");
+            set_error("dedisperse: 16-byte reads need FP32 or u8 input and a tile of 256k samples");
             return PPL_E_ARG;
         }
         // float4 per copy: the widest read reaches float4 span/4 + TT/4 - 1
@@ -1099,6 +1398,21 @@ int launch_batch(const Shape& s, const DDBatch& B, unsigned blocks, size_t lds, 
         PPL_DD16_CASE(16, 2)
 #undef PPL_DD16_CASE
     } else if constexpr (std::is_same<Tin, uint8_t>::value) {
+        if (s.rw == 16) {
+#define PPL_DD8R_CASE(ND_, NT_)                                                                   \
+    if (s.nd == ND_ && s.nt == NT_) {                                                             \
+        hipLaunchKernelGGL((dedisperse_u8_ring_kernel<ND_, NT_ / 4>), grid, block, 0, stream, B); \
+        PPL_LAUNCHED();                                                                           \
+        return PPL_OK;                                                                            \
+    }
+            PPL_DD8R_CASE(4, 8)
+            PPL_DD8R_CASE(8, 8)
+            PPL_DD8R_CASE(4, 4)
+            PPL_DD8R_CASE(8, 4)
+#undef PPL_DD8R_CASE
+            set_error("dedisperse: unsupported u8 ring tile %dx%d", 4 * s.nd, 64 * s.nt);
+            return PPL_E_ARG;
+        }
 #define PPL_DD8_CASE(ND_, NT_)                                                                  \
     if (s.nd == ND_ && s.nt == NT_) {                                                           \
         hipLaunchKernelGGL((dedisperse_u8_kernel<ND_, NT_ / 4>), grid, block, lds, stream, B); \
@@ -1209,8 +1523,12 @@ bool shape_known(int nd, int nt, int rw)
     return false;
 }
 
-// The shapes the autotuner tries for a dtype: 16-byte reads only for FP32.
-bool shape_for(const Shape& s, int dtype) { return s.rw == 4 || dtype == PPL_F32; }
+// The shapes the autotuner tries for a dtype: the ring kernels (read width 16)
+// for FP32 and, up to 32 trials, for u8.
+bool shape_for(const Shape& s, int dtype)
+{
+    return s.rw == 4 || dtype == PPL_F32 || (dtype == PPL_U8 && s.nd <= 8);
+}
 
 // Largest delay spread inside any DM tile of TD trials.
 int tile_span(const std::vector<int32_t>& d, int tdms, int nChan, int TD)
@@ -1374,7 +1692,8 @@ int parse_shape(int tileDM, int tileT, int chanBlock, int readBytes, int dtype, 
     PPL_CHECK_ARG(tileDM % 4 == 0 && tileT % 64 == 0, "tile %dx%d not a multiple of 4x64", tileDM, tileT);
     PPL_CHECK_ARG(chanBlock >= 0 && chanBlock <= 16, "channel block %d outside [0, 16]", chanBlock);
     PPL_CHECK_ARG(readBytes == 4 || readBytes == 16, "read width %d bytes (4 or 16)", readBytes);
-    PPL_CHECK_ARG(readBytes == 4 || dtype == PPL_F32, "16-byte reads need FP32 input");
+    PPL_CHECK_ARG(readBytes == 4 || dtype == PPL_F32 || (dtype == PPL_U8 && tileDM <= 32),
+                  "16-byte reads (the ring kernels) need FP32 input, or u8 input with tiles of up to 32 trials");
     PPL_CHECK_ARG(shape_known(tileDM / 4, tileT / 64, readBytes), "tile %dx%d with %d-byte reads not instantiated",
                   tileDM, tileT, readBytes);
     s = Shape{tileDM / 4, tileT / 64, chanBlock, readBytes};

@@ -213,8 +213,8 @@ def test_all_inputs_all_tiles_exact(orc, ppl, dtype, tile):
         host = orc.quantise_u8(x, np.float32(12.7))
         dev = torch.from_numpy(host).cuda()
     plan = ppl.DedispPlan(**cfg, dtype=dtype)
-    if len(tile) == 4 and dtype != "f32":
-        with pytest.raises(ppl.PPLError):  # 16-byte reads are built for FP32 input only
+    if len(tile) == 4 and (dtype == "f16" or (dtype == "u8" and tile[0] > 32)):
+        with pytest.raises(ppl.PPLError):  # the ring kernels take FP32, and u8 up to 32 trials
             plan.set_tile(*tile)
         return
     plan.set_tile(*tile)
@@ -258,6 +258,18 @@ def test_last_partial_dword_of_the_input(orc, ppl, dtype, tile):
     plan.set_tile(*tile)
     out = plan.run(dev).cpu().numpy()
     assert np.array_equal(out, orc.dedisperse(host, d, p["nout"]))
+
+
+def test_u8_ring_refuses_a_ragged_input(ppl):
+    """The u8 ring kernel stages whole dwords, so it is refused when the
+    input's byte count is not a multiple of 4 (the one-word u8 kernel takes
+    such inputs); the autotuner then keeps another tile."""
+    cfg = dict(n_chan=53, nsamp_buf=3001, tdms=40, dm_low=0.0, dm_step=0.3, tsamp=0.00032768, fch1=150.0,
+               foff=-0.05)
+    plan = ppl.DedispPlan(**cfg, dtype="u8")
+    plan.set_tile(32, 512, 0, 16)
+    with pytest.raises(ppl.PPLError, match="multiple of 4"):
+        plan.run(torch.zeros((53, 3001), dtype=torch.uint8, device="cuda"))
 
 
 def test_wide_reads_reject_a_spread_too_wide(ppl):
@@ -326,6 +338,9 @@ def test_batch_equals_separate_runs(ppl, dtype):
         tiles += [(16, 512, 0, 16), (16, 256, 0, 16), (32, 256, 0, 16)]
         with pytest.raises(ppl.PPLError, match="too large"):
             ppl.dedisperse_batch(plans, ins, tile=(32, 512, 2, 16))
+    elif dtype == "u8":
+        # the u8 ring stages a row of up to 1020 bytes: every ring tile fits
+        tiles += [(32, 512, 2, 16), (16, 512, 0, 16), (16, 256, 0, 16), (32, 256, 0, 16)]
     else:
         with pytest.raises(ppl.PPLError):
             ppl.dedisperse_batch(plans, ins, tile=(32, 512, 2, 16))
