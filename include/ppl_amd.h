@@ -113,9 +113,11 @@ int ppl_dedisp_plan_info(const ppl_dedisp_plan* plan, ppl_dedisp_info* info);
 int ppl_dedisp_set_tile(ppl_dedisp_plan* plan, int tileDM, int tileT);
 /* Same with the number of channels staged per LDS block (0: as many as fit). */
 int ppl_dedisp_set_shape(ppl_dedisp_plan* plan, int tileDM, int tileT, int chanBlock);
-/* Same with the kernel's LDS read width per lane: 4 bytes (one sample), or 16
- * (four consecutive samples per read, FP32 input only; tileDM 16, 32 or 64,
- * tileT 256 or 512).  Both give the same sums bit for bit. */
+/* Same with the kernel family: readBytes 4 for the one-word kernels, or 16
+ * for the ring kernels (FP32 input: four consecutive samples per 16-byte
+ * read, tileDM 16, 32 or 64; u8 input: four samples per dword, tileDM 16 or
+ * 32, byte count of the input a multiple of 4; tileT 256 or 512).  All give
+ * the same sums bit for bit. */
 int ppl_dedisp_set_kernel(ppl_dedisp_plan* plan, int tileDM, int tileT, int chanBlock, int readBytes);
 /* Autotuner: times every instantiated DM x time tile, channel block and read width on
  * the given buffers (reps launches each, HIP events on the stream), keeps the
