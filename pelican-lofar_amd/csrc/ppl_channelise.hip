@@ -56,6 +56,12 @@ __device__ __forceinline__ C2 load_sample(const float2* __restrict__ p, long idx
     const float2 v = p[idx];
     return {v.x, v.y};
 }
+// The same in two steps: the raw load (held in flight a step ahead) and the
+// conversion at use.
+__device__ __forceinline__ uint32_t load_raw(const uint32_t* __restrict__ p, long idx) { return p[idx]; }
+__device__ __forceinline__ float2 load_raw(const float2* __restrict__ p, long idx) { return p[idx]; }
+__device__ __forceinline__ C2 to_c2(uint32_t v) { return {(float)(int16_t)(v & 0xffffu), (float)(int16_t)(v >> 16)}; }
+__device__ __forceinline__ C2 to_c2(float2 v) { return {v.x, v.y}; }
 
 template <typename Tin> struct HistT;
 template <> struct HistT<uint32_t> { using type = uint32_t; };
@@ -274,24 +280,37 @@ __device__ __forceinline__ void ppf_wave(const Tin* __restrict__ in, const typen
     // every lane runs every step: the lane FFT shuffles need the whole wave.
     // Steps past a run's length (a shorter last run, and up to the next
     // multiple of TAPS) are not live: they store nothing.
+    // Every lane loads, from a block clamped into the chunk: a load under a
+    // branch would be waited for on its own.  The steps past a run's length
+    // follow all of its live steps and store nothing (integration windows
+    // never straddle the end of a run), so what they load reaches no output.
+    // A step's samples are loaded a step ahead, so the load is in flight
+    // while the previous block is transformed.
+    using Raw = decltype(load_raw(x0, 0));
+    Raw n0[R], n1[R];
+    auto load_step = [&](int s) {
+        const long b = b0 + s;
+        const long bl = b < nBlk ? b : nBlk - 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = l + G * r;
+            n0[r] = load_raw(x0, bl * N + n);
+            if constexpr (!PS) n1[r] = load_raw(x1, bl * N + n);
+        }
+    };
+    load_step(0);
     for (int s0 = 0; s0 < waveLen; s0 += TAPS) {
 #pragma unroll
         for (int u = 0; u < TAPS; ++u) {
             const int s = s0 + u;
             const bool live = s < len;
             const long b = b0 + s;
-            // Every lane loads, from a block clamped into the chunk: a load
-            // under a branch would be waited for on its own.  The steps past a
-            // run's length follow all of its live steps and store nothing
-            // (integration windows never straddle the end of a run), so what
-            // they load reaches no output.
-            const long bl = b < nBlk ? b : nBlk - 1;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int n = l + G * r;
-                w0[u][r] = load_sample(x0, bl * N + n);
-                if constexpr (!PS) w1[u][r] = load_sample(x1, bl * N + n);
+                w0[u][r] = to_c2(n0[r]);
+                if constexpr (!PS) w1[u][r] = to_c2(n1[r]);
             }
+            load_step(s + 1);
             // FIR, oldest block first, coefficient index t*N + n, one
             // multiply-add per tap (written out, as in lane_fft)
             C2 f0[R], f1[R];
