@@ -703,8 +703,8 @@ def multibeam(args, torch, ppl):
         bp = beam_bandpass(h, bm["n_sub"], bm["n_ch"], INTEG, FCH1, foff, SIGMA)
         specs.append(ppl.BeamSpec(bm["n_sub"], bm["n_ch"], N_TAPS, h, INTEG, bp, bm["nsamp"], bm["tdms"], 0.0,
                                   bm["dm_hi"] / bm["tdms"], FCH1, foff))
-    cands = [(32, 512, 1, 16), (16, 512, 1, 16), (32, 256, 1, 16), (16, 512, 4), (32, 512, 4), (16, 256, 4),
-             (32, 256, 8), (16, 512, 8), (8, 512, 8)]
+    cands = [(64, 512, 1, 16), (32, 512, 1, 16), (16, 512, 1, 16), (32, 256, 1, 16), (16, 512, 4), (32, 512, 4),
+             (16, 256, 4), (32, 256, 8), (16, 512, 8), (8, 512, 8)]
     report = {"beams": [{k: bm[k] for k in ("n_sub", "n_ch", "dm_hi", "tdms", "nsamp")} for bm in beams]}
     ref_events = None
     # the plain form (FP32 DFT, FP32 input) and the mixed form (BF16x3 DFT
