@@ -690,7 +690,7 @@ def draw_beams(seed, n):
 def multibeam(args, torch, ppl):
     """24 beams of different shapes through the batched chain, plain (FP32
     DFT and dedispersion input) and mixed (BF16x3 DFT on the matrix cores for
-    the 64- and 256-channel beams, FP16 dedispersion input): one clipper
+    the 64- and 256-channel beams, FP16 or u8 dedispersion input): one clipper
     launch per stage for all beams, one dedispersion launch for all beams
     (or per group of beams sharing a tile, when the autotuner finds that
     faster), HIP-graph replayed steps.  Returns the report object."""
@@ -707,10 +707,10 @@ def multibeam(args, torch, ppl):
              (16, 256, 4), (32, 256, 8), (16, 512, 8), (8, 512, 8)]
     report = {"beams": [{k: bm[k] for k in ("n_sub", "n_ch", "dm_hi", "tdms", "nsamp")} for bm in beams]}
     ref_events = None
-    # the plain form (FP32 DFT, FP32 input) and the mixed form (BF16x3 DFT
-    # where built, FP16 input); the autotuner keeps one batch tile or per-beam
-    # tiles, whichever is faster on the whole batch
-    for dtype in ("f32", "f16"):
+    # the plain form (FP32 DFT, FP32 input) and the mixed forms (BF16x3 DFT
+    # where built, FP16 or u8 input); the autotuner keeps one batch tile or
+    # per-beam tiles, whichever is faster on the whole batch
+    for dtype in ("f32", "f16", "u8"):
         log("multi-beam %d beams, %s" % (len(specs), dtype))
         mb = ppl.MultiBeam(specs, dtype=dtype, dft="fp32" if dtype == "f32" else "bf16x3")
         volts = []

@@ -23,7 +23,7 @@ def _specs(ppl, dtype):
     return specs
 
 
-@pytest.mark.parametrize("dtype,dft", [("f32", "fp32"), ("f16", "fp32"), ("f16", "bf16x3")])
+@pytest.mark.parametrize("dtype,dft", [("f32", "fp32"), ("f16", "fp32"), ("f16", "bf16x3"), ("u8", "bf16x3")])
 def test_multibeam_equals_single_beam_chains(ppl, dtype, dft):
     specs = _specs(ppl, dtype)
     mb = ppl.MultiBeam(specs, dtype=dtype, max_chunk_samples=1 << 15, dft=dft)
