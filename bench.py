@@ -247,7 +247,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the hot path is measured on one MI355X, and a "
-              "multi-GPU layout (beams or DM ranges per GPU) is described in DESIGN.md, "
+              "multi-GPU layout would place beams or DM ranges one per GPU; it is "
               "not launched here.  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

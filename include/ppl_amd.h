@@ -340,7 +340,7 @@ enum { PPL_WIN_KAISER = 0, PPL_WIN_GAUSSIAN = 1, PPL_WIN_BLACKMAN = 2, PPL_WIN_H
 
 /* PolyphaseCoefficients::genereateFilter (src/lib/src/PolyphaseCoefficients.cpp:
  * 308-364), host code in FP64: nTaps*nCh coefficients, index t*nCh+c, with
- * the 1/nCh scale and the (-1)^c sign.  PPL_WIN_HANN is this build's
+ * the 1/nCh scale and the (-1)^c sign.  PPL_WIN_HANN is this library's
  * addition (0.5 - 0.5 cos(2 pi i/(n-1))); the reference has no Hann window. */
 int ppl_ppf_coefficients(unsigned nTaps, unsigned nCh, int window, double* out);
 

@@ -73,7 +73,7 @@ void blackman(unsigned n, double* d)
     }
 }
 
-// Hann window.  The reference has no Hann window; this build defines it as
+// Hann window.  The reference has no Hann window; this library defines it as
 // 0.5 - 0.5 cos(2 pi i/(n-1)), on the same fir2 path (parity unpinned by
 // the reference; pinned by symmetry/endpoint tests).
 void hann(unsigned n, double* d)

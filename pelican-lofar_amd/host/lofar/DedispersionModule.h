@@ -30,7 +30,7 @@ class DedispersionModule : public AsyncronousModule {
     // (1984), dedispersionStepSize/value (0), dedispersionMinimum/value (0,
     // clamped at 0), frequencyChannel1/MHz (required), channelBandwidth/MHz
     // (1; must be negative: channels in descending frequency),
-    // numberOfBuffers/value (2); this build's inputPrecision/value
+    // numberOfBuffers/value (2); this module's own inputPrecision/value
     // (f32 | f16 | u8) and quantiseScale/value (127/10, u8 steps per unit).
     explicit DedispersionModule(const ConfigNode& config);
     ~DedispersionModule() override;

@@ -16,7 +16,7 @@ class PPFChanneliser : public AbstractModule {
   public:
     // Options: outputChannelsPerSubband/value (512), filter/nTaps (8),
     // filter/filterWindow (kaiser | gaussian | blackman | hamming | hann),
-    // processingThreads/value (accepted, unused), and this build's
+    // processingThreads/value (accepted, unused), and this module's own
     // dftPrecision/value (fp32 | bf16x3 | fp16x2).
     explicit PPFChanneliser(const ConfigNode& config);
     ~PPFChanneliser() override;

@@ -161,8 +161,9 @@ def test_plan_rejects_oversized_maxshift(ppl):
 
 
 def test_linearity_full_size(ppl):
-    """configs[1] shape: dedisperse(a + b) == dedisperse(a) + dedisperse(b) for
-    integer-valued inputs (exact in FP32), and an all-ones input sums to nChan."""
+    """LOFAR-beam shape (3968 channels, 2048 trials, 65536 samples):
+    dedisperse(a + b) == dedisperse(a) + dedisperse(b) for integer-valued
+    inputs (exact in FP32), and an all-ones input sums to nChan."""
     cfg = dict(n_chan=3968, nsamp_buf=65536, tdms=2048, dm_low=0.0, dm_step=500 / 2048,
                tsamp=5.12e-6 * 64 * 4, fch1=150.0, foff=-0.1953125 / 64)
     plan = ppl.DedispPlan(**cfg)
