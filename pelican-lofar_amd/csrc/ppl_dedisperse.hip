@@ -1199,6 +1199,28 @@ __global__ void build_delays_kernel(const float* __restrict__ shift, int nChan, 
     delayT[(size_t)c * tdmsPad + dm] = (int32_t)p;
 }
 
+// The FP32 ring kernel's table for DM tiles of TD trials, from a device delay
+// table [nChan][tdmsPad] (tdmsPad a multiple of TD, rows padded with the last
+// trial): entry (tile k, channel c, trial j) is the LDS byte offset of the
+// trial's reads, r = delay - first trial's delay, as copy (r & 3) of the
+// stage at float4 r >> 2; then per (k, c) the first trial's delay.  The same
+// layout the host builds from a plan's delays.
+__global__ void ring_table_kernel(const int32_t* __restrict__ delayT, int nChan, int tdmsPad, int TD,
+                                  int quads, int32_t* __restrict__ tab)
+{
+    const size_t n = (size_t)(tdmsPad / TD) * nChan * TD;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = (int)(i % TD);
+    const size_t kc = i / TD;  // k * nChan + c
+    const int c = (int)(kc % nChan), k = (int)(kc / nChan);
+    const int32_t* row = delayT + (size_t)c * tdmsPad + (size_t)k * TD;
+    const int32_t d0 = row[0];
+    const int r = row[j] - d0;
+    tab[i] = (r & 3) * quads * 16 + (r >> 2) * 16;
+    if (j == 0) tab[n + kc] = d0;
+}
+
 struct Shape {
     int nd, nt;  // DM trials per wave, 64-sample groups per lane
     int cb;      // channels per staged block (0: as many as fit)
@@ -1902,8 +1924,35 @@ extern "C" int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float*
     tmp.d_delayT = d_delayT;
     tmp.tdmsPad = pad;
     for (int k = 0; k < 4; ++k) tmp.spanTD[k] = span;
+    // The ring kernel (16-byte reads, 32 trials x 512 samples) where a DM
+    // tile's delay spread fits its staged row and the sizes fit its 32-bit
+    // offsets, with its table built here from the device delays; the
+    // one-sample kernel otherwise.  Both give the same sums.
+    const Shape ringShape{8, 8, 0, 16};
+    const int quads = ring_quads_of_tile(TD);
+    const bool ring = span / 4 + 64 * ringShape.nt / 4 <= quads && (long)i_nchans * numSamples < (1L << 31) &&
+                      (long)i_nchans * pad < (1L << 31);
+    int32_t* d_tab = nullptr;
+    if (ring) {
+        const size_t n = (size_t)(pad / TD) * i_nchans * TD;
+        const hipError_t e = hipMallocAsync((void**)&d_tab, (n + (size_t)(pad / TD) * i_nchans) * sizeof(int32_t), st);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(d_delayT, st);
+            set_error("cacheDedisperseLoop: ring table: %s", hipGetErrorString(e));
+            return PPL_E_HIP;
+        }
+        hipLaunchKernelGGL(ring_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_delayT,
+                           i_nchans, pad, TD, quads, d_tab);
+        tmp.d_ring[1] = d_tab;  // the 32-trial slot, so make_job takes this table
+    }
+    const hipError_t le = hipGetLastError();
     const ppl_dedisp_job job{&tmp, buff, outbuff};
-    int rc = run_jobs(&job, 1, s, st);
+    int rc = PPL_E_HIP;
+    if (le != hipSuccess)
+        set_error("cacheDedisperseLoop: ring table launch: %s", hipGetErrorString(le));
+    else
+        rc = run_jobs(&job, 1, ring ? ringShape : s, st);
+    if (d_tab) (void)hipFreeAsync(d_tab, st);
     (void)hipFreeAsync(d_delayT, st);
     return rc;
 }

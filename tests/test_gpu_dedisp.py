@@ -101,11 +101,20 @@ def test_f16_and_u8_exact(orc, ppl):
     assert np.array_equal(out8, orc.dedisperse(qh, d, p["nout"]))
 
 
+def _tile_spread(d, td=32):
+    """Largest delay spread inside any tile of td consecutive trials."""
+    return max(int((d[min(k + td, len(d)) - 1] - d[k]).max()) for k in range(0, len(d), td))
+
+
 def test_reference_c_seam(orc, ppl):
-    """cacheDedisperseLoop's own arguments, for tdms not a multiple of 40."""
+    """cacheDedisperseLoop's own arguments, for tdms not a multiple of 40.
+    The 32-trial tiles' delay spread (171 samples) fits the ring kernel's
+    staged row, so the seam runs the ring kernel with the table it builds
+    from its device delays."""
     rng = np.random.default_rng(13)
     x = rng.normal(size=(SMALL["n_chan"], SMALL["nsamp_buf"])).astype(np.float32)
     p, d = _oracle(orc, x, SMALL)
+    assert _tile_spread(d) // 4 + 512 // 4 <= 192  # the ring path
     shift = torch.from_numpy(p["shift"]).cuda()
     out = torch.full((SMALL["tdms"], p["nout"] + 5), -1.0, device="cuda")
     ppl.cache_dedisperse_loop(out, torch.from_numpy(x).cuda(), float(p["mstartdm"]), float(p["mdmstep"]),
@@ -115,6 +124,42 @@ def test_reference_c_seam(orc, ppl):
     n = SMALL["tdms"] * p["nout"]
     assert np.array_equal(flat[:n].reshape(SMALL["tdms"], p["nout"]), orc.dedisperse(x, d, p["nout"]))
     assert np.all(flat[n:] == 0.0)  # the rest of outbufSize is cleared, as cudaMemset did
+
+
+def test_reference_c_seam_wide_spread(orc, ppl):
+    """The seam with delay spreads too wide for the ring kernel's row (the
+    one-sample kernel path), tdms = 100."""
+    cfg = dict(SMALL, dm_step=300 / 128, nsamp_buf=16384, tdms=100)
+    rng = np.random.default_rng(14)
+    x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
+    p, d = _oracle(orc, x, cfg)
+    assert _tile_spread(d) // 4 + 512 // 4 > 192  # the one-sample path
+    out = torch.empty((cfg["tdms"], p["nout"]), device="cuda")
+    ppl.cache_dedisperse_loop(out, torch.from_numpy(x).cuda(), float(p["mstartdm"]), float(p["mdmstep"]),
+                              cfg["tdms"], cfg["nsamp_buf"], torch.from_numpy(p["shift"]).cuda(),
+                              p["maxshift"] + p["rem"], cfg["n_chan"])
+    assert np.array_equal(out.cpu().numpy(), orc.dedisperse(x, d, p["nout"]))
+
+
+def test_reference_c_seam_lofar_equals_plan(orc, ppl):
+    """LOFAR-beam shape (3968 channels, 2048 trials, 65536 samples) through
+    the seam (ring kernel, table built on the device) and through a plan
+    (table built on the host from the oracle-checked delays): the same
+    sums, bit for bit."""
+    cfg = dict(n_chan=3968, nsamp_buf=65536, tdms=2048, dm_low=0.0, dm_step=500 / 2048,
+               tsamp=5.12e-6 * 64 * 4, fch1=150.0, foff=-0.1953125 / 64)
+    p, d = _oracle(orc, None, cfg)
+    assert _tile_spread(d) // 4 + 512 // 4 <= 192  # the ring path
+    plan = ppl.DedispPlan(**cfg)
+    assert np.array_equal(plan.delays(), d)
+    g = torch.Generator(device="cuda").manual_seed(21)
+    x = torch.randn((cfg["n_chan"], cfg["nsamp_buf"]), device="cuda", generator=g)
+    want = plan.run(x)
+    out = torch.empty((cfg["tdms"], p["nout"]), device="cuda")
+    ppl.cache_dedisperse_loop(out, x, float(p["mstartdm"]), float(p["mdmstep"]), cfg["tdms"], cfg["nsamp_buf"],
+                              torch.from_numpy(p["shift"]).cuda(), p["maxshift"] + p["rem"], cfg["n_chan"])
+    assert torch.equal(out, want)
+    del x, out, want
 
 
 def test_plan_counts_fused_delay_differences(orc, ppl):
