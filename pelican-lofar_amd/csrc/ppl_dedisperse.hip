@@ -1660,6 +1660,19 @@ extern "C" int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_c
         const int span = tile_span(p->delays, (int)cfg->tdms, (int)N, TD);
         if (4 * 2 * ((64 * s.nt + span + 20) * elem + 4 * TD) <= kMaxLDS) break;
     }
+    // FP32: the ring kernel (16-byte reads) at 32 trials x 512 samples, the
+    // autotuner's usual choice, where its staged row holds a tile's delay
+    // spread and the sizes fit its 32-bit offsets
+    if (cfg->dtype == PPL_F32) {
+        const long pad = ((long)cfg->tdms + 63) / 64 * 64;
+        const int span32 = tile_span(p->delays, (int)cfg->tdms, (int)N, 32);
+        if (span32 / 4 + 512 / 4 <= ring_quads_of_tile(32) && (long)N * cfg->nsampBuf < (1L << 31) &&
+            (long)N * pad < (1L << 31)) {
+            p->nd = 8;
+            p->nt = 8;
+            p->rw = 16;
+        }
+    }
     int rc = upload_delays(p);
     if (rc != PPL_OK) {
         delete p;
@@ -1801,7 +1814,14 @@ extern "C" int ppl_dedisperse_batch(const ppl_dedisp_job* jobs, int n, int tileD
     if (tileDM != 0 || tileT != 0) {
         const int rc = parse_shape(tileDM, tileT, chanBlock, readBytes, jobs[0].plan->cfg.dtype, s);
         if (rc != PPL_OK) return rc;
+        return run_jobs(jobs, n, s, stream_of(stream));
     }
+    // job 0's own tile; when that is a ring tile some other job's delay
+    // spread does not fit (a refusal before any launch), the same tile with
+    // one-sample reads
+    const int rc = run_jobs(jobs, n, s, stream_of(stream));
+    if (rc != PPL_E_CONFIG || s.rw != 16) return rc;
+    s.rw = 4;
     return run_jobs(jobs, n, s, stream_of(stream));
 }
 

@@ -162,6 +162,28 @@ def test_reference_c_seam_lofar_equals_plan(orc, ppl):
     del x, out, want
 
 
+def test_default_fp32_tile_is_the_ring_and_batches_fall_back(orc, ppl):
+    """An FP32 plan starts on the ring tile (32 x 512, 16-byte reads) where
+    its delay spread fits, and on one-sample reads where it does not; a batch
+    under job 0's own tile falls back to one-sample reads when another job's
+    spread does not fit the ring.  Every output equals the oracle's."""
+    wide = dict(SMALL, dm_step=300 / 128, nsamp_buf=16384, tdms=100)
+    rng = np.random.default_rng(15)
+    xs, exps, plans = [], [], []
+    for cfg, ring in ((SMALL, True), (wide, False)):
+        x = rng.normal(size=(cfg["n_chan"], cfg["nsamp_buf"])).astype(np.float32)
+        p, d = _oracle(orc, x, cfg)
+        plan = ppl.DedispPlan(**cfg)
+        assert (plan.tile == (32, 512, 0, 16)) == ring, plan.tile
+        assert np.array_equal(plan.run(torch.from_numpy(x).cuda()).cpu().numpy(), orc.dedisperse(x, d, p["nout"]))
+        xs.append(torch.from_numpy(x).cuda())
+        exps.append(orc.dedisperse(x, d, p["nout"]))
+        plans.append(plan)
+    outs = ppl.dedisperse_batch(plans, xs)
+    for o, e in zip(outs, exps):
+        assert np.array_equal(o.cpu().numpy(), e)
+
+
 def test_plan_counts_fused_delay_differences(orc, ppl):
     """The plan reports how many delays a fused multiply-add trial value
     (DedispersionKernel.cu:29 under contraction) would change: none with
