@@ -103,7 +103,9 @@ typedef struct {
 /* DedispersionModule::resize (src/lib/src/DedispersionModule.cpp:109-190):
  * builds the FP32 delay table on the host from the FP64 shift table exactly
  * as the reference (:152-162, DedispersionKernel.cu:29,36), uploads it and
- * picks the tile shape.  Fails with PPL_E_CONFIG when nsampBuf <= maxshift
+ * picks the tile shape: for FP32 input the ring kernel at 32 trials x 512
+ * samples where a tile's delay spread fits its staged row, else (and for
+ * f16 and u8) the one-word kernel at 32 (or fewer) trials x 512.  Fails with PPL_E_CONFIG when nsampBuf <= maxshift
  * (the module's throw at :175-178), when dmStep < 0 (delays would decrease
  * with the trial index) or when a delayed read would leave the channel row. */
 int ppl_dedisp_plan_create(ppl_dedisp_plan** plan, const ppl_dedisp_config* cfg);
@@ -139,8 +141,10 @@ int ppl_dedisperse(const ppl_dedisp_plan* plan, const void* d_in, void* d_out, v
  * Each job is a plan with its own input buffer and output; plans may differ
  * in channels, buffer length, DM count and range, but share the dtype.  All
  * jobs run under one DM x time tile, channel block and read width (tileDM =
- * tileT = 0: the first plan's), each with its own LDS row length.  Results
- * equal n separate ppl_dedisperse calls, bit for bit. */
+ * tileT = 0: the first plan's; when that is a ring tile another job's delay
+ * spread does not fit, the same tile with one-word reads), each with its own
+ * LDS row length.  Results equal n separate ppl_dedisperse calls, bit for
+ * bit. */
 typedef struct {
     const ppl_dedisp_plan* plan;
     const void* d_in;
@@ -157,7 +161,9 @@ int ppl_dedisp_autotune_batch(const ppl_dedisp_job* jobs, int n, int reps, void*
 /* The reference's C seam itself, with the reference's arguments (device
  * pointers; maxshift includes the remainder; outbufSize in bytes), computed
  * for every tdms (the reference's launch covers all rows only when
- * tdms % 40 == 0).  Builds its delay table from dmShift on the device.
+ * tdms % 40 == 0).  Builds its delay table from dmShift on the device, and
+ * from it the FP32 ring kernel's table where a 32-trial tile's delay spread
+ * fits the ring (the one-word kernel otherwise).
  * PPL_E_CONFIG for a negative dmShift, mstartdm or mdmstep (delays that
  * decrease with the trial index) or a delayed read outside the buffer. */
 int ppl_cache_dedisperse_loop(float* outbuff, long outbufSize, float* buff, float mstartdm,
